@@ -1,0 +1,12 @@
+"""diffractom_amd: diffractom's single-phase forward/adjoint operator and SPDHG-Poisson solver, rebuilt for AMD
+Instinct MI355X (gfx950) with hand-written HIP kernels behind a C ABI (include/diffractom_hip.h), keeping the
+reference's Python call signatures."""
+from .crystallography import Material, group_reflections_into_rings
+from .grid import Grid, MatrixGrid
+from .operator import SinglePhaseForwardOperator, estimate_L_power
+from .quadrature import QuadratureRule
+from .spdhg import SPDHGPoisson
+from .support import fov_support_mask
+
+__all__ = ["Material", "Grid", "MatrixGrid", "SinglePhaseForwardOperator", "SPDHGPoisson", "estimate_L_power",
+           "QuadratureRule", "fov_support_mask", "group_reflections_into_rings"]

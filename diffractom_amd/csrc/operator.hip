@@ -1,0 +1,625 @@
+// Forward/adjoint kernels of the single-phase operator on gfx950: channel-fastest transposes, the parallel-beam
+// Radon projector over many channels, the dense pole-figure evaluation with its CSR conversion, and the CSR PF
+// products. Wave64 throughout; channels (orientations) run across the lanes, so every image and sinogram access of
+// a wave is one contiguous 1 KiB (float4 per lane) and the ray geometry is uniform across the wave.
+#include <stdarg.h>
+
+#include "dfm_common.h"
+
+namespace dfm {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ------------------------------------------------------------------------------------------------ transposes
+// 64 x 64 tiles through LDS (+1 padding against bank conflicts), 256 threads.
+constexpr int TT = 64;
+
+__global__ void __launch_bounds__(256) gather_k_kernel(const float* __restrict__ coeffs, float* __restrict__ img_k,
+                                                       int64_t npix, int64_t k0, int kb, int kstride) {
+  __shared__ float tile[TT][TT + 1];
+  const int64_t p0 = (int64_t)blockIdx.x * TT;
+  const int kk0 = blockIdx.y * TT;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // ty 0..3
+  // read coeffs[k0 + kk, p] rows kk (tile rows), pixels along the lanes
+  for (int r = ty; r < TT; r += 4) {
+    const int kk = kk0 + r;
+    const int64_t p = p0 + tx;
+    float v = 0.0f;
+    if (kk < kb && p < npix) v = coeffs[(k0 + kk) * npix + p];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  // write img_k[p, kk] with the channels along the lanes (zeros for kb <= kk < kstride)
+  for (int r = ty; r < TT; r += 4) {
+    const int64_t p = p0 + r;
+    const int kk = kk0 + tx;
+    if (p < npix && kk < kstride) img_k[p * kstride + kk] = tile[tx][r];
+  }
+}
+
+__global__ void __launch_bounds__(256) scatter_k_kernel(const float* __restrict__ img_k, float* __restrict__ coeffs,
+                                                        int64_t npix, int64_t k0, int kb, int kstride) {
+  __shared__ float tile[TT][TT + 1];
+  const int64_t p0 = (int64_t)blockIdx.x * TT;
+  const int kk0 = blockIdx.y * TT;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < TT; r += 4) {
+    const int64_t p = p0 + r;
+    const int kk = kk0 + tx;
+    float v = 0.0f;
+    if (p < npix && kk < kb) v = img_k[p * kstride + kk];
+    tile[tx][r] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < TT; r += 4) {
+    const int kk = kk0 + r;
+    const int64_t p = p0 + tx;
+    if (kk < kb && p < npix) coeffs[(k0 + kk) * npix + p] = tile[r][tx];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ Radon
+struct AngleGeo {
+  float X, Y, T0, wscale, hw, top, pad0, pad1;
+};
+
+// Forward: one thread per (channel quad, NB consecutive bins, angle). For every image row along the axis with the
+// larger |coefficient| (the one whose pixels are at most a bin apart on the detector) only the pixels whose footprint
+// reaches the thread's bins are visited.
+template <int NB>
+__global__ void __launch_bounds__(256) radon_forward_kernel(const float4* __restrict__ img, float4* __restrict__ sino,
+                                                            const AngleGeo* __restrict__ geo, int nx, int ny, int ns,
+                                                            int kstride4, int k4_off, int k4_n, int skstride4,
+                                                            float scale) {
+  const int kq = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int s0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * NB;
+  const int a = blockIdx.z;
+  if (kq >= k4_n || s0 >= ns) return;
+  const int k = kq + k4_off;
+  const AngleGeo g = geo[a];
+  const bool rows_y = fabsf(g.X) >= fabsf(g.Y);
+  const float A = rows_y ? g.X : g.Y;
+  const float B = rows_y ? g.Y : g.X;
+  const int nu = rows_y ? nx : ny;
+  const int nv = rows_y ? ny : nx;
+  const int64_t du = rows_y ? (int64_t)kstride4 : (int64_t)nx * kstride4;
+  const int64_t dv = rows_y ? (int64_t)nx * kstride4 : (int64_t)kstride4;
+  const float inv = 1.0f / A;
+  const float hw = g.hw, top = g.top;
+  float4 acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4* col = img + k;
+  for (int v = 0; v < nv; ++v) {
+    const float c = B * (float)v + g.T0;
+    const float e1 = ((float)s0 - hw - c) * inv;
+    const float e2 = ((float)(s0 + NB - 1) + hw - c) * inv;
+    const int u0 = max((int)floorf(fminf(e1, e2)), 0);
+    const int u1 = min((int)ceilf(fmaxf(e1, e2)), nu - 1);
+    const float4* row = col + (int64_t)v * dv;
+    for (int u = u0; u <= u1; ++u) {
+      const float4 val = row[(int64_t)u * du];
+      const float t = A * (float)u + c - (float)s0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const float w = fminf(fmaxf(hw - fabsf(t - (float)j), 0.0f), top);
+        acc[j].x += w * val.x;
+        acc[j].y += w * val.y;
+        acc[j].z += w * val.z;
+        acc[j].w += w * val.w;
+      }
+    }
+  }
+  const float f = scale * g.wscale;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (s0 + j < ns) {
+      float4 o;
+      o.x = f * acc[j].x;
+      o.y = f * acc[j].y;
+      o.z = f * acc[j].z;
+      o.w = f * acc[j].w;
+      sino[((int64_t)a * ns + s0 + j) * skstride4 + kq] = o;
+    }
+  }
+}
+
+// Backward: one thread per (channel quad, pixel); the taps of the pixel at every angle. MODE 0 writes the image,
+// MODE 1 applies the SPDHG primal update to (x, z) with the step T.
+template <int NTAP, int MODE>
+__global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __restrict__ sino, float4* __restrict__ out,
+                                                             const AngleGeo* __restrict__ geo, int nx, int ny, int ns,
+                                                             int na, int kstride4, int k4_off, int k4_n, int skstride4,
+                                                             float4* __restrict__ x, float4* __restrict__ z,
+                                                             const float4* __restrict__ T, float inv_p, float lam) {
+  const int kq = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t pix = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (kq >= k4_n || pix >= (int64_t)nx * ny) return;
+  const int k = kq + k4_off;
+  const float px = (float)(pix % nx), py = (float)(pix / nx);
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int a = 0; a < na; ++a) {
+    const AngleGeo g = geo[a];
+    const float t = g.X * px + g.Y * py + g.T0;
+    const int sf = (int)floorf(t - g.hw) + 1;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4* srow = sino + (int64_t)a * ns * skstride4 + kq;
+#pragma unroll
+    for (int j = 0; j < NTAP; ++j) {
+      const int s = sf + j;
+      if (s >= 0 && s < ns) {
+        const float w = fminf(fmaxf(g.hw - fabsf(t - (float)s), 0.0f), g.top);
+        const float4 sv = srow[(int64_t)s * skstride4];
+        v.x += w * sv.x;
+        v.y += w * sv.y;
+        v.z += w * sv.z;
+        v.w += w * sv.w;
+      }
+    }
+    acc.x += g.wscale * v.x;
+    acc.y += g.wscale * v.y;
+    acc.z += g.wscale * v.z;
+    acc.w += g.wscale * v.w;
+  }
+  const int64_t o = pix * kstride4 + k;
+  if (MODE == 0) {
+    out[o] = acc;
+  } else {
+    float4 zz = z[o], xx = x[o];
+    const float4 tt = T[o];
+    float4 zn, xn;
+#define DFM_UPD(c)                                                                  \
+  zn.c = zz.c + acc.c;                                                              \
+  xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zn.c + inv_p * acc.c) + lam)) : 0.0f;
+    DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
+#undef DFM_UPD
+    z[o] = zn;
+    x[o] = xn;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ dense PF
+// One thread per entry (r, k, c, p) of the batch; the sum over quadrature nodes and the ring's pole axes.
+__global__ void __launch_bounds__(256) pf_eval_kernel(const float* __restrict__ coords, const float* __restrict__ poles,
+                                                      const int* __restrict__ axis_start,
+                                                      const float* __restrict__ axis_count,
+                                                      const float* __restrict__ inv_sigma2,
+                                                      const float* __restrict__ normf, const float* __restrict__ qw,
+                                                      float* __restrict__ out, int R, int kmax, int C, int P, int A,
+                                                      int S, int Se, int k0, int kb, float pf_cut, int quad_weighted) {
+  const int64_t total = (int64_t)R * kmax * C * P;
+  for (int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total;
+       gid += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(gid % P);
+    int64_t t = gid / P;
+    const int c = (int)(t % C);
+    t /= C;
+    const int k = (int)(t % kmax);
+    const int r = (int)(t / kmax);
+    if (k >= kb) {
+      out[gid] = 0.0f;
+      continue;
+    }
+    const int kg = k0 + k;
+    const float is2 = inv_sigma2[kg];
+    const int a0 = axis_start[p], a1 = axis_start[p + 1];
+    const float* pk = poles + (int64_t)kg * A * 3;
+    float total_w = 0.0f;
+    for (int s = 0; s < S; ++s) {
+      for (int se = 0; se < Se; ++se) {
+        const float* v = coords + (((((int64_t)r * S + s) * C + c) * Se + se) * P + p) * 3;
+        const float vx = v[0], vy = v[1], vz = v[2];
+        float node = 0.0f;
+        for (int a = a0; a < a1; ++a) {
+          const float d = fabsf(pk[a * 3 + 0] * vx + pk[a * 3 + 1] * vy + pk[a * 3 + 2] * vz);
+          float term = 0.0f;
+          const float x1 = (1.0f - d) * is2;
+          if (x1 < pf_cut) term += expf(-x1);
+          const float x2 = (1.0f + d) * is2;
+          if (x2 < pf_cut) term += expf(-x2);
+          node += axis_count[a] * term;
+        }
+        total_w += quad_weighted ? qw[s * Se + se] * node : node;
+      }
+    }
+    if (!quad_weighted) total_w = total_w / (float)(S * Se);
+    out[gid] = total_w * normf[kg];
+  }
+}
+
+// Forward-CSR rows (r, j): one block per angle, threads over j, loop over k (coalesced reads of the (r, k, :) rows).
+__global__ void __launch_bounds__(256) pf_count_fwd_kernel(const float* __restrict__ pf, int* __restrict__ counts,
+                                                           int kmax, int CP, int kb) {
+  const int r = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= CP) return;
+  const float* base = pf + (int64_t)r * kmax * CP + j;
+  int n = 0;
+  for (int k = 0; k < kb; ++k) n += base[(int64_t)k * CP] != 0.0f;
+  counts[(int64_t)r * CP + j] = n;
+}
+
+__global__ void __launch_bounds__(256) pf_fill_fwd_kernel(const float* __restrict__ pf, const int* __restrict__ row_ptr,
+                                                          uint16_t* __restrict__ col_k, float* __restrict__ val,
+                                                          int kmax, int CP, int kb) {
+  const int r = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= CP) return;
+  const float* base = pf + (int64_t)r * kmax * CP + j;
+  int o = row_ptr[(int64_t)r * CP + j];
+  for (int k = 0; k < kb; ++k) {
+    const float v = base[(int64_t)k * CP];
+    if (v != 0.0f) {
+      col_k[o] = (uint16_t)k;
+      val[o] = v;
+      ++o;
+    }
+  }
+}
+
+// Adjoint-CSR rows (r, k): one wave per row; the lanes scan the row's CP entries in chunks of 64 and compact them
+// with a ballot prefix count, keeping j ascending.
+__global__ void __launch_bounds__(256) pf_count_adj_kernel(const float* __restrict__ pf, int* __restrict__ counts,
+                                                           int R, int kmax, int CP, int kb) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= (int64_t)R * kb) return;
+  const int r = (int)(row / kb), k = (int)(row % kb);
+  const float* base = pf + ((int64_t)r * kmax + k) * CP;
+  int n = 0;
+  for (int j0 = 0; j0 < CP; j0 += 64) {
+    const int j = j0 + lane;
+    const bool nz = j < CP && base[j] != 0.0f;
+    n += __popcll(__ballot(nz));
+  }
+  if (lane == 0) counts[row] = n;
+}
+
+__global__ void __launch_bounds__(256) pf_fill_adj_kernel(const float* __restrict__ pf, const int* __restrict__ row_ptr,
+                                                          uint16_t* __restrict__ col_j, float* __restrict__ val,
+                                                          int R, int kmax, int CP, int kb) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= (int64_t)R * kb) return;
+  const int r = (int)(row / kb), k = (int)(row % kb);
+  const float* base = pf + ((int64_t)r * kmax + k) * CP;
+  int o = row_ptr[row];
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int j0 = 0; j0 < CP; j0 += 64) {
+    const int j = j0 + lane;
+    const float v = j < CP ? base[j] : 0.0f;
+    const unsigned long long m = __ballot(v != 0.0f);
+    if (v != 0.0f) {
+      const int pos = o + __popcll(m & below);
+      col_j[pos] = (uint16_t)j;
+      val[pos] = v;
+    }
+    o += __popcll(m);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ PF products
+constexpr int SPMM_TY = 16;          // detector positions per block (registers per thread)
+constexpr int SPMM_LDS_FLOATS = 12288;  // 48 KiB of staged sinogram / data rows
+
+// Forward: a thread per CSR row (angle, j); the sinogram rows of SPMM_TY detector positions staged in LDS chunk by
+// chunk of orientations (the CSR rows list k ascending, so each row advances through the chunks in order).
+__global__ void __launch_bounds__(256) spmm_forward_kernel(const float* __restrict__ sino, const int* __restrict__ row_ptr,
+                                                           const uint16_t* __restrict__ col_k,
+                                                           const float* __restrict__ val, float* __restrict__ data,
+                                                           const int* __restrict__ omegas, int my, int cp, int kstride,
+                                                           int kb, int kc, int compact, int accumulate) {
+  extern __shared__ float lsino[];
+  const int lid = threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + lid;
+  const int y0 = blockIdx.y * SPMM_TY;
+  const int rs = blockIdx.z;
+  const int r = omegas ? omegas[rs] : rs;
+  const int ny = min(SPMM_TY, my - y0);
+  const int rr = compact ? rs : r;
+  int i = 0, e = 0;
+  if (j < cp) {
+    i = row_ptr[(int64_t)rr * cp + j];
+    e = row_ptr[(int64_t)rr * cp + j + 1];
+  }
+  float acc[SPMM_TY];
+#pragma unroll
+  for (int t = 0; t < SPMM_TY; ++t) acc[t] = 0.0f;
+  const float* sbase = sino + ((int64_t)rs * my + y0) * kstride;
+  for (int k0 = 0; k0 < kb; k0 += kc) {
+    const int kn = min(kc, kb - k0);
+    __syncthreads();
+    for (int t = 0; t < ny; ++t)
+      for (int kk = lid; kk < kn; kk += blockDim.x) lsino[t * kc + kk] = sbase[(int64_t)t * kstride + k0 + kk];
+    __syncthreads();
+    const int k1 = k0 + kn;
+    for (; i < e; ++i) {
+      const int k = col_k[i];
+      if (k >= k1) break;
+      const float v = val[i];
+      const float* ls = lsino + (k - k0);
+#pragma unroll
+      for (int t = 0; t < SPMM_TY; ++t) acc[t] += v * ls[t * kc];
+    }
+  }
+  if (j < cp) {
+    float* dst = data + ((int64_t)r * my + y0) * cp + j;
+    if (accumulate) {
+      for (int t = 0; t < ny; ++t) dst[(int64_t)t * cp] += acc[t];
+    } else {
+      for (int t = 0; t < ny; ++t) dst[(int64_t)t * cp] = acc[t];
+    }
+  }
+}
+
+// Adjoint: a thread per CSR row (angle, k); data rows staged in LDS by chunks of segments j.
+__global__ void __launch_bounds__(256) spmm_adjoint_kernel(const float* __restrict__ data, const int* __restrict__ row_ptr,
+                                                           const uint16_t* __restrict__ col_j,
+                                                           const float* __restrict__ val, float* __restrict__ sino,
+                                                           const int* __restrict__ omegas, int my, int cp, int kstride,
+                                                           int kb, int jc, float alpha, int compact) {
+  extern __shared__ float ldata[];
+  const int lid = threadIdx.x;
+  const int k = blockIdx.x * blockDim.x + lid;
+  const int y0 = blockIdx.y * SPMM_TY;
+  const int rs = blockIdx.z;
+  const int r = omegas ? omegas[rs] : rs;
+  const int ny = min(SPMM_TY, my - y0);
+  const int rr = compact ? rs : r;
+  int i = 0, e = 0;
+  if (k < kb) {
+    i = row_ptr[(int64_t)rr * kb + k];
+    e = row_ptr[(int64_t)rr * kb + k + 1];
+  }
+  float acc[SPMM_TY];
+#pragma unroll
+  for (int t = 0; t < SPMM_TY; ++t) acc[t] = 0.0f;
+  const float* dbase = data + ((int64_t)r * my + y0) * cp;
+  for (int j0 = 0; j0 < cp; j0 += jc) {
+    const int jn = min(jc, cp - j0);
+    __syncthreads();
+    for (int t = 0; t < ny; ++t)
+      for (int jj = lid; jj < jn; jj += blockDim.x) ldata[t * jc + jj] = dbase[(int64_t)t * cp + j0 + jj];
+    __syncthreads();
+    const int j1 = j0 + jn;
+    for (; i < e; ++i) {
+      const int jj = col_j[i];
+      if (jj >= j1) break;
+      const float v = val[i];
+      const float* ld = ldata + (jj - j0);
+#pragma unroll
+      for (int t = 0; t < SPMM_TY; ++t) acc[t] += v * ld[t * jc];
+    }
+  }
+  if (k < kstride) {
+    float* dst = sino + ((int64_t)rs * my + y0) * kstride + k;
+    for (int t = 0; t < ny; ++t) dst[(int64_t)t * kstride] = (k < kb) ? alpha * acc[t] : 0.0f;
+  }
+}
+
+__global__ void zero_rows_kernel(float* __restrict__ data, const int* __restrict__ omegas, int64_t row_len,
+                                 int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = i / row_len;
+    data[(int64_t)omegas[s] * row_len + (i - s * row_len)] = 0.0f;
+  }
+}
+
+static int check_radon(const char* who, int nx, int ny, int ns, int na, int kstride, int k_off, int k_n,
+                       int skstride, int ntap) {
+  DFM_CHECK_ARG(nx > 0 && ny > 0 && ns > 0 && na >= 0, "%s: bad image/detector sizes", who);
+  DFM_CHECK_ARG(kstride > 0 && kstride % 4 == 0 && k_off >= 0 && k_off % 4 == 0 && k_n >= 0 && k_n % 4 == 0 &&
+                    k_off + k_n <= kstride,
+                "%s: channel range must be multiples of 4 inside kstride (k_off=%d k_n=%d kstride=%d)", who, k_off,
+                k_n, kstride);
+  DFM_CHECK_ARG(ntap >= 2 && ntap <= 4, "%s: ntap must be 2..4 (got %d)", who, ntap);
+  DFM_CHECK_ARG(skstride % 4 == 0 && skstride >= k_n, "%s: sinogram stride %d must be a multiple of 4 >= k_n", who,
+                skstride);
+  DFM_CHECK_ARG((int64_t)nx * ny * (kstride / 4) < (int64_t)1 << 40, "%s: array too large", who);
+  return DFM_OK;
+}
+
+template <int MODE>
+static int launch_backward(const char* who, const float* sino, float* img_k, const float* geo, int nx, int ny, int ns,
+                           int na, int kstride, int k_off, int k_n, int skstride, int ntap, float* x, float* z,
+                           const float* T, float inv_p, float lam, hipStream_t stream) {
+  const int k4n = k_n / 4;
+  const int64_t npix = (int64_t)nx * ny;
+  dim3 grid((unsigned)((k4n + 63) / 64), (unsigned)((npix + 3) / 4));
+#define DFM_BWD(NT)                                                                                               \
+  hipLaunchKernelGGL((radon_backward_kernel<NT, MODE>), grid, dim3(256), 0, stream, (const float4*)sino,          \
+                     (float4*)img_k, (const AngleGeo*)geo, nx, ny, ns, na, kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, \
+                     (float4*)z, (const float4*)T, inv_p, lam)
+  if (ntap == 2)
+    DFM_BWD(2);
+  else if (ntap == 3)
+    DFM_BWD(3);
+  else
+    DFM_BWD(4);
+#undef DFM_BWD
+  DFM_LAUNCH_CHECK(who);
+  return DFM_OK;
+}
+
+}  // namespace dfm
+
+using namespace dfm;
+
+// ================================================================================================ C ABI
+extern "C" {
+
+int dfm_version(void) { return 100; }
+
+const char* dfm_last_error(void) { return dfm::g_err; }
+
+int dfm_gather_k(const float* coeffs, float* img_k, int64_t npix, int64_t k0, int32_t kb, int32_t kstride,
+                 dfm_stream_t stream) {
+  DFM_CHECK_ARG(coeffs && img_k, "dfm_gather_k: null pointer");
+  DFM_CHECK_ARG(npix > 0 && k0 >= 0 && kb >= 0 && kstride >= kb && kstride % 4 == 0,
+                "dfm_gather_k: bad sizes npix=%lld kb=%d kstride=%d", (long long)npix, kb, kstride);
+  dim3 grid((unsigned)((npix + TT - 1) / TT), (unsigned)((kstride + TT - 1) / TT));
+  hipLaunchKernelGGL(gather_k_kernel, grid, dim3(256), 0, (hipStream_t)stream, coeffs, img_k, npix, k0, kb, kstride);
+  DFM_LAUNCH_CHECK("dfm_gather_k");
+  return DFM_OK;
+}
+
+int dfm_scatter_k(const float* img_k, float* coeffs, int64_t npix, int64_t k0, int32_t kb, int32_t kstride,
+                  dfm_stream_t stream) {
+  DFM_CHECK_ARG(coeffs && img_k, "dfm_scatter_k: null pointer");
+  DFM_CHECK_ARG(npix > 0 && k0 >= 0 && kb >= 0 && kstride >= kb && kstride % 4 == 0,
+                "dfm_scatter_k: bad sizes npix=%lld kb=%d kstride=%d", (long long)npix, kb, kstride);
+  if (kb == 0) return DFM_OK;
+  dim3 grid((unsigned)((npix + TT - 1) / TT), (unsigned)((kb + TT - 1) / TT));
+  hipLaunchKernelGGL(scatter_k_kernel, grid, dim3(256), 0, (hipStream_t)stream, img_k, coeffs, npix, k0, kb, kstride);
+  DFM_LAUNCH_CHECK("dfm_scatter_k");
+  return DFM_OK;
+}
+
+int dfm_radon_forward(const float* img_k, float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns,
+                      int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, float scale,
+                      int32_t ntap, dfm_stream_t stream) {
+  DFM_CHECK_ARG(img_k && sino && geo, "dfm_radon_forward: null pointer");
+  int st = check_radon("dfm_radon_forward", nx, ny, ns, n_angles, kstride, k_off, k_n, skstride, ntap);
+  if (st) return st;
+  if (n_angles == 0 || k_n == 0) return DFM_OK;
+  constexpr int NB = 2;
+  const int k4n = k_n / 4;
+  const int groups = (ns + NB - 1) / NB;
+  dim3 grid((unsigned)((k4n + 63) / 64), (unsigned)((groups + 3) / 4), (unsigned)n_angles);
+  hipLaunchKernelGGL(radon_forward_kernel<NB>, grid, dim3(256), 0, (hipStream_t)stream, (const float4*)img_k,
+                     (float4*)sino, (const AngleGeo*)geo, nx, ny, ns, kstride / 4, k_off / 4, k4n, skstride / 4, scale);
+  DFM_LAUNCH_CHECK("dfm_radon_forward");
+  return DFM_OK;
+}
+
+int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_t nx, int32_t ny, int32_t ns,
+                       int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap,
+                       dfm_stream_t stream) {
+  DFM_CHECK_ARG(img_k && geo && (sino || n_angles == 0), "dfm_radon_backward: null pointer");
+  int st = check_radon("dfm_radon_backward", nx, ny, ns, n_angles, kstride, k_off, k_n, skstride, ntap);
+  if (st) return st;
+  if (k_n == 0) return DFM_OK;
+  return launch_backward<0>("dfm_radon_backward", sino, img_k, geo, nx, ny, ns, n_angles, kstride, k_off, k_n,
+                            skstride, ntap, nullptr, nullptr, nullptr, 0.f, 0.f, (hipStream_t)stream);
+}
+
+int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
+                             int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap, float* x,
+                             float* z, const float* T, float inv_p, float lam, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && z && T && geo && (sino || n_angles == 0), "dfm_radon_backward_spdhg: null pointer");
+  int st = check_radon("dfm_radon_backward_spdhg", nx, ny, ns, n_angles, kstride, k_off, k_n, skstride, ntap);
+  if (st) return st;
+  if (k_n == 0) return DFM_OK;
+  return launch_backward<1>("dfm_radon_backward_spdhg", sino, nullptr, geo, nx, ny, ns, n_angles, kstride, k_off, k_n,
+                            skstride, ntap, x, z, T, inv_p, lam, (hipStream_t)stream);
+}
+
+int dfm_pf_eval_dense(const float* coords, const float* poles, const int32_t* axis_start, const float* axis_count,
+                      const float* inv_sigma2, const float* norm, const float* qw, float* out, int32_t R,
+                      int32_t kmax, int32_t C, int32_t P, int32_t A, int32_t S, int32_t Se, int32_t k0, int32_t kb,
+                      float pf_cut, int32_t quad_weighted, dfm_stream_t stream) {
+  DFM_CHECK_ARG(coords && poles && axis_start && axis_count && inv_sigma2 && norm && out && (qw || !quad_weighted),
+                "dfm_pf_eval_dense: null pointer");
+  DFM_CHECK_ARG(R > 0 && kmax > 0 && C > 0 && P > 0 && A >= P && S > 0 && Se > 0 && k0 >= 0 && kb >= 0 &&
+                    kb <= kmax,
+                "dfm_pf_eval_dense: bad sizes");
+  const int64_t total = (int64_t)R * kmax * C * P;
+  hipLaunchKernelGGL(pf_eval_kernel, dim3(grid_stride_blocks(total, 256, 1u << 20)), dim3(256), 0, (hipStream_t)stream,
+                     coords, poles, axis_start, axis_count, inv_sigma2, norm, qw, out, R, kmax, C, P, A, S, Se, k0, kb,
+                     pf_cut, quad_weighted);
+  DFM_LAUNCH_CHECK("dfm_pf_eval_dense");
+  return DFM_OK;
+}
+
+int dfm_pf_count_fwd(const float* pf, int32_t* counts, int32_t R, int32_t kmax, int32_t CP, int32_t kb,
+                     dfm_stream_t stream) {
+  DFM_CHECK_ARG(pf && counts && R > 0 && kmax > 0 && CP > 0 && kb <= kmax && kb <= 65536 && CP <= 65536,
+                "dfm_pf_count_fwd: bad arguments");
+  hipLaunchKernelGGL(pf_count_fwd_kernel, dim3((CP + 255) / 256, R), dim3(256), 0, (hipStream_t)stream, pf, counts,
+                     kmax, CP, kb);
+  DFM_LAUNCH_CHECK("dfm_pf_count_fwd");
+  return DFM_OK;
+}
+
+int dfm_pf_fill_fwd(const float* pf, const int32_t* row_ptr, uint16_t* col_k, float* val, int32_t R, int32_t kmax,
+                    int32_t CP, int32_t kb, dfm_stream_t stream) {
+  DFM_CHECK_ARG(pf && row_ptr && col_k && val && R > 0 && kmax > 0 && CP > 0 && kb <= kmax && kb <= 65536,
+                "dfm_pf_fill_fwd: bad arguments");
+  hipLaunchKernelGGL(pf_fill_fwd_kernel, dim3((CP + 255) / 256, R), dim3(256), 0, (hipStream_t)stream, pf, row_ptr,
+                     col_k, val, kmax, CP, kb);
+  DFM_LAUNCH_CHECK("dfm_pf_fill_fwd");
+  return DFM_OK;
+}
+
+int dfm_pf_count_adj(const float* pf, int32_t* counts, int32_t R, int32_t kmax, int32_t CP, int32_t kb,
+                     dfm_stream_t stream) {
+  DFM_CHECK_ARG(pf && counts && R > 0 && kmax > 0 && CP > 0 && kb <= kmax && CP <= 65536,
+                "dfm_pf_count_adj: bad arguments");
+  const int64_t rows = (int64_t)R * kb;
+  if (rows == 0) return DFM_OK;
+  hipLaunchKernelGGL(pf_count_adj_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, (hipStream_t)stream, pf,
+                     counts, R, kmax, CP, kb);
+  DFM_LAUNCH_CHECK("dfm_pf_count_adj");
+  return DFM_OK;
+}
+
+int dfm_pf_fill_adj(const float* pf, const int32_t* row_ptr, uint16_t* col_j, float* val, int32_t R, int32_t kmax,
+                    int32_t CP, int32_t kb, dfm_stream_t stream) {
+  DFM_CHECK_ARG(pf && row_ptr && col_j && val && R > 0 && kmax > 0 && CP > 0 && kb <= kmax && CP <= 65536,
+                "dfm_pf_fill_adj: bad arguments");
+  const int64_t rows = (int64_t)R * kb;
+  if (rows == 0) return DFM_OK;
+  hipLaunchKernelGGL(pf_fill_adj_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, (hipStream_t)stream, pf,
+                     row_ptr, col_j, val, R, kmax, CP, kb);
+  DFM_LAUNCH_CHECK("dfm_pf_fill_adj");
+  return DFM_OK;
+}
+
+int dfm_spmm_forward(const float* sino, const int32_t* row_ptr, const uint16_t* col_k, const float* val, float* data,
+                     const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
+                     int32_t compact, int32_t accumulate, dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && row_ptr && col_k && val && data, "dfm_spmm_forward: null pointer");
+  DFM_CHECK_ARG(n_angles >= 0 && my > 0 && cp > 0 && kb >= 0 && kb <= kstride && kb <= 65536,
+                "dfm_spmm_forward: bad sizes");
+  if (n_angles == 0) return DFM_OK;
+  const int kc = max(1, min(max(kb, 1), SPMM_LDS_FLOATS / SPMM_TY));
+  dim3 grid((unsigned)((cp + 255) / 256), (unsigned)((my + SPMM_TY - 1) / SPMM_TY), (unsigned)n_angles);
+  hipLaunchKernelGGL(spmm_forward_kernel, grid, dim3(256), (size_t)SPMM_TY * kc * sizeof(float), (hipStream_t)stream,
+                     sino, row_ptr, col_k, val, data, omegas, my, cp, kstride, kb, kc, compact, accumulate);
+  DFM_LAUNCH_CHECK("dfm_spmm_forward");
+  return DFM_OK;
+}
+
+int dfm_spmm_adjoint(const float* data, const int32_t* row_ptr, const uint16_t* col_j, const float* val, float* sino,
+                     const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
+                     float alpha, int32_t compact, dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && row_ptr && col_j && val && data, "dfm_spmm_adjoint: null pointer");
+  DFM_CHECK_ARG(n_angles >= 0 && my > 0 && cp > 0 && kb >= 0 && kb <= kstride && cp <= 65536,
+                "dfm_spmm_adjoint: bad sizes");
+  if (n_angles == 0) return DFM_OK;
+  const int jc = max(1, min(cp, SPMM_LDS_FLOATS / SPMM_TY));
+  dim3 grid((unsigned)((kstride + 255) / 256), (unsigned)((my + SPMM_TY - 1) / SPMM_TY), (unsigned)n_angles);
+  hipLaunchKernelGGL(spmm_adjoint_kernel, grid, dim3(256), (size_t)SPMM_TY * jc * sizeof(float), (hipStream_t)stream,
+                     data, row_ptr, col_j, val, sino, omegas, my, cp, kstride, kb, jc, alpha, compact);
+  DFM_LAUNCH_CHECK("dfm_spmm_adjoint");
+  return DFM_OK;
+}
+
+int dfm_zero_rows(float* data, const int32_t* omegas, int32_t n_sub, int64_t row_len, dfm_stream_t stream) {
+  DFM_CHECK_ARG(data && omegas && n_sub >= 0 && row_len > 0, "dfm_zero_rows: bad arguments");
+  const int64_t n = (int64_t)n_sub * row_len;
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, data,
+                     omegas, row_len, n);
+  DFM_LAUNCH_CHECK("dfm_zero_rows");
+  return DFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,683 @@
+"""SinglePhaseForwardOperator on MI355X: data = PF @ Radon(coefficients) and its adjoint, with the reference's
+Python signature (operators/single_phase_forward_operator.py:158-2206).
+
+Device arrays are torch tensors on a ROCm device (float32, C-contiguous), where the reference takes pyopencl arrays;
+NumPy in gives NumPy out. The CPU part (geometry, quadrature, pole axes) runs in float64 NumPy and is cast to float32
+as in the reference; every device computation is a HIP kernel of libdiffractom_hip.so called through its C ABI.
+
+Device layout: the orientation-fastest ("channel-fastest") images (Npix, kstride) and sinograms (angles, My,
+kstride) the Radon kernels work on; coefficients in the reference's (K, Ny, Nx) layout are transposed in and out per
+orientation batch (dfm_gather_k / dfm_scatter_k). The SPDHG solver keeps its arrays in the channel-fastest layout for
+the whole run.
+"""
+from __future__ import annotations
+
+import time
+from typing import Any, Mapping
+
+import numpy as np
+import torch
+
+from . import _lib
+from .crystallography import Material, group_reflections_into_rings
+from .quadrature import DEFAULT_TOL, QuadratureRule, setup_quadrature  # noqa: F401
+from .support import fov_support_mask
+
+KEV_ANGSTROM = 12.398       # single_phase_forward_operator.py:554
+AUTO_FFT_PIXELS = 400 * 400
+
+
+def _round4(n: int) -> int:
+    return -(-int(n) // 4) * 4
+
+
+def _resolve_device(ctx):
+    if ctx is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("SinglePhaseForwardOperator needs a ROCm GPU (torch.cuda.is_available() is False); "
+                               "there is no CPU path")
+        return torch.device("cuda", torch.cuda.current_device())
+    if isinstance(ctx, torch.device):
+        return ctx
+    if isinstance(ctx, int):
+        return torch.device("cuda", ctx)
+    if isinstance(ctx, str):
+        return torch.device(ctx)
+    raise TypeError(f"ctx must be None, a torch.device, a device index or a device string, not {type(ctx).__name__}")
+
+
+def check_device(a, shape, name, device=None):
+    """A torch tensor of the given shape, float32, C-contiguous (on the operator's device); raises TypeError /
+    ValueError as the reference's utils/arrays.py:13-23."""
+    if not isinstance(a, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on the GPU (or a NumPy array where accepted), "
+                        f"not {type(a).__name__}")
+    if a.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, not {a.dtype}")
+    if tuple(a.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, not {tuple(a.shape)}")
+    if not a.is_contiguous():
+        raise ValueError(f"{name} must be C-contiguous")
+    if device is not None and a.device != device:
+        raise ValueError(f"{name} is on {a.device}, the operator on {device}")
+    return a
+
+
+def as_host(a, shape, name):
+    """C-contiguous float32 NumPy array of the given shape: a itself if it is one (utils/arrays.py:26-32)."""
+    out = np.ascontiguousarray(a, dtype=np.float32)
+    if out.shape != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, not {out.shape}")
+    return out
+
+
+class SinglePhaseForwardOperator:
+    """Single-material pole-figure forward operator: data = PF @ Radon(coeffs) and its adjoint."""
+
+    coefficient_layout = "dense"
+
+    def __init__(self, cfg: Mapping[str, Any], material: Material, grid, max_gb: float, verbose: bool = False,
+                 normalized: bool = False, ctx=None, queue=None, pf_mode: str = "auto", sparse_max_fill: float = 0.3,
+                 sparse_max_fill_partial: float = 0.1, sparse_max_gb: float | None = None, projector: str = "auto",
+                 radon_kernel: str | None = None, fft_max_gb: float = 1.5, reserve_coefficient_arrays: int = 3,
+                 pf_cutoff_sigma: float | None = 4.0, quadrature="auto", quadrature_tol: float = DEFAULT_TOL,
+                 intensity_model=None, sparse_batch_max: int = 1024, rings=None, ring_two_theta=None,
+                 reflection_two_theta: bool = False, window_parallax=None, **kwargs):
+        t0 = time.perf_counter()
+        self.device = _resolve_device(ctx)
+        self.queue = queue                      # a torch stream, or None for the current stream at call time
+        self.cfg = {**cfg, **kwargs}
+        self.verbose = verbose
+        self.normalized = normalized
+        self.material = material
+        self.grid = grid
+        self.pf_cutoff_sigma = pf_cutoff_sigma
+        self.pf_cut = 6.0 if pf_cutoff_sigma is None else float(pf_cutoff_sigma) ** 2 / 2
+        self.reserve_coefficient_arrays = int(reserve_coefficient_arrays)
+        self.sparse_batch_max = int(sparse_batch_max)
+        if not normalized:
+            # the ring intensities need structure factors; the materials here have no atomic basis
+            # (single_phase_forward_operator.py:627-656 raises the same ValueError for such a material)
+            raise ValueError("normalized=False needs the ring intensities predicted from structure factors, but the "
+                             "material has no atomic basis. Use normalized=True.")
+        if intensity_model is not None:
+            raise NotImplementedError("intensity_model applies only with normalized=False")
+        if rings is not None or ring_two_theta is not None:
+            raise NotImplementedError("custom rings / ring_two_theta are not implemented")
+        if reflection_two_theta:
+            raise NotImplementedError("reflection_two_theta is not implemented")
+        if window_parallax is not None:
+            raise NotImplementedError("window_parallax is not implemented")
+        if projector not in ("auto", "native", "fft", "gratopy"):
+            raise ValueError(f"projector must be 'auto', 'native', 'fft' or 'gratopy', not {projector!r}")
+        if projector in ("fft", "gratopy"):
+            raise NotImplementedError(f"projector {projector!r} is not implemented (the native projector is)")
+        self.projector = "native"
+        if radon_kernel is None:
+            radon_kernel = "trapezoid"
+        if radon_kernel not in ("trapezoid", "hat"):
+            raise ValueError(f"kernel must be 'hat' or 'trapezoid', not {radon_kernel!r}")
+        self.radon_kernel = radon_kernel
+        if pf_mode not in ("dense", "sparse", "generated", "auto"):
+            raise ValueError(f"pf_mode must be 'dense', 'sparse', 'generated' or 'auto', not {pf_mode!r}")
+
+        # ---- geometry (single_phase_forward_operator.py:376-441)
+        c = self.cfg
+        self.ring_reflections = group_reflections_into_rings(material)
+        self.N_peaks = len(self.ring_reflections)
+        self.N_eta = int(c["N_eta"])
+        self.N_seg = self.N_peaks * self.N_eta
+        self.Nx, self.Ny, self.My, self.N_Omega = int(c["Nx"]), int(c["Ny"]), int(c["My"]), int(c["N_Omega"])
+        self.cor_offset = float(c["cor_offset"])
+        vs = c.get("voxel_size")
+        self.voxel_size = None if vs is None else float(vs)
+        self.image_width = float(self.Nx) if vs is None else float(vs) * max(self.Nx, self.Ny)
+        self.angle_range = np.array(c["angle_range"]) / 180 * np.pi
+        delta = (self.angle_range[1] - self.angle_range[0]) / self.N_Omega
+        self.angles = np.linspace(self.angle_range[0], self.angle_range[1], self.N_Omega, endpoint=False) + delta / 2
+        self.eta_angle_range = np.array(c.get("eta_angle_range", [0, 360])) / 180 * np.pi
+        eta_delta = (self.eta_angle_range[1] - self.eta_angle_range[0]) / self.N_eta
+        self.eta_angles = np.linspace(self.eta_angle_range[0], self.eta_angle_range[1], self.N_eta,
+                                      endpoint=False) + eta_delta / 2
+        mats, sig = grid.leaf_arrays()
+        self.grid_rot_cpu = np.asarray(mats, np.float64)
+        self.sigma_cpu = np.asarray(sig, np.float32)
+        self.K = len(self.grid_rot_cpu)
+        setup_quadrature(self, quadrature, quadrature_tol, float(np.min(self.sigma_cpu)))
+        self._material_parameters()
+        self._detector_coordinates()
+        self._pole_axes()
+        self._projector_geometry()
+        self.pf_batch_max_gb = float(max_gb)
+        self._make_batches(max_gb)
+        self.omega_subsets = None
+
+        # ---- PF matrix (single_phase_forward_operator.py:500-528, 1207-1374)
+        self.pf_mode = pf_mode
+        self.sparse_batches = None
+        self.dense_batches = None
+        self.pf_storage = None
+        self.sparse_nnz = 0
+        self.sparse_fill = 0.0
+        self._build_pf(pf_mode, sparse_max_fill, sparse_max_gb)
+        self.setup_s = time.perf_counter() - t0
+        if verbose:
+            print(f"PF matrix: {self.pf_mode} ({self.pf_storage}), {len(self.batches)} batches of up to "
+                  f"{self.K_batch_max}; nnz {self.sparse_nnz} (fill {100 * self.sparse_fill:.3f} %); "
+                  f"set up in {self.setup_s:.1f} s")
+
+    # ================================================================== set-up
+    def _material_parameters(self):
+        """single_phase_forward_operator.py:627-661 (normalized: every ring intensity 1)."""
+        first = [r[0] for r in self.ring_reflections]
+        self.h_cpu = np.asarray(self.material.h_vecs, dtype=np.float32)[first].copy(order="C")
+        self.intens_cpu = np.ones(self.N_peaks, dtype=np.float32)
+
+    def _detector_coordinates(self):
+        """Probed directions v (R, S, C, Se, P, 3) = R(omega)^T u(eta, theta_p) as float32
+        (single_phase_forward_operator.py:549-622); two-theta in float32 arithmetic as the reference's."""
+        from scipy.spatial.transform import Rotation
+        lam = KEV_ANGSTROM / self.cfg["energy"]
+        self.two_theta_peaks = (2.0 * np.arcsin(np.linalg.norm(self.h_cpu, axis=1) / (4.0 * np.pi) * lam)
+                                ).astype(np.float32)
+        Se = self.N_eta_subdivisions
+        eta2 = self.eta_angles_subdivided.reshape(self.N_eta, Se)
+        d0 = np.array(self.cfg["detector_direction_origin"])
+        d90 = np.array(self.cfg["detector_direction_positive_90"])
+        p0 = np.array(self.cfg["p_direction_0"])
+        k0 = np.asarray(self.cfg["k_direction_0"])
+        Rm = Rotation.from_rotvec(self.angles_subdivided[:, None] * k0).as_matrix()
+        out = []
+        for tt in self.two_theta_peaks:
+            half = tt / 2.0
+            dz = np.cos(eta2)[..., None] * d0[None, None, :] + np.sin(eta2)[..., None] * d90[None, None, :]
+            dz = dz * np.cos(half) - np.sin(half) * p0
+            pr = np.einsum("oij,esi->oesj", Rm, dz)
+            out.append(pr.reshape(self.N_Omega, self.N_Omega_subdivisions, self.N_eta, Se, 3))
+        self.coords_cpu = np.ascontiguousarray(np.stack(out, axis=-2), dtype=np.float32)
+        self.coords_gpu = torch.from_numpy(self.coords_cpu).to(self.device)
+
+    def _pole_axes(self):
+        """Distinct pole axes per ring and their image counts, poles[k, a] = U_k axis_a
+        (single_phase_forward_operator.py:742-826)."""
+        h = np.asarray(self.material.h_vecs_normed, np.float64)
+        sym = np.asarray(self.material.point_group_matrices, np.float64).reshape(-1, 3, 3)
+        mult = np.asarray(self.material.reflections["multiplicity"], np.float64)
+        axes, counts, start = [], [], [0]
+        for refl in self.ring_reflections:
+            share = mult[refl] / mult[refl].sum()
+            imgs = np.concatenate([sym @ h[f] for f in refl])
+            wts = np.concatenate([np.full(len(sym), w) for w in share])
+            first = np.argmax(np.abs(imgs) > 1e-6, axis=1)
+            imgs = imgs * np.sign(imgs[np.arange(len(imgs)), first])[:, None]
+            uniq, inv = np.unique(np.round(imgs, 6), axis=0, return_inverse=True)
+            cnt = np.bincount(inv.ravel(), weights=wts, minlength=len(uniq))
+            axes.append(uniq / np.linalg.norm(uniq, axis=1, keepdims=True))
+            counts.append(cnt)
+            start.append(start[-1] + len(uniq))
+        self.pole_axes_cpu = np.concatenate(axes)
+        self.N_axes = len(self.pole_axes_cpu)
+        self.axis_start_cpu = np.asarray(start, np.int32)
+        self.axis_count_cpu = np.concatenate(counts).astype(np.float32)
+        if not np.allclose(np.add.reduceat(self.axis_count_cpu, self.axis_start_cpu[:-1]), len(sym)):
+            raise AssertionError("pole-axis counts do not sum to the point-group order")
+        poles = np.einsum("kij,aj->kai", self.grid_rot_cpu, self.pole_axes_cpu)
+        dev = self.device
+        self.poles_gpu = torch.from_numpy(np.ascontiguousarray(poles, np.float32)).to(dev)
+        self.axis_start_gpu = torch.from_numpy(self.axis_start_cpu).to(dev)
+        self.axis_count_gpu = torch.from_numpy(self.axis_count_cpu).to(dev)
+        s = self.sigma_cpu
+        self.inv_sigma2_gpu = torch.from_numpy((1.0 / (s * s)).astype(np.float32)).to(dev)
+        self.norm_factor_gpu = torch.from_numpy((1.0 / (8.0 * np.pi * s * s)).astype(np.float32)).to(dev)
+        qw = self.quad_weights_cpu if self.quad_weights_cpu is not None else np.ones(1, np.float32)
+        self.quad_weights_gpu = torch.from_numpy(np.asarray(qw, np.float32)).to(dev)
+
+    def _projector_geometry(self):
+        """ParallelRadon's geometry (parallel_radon.py:92-124): X, Y, T0 per angle, the forward scale dx^2/ds and
+        the backprojection weight pi/N_Omega (single_phase_forward_operator.py:457-466), as the 8-float records of
+        the C ABI."""
+        n_max = max(self.Nx, self.Ny)
+        dx = self.image_width / float(n_max)
+        ds = 1.0
+        rho = ds / dx
+        ang = self.angles
+        X = np.cos(ang - 0.5 * np.pi) / rho
+        Y = np.sin(ang - 0.5 * np.pi) / rho
+        shift = np.full(self.N_Omega, self.cor_offset)
+        T0 = (self.My - 1) / 2.0 - X * (self.Nx - 1) / 2.0 - Y * (self.Ny - 1) / 2.0 - shift / ds
+        self.radon_scale = float(np.float32(dx * dx / ds))
+        self._bp_scale = 1.0 if self.voxel_size is None else self.voxel_size ** 2
+        w = np.full(self.N_Omega, np.pi / self.N_Omega * self._bp_scale)
+        X32, Y32 = X.astype(np.float32), Y.astype(np.float32)
+        A = np.maximum(np.abs(X32), np.abs(Y32)).astype(np.float64)
+        if self.radon_kernel == "hat":
+            if A.max() > 1.0 + 1e-6:
+                raise ValueError("kernel='hat' needs pixels no larger than the detector bins; use 'trapezoid'")
+            hw, top, inv_a = np.ones_like(A), np.ones_like(A), np.ones_like(A)
+            self.n_taps = 2
+        else:
+            hw, top, inv_a = 0.5 * (1.0 + A), np.minimum(1.0, A), 1.0 / A
+            self.n_taps = max(2, int(np.ceil(1.0 + A.max() - 1e-6)))
+        if self.n_taps > 4:
+            raise NotImplementedError("pixels larger than 3 detector bins (voxel_size) are not implemented")
+        z = np.zeros_like(A)
+        geo_f = np.stack([X, Y, T0, inv_a, hw, top, z, z], 1).astype(np.float32)      # forward: scale 1/A
+        geo_b = np.stack([X, Y, T0, w * inv_a, hw, top, z, z], 1).astype(np.float32)  # backward: w/A
+        self.geo_fwd_cpu, self.geo_bwd_cpu = geo_f, geo_b
+        self.geo_fwd = torch.from_numpy(geo_f).to(self.device)
+        self.geo_bwd = torch.from_numpy(geo_b).to(self.device)
+
+    def _make_batches(self, max_gb, K_batch=None):
+        """Orientation batches (single_phase_forward_operator.py:1026-1093): as many orientations as a dense PF batch
+        (R, K_batch, C, P) of max_gb holds, at most 65532 (16-bit CSR columns)."""
+        bytes_per_k = self.N_Omega * self.N_eta * self.N_peaks * 4
+        if K_batch is not None:
+            kb = max(1, min(int(K_batch), self.K))
+        else:
+            kb = max(int(float(max_gb) * 1024 ** 3 // bytes_per_k), 1)
+        kb = min(kb, self.K, 65532)
+        self.batches = []
+        k0 = 0
+        while k0 < self.K:
+            k1 = min(k0 + kb, self.K)
+            self.batches.append({"k_start": k0, "k_end": k1, "K_batch": k1 - k0})
+            k0 = k1
+        self.K_batch_max = _round4(max(b["K_batch"] for b in self.batches))
+        # channel-fastest layout of a whole coefficient array: batch b at columns koff .. koff + round4(Kb)
+        off = 0
+        for b in self.batches:
+            b["kstride"] = _round4(b["K_batch"])
+            b["koff"] = off
+            off += b["kstride"]
+        self.K_stride = off
+        for name, n in (("sinogram batch (N_Omega, My, K_batch)", self.N_Omega * self.My * self.K_batch_max),
+                        ("image batch (Nx, Ny, K_batch)", self.Nx * self.Ny * self.K_batch_max)):
+            if n >= 2 ** 31:
+                raise ValueError(f"The {name} has {n} elements, more than 2^31 - 1; lower max_gb.")
+
+    # ------------------------------------------------------------------ PF build
+    def _stream(self):
+        return _lib.stream_handle(self.queue)
+
+    def _eval_dense_batch(self, b, out=None):
+        """Dense PF of batch b, (R, kstride_b, C, P), rows beyond K_batch zero (dfm_pf_eval_dense)."""
+        R, C, P = self.N_Omega, self.N_eta, self.N_peaks
+        km = b["kstride"]
+        if out is None:
+            out = torch.empty((R, km, C, P), dtype=torch.float32, device=self.device)
+        qw = self.quad_weights_cpu is not None
+        _lib.call("dfm_pf_eval_dense", _lib.ptr(self.coords_gpu), _lib.ptr(self.poles_gpu),
+                  _lib.ptr(self.axis_start_gpu), _lib.ptr(self.axis_count_gpu), _lib.ptr(self.inv_sigma2_gpu),
+                  _lib.ptr(self.norm_factor_gpu), _lib.ptr(self.quad_weights_gpu), _lib.ptr(out), R, km, C, P,
+                  self.N_axes, self.N_Omega_subdivisions, self.N_eta_subdivisions, b["k_start"], b["K_batch"],
+                  float(self.pf_cut), int(qw), self._stream())
+        return out
+
+    def _csr_from_dense(self, dense, b):
+        """Both CSR forms of a dense batch (count, exclusive scan, fill)."""
+        R, CP = self.N_Omega, self.N_seg
+        km, kb = b["kstride"], b["K_batch"]
+        st = self._stream()
+        dev = self.device
+        cnt_f = torch.empty(R * CP, dtype=torch.int32, device=dev)
+        _lib.call("dfm_pf_count_fwd", _lib.ptr(dense), _lib.ptr(cnt_f), R, km, CP, kb, st)
+        cnt_a = torch.empty(R * kb, dtype=torch.int32, device=dev)
+        _lib.call("dfm_pf_count_adj", _lib.ptr(dense), _lib.ptr(cnt_a), R, km, CP, kb, st)
+        out = {}
+        for key, cnt in (("f", cnt_f), ("a", cnt_a)):
+            ptr64 = torch.zeros(cnt.numel() + 1, dtype=torch.int64, device=dev)
+            torch.cumsum(cnt, 0, out=ptr64[1:])
+            nnz = int(ptr64[-1].item())
+            if nnz >= 2 ** 31:
+                raise ValueError("Too many non-zeros in one orientation batch for int32 indices; lower max_gb.")
+            out["row_ptr_" + key] = ptr64.to(torch.int32)
+            out["nnz"] = nnz
+        n = max(out["nnz"], 1)
+        out["col_k"] = torch.empty(n, dtype=torch.int16, device=dev)
+        out["val_f"] = torch.empty(n, dtype=torch.float32, device=dev)
+        out["col_j"] = torch.empty(n, dtype=torch.int16, device=dev)
+        out["val_a"] = torch.empty(n, dtype=torch.float32, device=dev)
+        _lib.call("dfm_pf_fill_fwd", _lib.ptr(dense), _lib.ptr(out["row_ptr_f"]), _lib.ptr(out["col_k"]),
+                  _lib.ptr(out["val_f"]), R, km, CP, kb, st)
+        _lib.call("dfm_pf_fill_adj", _lib.ptr(dense), _lib.ptr(out["row_ptr_a"]), _lib.ptr(out["col_j"]),
+                  _lib.ptr(out["val_a"]), R, km, CP, kb, st)
+        return out
+
+    def _build_pf(self, mode, max_fill, max_gb):
+        R, CP = self.N_Omega, self.N_seg
+        if mode == "dense":
+            self._store_dense()
+            return
+        # fill of the first batch decides "auto" (single_phase_forward_operator.py:1240-1265)
+        b0 = self.batches[0]
+        dense = self._eval_dense_batch(b0)
+        nnz0 = int((dense != 0).sum().item())
+        fill = nnz0 / (R * b0["K_batch"] * CP)
+        if mode == "auto" and fill > max_fill:
+            del dense
+            self.pf_mode = "dense"
+            if self.verbose:
+                print(f"Sparse PF matrix: fill fraction {100 * fill:.2f} % > {100 * max_fill:.2f} %, dense PF path")
+            self._store_dense()
+            return
+        if mode == "generated":
+            del dense
+            self.pf_mode = "generated"
+            self.pf_storage = "none"
+            self.sparse_nnz = int(round(fill * R * self.K * CP))
+            self.sparse_fill = fill
+            return
+        sbs, nnz = [], 0
+        for i, b in enumerate(self.batches):
+            if i > 0:
+                dense = self._eval_dense_batch(b, out=dense if dense.shape[1] == b["kstride"] else None)
+            sb = self._csr_from_dense(dense, b)
+            nnz += sb["nnz"]
+            sbs.append(sb)
+        del dense
+        self.sparse_batches = sbs
+        self.pf_mode = "sparse"
+        self.pf_storage = "both"
+        self.sparse_nnz = nnz
+        self.sparse_fill = nnz / (R * self.K * CP)
+
+    def _store_dense(self):
+        """Dense PF batches (R, kstride_b, C*P) kept on the device (the dense-PF mode)."""
+        self.dense_batches = [self._eval_dense_batch(b).reshape(self.N_Omega, b["kstride"], self.N_seg)
+                              for b in self.batches]
+        self.pf_storage = "dense"
+
+    def _pf_batch(self, ib):
+        """The CSR pair of batch ib: stored, or evaluated now (generated mode; valid until the next call)."""
+        if self.sparse_batches is not None:
+            return self.sparse_batches[ib]
+        b = self.batches[ib]
+        return self._csr_from_dense(self._eval_dense_batch(b), b)
+
+    # ================================================================== shapes and helpers
+    @property
+    def coeff_shape(self):
+        return (self.K, self.Ny, self.Nx)
+
+    @property
+    def data_shape(self):
+        return (self.N_Omega, self.My, self.N_seg)
+
+    def support_mask(self):
+        return fov_support_mask(self.Nx, self.Ny, self.My, angles=self.angles, image_width=self.image_width,
+                                detector_width=self.My, detector_shift=self.cor_offset)
+
+    def device_bytes(self):
+        n = 0
+        for t in (self.coords_gpu, self.poles_gpu):
+            n += t.numel() * t.element_size()
+        for sb in self.sparse_batches or []:
+            n += sum(v.numel() * v.element_size() for v in sb.values() if isinstance(v, torch.Tensor))
+        for d in self.dense_batches or []:
+            n += d.numel() * 4
+        return n
+
+    def free_memory(self):
+        self.sparse_batches = None
+        self.dense_batches = None
+        self._scratch = None
+
+    def _scratch_buffers(self, n_angles):
+        """Per-batch scratch: channel-fastest image and sinogram of the largest batch."""
+        need_s = n_angles * self.My * self.K_batch_max
+        sc = getattr(self, "_scratch", None)
+        if sc is None or sc["sino"].numel() < need_s:
+            sc = {"img": torch.empty(self.Nx * self.Ny * self.K_batch_max, dtype=torch.float32, device=self.device),
+                  "sino": torch.empty(max(need_s, self.N_Omega * self.My * self.K_batch_max), dtype=torch.float32,
+                                      device=self.device)}
+            self._scratch = sc
+        return sc
+
+    # ================================================================== subsets
+    def set_omega_subsets(self, subsets):
+        """Subsets of the rotation angles for direct_cl / adjoint_cl(subset=m) (single_phase_forward_operator.py:
+        1541-1599): an int M (interleaved, angle i in subset i % M) or a list of angle-index arrays; returns the
+        sorted index arrays; the same subsets again is a no-op."""
+        if self.pf_mode not in ("sparse", "generated"):
+            raise NotImplementedError(f"omega subsets need pf_mode 'sparse' or 'generated' (this operator: "
+                                      f"{self.pf_mode!r})")
+        R = self.N_Omega
+        if np.isscalar(subsets):
+            M = int(subsets)
+            if not 1 <= M <= R:
+                raise ValueError(f"the number of subsets must be between 1 and N_Omega = {R}")
+            subsets = [np.arange(m, R, M) for m in range(M)]
+        out = []
+        for s in subsets:
+            s = np.unique(np.asarray(s, dtype=np.int64))
+            if s.size == 0 or s[0] < 0 or s[-1] >= R:
+                raise ValueError(f"every subset must hold angle indices in 0 .. {R - 1}")
+            out.append(s)
+        old = self.omega_subsets
+        if old is not None and len(old) == len(out) and all(np.array_equal(a, b) for a, b in zip(old, out)):
+            return out
+        self.omega_subsets = out
+        self._sub_omegas = [torch.from_numpy(s.astype(np.int32)).to(self.device) for s in out]
+        self._sub_geo_fwd = [self.geo_fwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
+        self._sub_geo_bwd = [self.geo_bwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
+        return out
+
+    def release_omega_subsets(self):
+        self.omega_subsets = None
+
+    def _subset(self, subset):
+        if self.omega_subsets is None:
+            raise ValueError("no omega subsets defined: call set_omega_subsets first")
+        return int(subset)
+
+    def _angle_set(self, subset):
+        """(n_angles, omegas tensor or None, forward geo, backward geo) of the full operator or subset m."""
+        if subset is None:
+            return self.N_Omega, None, self.geo_fwd, self.geo_bwd
+        m = self._subset(subset)
+        return len(self.omega_subsets[m]), self._sub_omegas[m], self._sub_geo_fwd[m], self._sub_geo_bwd[m]
+
+    def zero_subset_rows(self, data, subset):
+        m = self._subset(subset)
+        _lib.call("dfm_zero_rows", _lib.ptr(data), _lib.ptr(self._sub_omegas[m]), len(self.omega_subsets[m]),
+                  self.My * self.N_seg, self._stream())
+
+    # ================================================================== forward
+    def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles):
+        _lib.call("dfm_radon_forward", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(geo), self.Nx, self.Ny, self.My,
+                  n_angles, img_kstride, koff, kn, kn, self.radon_scale, self.n_taps, self._stream())
+
+    def _pf_forward(self, ib, sino, data, subset, accumulate):
+        """data rows (+)= PF_b sino for batch ib (sparse CSR product or dense GEMM)."""
+        b = self.batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        if self.pf_mode == "dense":
+            self._dense_forward(ib, sino, data, subset, accumulate)
+            return
+        sb = self._pf_batch(ib)
+        _lib.call("dfm_spmm_forward", _lib.ptr(sino), _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]),
+                  _lib.ptr(sb["val_f"]), _lib.ptr(data), _lib.ptr(om), n_ang, self.My, self.N_seg, b["kstride"],
+                  b["K_batch"], 0, int(accumulate), self._stream())
+
+    def _pf_adjoint(self, ib, data, sino, subset):
+        b = self.batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        alpha = float(self.N_Omega / np.pi)
+        if self.pf_mode == "dense":
+            self._dense_adjoint(ib, data, sino, subset, alpha)
+            return
+        sb = self._pf_batch(ib)
+        _lib.call("dfm_spmm_adjoint", _lib.ptr(data), _lib.ptr(sb["row_ptr_a"]), _lib.ptr(sb["col_j"]),
+                  _lib.ptr(sb["val_a"]), _lib.ptr(sino), _lib.ptr(om), n_ang, self.My, self.N_seg, b["kstride"],
+                  b["K_batch"], alpha, 0, self._stream())
+
+    def _dense_forward(self, ib, sino, data, subset, accumulate):
+        """Dense PF path: data[r] (+)= sino[r] (My x Kb) @ PF[r] (Kb x CP), batched over the angles."""
+        b = self.batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        pf = self.dense_batches[ib]
+        if om is not None:
+            pf = pf.index_select(0, om.long())
+        s3 = sino[: n_ang * self.My * b["kstride"]].view(n_ang, self.My, b["kstride"])
+        out = torch.bmm(s3, pf)
+        rows = torch.arange(self.N_Omega, device=self.device) if om is None else om.long()
+        if accumulate:
+            data.index_add_(0, rows, out)
+        else:
+            data.index_copy_(0, rows, out)
+
+    def _dense_adjoint(self, ib, data, sino, subset, alpha):
+        b = self.batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        pf = self.dense_batches[ib]
+        d = data
+        if om is not None:
+            pf = pf.index_select(0, om.long())
+            d = data.index_select(0, om.long())
+        pft = pf.transpose(1, 2).contiguous()
+        s3 = sino[: n_ang * self.My * b["kstride"]].view(n_ang, self.My, b["kstride"])
+        torch.bmm(d, pft, out=s3)
+        s3.mul_(alpha)
+
+    def direct_cl(self, coeffs, data, subset=None):
+        """In-place forward: data = PF @ Radon(coeffs); with subset=m only the rows of subset m's angles are
+        written, the others are left as they are (single_phase_forward_operator.py:1646-1729)."""
+        check_device(coeffs, self.coeff_shape, "coeffs", self.device)
+        check_device(data, self.data_shape, "data", self.device)
+        n_ang, _, geo, _ = self._angle_set(subset)
+        sc = self._scratch_buffers(n_ang)
+        npix = self.Nx * self.Ny
+        st = self._stream()
+        for ib, b in enumerate(self.batches):
+            ks = b["kstride"]
+            _lib.call("dfm_gather_k", _lib.ptr(coeffs), _lib.ptr(sc["img"]), npix, b["k_start"], b["K_batch"], ks, st)
+            self._radon_forward(sc["img"], ks, 0, ks, sc["sino"], geo, n_ang)
+            self._pf_forward(ib, sc["sino"], data, subset, accumulate=ib > 0)
+
+    def adjoint_cl(self, data, coeffs, subset=None):
+        """In-place adjoint: coeffs = A^T data (with subset=m: A_m^T, reading only subset m's rows)
+        (single_phase_forward_operator.py:1800-1817, 1966-2033)."""
+        check_device(data, self.data_shape, "data", self.device)
+        check_device(coeffs, self.coeff_shape, "coeffs", self.device)
+        n_ang, _, _, geo = self._angle_set(subset)
+        sc = self._scratch_buffers(n_ang)
+        npix = self.Nx * self.Ny
+        st = self._stream()
+        for ib, b in enumerate(self.batches):
+            ks = b["kstride"]
+            self._pf_adjoint(ib, data, sc["sino"], subset)
+            _lib.call("dfm_radon_backward", _lib.ptr(sc["sino"]), _lib.ptr(sc["img"]), _lib.ptr(geo), self.Nx,
+                      self.Ny, self.My, n_ang, ks, 0, ks, ks, self.n_taps, st)
+            _lib.call("dfm_scatter_k", _lib.ptr(sc["img"]), _lib.ptr(coeffs), npix, b["k_start"], b["K_batch"], ks, st)
+
+    # channel-fastest forms (used by the solver): x_k (Npix, K_stride), batch b at columns koff .. koff + kstride
+    def direct_k(self, x_k, data, subset=None):
+        """data (rows of subset) = A x for a channel-fastest coefficient array."""
+        n_ang, _, geo, _ = self._angle_set(subset)
+        sc = self._scratch_buffers(n_ang)
+        for ib, b in enumerate(self.batches):
+            self._radon_forward(x_k, self.K_stride, b["koff"], b["kstride"], sc["sino"], geo, n_ang)
+            self._pf_forward(ib, sc["sino"], data, subset, accumulate=ib > 0)
+
+    def adjoint_k(self, data, out_k, subset=None):
+        """out_k = A^T data (channel-fastest)."""
+        n_ang, _, _, geo = self._angle_set(subset)
+        sc = self._scratch_buffers(n_ang)
+        st = self._stream()
+        for ib, b in enumerate(self.batches):
+            self._pf_adjoint(ib, data, sc["sino"], subset)
+            _lib.call("dfm_radon_backward", _lib.ptr(sc["sino"]), _lib.ptr(out_k), _lib.ptr(geo), self.Nx, self.Ny,
+                      self.My, n_ang, self.K_stride, b["koff"], b["kstride"], b["kstride"], self.n_taps, st)
+
+    def to_k(self, coeffs, out_k=None):
+        """(K, Ny, Nx) -> channel-fastest (Npix, K_stride)."""
+        npix = self.Nx * self.Ny
+        if out_k is None:
+            out_k = torch.empty(npix * self.K_stride, dtype=torch.float32, device=self.device)
+        for b in self.batches:
+            # gather into the batch's columns: a (Npix, kstride) view is not contiguous, so gather per batch into
+            # a scratch and copy when there are several batches
+            if len(self.batches) == 1:
+                _lib.call("dfm_gather_k", _lib.ptr(coeffs), _lib.ptr(out_k), npix, b["k_start"], b["K_batch"],
+                          b["kstride"], self._stream())
+            else:
+                sc = self._scratch_buffers(1)
+                _lib.call("dfm_gather_k", _lib.ptr(coeffs), _lib.ptr(sc["img"]), npix, b["k_start"], b["K_batch"],
+                          b["kstride"], self._stream())
+                out_k.view(npix, self.K_stride)[:, b["koff"]:b["koff"] + b["kstride"]].copy_(
+                    sc["img"][: npix * b["kstride"]].view(npix, b["kstride"]))
+        return out_k
+
+    def from_k(self, x_k, coeffs):
+        """channel-fastest (Npix, K_stride) -> (K, Ny, Nx)."""
+        npix = self.Nx * self.Ny
+        for b in self.batches:
+            if len(self.batches) == 1:
+                _lib.call("dfm_scatter_k", _lib.ptr(x_k), _lib.ptr(coeffs), npix, b["k_start"], b["K_batch"],
+                          b["kstride"], self._stream())
+            else:
+                sc = self._scratch_buffers(1)
+                sc["img"][: npix * b["kstride"]].view(npix, b["kstride"]).copy_(
+                    x_k.view(npix, self.K_stride)[:, b["koff"]:b["koff"] + b["kstride"]])
+                _lib.call("dfm_scatter_k", _lib.ptr(sc["img"]), _lib.ptr(coeffs), npix, b["k_start"], b["K_batch"],
+                          b["kstride"], self._stream())
+        return coeffs
+
+    # ================================================================== allocating forms
+    def direct(self, coeffs):
+        """The forward operator, allocating its output: a torch tensor in gives a torch tensor; a NumPy array gives
+        a NumPy array (single_phase_forward_operator.py:1107-1127)."""
+        if isinstance(coeffs, torch.Tensor):
+            data = torch.empty(self.data_shape, dtype=torch.float32, device=self.device)
+            self.direct_cl(coeffs, data)
+            return data
+        x = torch.from_numpy(as_host(coeffs, self.coeff_shape, "coeffs")).to(self.device)
+        data = torch.empty(self.data_shape, dtype=torch.float32, device=self.device)
+        self.direct_cl(x, data)
+        return data.cpu().numpy()
+
+    def adjoint(self, data):
+        """The adjoint, allocating its output (single_phase_forward_operator.py:1777-1796)."""
+        if isinstance(data, torch.Tensor):
+            coeffs = torch.empty(self.coeff_shape, dtype=torch.float32, device=self.device)
+            self.adjoint_cl(data, coeffs)
+            return coeffs
+        d = torch.from_numpy(as_host(data, self.data_shape, "data")).to(self.device)
+        out = torch.empty(self.coeff_shape, dtype=torch.float32, device=self.device)
+        self.adjoint_cl(d, out)
+        return out.cpu().numpy()
+
+
+def _dot64(a, b):
+    """float64 dot product of two float32 device tensors (dfm_dot_partial)."""
+    part = torch.empty(2 * 2048, dtype=torch.float64, device=a.device)
+    n = _lib.ctypes.c_int32(0)
+    _lib.call("dfm_dot_partial", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(part), _lib.ctypes.byref(n),
+              _lib.stream_handle())
+    return float(part[: n.value].sum().item())
+
+
+def estimate_L_power(op, niter: int = 20, seed: int = 0, eps: float = 1e-30, verbose: int = 1) -> float:
+    """Lipschitz constant L = ||A^T A|| by power iteration with float64 dot products
+    (single_phase_forward_operator.py:2136-2206): the start is drawn as (Nx, Ny, K) standard normals and transposed
+    to (K, Ny, Nx)."""
+    if niter < 1:
+        raise ValueError("niter must be >= 1")
+    rng = np.random.default_rng(seed)
+    xh = np.ascontiguousarray(rng.standard_normal((op.Nx, op.Ny, op.K)).astype(np.float32).transpose(2, 1, 0))
+    x = torch.from_numpy(xh).to(op.device)
+    Ax = torch.empty(op.data_shape, dtype=torch.float32, device=op.device)
+    z = torch.empty(op.coeff_shape, dtype=torch.float32, device=op.device)
+    xn = np.sqrt(_dot64(x, x)) + eps
+    _lib.call("dfm_scale", _lib.ptr(x), float(1.0 / xn), x.numel(), _lib.stream_handle())
+    L = 0.0
+    for it in range(niter):
+        op.direct_cl(x, Ax)
+        op.adjoint_cl(Ax, z)
+        L = _dot64(x, z) / (_dot64(x, x) + eps)
+        zn = np.sqrt(_dot64(z, z)) + eps
+        _lib.call("dfm_scale", _lib.ptr(z), float(1.0 / zn), z.numel(), _lib.stream_handle())
+        x, z = z, x
+        if verbose:
+            print(f"[power {it + 1:02d}] L_est={L:.6e}  ||z||={zn:.6e}")
+    return float(L)

@@ -1,0 +1,299 @@
+"""SPDHGPoisson on MI355X: stochastic primal-dual hybrid gradient over rotation-angle subsets for the Poisson
+deviance, with the reference's signature and iteration (optimization/spdhg_poisson.py:298-751).
+
+Iteration, for the next subset m of a random permutation per epoch (p_m = 1/M):
+    y_m' = prox_{S F_m*}(y_m + S A_m x)            (float64 arithmetic, dfm_dual_update_sub)
+    dz = A_m^T (y_m' - y_m); z += dz; x = max(0, x - T (z + dz/p_m + lam))   (fused into the backprojection)
+    D_{m_next} = A_{m_next} x                       (forward of the next subset, into the same data-sized array)
+with S = gamma/(A 1_S), T = min_m p_m/(gamma A_m^T 1_counted), gamma = sqrt(rho), rho = 3e4/mean(b)^2 ("auto").
+
+MI355X design: x, z and T stay on the GPU for the whole run in the orientation-fastest layout (Npix, K) the Radon
+kernels read, so a step is the subset's CSR adjoint, one backprojection kernel that also applies the primal update,
+and the next subset's forward projection and CSR product: no gather/scatter passes and no dz array.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .operator import as_host, check_device
+
+RHO_AUTO = 3e4            # pdhg_poisson.py:33
+NAMES = ("x", "z", "T")
+DFM_MAX_PARTIALS = 2048
+
+
+def _periodic(a, dshape, name, device):
+    """(device tensor, period) of a per-segment, per-(translation, segment) or per-point array
+    (fista_poisson.py:287-308); None -> (None, 0)."""
+    if a is None:
+        return None, 0
+    n_seg, n_ys, n_all = dshape[-1], dshape[-2] * dshape[-1], int(np.prod(dshape))
+    if isinstance(a, torch.Tensor):
+        if a.dtype != torch.float32 or not a.is_contiguous() or a.numel() not in (n_seg, n_ys, n_all):
+            raise ValueError(f"{name} must be C-contiguous float32 with N_seg = {n_seg}, My * N_seg = {n_ys} or "
+                             f"{n_all} elements; got shape {tuple(a.shape)}, {a.dtype}")
+        return a.reshape(-1).to(device), int(a.numel())
+    h = np.ascontiguousarray(a, dtype=np.float32)
+    if h.size == 1:
+        h = np.full(n_seg, float(h.ravel()[0]), np.float32)
+    if h.size not in (n_seg, n_ys, n_all):
+        raise ValueError(f"{name} must have N_seg = {n_seg} elements, My * N_seg = {n_ys} or the data shape "
+                         f"{tuple(dshape)}; got shape {h.shape}")
+    return torch.from_numpy(h.ravel().copy()).to(device), int(h.size)
+
+
+class SPDHGPoisson:
+    """SPDHG over omega subsets for the Poisson deviance."""
+
+    def __init__(self, operator, subsets=8, prox_kind="nonneg", lam=0.0, support="fov", eps_counts=1e-3, rho="auto",
+                 seed=0, resident="auto", tv=0.0, tv_prob=None, tv_tree=None):
+        if prox_kind not in ("nonneg", "nonneg_l1"):
+            raise ValueError("SPDHGPoisson supports prox_kind 'nonneg' and 'nonneg_l1'")
+        if not (resident in ("auto", "all", "none") or set(resident) <= set(NAMES + ("qx", "qy"))):
+            raise ValueError(f"resident must be 'auto', 'all', 'none' or a list of names from {NAMES + ('qx', 'qy')}")
+        if float(tv) < 0:
+            raise ValueError("tv must be >= 0")
+        if tv_tree is not None:
+            raise NotImplementedError("tv_tree needs an operator with the local coefficient layout")
+        if float(tv) > 0:
+            raise NotImplementedError("the total-variation block (tv > 0) is not implemented")
+        self.op = operator
+        self.subsets = operator.set_omega_subsets(subsets)
+        self.M = len(self.subsets)
+        self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
+        self.eps_counts = float(eps_counts)
+        self.rho = rho if rho == "auto" else float(rho)
+        self.rng = np.random.default_rng(seed)
+        self.resident = resident
+        self.tv = 0.0
+        self.tv_prob = 0.0
+        self._p_data = 1.0 / self.M
+        self._inv_p_data = float(self.M)
+        self._last = None
+        self.support_mask = self._support(support)
+        self.state = None
+        self.graphs = False
+
+    def _support(self, support):
+        """The support option as a (Ny, Nx) uint8 device mask or None (optimization/prox.py:75-99)."""
+        op = self.op
+        if support is None or support is False:
+            return None
+        if isinstance(support, str):
+            if support != "fov":
+                raise ValueError(f"support must be 'fov', None or an (Ny, Nx) array, not {support!r}")
+            mask = op.support_mask()
+        else:
+            mask = np.asarray(support)
+        if mask.shape != (op.Ny, op.Nx):
+            raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
+        return torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8).ravel()).to(op.device)
+
+    # ---------------------------------------------------------------- subset order
+    def _epoch_order(self):
+        """A random permutation that never starts with the last subset of the previous epoch
+        (spdhg_poisson.py:363-370)."""
+        perm = self.rng.permutation(self.M)
+        if self.M > 1 and int(perm[0]) == self._last:
+            perm[[0, 1]] = perm[[1, 0]]
+        self._last = int(perm[-1])
+        return [int(m) for m in perm]
+
+    def _epoch_sequence(self):
+        return self._epoch_order()
+
+    # ---------------------------------------------------------------- reductions
+    def _partials(self):
+        return self._part
+
+    def _deviance(self, mu):
+        n = ctypes.c_int32(0)
+        _lib.call("dfm_deviance_partial", _lib.ptr(mu), _lib.ptr(self.b), _lib.ptr(self.w), self.pw, _lib.ptr(self.g),
+                  self.pg, _lib.ptr(self.bg), self.pbg, float(self.eps_counts), mu.numel(), _lib.ptr(self._part),
+                  ctypes.byref(n), self.op._stream())
+        return float(self._part[: n.value].sum().item())
+
+    def _mean_counted(self, a):
+        n = ctypes.c_int32(0)
+        _lib.call("dfm_mean_counted_partial", _lib.ptr(a), _lib.ptr(self.w), self.pw, a.numel(), _lib.ptr(self._part),
+                  ctypes.byref(n), self.op._stream())
+        v = self._part[: 2 * n.value].view(2, n.value).sum(dim=1).cpu().numpy()
+        return float(v[0] / max(v[1], 1.0))
+
+    # ---------------------------------------------------------------- steps
+    def _dual(self, m):
+        op = self.op
+        row = op.My * op.N_seg
+        _lib.call("dfm_dual_update_sub", _lib.ptr(self.y), _lib.ptr(self.D), _lib.ptr(self.sig), _lib.ptr(self.b),
+                  _lib.ptr(self.w), self.pw, _lib.ptr(self.g), self.pg, _lib.ptr(self.bg), self.pbg,
+                  float(self.eps_counts), _lib.ptr(op._sub_omegas[m]), len(self.subsets[m]), row, op._stream())
+
+    def _adjoint_update(self, m):
+        """z += A_m^T D; x updated (all batches): the CSR adjoint into the sinogram scratch, then the fused
+        backprojection + primal update on the batch's columns of x, z, T."""
+        op = self.op
+        n_ang, _, _, geo = op._angle_set(m)
+        sc = op._scratch_buffers(n_ang)
+        st = op._stream()
+        for ib, bt in enumerate(op.batches):
+            op._pf_adjoint(ib, self.D, sc["sino"], m)
+            _lib.call("dfm_radon_backward_spdhg", _lib.ptr(sc["sino"]), _lib.ptr(geo), op.Nx, op.Ny, op.My, n_ang,
+                      op.K_stride, bt["koff"], bt["kstride"], bt["kstride"], op.n_taps, _lib.ptr(self.xk),
+                      _lib.ptr(self.zk), _lib.ptr(self.Tk), float(self._inv_p_data), float(self.lam), st)
+
+    def _forward(self, m):
+        """D (rows of subset m, or all rows for m None) = A x."""
+        self.op.direct_k(self.xk, self.D, subset=m)
+
+    # ---------------------------------------------------------------- run
+    def run(self, x0, b, n_epochs, weights=None, gain=None, background=None, verbose=0, diagnostics_interval=1,
+            callback=None, state=None):
+        """spdhg_poisson.py:513-751. x0: a torch tensor on the GPU (updated in place and returned) or a NumPy array
+        (updated in place if C-contiguous float32, else a converted copy is returned). callback(epoch, x, stats)
+        after every epoch; a true return stops the run. state: solver.state of an earlier run (z and y; without
+        "z", z = A^T y is recomputed)."""
+        t_start = time.perf_counter()
+        op = self.op
+        dev = op.device
+        cshape, dshape = tuple(op.coeff_shape), tuple(op.data_shape)
+        x_on_gpu = isinstance(x0, torch.Tensor)
+        host_x = None if x_on_gpu else as_host(x0, cshape, "x0")
+        x_dev = check_device(x0, cshape, "x0", dev) if x_on_gpu else torch.from_numpy(host_x).to(dev)
+        if isinstance(b, torch.Tensor):
+            self.b = check_device(b, dshape, "b", dev)
+        else:
+            self.b = torch.from_numpy(as_host(b, dshape, "b")).to(dev)
+        dummy = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.w, self.pw = _periodic(weights, dshape, "weights", dev)
+        self.g, self.pg = _periodic(gain, dshape, "gain", dev)
+        self.bg, self.pbg = _periodic(background, dshape, "background", dev)
+        self.w = self.w if self.w is not None else dummy
+        self.g = self.g if self.g is not None else dummy
+        self.bg = self.bg if self.bg is not None else dummy
+        self._part = torch.empty(2 * DFM_MAX_PARTIALS, dtype=torch.float64, device=dev)
+        nk = op.Nx * op.Ny * op.K_stride
+        npix = op.Nx * op.Ny
+        st = op._stream()
+        supp = self.support_mask
+        self.y = torch.zeros(dshape, dtype=torch.float32, device=dev)
+        self.sig = torch.empty(dshape, dtype=torch.float32, device=dev)
+        self.D = torch.empty(dshape, dtype=torch.float32, device=dev)
+        nd = int(np.prod(dshape))
+
+        rho = self.rho
+        if rho == "auto":
+            mean_b = self._mean_counted(self.b)
+            rho = RHO_AUTO / mean_b ** 2 if mean_b > 0 else 1.0
+        self.rho_used = float(rho)
+        gamma = float(np.sqrt(rho))
+        p = self._p_data
+
+        # resident channel-fastest arrays
+        self.xk = op.to_k(x_dev)
+        self.zk = torch.zeros(nk, dtype=torch.float32, device=dev)
+        self.Tk = torch.empty(nk, dtype=torch.float32, device=dev)
+        self.resident_used = NAMES
+
+        # S from the row sums of A 1_S (spdhg_poisson.py:621-631)
+        ones = torch.empty(nk, dtype=torch.float32, device=dev)
+        _lib.call("dfm_support_ones_k", _lib.ptr(ones), _lib.ptr(supp), npix, op.K_stride, op.K_stride, st)
+        self._zero_padding(ones)
+        op.direct_k(ones, self.D)
+        self.scales = {"row_sum_mean": self._mean_counted(self.D), "b_mean": self._mean_counted(self.b),
+                       "gain_mean": 1.0 if self.pg == 0 else float(self.g.mean().item())}
+        _lib.call("dfm_dual_steps", _lib.ptr(self.sig), _lib.ptr(self.D), _lib.ptr(self.w), self.pw, gamma, nd, st)
+        # T = min_m p / (gamma A_m^T 1_counted) on the support (spdhg_poisson.py:632-643)
+        _lib.call("dfm_set_mask_ones", _lib.ptr(self.D), _lib.ptr(self.w), self.pw, nd, st)
+        _lib.call("dfm_tinit", _lib.ptr(self.Tk), npix, op.K_stride, st)
+        colsum = ones
+        for m in range(self.M):
+            op.adjoint_k(self.D, colsum, subset=m)
+            _lib.call("dfm_tmin", _lib.ptr(self.Tk), _lib.ptr(colsum), _lib.ptr(supp), npix, op.K_stride,
+                      op.K_stride, float(np.float32(p / gamma)), st)
+        _lib.call("dfm_tfinal", _lib.ptr(self.Tk), npix, op.K_stride, op.K_stride, st)
+        self._zero_padding(self.Tk)
+        del ones, colsum
+        _lib.call("dfm_project_support_k", _lib.ptr(self.xk), _lib.ptr(supp), npix, op.K_stride, op.K_stride, st)
+        self._zero_padding(self.xk)
+        if state is not None:
+            self.y.copy_(torch.from_numpy(as_host(state["y"], dshape, "state['y']")).to(dev))
+            if state.get("z") is not None:
+                zc = torch.from_numpy(as_host(state["z"], cshape, "state['z']")).to(dev)
+                op.to_k(zc, self.zk)
+            else:
+                op.adjoint_k(self.y, self.zk)
+        t_setup = time.perf_counter() - t_start
+
+        ep0 = int(state["epoch"]) if state is not None else 0
+        self.iter_stats = []
+        f = float("nan")
+        seq = self._epoch_sequence()
+        op.zero_subset_rows(self.D, seq[0])
+        self._forward(seq[0])
+        ep = ep0
+        for e in range(n_epochs):
+            ep = ep0 + e + 1
+            last = e + 1 == n_epochs
+            want_f = (e + 1) % max(1, diagnostics_interval) == 0 or last
+            nseq = None
+            for j, blk in enumerate(seq):
+                end = j + 1 == len(seq)
+                if not end:
+                    nb = seq[j + 1]
+                elif not last:
+                    nseq = self._epoch_sequence()
+                    nb = nseq[0]
+                else:
+                    nb = None
+                self._dual(blk)
+                self._adjoint_update(blk)
+                if (end and want_f) or nb is None or nb == blk:
+                    if (end and want_f) or nb is not None:
+                        self._forward(None)                      # A x on every row
+                        if end and want_f:
+                            f = self._deviance(self.D)
+                else:
+                    self._forward(nb)
+            if nseq is not None:
+                seq = nseq
+            torch.cuda.synchronize(dev)
+            stats = {"epoch": ep, "f": f if want_f else None, "rho": self.rho_used,
+                     "time": time.perf_counter() - t_start}
+            self.iter_stats.append(stats)
+            stop = False
+            if callback is not None:
+                op.from_k(self.xk, x_dev)
+                if not x_on_gpu:
+                    host_x[...] = x_dev.cpu().numpy()
+                stop = bool(callback(ep, x_dev if x_on_gpu else host_x, stats))
+            if verbose and want_f:
+                print(f"[spdhg epoch {ep:4d}] f={f:.8e} {time.perf_counter() - t_start:.1f} s", flush=True)
+            if stop:
+                break
+        op.from_k(self.xk, x_dev)
+        zc = torch.empty(cshape, dtype=torch.float32, device=dev)
+        op.from_k(self.zk, zc)
+        torch.cuda.synchronize(dev)
+        self.state = {"epoch": ep0 + len(self.iter_stats), "z": zc.cpu().numpy(), "y": self.y.cpu().numpy()}
+        self.final_stats = {"epochs": len(self.iter_stats), "final_f": f, "setup_s": t_setup, "rho": self.rho_used,
+                            "subsets": self.M, "resident": self.resident_used,
+                            "time": time.perf_counter() - t_start, "tv": 0.0, "tv_prob": 0.0, **self.scales}
+        if not x_on_gpu:
+            host_x[...] = x_dev.cpu().numpy()
+        self.b = self.w = self.g = self.bg = self._part = self.y = self.sig = self.D = None
+        self.xk = self.zk = self.Tk = None
+        return x_dev if x_on_gpu else host_x
+
+    def _zero_padding(self, a_k):
+        """Padding channels (K_batch < kstride) of a channel-fastest array to 0."""
+        op = self.op
+        npix = op.Nx * op.Ny
+        v = a_k.view(npix, op.K_stride)
+        for bt in op.batches:
+            if bt["kstride"] > bt["K_batch"]:
+                v[:, bt["koff"] + bt["K_batch"]: bt["koff"] + bt["kstride"]] = 0.0

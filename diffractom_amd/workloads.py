@@ -1,0 +1,127 @@
+"""Synthetic workloads of the benchmark configurations: materials, scan geometry, orientation grids, Voronoi
+ground truths and Poisson data. Every generator is seeded and deterministic.
+
+  al_c0  Al (Fm-3m, a 4.05 A, lambda 0.2 A), 5 rings, 180 omega over 180 deg, My 103, 180 eta, 99 x 99 pixels,
+         K = 2000 random fundamental-zone orientations, sigma 2 deg; 40 Voronoi grains in a disc of radius 45 px
+         (one orientation each, coefficient 1); Poisson counts at a mean of 1000; SPDHG M = 10, 200 steps.
+  al_c1  The same scan with a uniform cubic fundamental-zone grid at 3 deg spacing (random orientations thinned to
+         0.6 x 3 deg: K = 29713), sigma 2 deg;
+         80 grains with coefficients U(0.5, 1.5); SPDHG M = 10, 300 steps.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.spatial import cKDTree
+from scipy.spatial.transform import Rotation
+
+from .crystallography import Material, point_group
+from .grid import MatrixGrid
+
+AL_RINGS = [[1, 1, 1], [2, 0, 0], [2, 2, 0], [3, 1, 1], [2, 2, 2]]
+TI_RINGS = [[1, 0, 0], [0, 0, 2], [1, 0, 1], [1, 0, 2], [1, 1, 0], [1, 0, 3], [2, 0, 0], [1, 1, 2]]
+
+
+def al_material(wavelength=0.2):
+    return Material.from_lattice_parameters(a=4.05, b=4.05, c=4.05, alpha=90, beta=90, gamma=90,
+                                            symmetry_group="cubic", wavelength_A=wavelength, hkl_list=AL_RINGS)
+
+
+def ti_material(wavelength=0.2):
+    return Material.from_lattice_parameters(a=2.95, b=2.95, c=4.68, alpha=90, beta=90, gamma=120,
+                                            symmetry_group="hexagonal", wavelength_A=wavelength, hkl_list=TI_RINGS)
+
+
+def scan_cfg(n, my, n_omega, n_eta, wavelength=0.2, angle_range=(0, 180), cor_offset=0.0):
+    """The reference's cfg (README conventions: rotation axis z, beam x, eta 0 along -y, eta 90 along -z)."""
+    return {"Nx": n, "Ny": n, "My": my, "N_Omega": n_omega, "angle_range": list(angle_range),
+            "cor_offset": cor_offset, "N_eta": n_eta, "eta_angle_range": [0, 360], "energy": 12.398 / wavelength,
+            "k_direction_0": [0, 0, 1], "p_direction_0": [1, 0, 0], "detector_direction_origin": [0, -1, 0],
+            "detector_direction_positive_90": [0, 0, -1]}
+
+
+def to_fundamental_zone(R, symmetry):
+    """Each rotation replaced by the equivalent R g with the largest trace (utils/grid.py:341-358)."""
+    pg = point_group(symmetry)
+    prod = np.einsum("nij,gjk->ngik", R, pg)
+    best = np.argmax(np.einsum("ngii->ng", prod), axis=1)
+    return prod[np.arange(len(R)), best]
+
+
+def random_fz(K, symmetry, seed):
+    """K uniform random orientations (normalised Gaussian quaternions from default_rng(seed)) in the FZ."""
+    q = np.random.default_rng(seed).standard_normal((K, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return to_fundamental_zone(Rotation.from_quat(q).as_matrix(), symmetry)
+
+
+def uniform_fz(spacing_deg, symmetry, seed=0, thin=0.7):
+    """Near-uniform FZ orientations: 3 x (SO(3) volume / spacing^3 / |G|) random orientations, thinned greedily to a
+    minimum misorientation of thin x spacing (the "random" method of odf_basis.py:117-139 thins at 0.7)."""
+    pg = point_group(symmetry)
+    n = int(3 * 8 * np.pi ** 2 / np.deg2rad(spacing_deg) ** 3 / len(pg))
+    R = random_fz(n, symmetry, seed)
+    rot = Rotation.from_matrix(R)
+    q = rot.as_quat()
+    eq = np.concatenate([(rot * Rotation.from_matrix(g)).as_quat() for g in pg])
+    eq = np.vstack([eq, -eq])
+    tree = cKDTree(eq)
+    radius = 2.0 * np.sin(np.deg2rad(thin * spacing_deg) / 4.0)
+    used = np.zeros(n, bool)
+    keep = np.zeros(n, bool)
+    for i in range(n):
+        if used[i]:
+            continue
+        keep[i] = True
+        used[np.asarray(tree.query_ball_point(q[i], r=radius), dtype=np.int64) % n] = True
+    return R[keep]
+
+
+def voronoi_truth(K, n, n_grains, radius, seed, coeff=None, orientations=None):
+    """(K, n, n) coefficients: n_grains Voronoi cells of seeds uniform in a centred disc of `radius` pixels; every
+    pixel of the disc gets its cell's orientation with coefficient 1 (coeff None) or U(coeff) per grain."""
+    rng = np.random.default_rng(seed)
+    c = (n - 1) / 2.0
+    r = radius * np.sqrt(rng.random(n_grains))
+    t = rng.uniform(0, 2 * np.pi, n_grains)
+    sx, sy = c + r * np.cos(t), c + r * np.sin(t)
+    ori = rng.choice(K, n_grains, replace=False) if orientations is None else np.asarray(orientations)
+    val = np.ones(n_grains) if coeff is None else rng.uniform(coeff[0], coeff[1], n_grains)
+    yy, xx = np.mgrid[:n, :n]
+    inside = (xx - c) ** 2 + (yy - c) ** 2 <= radius ** 2
+    d2 = (xx[..., None] - sx) ** 2 + (yy[..., None] - sy) ** 2
+    cell = np.argmin(d2, axis=-1)
+    x = np.zeros((K, n, n), np.float32)
+    iy, ix = np.nonzero(inside)
+    x[ori[cell[iy, ix]], iy, ix] = val[cell[iy, ix]]
+    return x
+
+
+def poisson_counts(ax, mean_counts, seed, weights=None):
+    """Poisson counts with the prediction scaled to a mean of mean_counts over the counted points."""
+    ax = np.asarray(ax, np.float64)
+    w = np.ones(ax.size) if weights is None else np.asarray(weights, np.float64).ravel()[np.arange(ax.size) %
+                                                                                      np.asarray(weights).size]
+    counted = w.reshape(ax.shape) != 0
+    s = mean_counts / ax[counted].mean()
+    return np.random.default_rng(seed).poisson(s * np.maximum(ax, 0)).astype(np.float32), s
+
+
+CONFIGS = {
+    "al_c0": dict(n=99, my=103, n_omega=180, n_eta=180, K=2000, grid="random", sigma_deg=2.0, grains=40,
+                  radius=45, coeff=None, mean_counts=1000, subsets=10, steps=200, seed=0),
+    "al_c1": dict(n=99, my=103, n_omega=180, n_eta=180, K=None, grid="uniform3", sigma_deg=2.0, grains=80,
+                  radius=45, coeff=(0.5, 1.5), mean_counts=1000, subsets=10, steps=300, seed=1),
+}
+
+
+def make_grid(name, K=None):
+    """(MatrixGrid, material, cfg) of a named configuration (K overrides the grid size of a random grid)."""
+    p = CONFIGS[name]
+    if p["grid"] == "random":
+        R = random_fz(K or p["K"], "cubic", p["seed"])
+    else:
+        R = uniform_fz(3.0, "cubic", seed=p["seed"], thin=0.6)      # 29713 orientations
+        if K is not None:
+            R = R[:K]
+    return MatrixGrid(R, np.deg2rad(p["sigma_deg"])), al_material(), scan_cfg(p["n"], p["my"], p["n_omega"],
+                                                                                p["n_eta"])

@@ -1,0 +1,154 @@
+/*
+ * C ABI of the MI355X (gfx950) kernels of the diffractom single-phase operator and its SPDHG-Poisson solver.
+ *
+ * Plain device pointers, sizes and a hipStream_t; no torch types. Every function is asynchronous on the given
+ * stream, allocates nothing, and returns 0 (DFM_OK) or a negative status; dfm_last_error() gives the message of
+ * the last failure on the calling thread.
+ *
+ * The reference (diffractom, Python + OpenCL) launches its kernels through pyopencl; each entry point below replaces
+ * the pyopencl launch(es) cited with it (paths relative to the reference's src/diffractom/). Layouts:
+ *   coefficients  (K, Ny, Nx) float32, pixel p = x + Nx*y       (utils/arrays.py:3-8)
+ *   channel-fastest images   img_k (Npix, kstride) float32, kstride a multiple of 4 (operators/parallel_radon.py:34-38)
+ *   sinograms     (n_angles, My, kstride) float32, channel fastest
+ *   data          (N_Omega, My, CP) float32, CP = N_eta * N_rings, segment j = c*N_rings + p
+ *   geometry      geo (n_angles, 8) float32 per angle: X, Y, T0, wscale, hw, top, 0, 0 (see dfm_radon_forward)
+ *   CSR           forward rows (angle, j) listing orientations k (uint16, ascending);
+ *                 adjoint rows (angle, k) listing segments j (uint16, ascending); float32 values; int32 row pointers
+ */
+#ifndef DIFFRACTOM_HIP_H
+#define DIFFRACTOM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* dfm_stream_t; /* the hipStream_t of the caller (NULL: the null stream) */
+
+enum {
+  DFM_OK = 0,
+  DFM_ERR_ARG = -1,         /* bad argument (shape, size, range): the reference raises ValueError/TypeError */
+  DFM_ERR_UNSUPPORTED = -2, /* unsupported combination: the reference raises NotImplementedError */
+  DFM_ERR_LAUNCH = -3,      /* kernel launch failure */
+  DFM_ERR_MEMORY = -4       /* out of memory: the reference raises MemoryError */
+};
+
+/* Library version (major*10000 + minor*100 + patch) and the message of the last failure on this thread. */
+int dfm_version(void);
+const char* dfm_last_error(void);
+
+/* ---------------------------------------------------------------- layout (operators/parallel_radon.py:221-236,
+ * radon_kernels.cl:192-244: gather_channels_k_fastest / scatter_channels_k_fastest) */
+/* img_k[p, k] = coeffs[k0 + k, p] for k < kb, 0 for kb <= k < kstride. */
+int dfm_gather_k(const float* coeffs, float* img_k, int64_t npix, int64_t k0, int32_t kb, int32_t kstride,
+                 dfm_stream_t stream);
+/* coeffs[k0 + k, p] = img_k[p, k] for k < kb. */
+int dfm_scatter_k(const float* img_k, float* coeffs, int64_t npix, int64_t k0, int32_t kb, int32_t kstride,
+                  dfm_stream_t stream);
+
+/* ---------------------------------------------------------------- Radon (operators/parallel_radon.py:283-318,
+ * radon_kernels.cl:44-141: radon_forward_k / radon_forward_tiled / radon_backward_k / radon_backward_tiled)
+ * Per angle a: geo[8a..8a+7] = X, Y, T0, wscale, hw, top, 0, 0. Pixel (x, y) lands at t = X x + Y y + T0 (bins) and
+ * adds K(t - s) to bin s, K(d) = clamp(hw - |d|, 0, top) (hw = (1 + A)/2, top = min(1, A), A = max(|X|, |Y|) for the
+ * trapezoid; hw = top = 1 for the hat). ntap: bins a pixel can touch (2 when A <= 1).
+ * forward:  sino[a, s, k] = scale * wscale_a * sum_pix K(t - s) img[pix, k]   (wscale_a = 1/A for the trapezoid)
+ * backward: img[pix, k]   = sum_a wscale_a * sum_s K(t - s) sino[a, s, k]     (wscale_a = w_a/A: the angle weight)
+ * Image channels k_off .. k_off + k_n - 1 of an (Npix, kstride) array are sinogram channels 0 .. k_n - 1 of an
+ * (n_angles, ns, skstride) array (k_off, k_n, kstride, skstride multiples of 4). */
+int dfm_radon_forward(const float* img_k, float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns,
+                      int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, float scale,
+                      int32_t ntap, dfm_stream_t stream);
+int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_t nx, int32_t ny, int32_t ns,
+                       int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap,
+                       dfm_stream_t stream);
+
+/* ---------------------------------------------------------------- pole-figure matrix
+ * Dense PF batch (create_pfo_matrix.py:108-199 pfmatrix_eval_poles): out (R, kmax, C, P) for orientations
+ * k0 .. k0+kb-1 (rows kb..kmax-1 zero). coords (R, S, C, Se, P, 3); poles (K, A, 3); axis_start (P+1);
+ * axis_count (A); inv_sigma2, norm (K); qw (S*Se) used when quad_weighted (else the node mean). */
+int dfm_pf_eval_dense(const float* coords, const float* poles, const int32_t* axis_start, const float* axis_count,
+                      const float* inv_sigma2, const float* norm, const float* qw, float* out, int32_t R,
+                      int32_t kmax, int32_t C, int32_t P, int32_t A, int32_t S, int32_t Se, int32_t k0, int32_t kb,
+                      float pf_cut, int32_t quad_weighted, dfm_stream_t stream);
+/* Dense batch -> CSR (pf_kernels.cl:482-548 pf_count_rows_* / pf_fill_rows_*): counts per forward row (r, j) /
+ * adjoint row (r, k); fill writes col and val from the row pointers (exclusive scan of the counts). */
+int dfm_pf_count_fwd(const float* pf, int32_t* counts, int32_t R, int32_t kmax, int32_t CP, int32_t kb,
+                     dfm_stream_t stream);
+int dfm_pf_count_adj(const float* pf, int32_t* counts, int32_t R, int32_t kmax, int32_t CP, int32_t kb,
+                     dfm_stream_t stream);
+int dfm_pf_fill_fwd(const float* pf, const int32_t* row_ptr, uint16_t* col_k, float* val, int32_t R, int32_t kmax,
+                    int32_t CP, int32_t kb, dfm_stream_t stream);
+int dfm_pf_fill_adj(const float* pf, const int32_t* row_ptr, uint16_t* col_j, float* val, int32_t R, int32_t kmax,
+                    int32_t CP, int32_t kb, dfm_stream_t stream);
+
+/* ---------------------------------------------------------------- PF products (pf_kernels.cl:597-687
+ * spmm_pf_forward_c / spmm_pf_adjoint_c; single_phase_forward_operator.py:38-123 the _sub forms)
+ * Sinogram row rs (0..n_angles-1) belongs to data row omegas[rs] (omegas NULL: rs). CSR rows: rs if compact, else
+ * omegas[rs].
+ * forward: data[r, y, j] (+)= sum_k PF[r, k, j] sino[rs, y, k]   (accumulate 0: overwrite every j; 1: add)
+ * adjoint: sino[rs, y, k] = alpha sum_j PF[r, k, j] data[r, y, j]  (k < kb; kb <= k < kstride set to 0) */
+int dfm_spmm_forward(const float* sino, const int32_t* row_ptr, const uint16_t* col_k, const float* val, float* data,
+                     const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
+                     int32_t compact, int32_t accumulate, dfm_stream_t stream);
+int dfm_spmm_adjoint(const float* data, const int32_t* row_ptr, const uint16_t* col_j, const float* val, float* sino,
+                     const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
+                     float alpha, int32_t compact, dfm_stream_t stream);
+/* data rows omegas[s] (each row_len values) <- 0 (single_phase_forward_operator.py:126-132 zero_rows). */
+int dfm_zero_rows(float* data, const int32_t* omegas, int32_t n_sub, int64_t row_len, dfm_stream_t stream);
+
+/* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
+ * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
+ * Dual update on the rows of a subset (spdhg_poisson.py:69-91 dual_update_sub), float64 arithmetic:
+ * d holds A x on entry and y' - y on exit; y <- y'. */
+int dfm_dual_update_sub(float* y, float* d, const float* sig, const float* b, const float* w, int64_t pw,
+                        const float* g, int64_t pg, const float* bg, int64_t pbg, float eps, const int32_t* omegas,
+                        int32_t n_sub, int64_t row_len, dfm_stream_t stream);
+/* Backprojection of the subset's dual change fused with the primal update (spdhg_poisson.py:95-106 spdhg_update
+ * applied to the output of radon_backward_k): dz = backward(sino); z += dz; x = T > 0 ? max(0, x - T (z + inv_p dz
+ * + lam)) : 0, on the channel-fastest arrays x, z, T (Npix, kstride), channels k_off .. k_off + k_n - 1. */
+int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
+                             int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap, float* x,
+                             float* z, const float* T, float inv_p, float lam, dfm_stream_t stream);
+/* The plain update (spdhg_poisson.py:95-106) on n elements: z += dz; x = T > 0 ? max(0, x - T (z + inv_p dz + lam)) : 0 */
+int dfm_spdhg_update(float* x, float* z, const float* T, const float* dz, float inv_p, float lam, int64_t n,
+                     dfm_stream_t stream);
+/* Primal steps (spdhg_poisson.py:252-272 tinit / tmin / tfinal) on channel-fastest (Npix, kstride) arrays,
+ * channels < kvalid (the others 0): tinit T = inf; tmin T = min(T, c / colsum) where supp[p] and colsum > 0
+ * (supp NULL: every pixel); tfinal T = 0 where T is inf. */
+int dfm_tinit(float* T, int64_t npix, int32_t kstride, dfm_stream_t stream);
+int dfm_tmin(float* T, const float* colsum, const uint8_t* supp, int64_t npix, int32_t kstride, int32_t kvalid,
+             float c, dfm_stream_t stream);
+int dfm_tfinal(float* T, int64_t npix, int32_t kstride, int32_t kvalid, dfm_stream_t stream);
+/* sig = (w != 0 && rowsum > 0) ? sigma0 / rowsum : 0 (pdhg_poisson.py:79-84 dual_steps). */
+int dfm_dual_steps(float* sig, const float* rowsum, const float* w, int64_t pw, float sigma0, int64_t n,
+                   dfm_stream_t stream);
+/* u = (w != 0) ? 1 : 0 (pdhg_poisson.py:113-116 set_mask_ones). */
+int dfm_set_mask_ones(float* u, const float* w, int64_t pw, int64_t n, dfm_stream_t stream);
+/* Channel-fastest ones on the support, 0 outside and on channels >= kvalid (fista_poisson.py:219-223). */
+int dfm_support_ones_k(float* x, const uint8_t* supp, int64_t npix, int32_t kstride, int32_t kvalid,
+                       dfm_stream_t stream);
+/* x = 0 outside the support, where x < 0 and on channels >= kvalid (fista_poisson.py:277-281 project_support). */
+int dfm_project_support_k(float* x, const uint8_t* supp, int64_t npix, int32_t kstride, int32_t kvalid,
+                          dfm_stream_t stream);
+/* float64 partial sums, one per work-group, into partial[0 .. *n_partial) (the caller sums them):
+ * deviance: sum over w != 0 of w/g (m - b + b log(b/m)), m = max(mu, 0) + bg + eps g (fista_poisson.py:72-109);
+ * mean_counted: partial[i] = sum b, partial[n_partial + i] = count (pdhg_poisson.py:97-111);
+ * dot: sum a*b (single_phase_forward_operator.py:2168-2189 vdot). n_partial receives the number of partials
+ * (at most DFM_MAX_PARTIALS). */
+#define DFM_MAX_PARTIALS 2048
+int dfm_deviance_partial(const float* mu, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                         const float* bg, int64_t pbg, float eps, int64_t n, double* partial, int32_t* n_partial,
+                         dfm_stream_t stream);
+int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t n, double* partial,
+                             int32_t* n_partial, dfm_stream_t stream);
+int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
+                    dfm_stream_t stream);
+/* a *= s (single_phase_forward_operator.py:2169, 2192 x *= 1/|x|). */
+int dfm_scale(float* a, float s, int64_t n, dfm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFRACTOM_HIP_H */

@@ -1,0 +1,638 @@
+"""FP64 NumPy oracle of diffractom's SinglePhaseForwardOperator (native projector) and SPDHGPoisson.
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker. The product (diffractom_amd) never imports it.
+
+Every function restates the reference's algorithm in float64 and cites the reference file:line it follows
+(paths relative to the diffractom source tree src/diffractom/). What pins each function is listed in its docstring
+(the test that checks it); a function with no such check would say "parity unpinned".
+
+Model (differentiable/reference.py:1-26):
+    d[a, s, j] = sum_k PF[a, k, j] S_k[a, s],   S_k[a, s] = scale sum_pix K(t_a(pix) - s) c_k(pix),   j = c P + p
+    PF[a, k, (c, p)] = I_p N(sigma_k) sum_nodes w_n sum_{axes h of ring p} m_h g(U_k h . v_n; sigma_k)
+    g(d; sigma) = exp(-(1 - |d|)/sigma^2) [+ exp(-(1 + |d|)/sigma^2)], each term dropped where its exponent >= cut
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass
+from math import comb, factorial
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.spatial.transform import Rotation
+
+KEV_ANGSTROM = 12.398      # operators/single_phase_forward_operator.py:554 (the operator's energy <-> wavelength)
+
+
+# ============================================================================ crystallography
+def lattice_matrix(a, b, c, alpha, beta, gamma):
+    """Direct lattice matrix, columns = lattice vectors (crystallography/material.py:72-105): a along x for a cell
+    with all angles 90 deg, else the Busing-Levy frame with a* along x and c along z.
+    Pinned by tests/test_oracle_crystal.py (notebook d-spacings and two-theta; hexagonal d-spacing closed form)."""
+    if alpha == 90.0 and beta == 90.0 and gamma == 90.0:
+        return np.diag([float(a), float(b), float(c)])
+    ca, cb, cg = (np.cos(np.deg2rad(t)) for t in (alpha, beta, gamma))
+    G = np.array([[a * a, a * b * cg, a * c * cb], [a * b * cg, b * b, b * c * ca], [a * c * cb, b * c * ca, c * c]])
+    Gi = np.linalg.inv(G)
+    ast, bst, cst = np.sqrt(np.diag(Gi))
+    cos_bs = Gi[0, 2] / (ast * cst)
+    cos_gs = Gi[0, 1] / (ast * bst)
+    B = np.array([[ast, bst * cos_gs, cst * cos_bs],
+                  [0.0, bst * np.sqrt(1 - cos_gs ** 2), -cst * np.sqrt(1 - cos_bs ** 2) * ca],
+                  [0.0, 0.0, 1.0 / c]])
+    return np.linalg.inv(B).T
+
+
+def reciprocal(A):
+    """B = 2 pi (A^-1)^T (crystallography/lattice.py:17-19)."""
+    return 2.0 * np.pi * np.linalg.inv(A).T
+
+
+def cubic_rotations():
+    """The 24 proper rotations of the cube (point group O; crystallography/point_groups.py:123-151): the signed
+    permutation matrices of determinant +1."""
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            m = np.zeros((3, 3))
+            for i, j in enumerate(perm):
+                m[i, j] = signs[i]
+            if np.linalg.det(m) > 0:
+                out.append(m)
+    return np.array(out)
+
+
+def hexagonal_rotations():
+    """The 12 proper rotations of 622 (point group D6; crystallography/point_groups.py:86-101): rotations by k 60 deg
+    about z and 2-fold axes in the xy plane every 30 deg."""
+    out = []
+    for k in range(6):
+        t = np.deg2rad(60.0 * k)
+        out.append(np.array([[np.cos(t), -np.sin(t), 0], [np.sin(t), np.cos(t), 0], [0, 0, 1.0]]))
+    for k in range(6):
+        f = np.deg2rad(30.0 * k)
+        out.append(np.array([[np.cos(2 * f), np.sin(2 * f), 0], [np.sin(2 * f), -np.cos(2 * f), 0], [0, 0, -1.0]]))
+    return np.array(out)
+
+
+ROTATIONS = {"cubic": cubic_rotations, "hexagonal": hexagonal_rotations}
+
+
+def hkl_space_ops(rot, B):
+    """The point-group rotations acting on Miller indices, B^-1 R B rounded to integers (material.py:1051-1058 uses
+    the equivalent integer tables of point_groups.py:163-268)."""
+    Bi = np.linalg.inv(B)
+    return [np.rint(Bi @ r @ B).astype(int) for r in rot]
+
+
+@dataclass
+class Reflections:
+    hkl: np.ndarray          # (N, 3) family representatives
+    multiplicity: np.ndarray
+    d_spacing: np.ndarray
+    two_theta: np.ndarray    # radians
+    h_vecs: np.ndarray       # (N, 3) B @ hkl (1/Angstrom, with 2 pi)
+
+
+def reflections(A, system, hkl_list, wavelength):
+    """Reflection families from an explicit hkl list (material.py:1000-1174): equivalents under the point group and
+    Friedel pairs, representative min(equivalents), multiplicity = number of equivalents, sorted by two-theta.
+    Pinned by tests/test_oracle_crystal.py (the notebook's Al reflection table: multiplicities 8, 6, 12, 24, 8)."""
+    B = reciprocal(A)
+    ops = hkl_space_ops(ROTATIONS[system](), B)
+    fams, mults, seen = [], [], set()
+    for h in hkl_list:
+        t = tuple(int(v) for v in h)
+        if t in seen:
+            continue
+        eq = []
+        for R in ops:
+            w = tuple(int(v) for v in R @ np.asarray(t))
+            for cand in (w, tuple(-v for v in w)):
+                if cand not in eq:
+                    eq.append(cand)
+        seen.update(eq)
+        fams.append(min(eq))
+        mults.append(len(eq))
+    hkl = np.array(fams, dtype=np.int64)
+    G = (B @ hkl.T).T
+    q = np.linalg.norm(G, axis=1)
+    tth = 2.0 * np.arcsin(np.clip(q * wavelength / (4.0 * np.pi), 0.0, 1.0))
+    order = np.argsort(tth)
+    return Reflections(hkl[order], np.array(mults)[order], (2 * np.pi / q)[order], tth[order], G[order])
+
+
+def group_rings(two_theta, rtol=1e-6):
+    """Reflections of equal two-theta form a ring (crystallography/intensity.py:156-169)."""
+    rings = []
+    for i in np.argsort(two_theta, kind="stable"):
+        if rings and np.isclose(two_theta[i], two_theta[rings[-1][0]], rtol=rtol, atol=0.0):
+            rings[-1].append(int(i))
+        else:
+            rings.append([int(i)])
+    return rings
+
+
+def pole_axes(h_vecs, multiplicity, rings, rot):
+    """Distinct pole axes of every ring and the number of symmetry images on each
+    (single_phase_forward_operator.py:742-826, normalized=True): images S_g h, sign made canonical (first component
+    with |.| > 1e-6 positive), merged after rounding to 6 decimals, renormalised.
+    Pinned by tests/test_oracle_pf.py (PF from the merged axes == PF summed over all |G| images; counts sum to |G|)."""
+    hn = h_vecs / (np.linalg.norm(h_vecs, axis=1, keepdims=True) + 1e-12)
+    axes, counts, start = [], [], [0]
+    for refl in rings:
+        share = multiplicity[refl].astype(np.float64)
+        share = share / share.sum()
+        imgs, wts = [], []
+        for f, w in zip(refl, share):
+            img = rot @ hn[f]
+            imgs.append(img)
+            wts.append(np.full(len(img), w))
+        imgs, wts = np.concatenate(imgs), np.concatenate(wts)
+        first = np.argmax(np.abs(imgs) > 1e-6, axis=1)
+        imgs = imgs * np.sign(imgs[np.arange(len(imgs)), first])[:, None]
+        uniq, inv = np.unique(np.round(imgs, 6), axis=0, return_inverse=True)
+        cnt = np.bincount(inv.ravel(), weights=wts, minlength=len(uniq))
+        axes.append(uniq / np.linalg.norm(uniq, axis=1, keepdims=True))
+        counts.append(cnt)
+        start.append(start[-1] + len(uniq))
+    return np.concatenate(axes), np.concatenate(counts), np.asarray(start)
+
+
+# ============================================================================ quadrature
+def gauss_constant(p):
+    """c_p of the p-point Gauss-Legendre error (operators/quadrature.py:32-34). Pinned: c_1 = 1/24, c_2 = 1/4320."""
+    return factorial(p) ** 4 / ((2 * p + 1) * factorial(2 * p) ** 3)
+
+
+def peak_derivative_factor(m, a):
+    """mu_m(a), the m-th moment of the symmetric Skellam distribution (quadrature.py:37-48), by the moment-cumulant
+    recursion with cumulants a (even) and 0 (odd). Pinned: closed forms mu_2..mu_8 and Bessel sums
+    (tests/test_oracle_quadrature.py)."""
+    mom = [1.0]
+    for n in range(1, m + 1):
+        mom.append(sum(comb(n - 1, j - 1) * a * mom[n - j] for j in range(2, n + 1, 2)))
+    return mom[m]
+
+
+def direction_bound(p, k, width, sigma):
+    h = abs(float(width)) / k
+    return gauss_constant(p) * h ** (2 * p) * peak_derivative_factor(2 * p, 1.0 / float(sigma) ** 2)
+
+
+def nodes_weights(p, k):
+    """k panels of p-point Gauss-Legendre on (0, 1) (quadrature.py:59-63). Pinned: exact for polynomials."""
+    g, w = np.polynomial.legendre.leggauss(p)
+    x = ((np.arange(k)[:, None] + (g[None, :] + 1.0) / 2.0) / k).ravel()
+    return x, np.tile(w / 2.0 / k, k)
+
+
+def choose_rule(d_omega, d_eta, sigma, tol=0.05, cut_share=0.0, p_max=8, n_max=64):
+    """Fewest nodes per bin whose bound meets tol - cut_share (quadrature.py:127-153): returns
+    (p_omega, k_omega, p_eta, k_eta). Pinned by the reference test's known answers (test_quadrature.py:67-85)."""
+    budget = float(tol) - float(cut_share)
+    if budget <= 0:
+        raise ValueError("tolerance below the cut share")
+
+    def options(width):
+        out = []
+        for n in range(1, n_max + 1):
+            best = None
+            for p in range(1, min(p_max, n) + 1):
+                if n % p:
+                    continue
+                bd = direction_bound(p, n // p, width, sigma)
+                if best is None or bd < best[0]:
+                    best = (bd, p, n // p)
+            out.append((n, *best))
+        return out
+
+    best = None
+    for nw, bw, pw, kw in options(d_omega):
+        for ne, be, pe, ke in options(d_eta):
+            if bw + be <= budget:
+                key = (nw * ne, bw + be)
+                if best is None or key < best[0]:
+                    best = (key, (pw, kw, pe, ke))
+                break
+    if best is None:
+        raise ValueError("no quadrature rule meets the tolerance")
+    return best[1]
+
+
+def rule_nodes(a0, a1, n, p, k):
+    """Node angles of n bins over [a0, a1] (quadrature.py:204-213): equal sub-bins for p = 1 (the linspace
+    expression), Gauss-Legendre panels otherwise; and the per-node weights within a bin."""
+    delta = (a1 - a0) / n
+    if p == 1:
+        nodes = np.linspace(a0, a1, n * k, endpoint=False) + delta / k / 2
+        w = np.full(k, 1.0 / k)
+    else:
+        x, w = nodes_weights(p, k)
+        nodes = (a0 + (np.arange(n)[:, None] + x[None, :]) * delta).ravel()
+    return nodes, w
+
+
+# ============================================================================ the operator
+def fov_support_mask(Nx, Ny, n_det, angles=None, image_width=None, detector_width=None, detector_shift=0.0):
+    """Pixels whose centre projects onto the detector at every angle (utils/support.py:5-51)."""
+    image_width = float(Nx if image_width is None else image_width)
+    detector_width = float(n_det if detector_width is None else detector_width)
+    dx = image_width / max(Nx, Ny)
+    x = (np.arange(Nx) - (Nx - 1) / 2.0)[None, :] * dx
+    y = (np.arange(Ny) - (Ny - 1) / 2.0)[:, None] * dx
+    half = detector_width / 2.0
+    if angles is None:
+        return x ** 2 + y ** 2 <= (half - abs(float(detector_shift))) ** 2
+    mask = np.ones((Ny, Nx), dtype=bool)
+    for a in np.asarray(angles, dtype=np.float64):
+        t = np.cos(a - 0.5 * np.pi) * x + np.sin(a - 0.5 * np.pi) * y - detector_shift
+        mask &= np.abs(t) <= half + 1e-9 * detector_width
+    return mask
+
+
+class OracleOperator:
+    """FP64 forward/adjoint of SinglePhaseForwardOperator with the native projector, normalized=True.
+
+    cfg: the reference's cfg dict (Nx, Ny, My, N_Omega, angle_range, cor_offset, N_eta, eta_angle_range, energy,
+    k_direction_0, p_direction_0, detector_direction_origin, detector_direction_positive_90).
+    lattice: (a, b, c, alpha, beta, gamma) or a 3x3 direct matrix (rows = lattice vectors); system: "cubic" or
+    "hexagonal"; hkl_list; wavelength of the material; U (K, 3, 3) orientation matrices (crystal -> sample);
+    sigma: kernel width(s) in radians.
+    """
+
+    def __init__(self, cfg, lattice, system, hkl_list, wavelength, U, sigma, pf_cutoff_sigma=4.0, quadrature="auto",
+                 quadrature_tol=0.05, radon_kernel="trapezoid", orthonormalize=True):
+        self.cfg = dict(cfg)
+        c = self.cfg
+        self.Nx, self.Ny, self.My, self.R = int(c["Nx"]), int(c["Ny"]), int(c["My"]), int(c["N_Omega"])
+        self.C = int(c["N_eta"])
+        self.cor = float(c.get("cor_offset", 0.0))
+        # material (material.py:639-755 with hkl_list)
+        if np.ndim(lattice) == 2:
+            M = np.asarray(lattice, np.float64)          # rows = direct lattice vectors
+            A = M.T
+        else:
+            A = lattice_matrix(*[float(v) for v in lattice])
+        self.refl = reflections(A, system, hkl_list, wavelength)
+        self.rings = group_rings(self.refl.two_theta)
+        self.P = len(self.rings)
+        self.CP = self.C * self.P
+        rot = ROTATIONS[system]()
+        self.axes, self.counts, self.start = pole_axes(self.refl.h_vecs, self.refl.multiplicity, self.rings, rot)
+        # orientations: scipy Rotation round trip as the reference's Grid (utils/grid.py:238-278, 665-700)
+        U = np.asarray(U, np.float64)
+        self.U = Rotation.from_matrix(U).as_matrix() if orthonormalize else U
+        self.K = len(self.U)
+        sig = np.broadcast_to(np.asarray(sigma, np.float64), (self.K,))
+        self.sigma = sig.astype(np.float32).astype(np.float64)      # the operator stores float32 widths
+        self.cut = float(pf_cutoff_sigma) ** 2 / 2 if pf_cutoff_sigma is not None else 6.0
+        # angles (single_phase_forward_operator.py:407-414)
+        ar = np.array(c["angle_range"], np.float64) / 180 * np.pi
+        er = np.array(c.get("eta_angle_range", [0, 360]), np.float64) / 180 * np.pi
+        self.angles = np.linspace(ar[0], ar[1], self.R, endpoint=False) + (ar[1] - ar[0]) / self.R / 2
+        # quadrature (quadrature.py:168-222)
+        d_om, d_eta = (ar[1] - ar[0]) / self.R, (er[1] - er[0]) / self.C
+        share = float(np.exp(-self.cut))
+        if quadrature == "auto":
+            self.rule = choose_rule(d_om, d_eta, float(self.sigma.min()), quadrature_tol, share)
+        elif quadrature == "midpoint":
+            self.rule = (1, int(c.get("N_Omega_subdivisions", 1)), 1, int(c.get("N_eta_subdivisions", 1)))
+        else:
+            self.rule = tuple(quadrature)
+        pw, kw, pe, ke = self.rule
+        self.S, self.Se = pw * kw, pe * ke
+        self.om_nodes, ww = rule_nodes(ar[0], ar[1], self.R, pw, kw)
+        self.eta_nodes, we = rule_nodes(er[0], er[1], self.C, pe, ke)
+        self.qw = ww[:, None] * we[None, :]                          # (S, Se), sums to 1
+        # two-theta of the rings: float32 arithmetic on float32 h vectors (single_phase_forward_operator.py:553-557)
+        lam = KEV_ANGSTROM / float(c["energy"])
+        h32 = np.asarray(self.refl.h_vecs, np.float32)[[r[0] for r in self.rings]]
+        self.two_theta = (2.0 * np.arcsin(np.linalg.norm(h32, axis=1) / (4.0 * np.pi) * lam)).astype(np.float32)
+        # probed directions v (R, S, C, Se, P, 3) = R(omega)^T u (single_phase_forward_operator.py:569-616),
+        # trig of the float32 two-theta in float64 (differentiable/reference.py:131-141)
+        k0 = np.asarray(c["k_direction_0"], np.float64)
+        p0 = np.asarray(c["p_direction_0"], np.float64)
+        d0 = np.asarray(c["detector_direction_origin"], np.float64)
+        d90 = np.asarray(c["detector_direction_positive_90"], np.float64)
+        eta = self.eta_nodes.reshape(self.C, self.Se)
+        e = np.cos(eta)[..., None] * d0 + np.sin(eta)[..., None] * d90                    # (C, Se, 3)
+        th = 0.5 * self.two_theta.astype(np.float64)
+        u = np.cos(th)[None, None, :, None] * e[:, :, None, :] - np.sin(th)[None, None, :, None] * p0   # (C,Se,P,3)
+        Rm = Rotation.from_rotvec(self.om_nodes[:, None] * k0).as_matrix().reshape(self.R, self.S, 3, 3)
+        self.v = np.einsum("rsij,cepi->rscepj", Rm, u)
+        self.poles = np.einsum("kij,aj->kai", self.U, self.axes)                         # (K, A, 3)
+        # projector (operators/parallel_radon.py:92-124)
+        self.kernel = radon_kernel
+        dx = float(self.Nx) / max(self.Nx, self.Ny)
+        rho = 1.0 / dx
+        self.X = np.cos(self.angles - 0.5 * np.pi) / rho
+        self.Y = np.sin(self.angles - 0.5 * np.pi) / rho
+        self.T0 = (self.My - 1) / 2.0 - self.X * (self.Nx - 1) / 2.0 - self.Y * (self.Ny - 1) / 2.0 - self.cor
+        self.scale = dx * dx
+        self._pf = {}
+        self._P = {}
+        self.subsets = None
+
+    # ------------------------------------------------------------------ shapes
+    @property
+    def coeff_shape(self):
+        return (self.K, self.Ny, self.Nx)
+
+    @property
+    def data_shape(self):
+        return (self.R, self.My, self.CP)
+
+    def support_mask(self):
+        return fov_support_mask(self.Nx, self.Ny, self.My, self.angles, float(self.Nx), self.My, self.cor)
+
+    # ------------------------------------------------------------------ pole figure
+    def pf_angle(self, r, k=None):
+        """PF[r] as a dense (K, C, P) float64 array (create_pfo_matrix.py:108-199 / pf_generate.cl:62-140; the
+        reference model's pf, differentiable/reference.py:189-211). Only entries near a pole get exp evaluated: the
+        cut makes all others exactly zero. Pinned by tests/test_oracle_pf.py (all-|G|-images brute force, sign
+        evenness, peak value) and the notebook golden (tests/test_oracle_golden.py)."""
+        ks = np.arange(self.K) if k is None else np.asarray(k)
+        s = 1.0 / self.sigma[ks] ** 2
+        norm = 1.0 / (8.0 * np.pi * self.sigma[ks] ** 2)
+        out = np.zeros((len(ks), self.C, self.P))
+        for p in range(self.P):
+            a0, a1 = self.start[p], self.start[p + 1]
+            na = a1 - a0
+            V = self.v[r, :, :, :, p, :].reshape(-1, 3)                           # (S*C*Se, 3)
+            Pn = self.poles[ks, a0:a1, :].reshape(-1, 3)                          # (K*na, 3)
+            ad = np.abs(V @ Pn.T)                                                 # (nodes, K*na)
+            sc = np.repeat(s, na)[None, :]
+            x1 = (1.0 - ad) * sc
+            g = np.zeros_like(ad)
+            m1 = x1 < self.cut
+            g[m1] = np.exp(-x1[m1])
+            x2 = (1.0 + ad) * sc
+            m2 = x2 < self.cut
+            if m2.any():
+                g[m2] += np.exp(-x2[m2])
+            g = g.reshape(self.S, self.C, self.Se, len(ks), na) * self.counts[a0:a1]
+            node = g.sum(axis=-1)                                                 # (S, C, Se, K)
+            val = np.einsum("scek,se->kc", node, self.qw)
+            out[:, :, p] = val * norm[:, None]
+        return out
+
+    def pf_sparse(self, r):
+        """PF[r] as a sparse (K, C*P) CSR matrix (cached)."""
+        if r not in self._pf:
+            self._pf[r] = sp.csr_matrix(self.pf_angle(r).reshape(self.K, self.CP))
+        return self._pf[r]
+
+    # ------------------------------------------------------------------ projector
+    def _kernel(self, d, A):
+        if self.kernel == "hat":
+            return np.maximum(1.0 - np.abs(d), 0.0)
+        hw, top = 0.5 * (1.0 + A), min(1.0, A)
+        return np.clip(hw - np.abs(d), 0.0, top) / A
+
+    def radon_matrix(self, r):
+        """P[r] (My, Npix) as CSR: S_k[r] = P[r] @ c_k (differentiable/reference.py:225-238; radon_kernels.cl:1-101).
+        Every bin a pixel's footprint can reach (|t - s| < (1 + A) / 2 <= 1) is among floor(t) - 1 .. floor(t) + 2.
+        Pinned by tests/test_oracle_radon.py (dense brute force over all bins, trapezoid constant row sum, hat ==
+        trapezoid at 0/90 degrees)."""
+        if r in self._P:
+            return self._P[r]
+        yy, xx = np.meshgrid(np.arange(self.Ny), np.arange(self.Nx), indexing="ij")
+        px, py = xx.ravel().astype(np.float64), yy.ravel().astype(np.float64)
+        A = max(abs(self.X[r]), abs(self.Y[r]))
+        t = self.X[r] * px + self.Y[r] * py + self.T0[r]
+        rows, cols, vals = [], [], []
+        base = np.floor(t).astype(np.int64)
+        for off in (-1, 0, 1, 2):
+            sidx = base + off
+            w = self._kernel(t - sidx, A)
+            ok = (w > 0) & (sidx >= 0) & (sidx < self.My)
+            rows.append(sidx[ok])
+            cols.append(np.nonzero(ok)[0])
+            vals.append(self.scale * w[ok])
+        M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(self.My, self.Nx * self.Ny))
+        self._P[r] = M
+        return M
+
+    # ------------------------------------------------------------------ forward / adjoint
+    def _angles(self, subset):
+        return np.arange(self.R) if subset is None else self.subsets[subset]
+
+    def direct(self, x, subset=None):
+        """A x (R, My, C P); with subset m only its rows (others 0) (single_phase_forward_operator.py:1646-1729)."""
+        xf = np.asarray(x, np.float64).reshape(self.K, -1)
+        d = np.zeros(self.data_shape)
+        for r in self._angles(subset):
+            S = self.radon_matrix(r) @ xf.T                  # (My, K)
+            d[r] = (self.pf_sparse(r).T @ S.T).T             # (My, CP)
+        return d
+
+    def adjoint(self, y, subset=None):
+        """A^T y (K, Ny, Nx), the exact transpose (single_phase_forward_operator.py:1800-2033: alpha R/pi times the
+        backprojection weight pi/R is 1); with subset m only its rows are read."""
+        yf = np.asarray(y, np.float64)
+        out = np.zeros((self.K, self.Nx * self.Ny))
+        for r in self._angles(subset):
+            Z = self.pf_sparse(r) @ yf[r].T                  # (K, My)
+            out += (self.radon_matrix(r).T @ Z.T).T          # (K, Npix)
+        return out.reshape(self.coeff_shape)
+
+    def set_omega_subsets(self, subsets):
+        """Interleaved subsets np.arange(m, R, M) or given lists, each sorted and unique
+        (single_phase_forward_operator.py:1541-1599)."""
+        if np.isscalar(subsets):
+            M = int(subsets)
+            subsets = [np.arange(m, self.R, M) for m in range(M)]
+        self.subsets = [np.unique(np.asarray(s, dtype=np.int64)) for s in subsets]
+        return self.subsets
+
+
+# ============================================================================ Poisson pieces
+def periodic(a, n):
+    """A per-segment / per-(translation, segment) / per-point array expanded to n points (fista_poisson.py:287-308)."""
+    if a is None:
+        return None
+    a = np.asarray(a, np.float64).ravel()
+    return a[np.arange(n) % a.size]
+
+
+def deviance(mu, b, w=None, g=None, bg=None, eps=1e-3):
+    """Poisson deviance sum w/g (m - b + b log(b/m)), m = max(mu, 0) + bg + eps g, over points with w != 0
+    (fista_poisson.py:72-109, mode 0). Pinned by the notebook golden objectives (tests/test_oracle_golden.py)."""
+    mu = np.asarray(mu, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    n = mu.size
+    w = np.ones(n) if w is None else periodic(w, n)
+    g = np.ones(n) if g is None else periodic(g, n)
+    bg = np.zeros(n) if bg is None else periodic(bg, n)
+    eps32 = float(np.float32(eps))
+    m = np.maximum(mu, 0.0) + bg + eps32 * g
+    v = m - b
+    pos = b > 0
+    v[pos] += b[pos] * np.log(b[pos] / m[pos])
+    cnt = w != 0
+    return float(np.sum((w[cnt] / g[cnt]) * v[cnt]))
+
+
+def mean_counted(b, w=None):
+    """Mean of b over points with w != 0 (pdhg_poisson.py:97-111)."""
+    b = np.asarray(b, np.float64).ravel()
+    w = np.ones(b.size) if w is None else periodic(w, b.size)
+    c = w != 0
+    return float(b[c].sum() / max(c.sum(), 1))
+
+
+def dual_prox(y, Ax, s, b, w, g, bg, eps):
+    """y' = prox_{s F*}(y + s Ax), F_i(u) = sc (u + r - b log(u + r)), sc = w/g, r = bg + eps g; 0 where w = 0 or
+    s = 0 (spdhg_poisson.py:69-91; pdhg_poisson.py:33-46). Pinned: tests/test_oracle_spdhg.py (the prox optimality
+    condition, and the solver reaching an independent minimiser)."""
+    sc = w / g
+    r = bg + eps * g
+    v = y + s * Ax + s * r
+    dd = v - sc
+    yn = 0.5 * (v + sc - np.sqrt(dd * dd + 4.0 * s * sc * b))
+    return np.where((w != 0) & (s != 0), yn, 0.0)
+
+
+class OracleSPDHG:
+    """SPDHGPoisson (no TV, dense layout, every array resident) in float64 (spdhg_poisson.py:298-751).
+
+    The control flow, the random subset order (numpy default_rng(seed).permutation per epoch, the first subset of an
+    epoch never the last of the previous one) and the step sizes follow the reference; the arithmetic is float64."""
+
+    def __init__(self, op, subsets=8, lam=0.0, prox_kind="nonneg", support="fov", eps_counts=1e-3, rho="auto", seed=0):
+        self.op = op
+        self.subsets = op.set_omega_subsets(subsets)
+        self.M = len(self.subsets)
+        self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
+        self.eps = float(np.float32(eps_counts))
+        self.rho = rho
+        self.rng = np.random.default_rng(seed)
+        self._last = None
+        if support is None or support is False:
+            self.supp = None
+        elif isinstance(support, str):
+            self.supp = op.support_mask()
+        else:
+            self.supp = np.asarray(support, bool)
+        self.state = None
+
+    def _epoch_order(self):
+        """spdhg_poisson.py:363-370."""
+        perm = self.rng.permutation(self.M)
+        if self.M > 1 and int(perm[0]) == self._last:
+            perm[[0, 1]] = perm[[1, 0]]
+        self._last = int(perm[-1])
+        return [int(m) for m in perm]
+
+    def run(self, x0, b, n_epochs, weights=None, gain=None, background=None, diagnostics_interval=1, state=None,
+            record=False):
+        op = self.op
+        n = int(np.prod(op.data_shape))
+        b = np.asarray(b, np.float64).reshape(op.data_shape)
+        w = np.ones(n) if weights is None else periodic(weights, n)
+        g = np.ones(n) if gain is None else periodic(gain, n)
+        bg = np.zeros(n) if background is None else periodic(background, n)
+        w, g, bg = (a.reshape(op.data_shape) for a in (w, g, bg))
+        # rho (spdhg_poisson.py:558-567; RHO_AUTO = 3e4, pdhg_poisson.py:33)
+        if self.rho == "auto":
+            mb = mean_counted(b, w)
+            rho = 3e4 / mb ** 2 if mb > 0 else 1.0
+        else:
+            rho = float(self.rho)
+        self.rho_used = rho
+        gamma = np.sqrt(rho)
+        p = 1.0 / self.M
+        sup = np.ones((op.Ny, op.Nx), bool) if self.supp is None else self.supp
+        # S from the row sums of A 1_S (spdhg_poisson.py:621-631; dual_steps pdhg_poisson.py:79-84)
+        ones = np.broadcast_to(sup, op.coeff_shape).astype(np.float64)
+        D = op.direct(ones)
+        sig = np.where((w != 0) & (D > 0), gamma / np.where(D > 0, D, 1.0), 0.0)
+        # T = min_m p / (gamma A_m^T 1_counted) on the support (spdhg_poisson.py:632-643, 252-272)
+        mask = (w != 0).astype(np.float64)
+        T = np.full(op.coeff_shape, np.inf)
+        for m in range(self.M):
+            col = op.adjoint(mask, subset=m)
+            ok = (col > 0) & sup[None]
+            T = np.where(ok, np.minimum(T, (p / gamma) / np.where(col > 0, col, 1.0)), T)
+        T[np.isinf(T)] = 0.0
+        self.T, self.sig = T, sig
+        x = np.maximum(np.asarray(x0, np.float64).reshape(op.coeff_shape), 0.0) * sup[None]
+        y = np.zeros(op.data_shape) if state is None else np.asarray(state["y"], np.float64).reshape(op.data_shape)
+        if state is not None and state.get("z") is not None:
+            z = np.asarray(state["z"], np.float64).reshape(op.coeff_shape).copy()
+        elif state is not None:
+            z = op.adjoint(y)
+        else:
+            z = np.zeros(op.coeff_shape)
+        ep0 = int(state["epoch"]) if state is not None else 0
+        inv_p = float(self.M)
+        self.iter_stats = []
+        self.iterates = []
+        f = float("nan")
+
+        def dual(m, D):
+            rows = self.subsets[m]
+            yn = dual_prox(y[rows], D[rows], sig[rows], b[rows], w[rows], g[rows], bg[rows], self.eps)
+            D[rows] = yn - y[rows]
+            y[rows] = yn
+
+        def update(dz):
+            nonlocal x, z
+            z = z + dz
+            zb = z + inv_p * dz
+            x = np.where(T > 0, np.maximum(0.0, x - T * (zb + self.lam)), 0.0)
+
+        seq = self._epoch_order()
+        D = np.zeros(op.data_shape)
+        D[self.subsets[seq[0]]] = op.direct(x, subset=seq[0])[self.subsets[seq[0]]]
+        for e in range(n_epochs):
+            last = e + 1 == n_epochs
+            want_f = (e + 1) % max(1, diagnostics_interval) == 0 or last
+            nseq = None
+            for j, blk in enumerate(seq):
+                end = j + 1 == len(seq)
+                if not end:
+                    nb = seq[j + 1]
+                elif not last:
+                    nseq = self._epoch_order()
+                    nb = nseq[0]
+                else:
+                    nb = None
+                dual(blk, D)
+                update(op.adjoint(D, subset=blk))
+                if record:
+                    self.iterates.append(x.copy())
+                if (end and want_f) or nb is None or nb == blk:
+                    if (end and want_f) or nb is not None:
+                        D = op.direct(x)
+                        if end and want_f:
+                            f = deviance(D, b, w, g, bg, self.eps)
+                else:
+                    rows = self.subsets[nb]
+                    D[rows] = op.direct(x, subset=nb)[rows]
+            if nseq is not None:
+                seq = nseq
+            self.iter_stats.append({"epoch": ep0 + e + 1, "f": f if want_f else None, "rho": rho})
+        self.state = {"epoch": ep0 + n_epochs, "z": z, "y": y}
+        self.final_stats = {"epochs": n_epochs, "final_f": f, "rho": rho, "subsets": self.M}
+        return x
+
+
+def estimate_L_power(op, niter=20, seed=0, eps=1e-30):
+    """Power iteration for ||A^T A|| (single_phase_forward_operator.py:2136-2206): start drawn as (Nx, Ny, K) float32
+    normals and transposed; float64 dot products here. Pinned by tests/test_oracle_power.py (the largest eigenvalue
+    of the explicit A^T A on a tiny operator)."""
+    rng = np.random.default_rng(seed)
+    x = np.ascontiguousarray(rng.standard_normal((op.Nx, op.Ny, op.K)).astype(np.float32).transpose(2, 1, 0))
+    x = x.astype(np.float64)
+    x /= np.sqrt(np.vdot(x, x)) + eps
+    L = 0.0
+    for _ in range(niter):
+        z = op.adjoint(op.direct(x))
+        L = float(np.vdot(x, z) / (np.vdot(x, x) + eps))
+        z /= np.sqrt(np.vdot(z, z)) + eps
+        x = z
+    return L

@@ -1,0 +1,112 @@
+"""HIP operator vs the FP64 oracle: forward, adjoint, transpose, subsets, storage forms, power iteration.
+Tolerances: FP32 forward/adjoint within 2e-5 of the largest entry (the reference's own GPU-vs-FP64 bar,
+tests/test_differentiable.py:36-41); adjointness 1e-5 (tests/test_adjoint.py:43-47)."""
+import numpy as np
+import pytest
+
+from pairs import build, cfg, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(op, seed=1):
+    rng = np.random.default_rng(seed)
+    x = (rng.random(op.coeff_shape) * op.support_mask()[None]).astype(np.float32)
+    y = rng.random(op.data_shape).astype(np.float32)
+    return x, y
+
+
+@pytest.mark.parametrize("angle_range", [(0, 180), (0, 360), (0, 90), (180, 0)])
+@pytest.mark.parametrize("kernel", ["trapezoid", "hat"])
+def test_forward_adjoint_match_oracle(angle_range, kernel):
+    print(f"case angle_range={angle_range} kernel={kernel}", flush=True)
+    op, orc, _ = build(cfg(angle_range=angle_range), radon_kernel=kernel, pf_mode="sparse")
+    x, y = _inputs(op)
+    d = op.direct(x)
+    d_ref = orc.direct(x)
+    assert rel_max(d, d_ref) < 2e-5
+    a = op.adjoint(y)
+    a_ref = orc.adjoint(y)
+    assert rel_max(a, a_ref) < 2e-5
+    lhs = float(np.dot(x.ravel().astype(np.float64), a.ravel()))
+    rhs = float(np.dot(d.ravel().astype(np.float64), y.ravel()))
+    assert rhs > 0 and abs(lhs / rhs - 1) < 1e-5
+
+
+@pytest.mark.parametrize("pf_mode", ["sparse", "dense", "generated"])
+def test_storage_forms_and_rectangular_grid(pf_mode):
+    """Nx != Ny, a centre-of-rotation offset, small kernels (Gauss-Legendre quadrature), 4 orientation batches."""
+    print(f"case pf_mode={pf_mode}", flush=True)
+    c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
+    op, orc, _ = build(c, K=12, sigma_deg=1.0, pf_mode=pf_mode, max_gb=3e-5)
+    assert len(op.batches) > 1
+    assert not op.quadrature.is_midpoint
+    x, y = _inputs(op, 2)
+    assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
+    assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5
+
+
+@pytest.mark.parametrize("pf_mode", ["sparse", "generated"])
+@pytest.mark.parametrize("subsets", [4, "blocks"])
+def test_subset_forward_and_adjoint(pf_mode, subsets):
+    """tests/test_subsets.py:49-86: subset rows equal the full rows, other rows untouched, each subset adjoint is
+    the transpose of its forward, the subset adjoints of a partition sum to the full adjoint."""
+    import torch
+    print(f"case pf_mode={pf_mode} subsets={subsets}", flush=True)
+    c = cfg(nx=20, ny=20, my=21, n_omega=30, angle_range=(0, 360), cor=0.5)
+    op, orc, _ = build(c, K=30, pf_mode=pf_mode, max_gb=1e-4, oracle=True)
+    assert len(op.batches) > 1
+    R = op.N_Omega
+    subs = op.set_omega_subsets(4 if subsets == 4 else [np.arange(0, 7), np.arange(7, 20), np.arange(20, R)])
+    x, y = _inputs(op, 6)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    full = op.direct(x)
+    adj = op.adjoint(y)
+    adj_sum = np.zeros_like(adj, dtype=np.float64)
+    for m, s in enumerate(subs):
+        d = torch.full(op.data_shape, -7.0, dtype=torch.float32, device="cuda")
+        op.direct_cl(xd, d, subset=m)
+        d = d.cpu().numpy()
+        np.testing.assert_allclose(d[s], full[s], rtol=1e-5, atol=1e-6 * np.abs(full).max())
+        rest = np.setdiff1d(np.arange(R), s)
+        assert np.all(d[rest] == -7.0)
+        a_m = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
+        op.adjoint_cl(yd, a_m, subset=m)
+        a_m = a_m.cpu().numpy()
+        adj_sum += a_m
+        lhs = float(np.dot(d[s].ravel().astype(np.float64), y[s].ravel()))
+        rhs = float(np.dot(x.ravel().astype(np.float64), a_m.ravel()))
+        assert abs(lhs / rhs - 1) < 1e-5
+    np.testing.assert_allclose(adj_sum, adj, rtol=1e-4, atol=1e-5 * np.abs(adj).max())
+    orc.set_omega_subsets(subs)
+    assert rel_max(op.adjoint(np.where(np.isin(np.arange(R), subs[0])[:, None, None], y, 0)),
+                   orc.adjoint(y, subset=0)) < 2e-5
+
+
+def test_subsets_need_sparse():
+    op, _, _ = build(pf_mode="dense", oracle=False)
+    with pytest.raises(NotImplementedError):
+        op.set_omega_subsets(2)
+
+
+def test_power_iteration_matches_oracle():
+    from diffractom_amd import estimate_L_power
+    from oracle.model import estimate_L_power as oracle_power
+    op, orc, _ = build(pf_mode="sparse")
+    L = estimate_L_power(op, niter=15, seed=3, verbose=0)
+    L_ref = oracle_power(orc, niter=15, seed=3)
+    assert abs(L / L_ref - 1) < 1e-5
+
+
+def test_errors_raise_like_the_reference():
+    import torch
+    op, _, _ = build(pf_mode="sparse", oracle=False)
+    with pytest.raises(TypeError):
+        op.direct_cl(torch.zeros(op.coeff_shape, dtype=torch.float64, device="cuda"),
+                     torch.zeros(op.data_shape, device="cuda"))
+    with pytest.raises(ValueError):
+        op.direct_cl(torch.zeros((1, 2, 3), device="cuda"), torch.zeros(op.data_shape, device="cuda"))
+    with pytest.raises(ValueError):
+        op.set_omega_subsets(0)
+    with pytest.raises(ValueError):
+        op.direct_cl(torch.zeros(op.coeff_shape, device="cuda"), torch.zeros(op.data_shape, device="cuda"), subset=0)
