@@ -186,12 +186,15 @@ __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __res
 }
 
 // ------------------------------------------------------------------------------------------------ dense PF
-// One thread per entry (r, k, c, p) of the batch; the sum over quadrature nodes and the ring's pole axes.
-__global__ void __launch_bounds__(256) pf_eval_kernel(const float* __restrict__ coords, const float* __restrict__ poles,
+// One thread per entry (r, k, c, p) of the batch; the sum over quadrature nodes and the ring's pole axes. Directions
+// and poles are float64 and so is the arithmetic: near a pole 1 - |n.v| is a difference of nearly equal numbers
+// and the kernel amplifies its error by 1/sigma^2 (3300 at sigma = 1 deg), so float32 inputs alone would cost ~1e-4
+// of relative accuracy. Setting up the PF is not on the iteration path, and gfx950 runs float64 at full rate.
+__global__ void __launch_bounds__(256) pf_eval_kernel(const double* __restrict__ coords, const double* __restrict__ poles,
                                                       const int* __restrict__ axis_start,
                                                       const float* __restrict__ axis_count,
-                                                      const float* __restrict__ inv_sigma2,
-                                                      const float* __restrict__ normf, const float* __restrict__ qw,
+                                                      const double* __restrict__ inv_sigma2,
+                                                      const double* __restrict__ normf, const float* __restrict__ qw,
                                                       float* __restrict__ out, int R, int kmax, int C, int P, int A,
                                                       int S, int Se, int k0, int kb, float pf_cut, int quad_weighted) {
   const int64_t total = (int64_t)R * kmax * C * P;
@@ -208,29 +211,30 @@ __global__ void __launch_bounds__(256) pf_eval_kernel(const float* __restrict__ 
       continue;
     }
     const int kg = k0 + k;
-    const float is2 = inv_sigma2[kg];
+    const double is2 = inv_sigma2[kg];
+    const double cut = (double)pf_cut;
     const int a0 = axis_start[p], a1 = axis_start[p + 1];
-    const float* pk = poles + (int64_t)kg * A * 3;
-    float total_w = 0.0f;
+    const double* pk = poles + (int64_t)kg * A * 3;
+    double total_w = 0.0;
     for (int s = 0; s < S; ++s) {
       for (int se = 0; se < Se; ++se) {
-        const float* v = coords + (((((int64_t)r * S + s) * C + c) * Se + se) * P + p) * 3;
-        const float vx = v[0], vy = v[1], vz = v[2];
-        float node = 0.0f;
+        const double* v = coords + (((((int64_t)r * S + s) * C + c) * Se + se) * P + p) * 3;
+        const double vx = v[0], vy = v[1], vz = v[2];
+        double node = 0.0;
         for (int a = a0; a < a1; ++a) {
-          const float d = fabsf(pk[a * 3 + 0] * vx + pk[a * 3 + 1] * vy + pk[a * 3 + 2] * vz);
-          float term = 0.0f;
-          const float x1 = (1.0f - d) * is2;
-          if (x1 < pf_cut) term += expf(-x1);
-          const float x2 = (1.0f + d) * is2;
-          if (x2 < pf_cut) term += expf(-x2);
-          node += axis_count[a] * term;
+          const double d = fabs(pk[a * 3 + 0] * vx + pk[a * 3 + 1] * vy + pk[a * 3 + 2] * vz);
+          double term = 0.0;
+          const double x1 = (1.0 - d) * is2;
+          if (x1 < cut) term += exp(-x1);
+          const double x2 = (1.0 + d) * is2;
+          if (x2 < cut) term += exp(-x2);
+          node += (double)axis_count[a] * term;
         }
-        total_w += quad_weighted ? qw[s * Se + se] * node : node;
+        total_w += quad_weighted ? (double)qw[s * Se + se] * node : node;
       }
     }
-    if (!quad_weighted) total_w = total_w / (float)(S * Se);
-    out[gid] = total_w * normf[kg];
+    if (!quad_weighted) total_w = total_w / (double)(S * Se);
+    out[gid] = (float)(total_w * normf[kg]);
   }
 }
 
@@ -521,8 +525,8 @@ int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, in
                             skstride, ntap, x, z, T, inv_p, lam, (hipStream_t)stream);
 }
 
-int dfm_pf_eval_dense(const float* coords, const float* poles, const int32_t* axis_start, const float* axis_count,
-                      const float* inv_sigma2, const float* norm, const float* qw, float* out, int32_t R,
+int dfm_pf_eval_dense(const double* coords, const double* poles, const int32_t* axis_start, const float* axis_count,
+                      const double* inv_sigma2, const double* norm, const float* qw, float* out, int32_t R,
                       int32_t kmax, int32_t C, int32_t P, int32_t A, int32_t S, int32_t Se, int32_t k0, int32_t kb,
                       float pf_cut, int32_t quad_weighted, dfm_stream_t stream) {
   DFM_CHECK_ARG(coords && poles && axis_start && axis_count && inv_sigma2 && norm && out && (qw || !quad_weighted),

@@ -189,13 +189,17 @@ class SinglePhaseForwardOperator:
         Rm = Rotation.from_rotvec(self.angles_subdivided[:, None] * k0).as_matrix()
         out = []
         for tt in self.two_theta_peaks:
-            half = tt / 2.0
+            # the float32 two-theta, its trig in float64 (differentiable/reference.py:131-141; the reference's
+            # operator takes cos/sin of the float32 half angle in float32, a 3e-8 error that 1/sigma^2 amplifies)
+            half = float(tt) / 2.0
             dz = np.cos(eta2)[..., None] * d0[None, None, :] + np.sin(eta2)[..., None] * d90[None, None, :]
             dz = dz * np.cos(half) - np.sin(half) * p0
             pr = np.einsum("oij,esi->oesj", Rm, dz)
             out.append(pr.reshape(self.N_Omega, self.N_Omega_subdivisions, self.N_eta, Se, 3))
-        self.coords_cpu = np.ascontiguousarray(np.stack(out, axis=-2), dtype=np.float32)
-        self.coords_gpu = torch.from_numpy(self.coords_cpu).to(self.device)
+        coords = np.stack(out, axis=-2)
+        self.coords_cpu = np.ascontiguousarray(coords, dtype=np.float32)
+        # the PF kernel reads float64 directions (the float32 copy above is the reference's coords_cpu)
+        self.coords_gpu = torch.from_numpy(np.ascontiguousarray(coords, dtype=np.float64)).to(self.device)
 
     def _pole_axes(self):
         """Distinct pole axes per ring and their image counts, poles[k, a] = U_k axis_a
@@ -223,12 +227,12 @@ class SinglePhaseForwardOperator:
             raise AssertionError("pole-axis counts do not sum to the point-group order")
         poles = np.einsum("kij,aj->kai", self.grid_rot_cpu, self.pole_axes_cpu)
         dev = self.device
-        self.poles_gpu = torch.from_numpy(np.ascontiguousarray(poles, np.float32)).to(dev)
+        self.poles_gpu = torch.from_numpy(np.ascontiguousarray(poles, np.float64)).to(dev)
         self.axis_start_gpu = torch.from_numpy(self.axis_start_cpu).to(dev)
         self.axis_count_gpu = torch.from_numpy(self.axis_count_cpu).to(dev)
-        s = self.sigma_cpu
-        self.inv_sigma2_gpu = torch.from_numpy((1.0 / (s * s)).astype(np.float32)).to(dev)
-        self.norm_factor_gpu = torch.from_numpy((1.0 / (8.0 * np.pi * s * s)).astype(np.float32)).to(dev)
+        s = self.sigma_cpu.astype(np.float64)         # the float32 widths of the grid, in float64 arithmetic
+        self.inv_sigma2_gpu = torch.from_numpy(1.0 / (s * s)).to(dev)
+        self.norm_factor_gpu = torch.from_numpy(1.0 / (8.0 * np.pi * s * s)).to(dev)
         qw = self.quad_weights_cpu if self.quad_weights_cpu is not None else np.ones(1, np.float32)
         self.quad_weights_gpu = torch.from_numpy(np.asarray(qw, np.float32)).to(dev)
 

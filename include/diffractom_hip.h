@@ -67,9 +67,11 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
 /* ---------------------------------------------------------------- pole-figure matrix
  * Dense PF batch (create_pfo_matrix.py:108-199 pfmatrix_eval_poles): out (R, kmax, C, P) for orientations
  * k0 .. k0+kb-1 (rows kb..kmax-1 zero). coords (R, S, C, Se, P, 3); poles (K, A, 3); axis_start (P+1);
- * axis_count (A); inv_sigma2, norm (K); qw (S*Se) used when quad_weighted (else the node mean). */
-int dfm_pf_eval_dense(const float* coords, const float* poles, const int32_t* axis_start, const float* axis_count,
-                      const float* inv_sigma2, const float* norm, const float* qw, float* out, int32_t R,
+ * axis_count (A); inv_sigma2, norm (K); qw (S*Se) used when quad_weighted (else the node mean). Directions, poles
+ * and kernel widths are float64 (the reference uses float32 copies; float64 keeps the entries of narrow kernels
+ * accurate to float32 rounding). */
+int dfm_pf_eval_dense(const double* coords, const double* poles, const int32_t* axis_start, const float* axis_count,
+                      const double* inv_sigma2, const double* norm, const float* qw, float* out, int32_t R,
                       int32_t kmax, int32_t C, int32_t P, int32_t A, int32_t S, int32_t Se, int32_t k0, int32_t kb,
                       float pf_cut, int32_t quad_weighted, dfm_stream_t stream);
 /* Dense batch -> CSR (pf_kernels.cl:482-548 pf_count_rows_* / pf_fill_rows_*): counts per forward row (r, j) /
