@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""SPDHG-Poisson subset steps per second on one MI355X (workload "al_c1": Al, 180 omega x 103 translations x 180 eta x
+5 rings, 99 x 99 pixels, K = 29713 orientations of a uniform 3-degree cubic fundamental-zone grid, sigma 2 deg, sparse
+CSR pole-figure matrix in float32, 10 omega subsets, Poisson data at a mean of 1000 counts).
+
+A step is one SPDHG subset iteration: the dual update of subset m, the CSR adjoint and the backprojection fused with
+the primal update, the forward projection and CSR product of the next subset. The warm-up and timed steps continue
+one run (the solver's own random subset order). Inputs and all solver arrays are resident in HBM before timing.
+
+Prints one JSON line (metric, value, roofline of the dominant kernel from HIP events over the timed steps,
+cpu_baseline from the C/OpenMP restatement on the host cores, forward/adjoint GB/s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="al_c1")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=512, help="orientations of the host-core baseline sample")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(name, K=None, mean_counts=None):
+    from diffractom_amd import SinglePhaseForwardOperator
+    from diffractom_amd.workloads import CONFIGS, make_grid, poisson_counts, voronoi_truth
+    p = CONFIGS[name]
+    grid, mat, cfg = make_grid(name, K=K)
+    op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="sparse")
+    xt = voronoi_truth(op.K, p["n"], p["grains"], p["radius"], p["seed"], p["coeff"])
+    ax = op.direct(xt)
+    b, _ = poisson_counts(ax, mean_counts or p["mean_counts"], p["seed"])
+    return op, b, p
+
+
+def kernel_times(solver, n_steps):
+    """Average duration (ms) of every kernel class over n_steps, from HIP events recorded on the stream the kernels
+    are launched on (torch's current stream)."""
+    import torch
+    op = solver.op
+    names = ["dual_update", "spmm_adjoint", "radon_backward_spdhg", "radon_forward", "spmm_forward"]
+    tot = {n: 0.0 for n in names}
+    cnt = {n: 0 for n in names}
+    from diffractom_amd import _lib
+    evs = []
+    for _ in range(n_steps):
+        seq, pos = solver._seq, solver._pos
+        blk = seq[pos]
+        if pos + 1 < len(seq):
+            nb = seq[pos + 1]
+            solver._pos += 1
+        else:
+            nseq = solver._epoch_sequence()
+            nb = nseq[0]
+            solver._seq, solver._pos = nseq, 0
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        e[0].record()
+        solver._dual(blk)
+        e[1].record()
+        n_ang, _, _, geo = op._angle_set(blk)
+        sc = op._scratch_buffers(n_ang)
+        bt = op.batches[0]
+        op._pf_adjoint(0, solver.D, sc["sino"], blk)
+        e[2].record()
+        _lib.call("dfm_radon_backward_spdhg", _lib.ptr(sc["sino"]), _lib.ptr(geo), op.Nx, op.Ny, op.My, n_ang,
+                  op.K_stride, bt["koff"], bt["kstride"], bt["kstride"], op.n_taps, _lib.ptr(solver.xk),
+                  _lib.ptr(solver.zk), _lib.ptr(solver.Tk), float(solver._inv_p_data), float(solver.lam),
+                  op._stream())
+        e[3].record()
+        n2, _, geo_f, _ = op._angle_set(nb)
+        op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2)
+        e[4].record()
+        op._pf_forward(0, sc["sino"], solver.D, nb, accumulate=False)
+        e[5].record()
+        evs.append(e)
+    torch.cuda.synchronize()
+    for e in evs:
+        for i, n in enumerate(names):
+            tot[n] += e[i].elapsed_time(e[i + 1])
+            cnt[n] += 1
+    return {n: tot[n] / max(cnt[n], 1) for n in names}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: every GPU box here has one GPU; the multi-GPU "
+                                   "sharding over orientations is a design in DESIGN.md, not a measured path",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+    import torch
+    from diffractom_amd import SPDHGPoisson
+    t0 = time.perf_counter()
+    op, b, p = build_problem(args.config)
+    M = p["subsets"]
+    log(f"[bench] operator K={op.K} nnz={op.sparse_nnz} fill={op.sparse_fill:.4f} batches={len(op.batches)} "
+        f"setup {time.perf_counter() - t0:.1f} s")
+    solver = SPDHGPoisson(op, subsets=M, seed=0)
+    solver.start(np.zeros(op.coeff_shape, np.float32), b)
+    torch.cuda.synchronize()
+    solver.advance(args.warmup)
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+    ts = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    solver.advance(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - ts
+    ms_step = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall:.3f} s)")
+
+    npix = op.Nx * op.Ny
+    K, R, My, CP = op.K, op.N_Omega, op.My, op.N_seg
+    Rm = R / M
+    nnz_m = op.sparse_nnz / M
+    step_bytes = 24.0 * K * npix + 16.0 * Rm * My * CP + 12.0 * nnz_m
+
+    roofline = None
+    ktimes = None
+    if not args.no_kernel_timing:
+        ktimes = kernel_times(solver, args.steps)
+        log("[bench] kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in ktimes.items()))
+        kbytes = {
+            "radon_backward_spdhg": 20.0 * K * npix + 4.0 * Rm * My * K,
+            "radon_forward": 4.0 * K * npix + 4.0 * Rm * My * K,
+            "spmm_adjoint": 6.0 * nnz_m + 4.0 * Rm * My * CP + 4.0 * Rm * My * K,
+            "spmm_forward": 6.0 * nnz_m + 4.0 * Rm * My * CP + 4.0 * Rm * My * K,
+            "dual_update": 20.0 * Rm * My * CP,
+        }
+        dom = max(ktimes, key=ktimes.get)
+        ach = kbytes[dom] / (ktimes[dom] * 1e-3) / 1e9
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_ms": round(ktimes[dom], 5),
+                    "algorithmic_bytes": kbytes[dom]}
+    # forward / adjoint GB/s of the whole operator (API layout, median of 10)
+    xd = solver.finish()
+    xt = torch.from_numpy(np.asarray(xd)).cuda()
+    data = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
+    out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
+    fb = 4.0 * K * npix + 4.0 * R * My * CP + 6.0 * op.sparse_nnz
+    tf, ta = [], []
+    for _ in range(12):
+        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a0.record()
+        op.direct_cl(xt, data)
+        a1.record()
+        op.adjoint_cl(data, out)
+        a2.record()
+        torch.cuda.synchronize()
+        tf.append(a0.elapsed_time(a1))
+        ta.append(a1.elapsed_time(a2))
+    t_fwd, t_adj = float(np.median(tf[2:])), float(np.median(ta[2:]))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            sys.path.insert(0, ROOT)
+            from oracle import cpu_baseline
+            Ks = min(args.cpu_sample, op.K)
+            op_s, b_s, _ = build_problem(args.config, K=Ks)
+            sol_s = SPDHGPoisson(op_s, subsets=M, seed=0)
+            sol_s.start(np.zeros(op_s.coeff_shape, np.float32), b_s)
+            rate, n, thr = cpu_baseline.time_steps(sol_s, budget_s=15.0)
+            scaled = rate * Ks / op.K
+            cpu = {"value": round(scaled, 4), "unit": "SPDHG steps/s", "cores": thr, "kind": "port",
+                   "sample": f"{n} steps of the C/OpenMP FP32 restatement on {Ks} of the {op.K} orientations "
+                             f"({rate:.3f} steps/s), scaled by {Ks}/{op.K} (the step's work is linear in K)"}
+            log(f"[bench] cpu baseline {rate:.3f} steps/s on {Ks} orientations with {thr} threads")
+        except Exception as exc:                                    # the GPU figure stands without it
+            log(f"[bench] cpu baseline failed: {exc!r}")
+
+    value = 1000.0 / ms_step
+    line = {
+        "metric": "SPDHG-Poisson iters/s (subset steps/s), FP32 CSR path",
+        "value": round(value, 3), "unit": "SPDHG subset steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (Voronoi grains, Poisson counts)",
+        "config": {"workload": f"{args.config}: Al 5 rings, 180x103x180, 99x99 px, K={op.K}, sigma 2 deg, "
+                               f"M={M} subsets, sparse CSR PF (nnz {op.sparse_nnz})",
+                   "K": op.K, "nnz": op.sparse_nnz, "subsets": M},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "step_GBps_algorithmic": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+        "forward_ms": round(t_fwd, 4), "adjoint_ms": round(t_adj, 4),
+        "forward_GBps": round(fb / (t_fwd * 1e-3) / 1e9, 1), "adjoint_GBps": round(fb / (t_adj * 1e-3) / 1e9, 1),
+        "kernel_ms": {k: round(v, 5) for k, v in (ktimes or {}).items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -151,19 +151,17 @@ class SPDHGPoisson:
         self.op.direct_k(self.xk, self.D, subset=m)
 
     # ---------------------------------------------------------------- run
-    def run(self, x0, b, n_epochs, weights=None, gain=None, background=None, verbose=0, diagnostics_interval=1,
-            callback=None, state=None):
-        """spdhg_poisson.py:513-751. x0: a torch tensor on the GPU (updated in place and returned) or a NumPy array
-        (updated in place if C-contiguous float32, else a converted copy is returned). callback(epoch, x, stats)
-        after every epoch; a true return stops the run. state: solver.state of an earlier run (z and y; without
-        "z", z = A^T y is recomputed)."""
+    def start(self, x0, b, weights=None, gain=None, background=None, state=None):
+        """Set-up of a run (spdhg_poisson.py:529-662): inputs on the device, rho, the step sizes S and T, x on the
+        support, the dual state, and D = A x on the rows of the first subset. The steps then run with advance()."""
         t_start = time.perf_counter()
         op = self.op
         dev = op.device
         cshape, dshape = tuple(op.coeff_shape), tuple(op.data_shape)
-        x_on_gpu = isinstance(x0, torch.Tensor)
-        host_x = None if x_on_gpu else as_host(x0, cshape, "x0")
-        x_dev = check_device(x0, cshape, "x0", dev) if x_on_gpu else torch.from_numpy(host_x).to(dev)
+        self._x_on_gpu = isinstance(x0, torch.Tensor)
+        self._host_x = None if self._x_on_gpu else as_host(x0, cshape, "x0")
+        self._x_dev = check_device(x0, cshape, "x0", dev) if self._x_on_gpu else \
+            torch.from_numpy(self._host_x).to(dev)
         if isinstance(b, torch.Tensor):
             self.b = check_device(b, dshape, "b", dev)
         else:
@@ -194,7 +192,7 @@ class SPDHGPoisson:
         p = self._p_data
 
         # resident channel-fastest arrays
-        self.xk = op.to_k(x_dev)
+        self.xk = op.to_k(self._x_dev)
         self.zk = torch.zeros(nk, dtype=torch.float32, device=dev)
         self.Tk = torch.empty(nk, dtype=torch.float32, device=dev)
         self.resident_used = NAMES
@@ -227,21 +225,76 @@ class SPDHGPoisson:
                 op.to_k(zc, self.zk)
             else:
                 op.adjoint_k(self.y, self.zk)
-        t_setup = time.perf_counter() - t_start
+        self._ep0 = int(state["epoch"]) if state is not None else 0
+        self._seq = self._epoch_sequence()
+        self._pos = 0
+        op.zero_subset_rows(self.D, self._seq[0])
+        self._forward(self._seq[0])
+        self._t_start = t_start
+        self.t_setup = time.perf_counter() - t_start
 
-        ep0 = int(state["epoch"]) if state is not None else 0
+    def step(self, next_block=None, full_forward=False):
+        """One subset step of the current block: dual update, adjoint + primal update, and the forward of
+        next_block (or of every row with full_forward) into D."""
+        blk = self._seq[self._pos]
+        self._dual(blk)
+        self._adjoint_update(blk)
+        if full_forward:
+            self._forward(None)
+        elif next_block is not None:
+            self._forward(next_block)
+
+    def advance(self, n_steps):
+        """n subset steps in the epoch order (no objective evaluations): each step ends with the forward of the next
+        subset, as in the middle of an epoch."""
+        for _ in range(int(n_steps)):
+            if self._pos + 1 < len(self._seq):
+                nb = self._seq[self._pos + 1]
+                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self._pos += 1
+            else:
+                nseq = self._epoch_sequence()
+                nb = nseq[0]
+                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self._seq, self._pos = nseq, 0
+
+    def finish(self):
+        """Write x back (and the state), free the run's arrays; returns x as run() does."""
+        op = self.op
+        dev = op.device
+        cshape = tuple(op.coeff_shape)
+        op.from_k(self.xk, self._x_dev)
+        zc = torch.empty(cshape, dtype=torch.float32, device=dev)
+        op.from_k(self.zk, zc)
+        torch.cuda.synchronize(dev)
+        self.state = {"epoch": self._ep0 + len(getattr(self, "iter_stats", [])), "z": zc.cpu().numpy(),
+                      "y": self.y.cpu().numpy()}
+        if not self._x_on_gpu:
+            self._host_x[...] = self._x_dev.cpu().numpy()
+        out = self._x_dev if self._x_on_gpu else self._host_x
+        self.b = self.w = self.g = self.bg = self._part = self.y = self.sig = self.D = None
+        self.xk = self.zk = self.Tk = None
+        return out
+
+    def run(self, x0, b, n_epochs, weights=None, gain=None, background=None, verbose=0, diagnostics_interval=1,
+            callback=None, state=None):
+        """spdhg_poisson.py:513-751. x0: a torch tensor on the GPU (updated in place and returned) or a NumPy array
+        (updated in place if C-contiguous float32, else a converted copy is returned). callback(epoch, x, stats)
+        after every epoch; a true return stops the run. state: solver.state of an earlier run (z and y; without
+        "z", z = A^T y is recomputed)."""
+        op = self.op
+        dev = op.device
+        self.start(x0, b, weights, gain, background, state)
         self.iter_stats = []
         f = float("nan")
-        seq = self._epoch_sequence()
-        op.zero_subset_rows(self.D, seq[0])
-        self._forward(seq[0])
-        ep = ep0
+        seq = self._seq
         for e in range(n_epochs):
-            ep = ep0 + e + 1
+            ep = self._ep0 + e + 1
             last = e + 1 == n_epochs
             want_f = (e + 1) % max(1, diagnostics_interval) == 0 or last
             nseq = None
             for j, blk in enumerate(seq):
+                self._pos = j
                 end = j + 1 == len(seq)
                 if not end:
                     nb = seq[j + 1]
@@ -250,44 +303,35 @@ class SPDHGPoisson:
                     nb = nseq[0]
                 else:
                     nb = None
-                self._dual(blk)
-                self._adjoint_update(blk)
                 if (end and want_f) or nb is None or nb == blk:
-                    if (end and want_f) or nb is not None:
-                        self._forward(None)                      # A x on every row
-                        if end and want_f:
-                            f = self._deviance(self.D)
+                    full = (end and want_f) or nb is not None
+                    self.step(None, full_forward=full)
+                    if end and want_f:
+                        f = self._deviance(self.D)
                 else:
-                    self._forward(nb)
+                    self.step(nb)
             if nseq is not None:
                 seq = nseq
+                self._seq = seq
             torch.cuda.synchronize(dev)
             stats = {"epoch": ep, "f": f if want_f else None, "rho": self.rho_used,
-                     "time": time.perf_counter() - t_start}
+                     "time": time.perf_counter() - self._t_start}
             self.iter_stats.append(stats)
             stop = False
             if callback is not None:
-                op.from_k(self.xk, x_dev)
-                if not x_on_gpu:
-                    host_x[...] = x_dev.cpu().numpy()
-                stop = bool(callback(ep, x_dev if x_on_gpu else host_x, stats))
+                op.from_k(self.xk, self._x_dev)
+                if not self._x_on_gpu:
+                    self._host_x[...] = self._x_dev.cpu().numpy()
+                stop = bool(callback(ep, self._x_dev if self._x_on_gpu else self._host_x, stats))
             if verbose and want_f:
-                print(f"[spdhg epoch {ep:4d}] f={f:.8e} {time.perf_counter() - t_start:.1f} s", flush=True)
+                print(f"[spdhg epoch {ep:4d}] f={f:.8e} {time.perf_counter() - self._t_start:.1f} s", flush=True)
             if stop:
                 break
-        op.from_k(self.xk, x_dev)
-        zc = torch.empty(cshape, dtype=torch.float32, device=dev)
-        op.from_k(self.zk, zc)
-        torch.cuda.synchronize(dev)
-        self.state = {"epoch": ep0 + len(self.iter_stats), "z": zc.cpu().numpy(), "y": self.y.cpu().numpy()}
-        self.final_stats = {"epochs": len(self.iter_stats), "final_f": f, "setup_s": t_setup, "rho": self.rho_used,
-                            "subsets": self.M, "resident": self.resident_used,
-                            "time": time.perf_counter() - t_start, "tv": 0.0, "tv_prob": 0.0, **self.scales}
-        if not x_on_gpu:
-            host_x[...] = x_dev.cpu().numpy()
-        self.b = self.w = self.g = self.bg = self._part = self.y = self.sig = self.D = None
-        self.xk = self.zk = self.Tk = None
-        return x_dev if x_on_gpu else host_x
+        out = self.finish()
+        self.final_stats = {"epochs": len(self.iter_stats), "final_f": f, "setup_s": self.t_setup,
+                            "rho": self.rho_used, "subsets": self.M, "resident": self.resident_used,
+                            "time": time.perf_counter() - self._t_start, "tv": 0.0, "tv_prob": 0.0, **self.scales}
+        return out
 
     def _zero_padding(self, a_k):
         """Padding channels (K_batch < kstride) of a channel-fastest array to 0."""
