@@ -363,14 +363,16 @@ class OracleOperator:
             V = self.v[r, :, :, :, p, :].reshape(-1, 3)                           # (S*C*Se, 3)
             Pn = self.poles[ks, a0:a1, :].reshape(-1, 3)                          # (K*na, 3)
             ad = np.abs(V @ Pn.T)                                                 # (nodes, K*na)
-            sc = np.repeat(s, na)[None, :]
-            x1 = (1.0 - ad) * sc
+            sc = np.repeat(s, na)
+            # the first term is kept where (1 - |d|) s < cut, i.e. |d| > 1 - cut / s: exp only there
+            ii, jj = np.nonzero(ad > (1.0 - self.cut / sc)[None, :])
+            x1 = (1.0 - ad[ii, jj]) * sc[jj]
+            ok = x1 < self.cut
             g = np.zeros_like(ad)
-            m1 = x1 < self.cut
-            g[m1] = np.exp(-x1[m1])
-            x2 = (1.0 + ad) * sc
-            m2 = x2 < self.cut
-            if m2.any():
+            g[ii[ok], jj[ok]] = np.exp(-x1[ok])
+            if sc.min() < self.cut:                     # the second term (1 + |d|) s < cut: only for very wide kernels
+                x2 = (1.0 + ad) * sc[None, :]
+                m2 = x2 < self.cut
                 g[m2] += np.exp(-x2[m2])
             g = g.reshape(self.S, self.C, self.Se, len(ks), na) * self.counts[a0:a1]
             node = g.sum(axis=-1)                                                 # (S, C, Se, K)
@@ -422,22 +424,39 @@ class OracleOperator:
 
     def direct(self, x, subset=None):
         """A x (R, My, C P); with subset m only its rows (others 0) (single_phase_forward_operator.py:1646-1729)."""
-        xf = np.asarray(x, np.float64).reshape(self.K, -1)
         d = np.zeros(self.data_shape)
-        for r in self._angles(subset):
-            S = self.radon_matrix(r) @ xf.T                  # (My, K)
-            d[r] = (self.pf_sparse(r).T @ S.T).T             # (My, CP)
+        rows = self._angles(subset)
+        d[rows] = self.direct_rows(x, rows)
         return d
+
+    def _stacked(self, rows):
+        """The projection matrices of the given angles stacked, (len(rows) My, Npix), and its transpose (cached)."""
+        key = tuple(int(r) for r in rows)
+        c = self.__dict__.setdefault("_stack", {})
+        if key not in c:
+            P = sp.vstack([self.radon_matrix(r) for r in key]).tocsr()
+            c[key] = (P, P.T.tocsr())
+        return c[key]
+
+    def direct_rows(self, x, rows):
+        """(len(rows), My, C P): the rows of A x of the given angles."""
+        xt = np.ascontiguousarray(np.asarray(x, np.float64).reshape(self.K, -1).T)     # (Npix, K)
+        S = (self._stacked(rows)[0] @ xt).reshape(len(rows), self.My, self.K)
+        out = np.empty((len(rows), self.My, self.CP))
+        for i, r in enumerate(rows):
+            out[i] = (self.pf_sparse(r).T @ S[i].T).T        # (My, CP)
+        return out
 
     def adjoint(self, y, subset=None):
         """A^T y (K, Ny, Nx), the exact transpose (single_phase_forward_operator.py:1800-2033: alpha R/pi times the
         backprojection weight pi/R is 1); with subset m only its rows are read."""
         yf = np.asarray(y, np.float64)
-        out = np.zeros((self.K, self.Nx * self.Ny))
-        for r in self._angles(subset):
-            Z = self.pf_sparse(r) @ yf[r].T                  # (K, My)
-            out += (self.radon_matrix(r).T @ Z.T).T          # (K, Npix)
-        return out.reshape(self.coeff_shape)
+        rows = self._angles(subset)
+        Z = np.empty((len(rows), self.My, self.K))
+        for i, r in enumerate(rows):
+            Z[i] = (self.pf_sparse(r) @ yf[r].T).T           # (My, K)
+        out = self._stacked(rows)[1] @ Z.reshape(-1, self.K)  # (Npix, K)
+        return np.ascontiguousarray(out.T).reshape(self.coeff_shape)
 
     def set_omega_subsets(self, subsets):
         """Interleaved subsets np.arange(m, R, M) or given lists, each sorted and unique
@@ -587,7 +606,7 @@ class OracleSPDHG:
 
         seq = self._epoch_order()
         D = np.zeros(op.data_shape)
-        D[self.subsets[seq[0]]] = op.direct(x, subset=seq[0])[self.subsets[seq[0]]]
+        D[self.subsets[seq[0]]] = op.direct_rows(x, self.subsets[seq[0]])
         for e in range(n_epochs):
             last = e + 1 == n_epochs
             want_f = (e + 1) % max(1, diagnostics_interval) == 0 or last
@@ -612,7 +631,7 @@ class OracleSPDHG:
                             f = deviance(D, b, w, g, bg, self.eps)
                 else:
                     rows = self.subsets[nb]
-                    D[rows] = op.direct(x, subset=nb)[rows]
+                    D[rows] = op.direct_rows(x, rows)
             if nseq is not None:
                 seq = nseq
             self.iter_stats.append({"epoch": ep0 + e + 1, "f": f if want_f else None, "rho": rho})
