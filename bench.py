@@ -83,13 +83,11 @@ def kernel_times(solver, n_steps):
         bt = op.batches[0]
         op._pf_adjoint(0, solver.D, sc["sino"], blk)
         e[2].record()
-        _lib.call("dfm_radon_backward_spdhg", _lib.ptr(sc["sino"]), _lib.ptr(geo), op.Nx, op.Ny, op.My, n_ang,
-                  op.K_stride, bt["koff"], bt["kstride"], bt["kstride"], op.n_taps, _lib.ptr(solver.xk),
-                  _lib.ptr(solver.zk), _lib.ptr(solver.Tk), float(solver._inv_p_data), float(solver.lam),
-                  op._stream())
+        op._radon_backward(sc["sino"], geo, op._tables(blk), n_ang, None, op.K_stride, bt["koff"], bt["kstride"],
+                           x=solver.xk, z=solver.zk, T=solver.Tk, inv_p=solver._inv_p_data, lam=solver.lam)
         e[3].record()
         n2, _, geo_f, _ = op._angle_set(nb)
-        op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2)
+        op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2, op._tables(nb))
         e[4].record()
         op._pf_forward(0, sc["sino"], solver.D, nb, accumulate=False)
         e[5].record()

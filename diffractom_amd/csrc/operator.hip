@@ -70,17 +70,43 @@ struct AngleGeo {
   float X, Y, T0, wscale, hw, top, pad0, pad1;
 };
 
-// Forward: one thread per (channel quad, NB consecutive bins, angle). For every image row along the axis with the
-// larger |coefficient| (the one whose pixels are at most a bin apart on the detector) only the pixels whose footprint
-// reaches the thread's bins are visited.
+// XCD-aware work split. A launch of n_blk channel blocks (16 float4 quads = 64 orientations each) x per_blk
+// workgroups per block is a 1-D grid whose workgroup i runs on XCD i % 8 (the dispatcher's round robin). Block c is
+// mapped to XCD c % 8 and its per_blk workgroups run there back to back, so the block's working set (its image
+// columns, 2.5 MB at 99 x 99 pixels, or its sinogram slab) stays in that XCD's 4 MB L2 while every angle / pixel
+// of the block is processed, instead of being re-read from HBM once per angle.
+constexpr int N_XCD = 8;
+constexpr int QB = 16;   // float4 quads per channel block
+
+__device__ __forceinline__ bool xcd_split(int per_blk, int n_blk, int& blk, int& local) {
+  const int i = blockIdx.x;
+  const int xcd = i % N_XCD;
+  const int j = i / N_XCD;
+  blk = (j / per_blk) * N_XCD + xcd;
+  local = j % per_blk;
+  return blk < n_blk;
+}
+
+inline unsigned xcd_grid(int per_blk, int n_blk) {
+  const int rounds = (n_blk + N_XCD - 1) / N_XCD;
+  return (unsigned)rounds * N_XCD * per_blk;
+}
+
+// Forward: a thread per (channel quad, NB consecutive bins) of one angle; a workgroup covers 16 quads x 16 bin groups
+// (16 NB bins) of one angle. For every image row along the axis with the larger |coefficient| only the pixels whose
+// footprint reaches the thread's bins are visited.
 template <int NB>
 __global__ void __launch_bounds__(256) radon_forward_kernel(const float4* __restrict__ img, float4* __restrict__ sino,
                                                             const AngleGeo* __restrict__ geo, int nx, int ny, int ns,
-                                                            int kstride4, int k4_off, int k4_n, int skstride4,
+                                                            int na, int kstride4, int k4_off, int k4_n, int skstride4,
                                                             float scale) {
-  const int kq = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int s0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * NB;
-  const int a = blockIdx.z;
+  const int sblocks = (ns + 16 * NB - 1) / (16 * NB);
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, local;
+  if (!xcd_split(na * sblocks, n_blk, blk, local)) return;
+  const int a = local / sblocks;
+  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const int s0 = ((local % sblocks) * 16 + (threadIdx.x >> 4)) * NB;
   if (kq >= k4_n || s0 >= ns) return;
   const int k = kq + k4_off;
   const AngleGeo g = geo[a];
@@ -131,17 +157,23 @@ __global__ void __launch_bounds__(256) radon_forward_kernel(const float4* __rest
   }
 }
 
-// Backward: one thread per (channel quad, pixel); the taps of the pixel at every angle. MODE 0 writes the image,
-// MODE 1 applies the SPDHG primal update to (x, z) with the step T.
+// Backward: a thread per (channel quad, pixel); a workgroup covers 16 quads x 16 consecutive pixels, which share
+// most of their sinogram bins at every angle. MODE 0 writes the image, MODE 1 applies the SPDHG primal update to
+// (x, z) with the step T.
 template <int NTAP, int MODE>
 __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __restrict__ sino, float4* __restrict__ out,
                                                              const AngleGeo* __restrict__ geo, int nx, int ny, int ns,
                                                              int na, int kstride4, int k4_off, int k4_n, int skstride4,
                                                              float4* __restrict__ x, float4* __restrict__ z,
                                                              const float4* __restrict__ T, float inv_p, float lam) {
-  const int kq = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int64_t pix = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (kq >= k4_n || pix >= (int64_t)nx * ny) return;
+  const int64_t npix = (int64_t)nx * ny;
+  const int pblocks = (int)((npix + 15) / 16);
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, local;
+  if (!xcd_split(pblocks, n_blk, blk, local)) return;
+  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const int64_t pix = (int64_t)local * 16 + (threadIdx.x >> 4);
+  if (kq >= k4_n || pix >= npix) return;
   const int k = kq + k4_off;
   const float px = (float)(pix % nx), py = (float)(pix / nx);
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -169,6 +201,144 @@ __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __res
     acc.w += g.wscale * v.w;
   }
   const int64_t o = pix * kstride4 + k;
+  if (MODE == 0) {
+    out[o] = acc;
+  } else {
+    float4 zz = z[o], xx = x[o];
+    const float4 tt = T[o];
+    float4 zn, xn;
+#define DFM_UPD(c)                                                                  \
+  zn.c = zz.c + acc.c;                                                              \
+  xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zn.c + inv_p * acc.c) + lam)) : 0.0f;
+    DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
+#undef DFM_UPD
+    z[o] = zn;
+    x[o] = xn;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ Radon, tables
+// The projector's geometry depends on the angles and the pixel grid only, not on the orientations, so it is computed
+// once per angle set (on the host, in float64) and the kernels spend their instructions on the orientations:
+//   forward table:  per (angle, bin) the pixels whose footprint reaches the bin and their weights (a CSR of the
+//                   projection matrix, the scale and 1/A folded in);
+//   pixel table:    per (angle, pixel) its two bins and the backprojection weights (angle weight and 1/A folded in);
+//   tile bases:     per (angle, 4 x 4 pixel tile) the first of the TB bins the tile reaches.
+
+// Forward from the table: a thread per (channel quad, bin); a workgroup covers 16 quads x 16 consecutive bins of one
+// angle; the channel blocks are split over the XCDs as above. UNR entries are fetched before they are used, so a
+// wave keeps UNR independent loads in flight instead of one dependent chain per entry.
+template <int UNR>
+__global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* __restrict__ img,
+                                                                  float4* __restrict__ sino,
+                                                                  const int* __restrict__ row_ptr,
+                                                                  const int* __restrict__ ent_pix,
+                                                                  const float* __restrict__ ent_w, int ns, int na,
+                                                                  int kstride4, int k4_off, int k4_n, int skstride4) {
+  const int sblocks = (ns + 15) / 16;
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, local;
+  if (!xcd_split(na * sblocks, n_blk, blk, local)) return;
+  const int a = local / sblocks;
+  const int s = (local % sblocks) * 16 + (threadIdx.x >> 4);
+  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  if (kq >= k4_n || s >= ns) return;
+  const float4* col = img + kq + k4_off;
+  const int r = a * ns + s;
+  const int e1 = row_ptr[r + 1];
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  int e = row_ptr[r];
+  for (; e + UNR <= e1; e += UNR) {
+    int p[UNR];
+    float w[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      p[u] = ent_pix[e + u];
+      w[u] = ent_w[e + u];
+    }
+    float4 v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) v[u] = col[(int64_t)p[u] * kstride4];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      acc.x += w[u] * v[u].x;
+      acc.y += w[u] * v[u].y;
+      acc.z += w[u] * v[u].z;
+      acc.w += w[u] * v[u].w;
+    }
+  }
+  for (; e < e1; ++e) {
+    const float w = ent_w[e];
+    const float4 v = col[(int64_t)ent_pix[e] * kstride4];
+    acc.x += w * v.x;
+    acc.y += w * v.y;
+    acc.z += w * v.z;
+    acc.w += w * v.w;
+  }
+  sino[(int64_t)r * skstride4 + kq] = acc;
+}
+
+// Backward from the tables, one 4 x 4 pixel tile x 16 channel quads per workgroup: the TB bins of every angle the
+// tile reaches are staged in LDS (TB x 16 quads x 16 B per angle, angles in chunks of ACH), then every thread (pixel,
+// quad) reads its two taps from LDS. MODE 0 writes the image, MODE 1 applies the SPDHG primal update.
+constexpr int TB = 8;     // bins per (angle, tile)
+
+template <int MODE, int ACH>
+__global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* __restrict__ sino,
+                                                                  float4* __restrict__ out,
+                                                                  const int4* __restrict__ ptab,
+                                                                  const int* __restrict__ tbase, int nx, int ny,
+                                                                  int ns, int na, int kstride4, int k4_off, int k4_n,
+                                                                  int skstride4, float4* __restrict__ x,
+                                                                  float4* __restrict__ z, const float4* __restrict__ T,
+                                                                  float inv_p, float lam) {
+  __shared__ float4 lds[ACH * TB * QB];   // ACH angles x TB bins x 16 quads
+  const int tnx = (nx + 3) / 4, tny = (ny + 3) / 4;
+  const int ntiles = tnx * tny;
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, tile;
+  if (!xcd_split(ntiles, n_blk, blk, tile)) return;
+  const int q = threadIdx.x & (QB - 1);
+  const int li = threadIdx.x >> 4;                       // pixel of the tile, row-major 4 x 4
+  const int px = (tile % tnx) * 4 + (li & 3), py = (tile / tnx) * 4 + (li >> 2);
+  const bool inside = px < nx && py < ny;
+  const int kq = blk * QB + q;
+  const bool live = inside && kq < k4_n;
+  const int64_t pix = (int64_t)py * nx + px;
+  const int64_t npix = (int64_t)nx * ny;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int a0 = 0; a0 < na; a0 += ACH) {
+    const int nan = min(ACH, na - a0);
+    __syncthreads();
+    // stage: element i = (angle al, bin b, quad qq)
+    for (int i = threadIdx.x; i < nan * TB * QB; i += 256) {
+      const int qq = i & (QB - 1);
+      const int b = (i >> 4) % TB;
+      const int al = i / (TB * QB);
+      const int s = tbase[(int64_t)(a0 + al) * ntiles + tile] + b;
+      const int kk = blk * QB + qq;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (s >= 0 && s < ns && kk < k4_n) v = sino[((int64_t)(a0 + al) * ns + s) * skstride4 + kk];
+      lds[i] = v;
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll 4
+      for (int al = 0; al < nan; ++al) {
+        const int4 e = ptab[(int64_t)(a0 + al) * npix + pix];
+        const int base = tbase[(int64_t)(a0 + al) * ntiles + tile];
+        const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
+        const float4 s0 = lds[(al * TB + (e.x - base)) * QB + q];
+        const float4 s1 = lds[(al * TB + (e.y - base)) * QB + q];
+        acc.x += w0 * s0.x + w1 * s1.x;
+        acc.y += w0 * s0.y + w1 * s1.y;
+        acc.z += w0 * s0.z + w1 * s1.z;
+        acc.w += w0 * s0.w + w1 * s1.w;
+      }
+    }
+  }
+  if (!live) return;
+  const int64_t o = pix * kstride4 + kq + k4_off;
   if (MODE == 0) {
     out[o] = acc;
   } else {
@@ -436,7 +606,7 @@ static int launch_backward(const char* who, const float* sino, float* img_k, con
                            const float* T, float inv_p, float lam, hipStream_t stream) {
   const int k4n = k_n / 4;
   const int64_t npix = (int64_t)nx * ny;
-  dim3 grid((unsigned)((k4n + 63) / 64), (unsigned)((npix + 3) / 4));
+  const dim3 grid(xcd_grid((int)((npix + 15) / 16), (k4n + QB - 1) / QB));
 #define DFM_BWD(NT)                                                                                               \
   hipLaunchKernelGGL((radon_backward_kernel<NT, MODE>), grid, dim3(256), 0, stream, (const float4*)sino,          \
                      (float4*)img_k, (const AngleGeo*)geo, nx, ny, ns, na, kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, \
@@ -495,10 +665,11 @@ int dfm_radon_forward(const float* img_k, float* sino, const float* geo, int32_t
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   constexpr int NB = 2;
   const int k4n = k_n / 4;
-  const int groups = (ns + NB - 1) / NB;
-  dim3 grid((unsigned)((k4n + 63) / 64), (unsigned)((groups + 3) / 4), (unsigned)n_angles);
-  hipLaunchKernelGGL(radon_forward_kernel<NB>, grid, dim3(256), 0, (hipStream_t)stream, (const float4*)img_k,
-                     (float4*)sino, (const AngleGeo*)geo, nx, ny, ns, kstride / 4, k_off / 4, k4n, skstride / 4, scale);
+  const int sblocks = (ns + 16 * NB - 1) / (16 * NB);
+  const unsigned grid = xcd_grid(n_angles * sblocks, (k4n + QB - 1) / QB);
+  hipLaunchKernelGGL(radon_forward_kernel<NB>, dim3(grid), dim3(256), 0, (hipStream_t)stream, (const float4*)img_k,
+                     (float4*)sino, (const AngleGeo*)geo, nx, ny, ns, n_angles, kstride / 4, k_off / 4, k4n,
+                     skstride / 4, scale);
   DFM_LAUNCH_CHECK("dfm_radon_forward");
   return DFM_OK;
 }
@@ -523,6 +694,64 @@ int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, in
   if (k_n == 0) return DFM_OK;
   return launch_backward<1>("dfm_radon_backward_spdhg", sino, nullptr, geo, nx, ny, ns, n_angles, kstride, k_off, k_n,
                             skstride, ntap, x, z, T, inv_p, lam, (hipStream_t)stream);
+}
+
+int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
+                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
+                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(img_k && sino && row_ptr && ent_pix && ent_w, "dfm_radon_forward_table: null pointer");
+  DFM_CHECK_ARG(ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
+                    k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
+                "dfm_radon_forward_table: bad sizes");
+  DFM_CHECK_ARG(variant == 1 || variant == 4 || variant == 8, "dfm_radon_forward_table: variant must be 1, 4 or 8");
+  if (n_angles == 0 || k_n == 0) return DFM_OK;
+  const int k4n = k_n / 4;
+  const unsigned grid = xcd_grid(n_angles * ((ns + 15) / 16), (k4n + QB - 1) / QB);
+#define DFM_FT(U)                                                                                                 \
+  hipLaunchKernelGGL(radon_forward_table_kernel<U>, dim3(grid), dim3(256), 0, (hipStream_t)stream,               \
+                     (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,      \
+                     k_off / 4, k4n, skstride / 4)
+  if (variant == 1)
+    DFM_FT(1);
+  else if (variant == 4)
+    DFM_FT(4);
+  else
+    DFM_FT(8);
+#undef DFM_FT
+  DFM_LAUNCH_CHECK("dfm_radon_forward_table");
+  return DFM_OK;
+}
+
+int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,
+                             int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
+                             int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
+                             int32_t mode, int32_t variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && ptab && tbase && (mode == 0 ? img_k != nullptr : (x && z && T)),
+                "dfm_radon_backward_table: null pointer");
+  DFM_CHECK_ARG(nx > 0 && ny > 0 && ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 &&
+                    k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
+                    skstride >= k_n && (mode == 0 || mode == 1),
+                "dfm_radon_backward_table: bad sizes");
+  DFM_CHECK_ARG(variant == 4 || variant == 8 || variant == 16 || variant == 32,
+                "dfm_radon_backward_table: variant (angles staged per pass) must be 4, 8, 16 or 32");
+  if (k_n == 0) return DFM_OK;
+  const int k4n = k_n / 4;
+  const int ntiles = ((nx + 3) / 4) * ((ny + 3) / 4);
+  const unsigned grid = xcd_grid(ntiles, (k4n + QB - 1) / QB);
+#define DFM_BT(M, A)                                                                                              \
+  hipLaunchKernelGGL((radon_backward_tile_kernel<M, A>), dim3(grid), dim3(256), 0, (hipStream_t)stream,          \
+                     (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
+                     kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
+  if (mode == 0) {
+    if (variant == 4) DFM_BT(0, 4); else if (variant == 8) DFM_BT(0, 8); else if (variant == 16) DFM_BT(0, 16);
+    else DFM_BT(0, 32);
+  } else {
+    if (variant == 4) DFM_BT(1, 4); else if (variant == 8) DFM_BT(1, 8); else if (variant == 16) DFM_BT(1, 16);
+    else DFM_BT(1, 32);
+  }
+#undef DFM_BT
+  DFM_LAUNCH_CHECK("dfm_radon_backward_table");
+  return DFM_OK;
 }
 
 int dfm_pf_eval_dense(const double* coords, const double* poles, const int32_t* axis_start, const float* axis_count,

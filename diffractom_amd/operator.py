@@ -71,6 +71,57 @@ def as_host(a, shape, name):
     return out
 
 
+def radon_tables(geo64, idx, nx, ny, ns, radon_scale):
+    """Geometry tables of the projector for the angles idx (compact order), in float64 NumPy: the forward CSR over
+    (angle, bin) -> (pixel, weight), the pixel table (bin, bin, w0, w1) of the backprojection and the first bin of
+    every 4 x 4 pixel tile (the C ABI's dfm_radon_forward_table / dfm_radon_backward_table). Weights are those of
+    ParallelRadon (parallel_radon.py:92-124): forward scale/A K(t - s), backward (pi/N_Omega)/A K(t - s),
+    K(d) = clamp(hw - |d|, 0, top); pixels touch two bins (A <= 1)."""
+    g = geo64
+    idx = np.asarray(idx, np.int64)
+    yy, xx = np.meshgrid(np.arange(ny, dtype=np.float64), np.arange(nx, dtype=np.float64), indexing="ij")
+    px, py = xx.ravel(), yy.ravel()
+    npix = nx * ny
+    t = g["X"][idx, None] * px[None] + g["Y"][idx, None] * py[None] + g["T0"][idx, None]      # (na, npix)
+    hw, top = g["hw"][idx, None], g["top"][idx, None]
+    sf = np.floor(t - hw).astype(np.int64) + 1
+    taps = [sf, sf + 1]
+    kern = [np.clip(hw - np.abs(t - s_), 0.0, top) for s_ in taps]
+    valid = [(s_ >= 0) & (s_ < ns) & (k_ > 0) for s_, k_ in zip(taps, kern)]
+    fwd_scale = (radon_scale * g["inv_a"][idx])[:, None]
+    bwd_scale = (g["w"][idx] * g["inv_a"][idx])[:, None]
+    na = len(idx)
+    rows, pix, wts = [], [], []
+    a_idx = np.broadcast_to(np.arange(na)[:, None], t.shape)
+    p_idx = np.broadcast_to(np.arange(npix)[None], t.shape)
+    for s_, k_, v_ in zip(taps, kern, valid):
+        rows.append((a_idx * ns + s_)[v_])
+        pix.append(p_idx[v_])
+        wts.append((fwd_scale * k_)[v_])
+    rows, pix, wts = np.concatenate(rows), np.concatenate(pix), np.concatenate(wts)
+    order = np.lexsort((pix, rows))
+    row_ptr = np.zeros(na * ns + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=na * ns), out=row_ptr[1:])
+    sc = [np.clip(s_, 0, ns - 1) for s_ in taps]
+    wb = [np.where(v_, bwd_scale * k_, 0.0) for k_, v_ in zip(kern, valid)]
+    ptab = np.empty((na, npix, 4), np.int32)
+    ptab[..., 0], ptab[..., 1] = sc[0], sc[1]
+    ptab[..., 2] = wb[0].astype(np.float32).view(np.int32)
+    ptab[..., 3] = wb[1].astype(np.float32).view(np.int32)
+    tnx, tny = -(-nx // 4), -(-ny // 4)
+    tile = ((py // 4) * tnx + (px // 4)).astype(np.int64)
+    lo, hi = np.minimum(sc[0], sc[1]), np.maximum(sc[0], sc[1])
+    base = np.full((na, tnx * tny), np.iinfo(np.int64).max)
+    top_ = np.full((na, tnx * tny), np.iinfo(np.int64).min)
+    for a in range(na):
+        np.minimum.at(base[a], tile, lo[a])
+        np.maximum.at(top_[a], tile, hi[a])
+    if np.any(top_ - base >= 8):
+        raise AssertionError("a 4 x 4 pixel tile reaches more than 8 bins at some angle")
+    return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32),
+            "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32)}
+
+
 class SinglePhaseForwardOperator:
     """Single-material pole-figure forward operator: data = PF @ Radon(coeffs) and its adjoint."""
 
@@ -151,6 +202,7 @@ class SinglePhaseForwardOperator:
         self.pf_batch_max_gb = float(max_gb)
         self._make_batches(max_gb)
         self.omega_subsets = None
+        self._sub_tables = None
 
         # ---- PF matrix (single_phase_forward_operator.py:500-528, 1207-1374)
         self.pf_mode = pf_mode
@@ -265,11 +317,20 @@ class SinglePhaseForwardOperator:
         if self.n_taps > 4:
             raise NotImplementedError("pixels larger than 3 detector bins (voxel_size) are not implemented")
         z = np.zeros_like(A)
+        self._geo64 = dict(X=X, Y=Y, T0=T0, hw=hw, top=top, inv_a=inv_a, w=w)
         geo_f = np.stack([X, Y, T0, inv_a, hw, top, z, z], 1).astype(np.float32)      # forward: scale 1/A
         geo_b = np.stack([X, Y, T0, w * inv_a, hw, top, z, z], 1).astype(np.float32)  # backward: w/A
         self.geo_fwd_cpu, self.geo_bwd_cpu = geo_f, geo_b
         self.geo_fwd = torch.from_numpy(geo_f).to(self.device)
         self.geo_bwd = torch.from_numpy(geo_b).to(self.device)
+        # kernel variants (identical results; set by autotune_projector)
+        self.radon_variant = {"fwd": 8, "bwd": 8, "fwd_kernel": "table", "bwd_kernel": "table"}
+        self._tables_full = self._radon_tables(np.arange(self.N_Omega)) if self.n_taps == 2 else None
+
+    def _radon_tables(self, idx):
+        """Device copies of radon_tables() for the angles idx (compact order)."""
+        t = radon_tables(self._geo64, idx, self.Nx, self.Ny, self.My, self.radon_scale)
+        return {k: torch.from_numpy(v).to(self.device) for k, v in t.items()}
 
     def _make_batches(self, max_gb, K_batch=None):
         """Orientation batches (single_phase_forward_operator.py:1026-1093): as many orientations as a dense PF batch
@@ -464,6 +525,7 @@ class SinglePhaseForwardOperator:
         self._sub_omegas = [torch.from_numpy(s.astype(np.int32)).to(self.device) for s in out]
         self._sub_geo_fwd = [self.geo_fwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
         self._sub_geo_bwd = [self.geo_bwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
+        self._sub_tables = [self._radon_tables(s) for s in out] if self.n_taps == 2 else None
         return out
 
     def release_omega_subsets(self):
@@ -481,13 +543,42 @@ class SinglePhaseForwardOperator:
         m = self._subset(subset)
         return len(self.omega_subsets[m]), self._sub_omegas[m], self._sub_geo_fwd[m], self._sub_geo_bwd[m]
 
+    def _tables(self, subset):
+        """The geometry tables of the full angle set or subset m (None when the projector needs more than 2
+        taps per pixel: the arithmetic kernels then run)."""
+        if subset is None:
+            return self._tables_full
+        return None if self._sub_tables is None else self._sub_tables[self._subset(subset)]
+
+    def _radon_backward(self, sino, geo, tab, n_ang, out_k, kstride, koff, kn, x=None, z=None, T=None, inv_p=0.0,
+                        lam=0.0):
+        """img (mode 0, out_k) or the SPDHG update of x, z (mode 1) from the sinogram of n_ang angles."""
+        st = self._stream()
+        mode = 0 if x is None else 1
+        if tab is not None:
+            _lib.call("dfm_radon_backward_table", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(tab["ptab"]),
+                      _lib.ptr(tab["tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff, kn, kn, _lib.ptr(x),
+                      _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), mode, self.radon_variant["bwd"], st)
+        elif mode == 0:
+            _lib.call("dfm_radon_backward", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(geo), self.Nx, self.Ny,
+                      self.My, n_ang, kstride, koff, kn, kn, self.n_taps, st)
+        else:
+            _lib.call("dfm_radon_backward_spdhg", _lib.ptr(sino), _lib.ptr(geo), self.Nx, self.Ny, self.My, n_ang,
+                      kstride, koff, kn, kn, self.n_taps, _lib.ptr(x), _lib.ptr(z), _lib.ptr(T), float(inv_p),
+                      float(lam), st)
+
     def zero_subset_rows(self, data, subset):
         m = self._subset(subset)
         _lib.call("dfm_zero_rows", _lib.ptr(data), _lib.ptr(self._sub_omegas[m]), len(self.omega_subsets[m]),
                   self.My * self.N_seg, self._stream())
 
     # ================================================================== forward
-    def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles):
+    def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles, tab=None):
+        if tab is not None:
+            _lib.call("dfm_radon_forward_table", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["row_ptr"]),
+                      _lib.ptr(tab["pix"]), _lib.ptr(tab["w"]), self.My, n_angles, img_kstride, koff, kn, kn,
+                      self.radon_variant["fwd"], self._stream())
+            return
         _lib.call("dfm_radon_forward", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(geo), self.Nx, self.Ny, self.My,
                   n_angles, img_kstride, koff, kn, kn, self.radon_scale, self.n_taps, self._stream())
 
@@ -549,13 +640,14 @@ class SinglePhaseForwardOperator:
         check_device(coeffs, self.coeff_shape, "coeffs", self.device)
         check_device(data, self.data_shape, "data", self.device)
         n_ang, _, geo, _ = self._angle_set(subset)
+        tab = self._tables(subset)
         sc = self._scratch_buffers(n_ang)
         npix = self.Nx * self.Ny
         st = self._stream()
         for ib, b in enumerate(self.batches):
             ks = b["kstride"]
             _lib.call("dfm_gather_k", _lib.ptr(coeffs), _lib.ptr(sc["img"]), npix, b["k_start"], b["K_batch"], ks, st)
-            self._radon_forward(sc["img"], ks, 0, ks, sc["sino"], geo, n_ang)
+            self._radon_forward(sc["img"], ks, 0, ks, sc["sino"], geo, n_ang, tab)
             self._pf_forward(ib, sc["sino"], data, subset, accumulate=ib > 0)
 
     def adjoint_cl(self, data, coeffs, subset=None):
@@ -564,34 +656,34 @@ class SinglePhaseForwardOperator:
         check_device(data, self.data_shape, "data", self.device)
         check_device(coeffs, self.coeff_shape, "coeffs", self.device)
         n_ang, _, _, geo = self._angle_set(subset)
+        tab = self._tables(subset)
         sc = self._scratch_buffers(n_ang)
         npix = self.Nx * self.Ny
         st = self._stream()
         for ib, b in enumerate(self.batches):
             ks = b["kstride"]
             self._pf_adjoint(ib, data, sc["sino"], subset)
-            _lib.call("dfm_radon_backward", _lib.ptr(sc["sino"]), _lib.ptr(sc["img"]), _lib.ptr(geo), self.Nx,
-                      self.Ny, self.My, n_ang, ks, 0, ks, ks, self.n_taps, st)
+            self._radon_backward(sc["sino"], geo, tab, n_ang, sc["img"], ks, 0, ks)
             _lib.call("dfm_scatter_k", _lib.ptr(sc["img"]), _lib.ptr(coeffs), npix, b["k_start"], b["K_batch"], ks, st)
 
     # channel-fastest forms (used by the solver): x_k (Npix, K_stride), batch b at columns koff .. koff + kstride
     def direct_k(self, x_k, data, subset=None):
         """data (rows of subset) = A x for a channel-fastest coefficient array."""
         n_ang, _, geo, _ = self._angle_set(subset)
+        tab = self._tables(subset)
         sc = self._scratch_buffers(n_ang)
         for ib, b in enumerate(self.batches):
-            self._radon_forward(x_k, self.K_stride, b["koff"], b["kstride"], sc["sino"], geo, n_ang)
+            self._radon_forward(x_k, self.K_stride, b["koff"], b["kstride"], sc["sino"], geo, n_ang, tab)
             self._pf_forward(ib, sc["sino"], data, subset, accumulate=ib > 0)
 
     def adjoint_k(self, data, out_k, subset=None):
         """out_k = A^T data (channel-fastest)."""
         n_ang, _, _, geo = self._angle_set(subset)
+        tab = self._tables(subset)
         sc = self._scratch_buffers(n_ang)
-        st = self._stream()
         for ib, b in enumerate(self.batches):
             self._pf_adjoint(ib, data, sc["sino"], subset)
-            _lib.call("dfm_radon_backward", _lib.ptr(sc["sino"]), _lib.ptr(out_k), _lib.ptr(geo), self.Nx, self.Ny,
-                      self.My, n_ang, self.K_stride, b["koff"], b["kstride"], b["kstride"], self.n_taps, st)
+            self._radon_backward(sc["sino"], geo, tab, n_ang, out_k, self.K_stride, b["koff"], b["kstride"])
 
     def to_k(self, coeffs, out_k=None):
         """(K, Ny, Nx) -> channel-fastest (Npix, K_stride)."""

@@ -138,13 +138,12 @@ class SPDHGPoisson:
         backprojection + primal update on the batch's columns of x, z, T."""
         op = self.op
         n_ang, _, _, geo = op._angle_set(m)
+        tab = op._tables(m)
         sc = op._scratch_buffers(n_ang)
-        st = op._stream()
         for ib, bt in enumerate(op.batches):
             op._pf_adjoint(ib, self.D, sc["sino"], m)
-            _lib.call("dfm_radon_backward_spdhg", _lib.ptr(sc["sino"]), _lib.ptr(geo), op.Nx, op.Ny, op.My, n_ang,
-                      op.K_stride, bt["koff"], bt["kstride"], bt["kstride"], op.n_taps, _lib.ptr(self.xk),
-                      _lib.ptr(self.zk), _lib.ptr(self.Tk), float(self._inv_p_data), float(self.lam), st)
+            op._radon_backward(sc["sino"], geo, tab, n_ang, None, op.K_stride, bt["koff"], bt["kstride"],
+                               x=self.xk, z=self.zk, T=self.Tk, inv_p=self._inv_p_data, lam=self.lam)
 
     def _forward(self, m):
         """D (rows of subset m, or all rows for m None) = A x."""

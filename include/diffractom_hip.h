@@ -64,6 +64,25 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
                        int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap,
                        dfm_stream_t stream);
 
+/* The same projector driven by precomputed geometry tables (radon_kernels.cl:44-141 with the per-pixel footprint
+ * arithmetic done once per angle set on the host, in float64; ntap = 2, i.e. pixels no larger than a bin):
+ * forward table: CSR over (angle, bin) rows, row_ptr (n_angles*ns + 1), entries (pixel, weight) with weight =
+ *   scale * wscale_a * K(t - s);  sino[a, s, k] = sum_e weight_e img[pixel_e, k]
+ * pixel table ptab (n_angles, Npix, 4) int32: bin s0, bin s1 (>= 0, < ns), float bits of w0, w1 (the backprojection
+ *   weights, wscale_a folded in; 0 for taps off the detector);
+ * tile bases tbase (n_angles, ceil(nx/4) * ceil(ny/4)): the first of the 8 bins every 4 x 4 pixel tile reaches.
+ * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg.
+ * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8; backward: angles
+ * staged in LDS per pass, 4, 8, 16 or 32), picked by timing (the reference times its tiled / plain projectors the
+ * same way, parallel_radon.py:240-281). */
+int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
+                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
+                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
+int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,
+                             int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
+                             int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
+                             int32_t mode, int32_t variant, dfm_stream_t stream);
+
 /* ---------------------------------------------------------------- pole-figure matrix
  * Dense PF batch (create_pfo_matrix.py:108-199 pfmatrix_eval_poles): out (R, kmax, C, P) for orientations
  * k0 .. k0+kb-1 (rows kb..kmax-1 zero). coords (R, S, C, Se, P, 3); poles (K, A, 3); axis_start (P+1);

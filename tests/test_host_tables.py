@@ -1,0 +1,51 @@
+"""The projector's geometry tables (host, float64) reproduce the oracle's projection matrices: the forward CSR is
+scale/A K(t - s) of every pixel and bin, the pixel table the same weights times the backprojection weight pi/R, and
+every pixel's two bins lie in the 8 bins of its tile (CPU)."""
+import numpy as np
+import pytest
+
+from diffractom_amd.operator import radon_tables
+from oracle import model as O
+from pairs import AL3, cfg, orientations
+
+
+def _geo(op, kernel):
+    X, Y, T0 = op.X, op.Y, op.T0
+    A = np.maximum(np.abs(X.astype(np.float32)), np.abs(Y.astype(np.float32))).astype(np.float64)
+    if kernel == "hat":
+        hw, top, inv_a = np.ones_like(A), np.ones_like(A), np.ones_like(A)
+    else:
+        hw, top, inv_a = 0.5 * (1 + A), np.minimum(1, A), 1 / A
+    return dict(X=X, Y=Y, T0=T0, hw=hw, top=top, inv_a=inv_a, w=np.full(op.R, np.pi / op.R))
+
+
+@pytest.mark.parametrize("kernel", ["trapezoid", "hat"])
+@pytest.mark.parametrize("shape", [(16, 16, 17, (0, 180)), (14, 12, 17, (0, 360)), (99, 99, 103, (0, 180))])
+def test_tables_equal_projection_matrices(kernel, shape):
+    nx, ny, my, ar = shape
+    op = O.OracleOperator(cfg(nx=nx, ny=ny, my=my, n_omega=30, angle_range=ar, cor=0.7), (4.05,) * 3 + (90,) * 3,
+                          "cubic", AL3, 0.35, orientations(2, 1), np.deg2rad(4.0), radon_kernel=kernel,
+                          orthonormalize=False)
+    idx = np.array([0, 3, 7, 11, 14, 22, 29])
+    tab = radon_tables(_geo(op, kernel), idx, nx, ny, my, op.scale)
+    npix = nx * ny
+    for i, r in enumerate(idx):
+        P = op.radon_matrix(r).toarray()                                   # (My, Npix) with scale
+        F = np.zeros_like(P)
+        lo, hi = tab["row_ptr"][i * my], tab["row_ptr"][(i + 1) * my]
+        rows = np.repeat(np.arange(my), np.diff(tab["row_ptr"][i * my:(i + 1) * my + 1]))
+        F[rows, tab["pix"][lo:hi]] = tab["w"][lo:hi]
+        A = max(abs(np.float32(op.X[r])), abs(np.float32(op.Y[r])))
+        # the forward table carries 1/A and the scale of the trapezoid; the oracle's K already holds 1/A
+        np.testing.assert_allclose(F, P, rtol=2e-6, atol=1e-7 * P.max())
+        B = np.zeros_like(P)
+        pt = tab["ptab"][i]
+        w0, w1 = pt[:, 2].view(np.float32), pt[:, 3].view(np.float32)
+        np.add.at(B, (pt[:, 0], np.arange(npix)), w0)
+        np.add.at(B, (pt[:, 1], np.arange(npix)), w1)
+        np.testing.assert_allclose(B, P / op.scale * np.pi / op.R, rtol=2e-6, atol=1e-7 * B.max())
+        tnx = -(-nx // 4)
+        yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        tile = (yy.ravel() // 4) * tnx + xx.ravel() // 4
+        base = tab["tbase"][i][tile]
+        assert np.all((pt[:, 0] - base >= 0) & (pt[:, 0] - base < 8) & (pt[:, 1] - base >= 0) & (pt[:, 1] - base < 8))
