@@ -47,9 +47,10 @@ SIGNATURES = {
     "dfm_mean_counted_partial": [P, P, I64, I64, P, P, P],
     "dfm_dot_partial": [P, P, I64, P, P, P],
     "dfm_scale": [P, F32, I64, P],
-    "dfm_dense_forward": [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P],
-    "dfm_dense_adjoint": [P, P, P, P, I32, I32, I32, I32, I32, I32, F32, I32, P],
-    "dfm_dense_split_pf": [P, P, P, I64, P],
+    "dfm_mfma_pack_fwd": [P, P, P, I32, I32, I32, I32, I32, I32, P],
+    "dfm_mfma_pack_adj": [P, P, P, I32, I32, I32, I32, I32, I32, P],
+    "dfm_mfma_forward": [P, I32, I32, P, P, P, I32, I32, I32, I32, I32, P, P, I32, I32, I32, P],
+    "dfm_mfma_adjoint": [P, P, P, P, I32, I32, I32, I32, I32, P, I32, F32, I32, P],
 }
 
 DFM_ERR_ARG, DFM_ERR_UNSUPPORTED, DFM_ERR_LAUNCH, DFM_ERR_MEMORY = -1, -2, -3, -4

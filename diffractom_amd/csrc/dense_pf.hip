@@ -1,0 +1,428 @@
+// Pole-figure contraction on the matrix cores (dense PF mode): BF16 operands, FP32 accumulation, optional
+// residual correction (hi/lo splitting of both operands).
+//
+// Forward, per rotation angle r:  data[r, y, j] (+)= sum_k sino[a, y, k] * PF[r, k, j]     (M = y, N = j, K = k)
+// Adjoint, per rotation angle r:  sino[a, y, k]    = alpha * sum_j data[r, y, j] * PF[r, k, j]   (M = y, N = k, K = j)
+// with a the position of r in the angle list (single_phase_forward_operator.py:2063-2127, CLBlast
+// gemmStridedBatched in the reference's dense mode).
+//
+// The PF matrix is stored twice, each copy already in the operand order of v_mfma_f32_16x16x32_bf16, so every
+// fragment is one coalesced 1 KiB load (lane l holds 8 consecutive bf16 of the contraction index):
+//   forward fragment (r, nt, kc): lane l holds PF[k = 32 kc + 8 (l / 16) + i][j = 16 nt + l % 16], i = 0..7
+//   adjoint fragment (r, kt, jc): lane l holds PF[k = 16 kt + l % 16][j = 32 jc + 8 (l / 16) + i]
+// at element offset ((r * n_tiles + tile) * n_chunks + chunk) * 512 + 8 l.  Entries beyond K or C*P are zero.
+// The y-side operand (sinogram or data, FP32 in HBM) is split into bf16 hi (+ lo) while it is staged in LDS. With the
+// residual the product is hi*hi + hi*lo + lo*hi (the lo*lo term, 2^-16 relative, is dropped).
+#include "dfm_common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MT_MAX = 8;   // 16-row y tiles held by one wave
+constexpr int N_XCD = 8;
+
+__device__ __forceinline__ f32x4 mma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Logical workgroup index such that indices [x * nb / 8, (x + 1) * nb / 8) run on XCD x (workgroup i is dispatched
+// to XCD i % 8); nb = gridDim.x is a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, i = blockIdx.x;
+  return (i % N_XCD) * (nb / N_XCD) + i / N_XCD;
+}
+
+// row[j0 .. j0 + 7], zero at and beyond n; vec: row is 16-byte aligned.
+__device__ __forceinline__ void load8(const float* __restrict__ row, int j0, int n, bool vec, float v[8]) {
+  if (vec && j0 + 8 <= n) {
+    const float4 p = *reinterpret_cast<const float4*>(row + j0);
+    const float4 q = *reinterpret_cast<const float4*>(row + j0 + 4);
+    v[0] = p.x; v[1] = p.y; v[2] = p.z; v[3] = p.w;
+    v[4] = q.x; v[5] = q.y; v[6] = q.z; v[7] = q.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = (j0 + i < n) ? row[j0 + i] : 0.f;
+  }
+}
+
+// v ~= hi + lo with hi = bf16(v) (round to nearest even) and lo = bf16(v - hi); v - hi is exact in FP32.
+template <bool RES>
+__device__ __forceinline__ void split8(const float v[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)v[i];
+    hi[i] = h;
+    if (RES) lo[i] = (__bf16)(v[i] - (float)h);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------- packing
+__global__ void pack_fwd_kernel(const float* __restrict__ dense, bf16x8* __restrict__ hi, bf16x8* __restrict__ lo,
+                                int R, int km, int kb, int cp, int n_nt, int n_kc) {
+  const int64_t total = (int64_t)R * n_nt * n_kc * 64;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const int64_t f = idx >> 6;
+    const int kc = (int)(f % n_kc);
+    const int64_t t = f / n_kc;
+    const int nt = (int)(t % n_nt);
+    const int r = (int)(t / n_nt);
+    const int j = nt * 16 + (lane & 15);
+    const int k0 = kc * 32 + (lane >> 4) * 8;
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      v[i] = (k0 + i < kb && j < cp) ? dense[((int64_t)r * km + k0 + i) * cp + j] : 0.f;
+    bf16x8 h, l;
+    split8<true>(v, h, l);
+    hi[idx] = h;
+    if (lo) lo[idx] = l;
+  }
+}
+
+__global__ void pack_adj_kernel(const float* __restrict__ dense, bf16x8* __restrict__ hi, bf16x8* __restrict__ lo,
+                                int R, int km, int kb, int cp, int n_kt, int n_jc) {
+  const int64_t total = (int64_t)R * n_kt * n_jc * 64;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const int64_t f = idx >> 6;
+    const int jc = (int)(f % n_jc);
+    const int64_t t = f / n_jc;
+    const int kt = (int)(t % n_kt);
+    const int r = (int)(t / n_kt);
+    const int k = kt * 16 + (lane & 15);
+    const int j0 = jc * 32 + (lane >> 4) * 8;
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      v[i] = (k < kb && j0 + i < cp) ? dense[((int64_t)r * km + k) * cp + j0 + i] : 0.f;
+    bf16x8 h, l;
+    split8<true>(v, h, l);
+    hi[idx] = h;
+    if (lo) lo[idx] = l;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------- products
+// One kernel for both directions. A workgroup (4 waves) owns (angle a, k split ks, y group yg) and 4 NT tiles of the
+// output columns (NT per wave). Per contraction chunk (32 orientations forward, 32 segments adjoint) the y-side
+// operand (mt_n 16-row tiles x 32, FP32 in HBM) is loaded once by the workgroup, split into bf16 hi (+ lo) and
+// stored in LDS in fragment order (double-buffered); each wave reads its A fragments from LDS (one ds_read_b128 per
+// lane) and streams its PF fragments from HBM one chunk ahead.
+struct MfmaArgs {
+  const float* src;        // forward: sino (n_ang, my, sks); adjoint: data (R, my, cp)
+  const bf16x8* hi;
+  const bf16x8* lo;
+  const int* omegas;
+  float* out;              // forward: data (R, my, cp) or the partial sums (ksplit, n_ang, my, cp); adjoint: sino
+  int n_ang, my, cp, sks, kb;
+  int n_tiles, n_chunks;   // forward: j tiles of 16, k chunks of 32; adjoint: k tiles of 16, j chunks of 32
+  int mt_n, n_yg, n_tg;    // y tiles per group, y groups, groups of 4 NT output tiles
+  int ksplit, ch_per;      // chunks split over ksplit workgroups (forward)
+  int direct, accumulate;
+  float alpha;
+};
+
+template <int NT, bool RES, bool FWD>
+__global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
+  __shared__ bf16x8 lds_hi[2][MT_MAX * 64];
+  __shared__ bf16x8 lds_lo[RES ? 2 : 1][RES ? MT_MAX * 64 : 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  int64_t w = xcd_block();
+  const int tg = (int)(w % p.n_tg);
+  w /= p.n_tg;
+  const int yg = (int)(w % p.n_yg);
+  w /= p.n_yg;
+  const int ks = (int)(w % p.ksplit);
+  w /= p.ksplit;
+  if (w >= p.n_ang) return;                            // uniform over the workgroup
+  const int a = (int)w;
+  const int r = p.omegas ? p.omegas[a] : a;
+  const int t0 = (tg * 4 + (tid >> 6)) * NT;           // this wave's first output tile
+  const int y0 = yg * p.mt_n * 16;
+  const int c0 = ks * p.ch_per;
+  const int c1 = min(p.n_chunks, c0 + p.ch_per);
+  if (c0 >= c1) return;                                // uniform
+
+  // y-side operand slots of this thread: s = tid, tid + 256 < mt_n * 64 (fragment mt = s / 64, lane s % 64)
+  const int n_slots = p.mt_n * 64;
+  const float* srow[2];
+  bool sok[2];
+  int scol[2];
+  const int64_t src_rows = FWD ? (int64_t)a * p.my : (int64_t)r * p.my;
+  const int src_stride = FWD ? p.sks : p.cp;
+  const int src_n = FWD ? p.kb : p.cp;
+  const bool vec = (src_stride & 3) == 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int s = tid + 256 * i;
+    const int y = y0 + (s >> 6) * 16 + (s & 15);
+    sok[i] = s < n_slots && y < p.my;
+    srow[i] = p.src + (src_rows + (sok[i] ? y : 0)) * src_stride;
+    scol[i] = ((s & 63) >> 4) * 8;
+  }
+  // PF fragment streams of this wave's tiles
+  const bf16x8* bh_p[NT];
+  const bf16x8* bl_p[NT];
+  bool tok[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    tok[t] = t0 + t < p.n_tiles;
+    const int64_t base = (((int64_t)r * p.n_tiles + (tok[t] ? t0 + t : 0)) * p.n_chunks) * 64 + lane;
+    bh_p[t] = p.hi + base;
+    bl_p[t] = RES ? p.lo + base : p.hi;
+  }
+  const bf16x8 zero8 = {};
+
+  float raw[2][8];
+  auto load_raw = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (sok[i]) {
+        load8(srow[i], c * 32 + scol[i], src_n, vec, raw[i]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) raw[i][e] = 0.f;
+      }
+    }
+  };
+  auto store_lds = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int s = tid + 256 * i;
+      if (s < n_slots) {
+        bf16x8 h, l;
+        split8<RES>(raw[i], h, l);
+        lds_hi[buf][s] = h;
+        if (RES) lds_lo[buf][s] = l;
+      }
+    }
+  };
+  bf16x8 bh[NT], bl[NT], nh[NT], nl[NT];
+  auto load_b = [&](int c, bf16x8 (&h)[NT], bf16x8 (&l)[NT]) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      h[t] = tok[t] ? bh_p[t][(int64_t)c * 64] : zero8;
+      l[t] = (RES && tok[t]) ? bl_p[t][(int64_t)c * 64] : zero8;
+    }
+  };
+
+  f32x4 acc[MT_MAX][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT_MAX; ++mt)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  load_raw(c0);
+  load_b(c0, bh, bl);
+  store_lds(0);
+  __syncthreads();
+  int cur = 0;
+  for (int c = c0; c < c1; ++c) {
+    const bool more = c + 1 < c1;
+    if (more) {
+      load_raw(c + 1);
+      load_b(c + 1, nh, nl);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT_MAX; ++mt) {
+      if (mt < p.mt_n) {
+        const bf16x8 ah = lds_hi[cur][mt * 64 + lane];
+        const bf16x8 al = RES ? lds_lo[RES ? cur : 0][RES ? mt * 64 + lane : 0] : zero8;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          acc[mt][t] = mma(ah, bh[t], acc[mt][t]);
+          if (RES) {
+            acc[mt][t] = mma(ah, bl[t], acc[mt][t]);
+            acc[mt][t] = mma(al, bh[t], acc[mt][t]);
+          }
+        }
+      }
+    }
+    if (more) store_lds(cur ^ 1);
+    __syncthreads();
+    if (more) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bh[t] = nh[t];
+        bl[t] = nl[t];
+      }
+    }
+    cur ^= 1;
+  }
+
+  // D fragment: lane l holds rows 4 (l / 16) + i, column l % 16 of each 16 x 16 tile
+  const int row = lane & 15, grp = lane >> 4;
+#pragma unroll
+  for (int mt = 0; mt < MT_MAX; ++mt) {
+    if (mt >= p.mt_n) continue;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (!tok[t]) continue;
+      const int col = (t0 + t) * 16 + row;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int y = y0 + mt * 16 + grp * 4 + i;
+        if (y >= p.my) continue;
+        const float v = acc[mt][t][i];
+        if (FWD) {
+          if (col >= p.cp) continue;
+          if (p.direct) {
+            float* d = p.out + ((int64_t)r * p.my + y) * p.cp + col;
+            *d = p.accumulate ? *d + v : v;
+          } else {
+            p.out[(((int64_t)ks * p.n_ang + a) * p.my + y) * p.cp + col] = v;
+          }
+        } else {
+          if (col < p.sks) p.out[((int64_t)a * p.my + y) * p.sks + col] = p.alpha * v;
+        }
+      }
+    }
+  }
+}
+
+// Sum of the k-split partial results into the data rows of the angles (fixed order: deterministic).
+__global__ void splitk_reduce_kernel(const float* __restrict__ part, float* __restrict__ data,
+                                     const int* __restrict__ omegas, int n_ang, int64_t row_len, int ksplit,
+                                     int accumulate) {
+  const int64_t total = (int64_t)n_ang * row_len;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int ks = 0; ks < ksplit; ++ks) s += part[(int64_t)ks * total + i];
+    const int a = (int)(i / row_len);
+    const int64_t rem = i - (int64_t)a * row_len;
+    const int r = omegas ? omegas[a] : a;
+    float* d = data + (int64_t)r * row_len + rem;
+    *d = accumulate ? *d + s : s;
+  }
+}
+
+template <bool FWD>
+void launch_mfma(const MfmaArgs& p, int variant, bool res, int64_t n_wg, hipStream_t st) {
+  const dim3 grid((unsigned)((n_wg + N_XCD - 1) / N_XCD * N_XCD)), block(256);
+  if (variant == 4) {
+    if (res) hipLaunchKernelGGL((mfma_pf_kernel<4, true, FWD>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((mfma_pf_kernel<4, false, FWD>), grid, block, 0, st, p);
+  } else {
+    if (res) hipLaunchKernelGGL((mfma_pf_kernel<2, true, FWD>), grid, block, 0, st, p);
+    else hipLaunchKernelGGL((mfma_pf_kernel<2, false, FWD>), grid, block, 0, st, p);
+  }
+}
+
+// y tiling shared by both directions: n_yg groups of mt_n 16-row tiles (mt_n <= MT_MAX).
+void y_tiling(int my, int& mt_n, int& n_yg) {
+  const int tiles = (my + 15) / 16;
+  n_yg = (tiles + MT_MAX - 1) / MT_MAX;
+  mt_n = (tiles + n_yg - 1) / n_yg;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfm_mfma_pack_fwd(const float* dense, void* hi, void* lo, int R, int km, int kb, int cp, int n_nt, int n_kc,
+                      dfm_stream_t stream) {
+  DFM_CHECK_ARG(dense && hi, "dfm_mfma_pack_fwd: null pointer");
+  DFM_CHECK_ARG(R > 0 && kb > 0 && km >= kb && cp > 0, "dfm_mfma_pack_fwd: bad sizes");
+  DFM_CHECK_ARG(n_nt * 16 >= cp && n_kc * 32 >= kb, "dfm_mfma_pack_fwd: %d x %d fragments do not cover %d x %d",
+                n_kc, n_nt, kb, cp);
+  const int64_t total = (int64_t)R * n_nt * n_kc * 64;
+  hipLaunchKernelGGL(pack_fwd_kernel, dim3(dfm::grid_stride_blocks(total, 256, 1u << 20)), dim3(256), 0,
+                     (hipStream_t)stream, dense, (bf16x8*)hi, (bf16x8*)lo, R, km, kb, cp, n_nt, n_kc);
+  DFM_LAUNCH_CHECK("dfm_mfma_pack_fwd");
+  return DFM_OK;
+}
+
+int dfm_mfma_pack_adj(const float* dense, void* hi, void* lo, int R, int km, int kb, int cp, int n_kt, int n_jc,
+                      dfm_stream_t stream) {
+  DFM_CHECK_ARG(dense && hi, "dfm_mfma_pack_adj: null pointer");
+  DFM_CHECK_ARG(R > 0 && kb > 0 && km >= kb && cp > 0, "dfm_mfma_pack_adj: bad sizes");
+  DFM_CHECK_ARG(n_kt * 16 >= kb && n_jc * 32 >= cp, "dfm_mfma_pack_adj: %d x %d fragments do not cover %d x %d",
+                n_kt, n_jc, kb, cp);
+  const int64_t total = (int64_t)R * n_kt * n_jc * 64;
+  hipLaunchKernelGGL(pack_adj_kernel, dim3(dfm::grid_stride_blocks(total, 256, 1u << 20)), dim3(256), 0,
+                     (hipStream_t)stream, dense, (bf16x8*)hi, (bf16x8*)lo, R, km, kb, cp, n_kt, n_jc);
+  DFM_LAUNCH_CHECK("dfm_mfma_pack_adj");
+  return DFM_OK;
+}
+
+int dfm_mfma_forward(const float* sino, int sks, int kb, const void* hi, const void* lo, const int* omegas,
+                     int n_ang, int my, int cp, int n_nt, int n_kc, float* data, float* part, int ksplit,
+                     int accumulate, int variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && hi && data, "dfm_mfma_forward: null pointer");
+  DFM_CHECK_ARG(n_ang > 0 && my > 0 && cp > 0 && kb > 0 && sks >= kb && (sks & 3) == 0,
+                "dfm_mfma_forward: bad sizes (n_ang %d, My %d, CP %d, Kb %d, stride %d)", n_ang, my, cp, kb, sks);
+  DFM_CHECK_ARG(n_nt * 16 >= cp && n_kc * 32 >= kb, "dfm_mfma_forward: fragments do not cover the matrix");
+  DFM_CHECK_ARG(ksplit >= 1 && ksplit <= n_kc, "dfm_mfma_forward: k split %d not in 1 .. %d", ksplit, n_kc);
+  DFM_CHECK_ARG(ksplit == 1 || part, "dfm_mfma_forward: a k split needs the partial-sum workspace");
+  DFM_CHECK_ARG(variant == 2 || variant == 4, "dfm_mfma_forward: variant (j tiles per wave) must be 2 or 4");
+  MfmaArgs p{};
+  p.src = sino;
+  p.hi = (const bf16x8*)hi;
+  p.lo = (const bf16x8*)lo;
+  p.omegas = omegas;
+  p.n_ang = n_ang;
+  p.my = my;
+  p.cp = cp;
+  p.sks = sks;
+  p.kb = kb;
+  p.n_tiles = n_nt;
+  p.n_chunks = n_kc;
+  y_tiling(my, p.mt_n, p.n_yg);
+  p.n_tg = (n_nt + 4 * variant - 1) / (4 * variant);
+  p.ch_per = (n_kc + ksplit - 1) / ksplit;
+  p.ksplit = (n_kc + p.ch_per - 1) / p.ch_per;     // every split holds at least one chunk
+  p.direct = p.ksplit == 1;
+  p.accumulate = accumulate;
+  p.out = p.direct ? data : part;
+  p.alpha = 1.f;
+  hipStream_t st = (hipStream_t)stream;
+  launch_mfma<true>(p, variant, lo != nullptr, (int64_t)n_ang * p.ksplit * p.n_yg * p.n_tg, st);
+  DFM_LAUNCH_CHECK("dfm_mfma_forward");
+  if (!p.direct) {
+    const int64_t total = (int64_t)n_ang * my * cp;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(dfm::grid_stride_blocks(total, 256, 1u << 16)), dim3(256), 0, st,
+                       part, data, omegas, n_ang, (int64_t)my * cp, p.ksplit, accumulate);
+    DFM_LAUNCH_CHECK("dfm_mfma_forward (reduce)");
+  }
+  return DFM_OK;
+}
+
+int dfm_mfma_adjoint(const float* data, const void* hi, const void* lo, const int* omegas, int n_ang, int my, int cp,
+                     int n_kt, int n_jc, float* sino, int sks, float alpha, int variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(data && hi && sino, "dfm_mfma_adjoint: null pointer");
+  DFM_CHECK_ARG(n_ang > 0 && my > 0 && cp > 0 && sks > 0 && (sks & 3) == 0 && n_kt * 16 >= sks,
+                "dfm_mfma_adjoint: bad sizes (n_ang %d, My %d, CP %d, stride %d, k tiles %d)", n_ang, my, cp, sks,
+                n_kt);
+  DFM_CHECK_ARG(n_jc * 32 >= cp, "dfm_mfma_adjoint: fragments do not cover the matrix");
+  DFM_CHECK_ARG(variant == 2 || variant == 4, "dfm_mfma_adjoint: variant (k tiles per wave) must be 2 or 4");
+  MfmaArgs p{};
+  p.src = data;
+  p.hi = (const bf16x8*)hi;
+  p.lo = (const bf16x8*)lo;
+  p.omegas = omegas;
+  p.out = sino;
+  p.n_ang = n_ang;
+  p.my = my;
+  p.cp = cp;
+  p.sks = sks;
+  p.kb = sks;
+  p.n_tiles = n_kt;
+  p.n_chunks = n_jc;
+  y_tiling(my, p.mt_n, p.n_yg);
+  p.n_tg = (n_kt + 4 * variant - 1) / (4 * variant);
+  p.ksplit = 1;
+  p.ch_per = n_jc;
+  p.alpha = alpha;
+  launch_mfma<false>(p, variant, lo != nullptr, (int64_t)n_ang * p.n_yg * p.n_tg, (hipStream_t)stream);
+  DFM_LAUNCH_CHECK("dfm_mfma_adjoint");
+  return DFM_OK;
+}
+
+}  // extern "C"

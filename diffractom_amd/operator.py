@@ -133,7 +133,7 @@ class SinglePhaseForwardOperator:
                  radon_kernel: str | None = None, fft_max_gb: float = 1.5, reserve_coefficient_arrays: int = 3,
                  pf_cutoff_sigma: float | None = 4.0, quadrature="auto", quadrature_tol: float = DEFAULT_TOL,
                  intensity_model=None, sparse_batch_max: int = 1024, rings=None, ring_two_theta=None,
-                 reflection_two_theta: bool = False, window_parallax=None, **kwargs):
+                 reflection_two_theta: bool = False, window_parallax=None, pf_precision: str = "fp32", **kwargs):
         t0 = time.perf_counter()
         self.device = _resolve_device(ctx)
         self.queue = queue                      # a torch stream, or None for the current stream at call time
@@ -171,6 +171,15 @@ class SinglePhaseForwardOperator:
         self.radon_kernel = radon_kernel
         if pf_mode not in ("dense", "sparse", "generated", "auto"):
             raise ValueError(f"pf_mode must be 'dense', 'sparse', 'generated' or 'auto', not {pf_mode!r}")
+        # PF product arithmetic: "fp32" (CSR or dense FP32), "bf16" (dense PF on the matrix cores, bf16 operands,
+        # FP32 accumulation) or "bf16_residual" (the same plus the hi/lo residual correction, ~2^-16 relative)
+        if pf_precision not in ("fp32", "bf16", "bf16_residual"):
+            raise ValueError(f"pf_precision must be 'fp32', 'bf16' or 'bf16_residual', not {pf_precision!r}")
+        if pf_precision != "fp32" and pf_mode not in ("dense", "auto"):
+            raise ValueError(f"pf_precision {pf_precision!r} runs the dense PF product on the matrix cores; "
+                             f"pf_mode must be 'dense' or 'auto', not {pf_mode!r}")
+        self.pf_precision = pf_precision
+        self.mfma_variant = {"fwd": 2, "adj": 2}
 
         # ---- geometry (single_phase_forward_operator.py:376-441)
         c = self.cfg
@@ -208,6 +217,7 @@ class SinglePhaseForwardOperator:
         self.pf_mode = pf_mode
         self.sparse_batches = None
         self.dense_batches = None
+        self.mfma_batches = None
         self.pf_storage = None
         self.sparse_nnz = 0
         self.sparse_fill = 0.0
@@ -410,7 +420,8 @@ class SinglePhaseForwardOperator:
 
     def _build_pf(self, mode, max_fill, max_gb):
         R, CP = self.N_Omega, self.N_seg
-        if mode == "dense":
+        if mode == "dense" or self.pf_precision != "fp32":
+            self.pf_mode = "dense"
             self._store_dense()
             return
         # fill of the first batch decides "auto" (single_phase_forward_operator.py:1240-1265)
@@ -447,10 +458,38 @@ class SinglePhaseForwardOperator:
         self.sparse_fill = nnz / (R * self.K * CP)
 
     def _store_dense(self):
-        """Dense PF batches (R, kstride_b, C*P) kept on the device (the dense-PF mode)."""
-        self.dense_batches = [self._eval_dense_batch(b).reshape(self.N_Omega, b["kstride"], self.N_seg)
-                              for b in self.batches]
-        self.pf_storage = "dense"
+        """Dense PF batches kept on the device (the dense-PF mode): FP32 (R, kstride_b, C*P), or bf16 fragments in
+        the matrix-core operand order, one copy per product direction (hi, and lo for the residual)."""
+        if self.pf_precision == "fp32":
+            self.dense_batches = [self._eval_dense_batch(b).reshape(self.N_Omega, b["kstride"], self.N_seg)
+                                  for b in self.batches]
+            self.pf_storage = "dense"
+            return
+        R, CP = self.N_Omega, self.N_seg
+        res = self.pf_precision == "bf16_residual"
+        st = self._stream()
+        self.mfma_batches = []
+        dense = None
+        for b in self.batches:
+            if dense is None or dense.shape[1] != b["kstride"]:
+                dense = None
+                dense = self._eval_dense_batch(b)
+            else:
+                self._eval_dense_batch(b, out=dense)
+            kb, ks = b["K_batch"], b["kstride"]
+            mb = {"n_nt": -(-CP // 16), "n_kc": -(-kb // 32), "n_kt": -(-ks // 16), "n_jc": -(-CP // 32)}
+            nf = R * mb["n_nt"] * mb["n_kc"] * 512
+            na = R * mb["n_kt"] * mb["n_jc"] * 512
+            for key, n in (("fwd", nf), ("adj", na)):
+                mb[key + "_hi"] = torch.empty(n, dtype=torch.bfloat16, device=self.device)
+                mb[key + "_lo"] = torch.empty(n, dtype=torch.bfloat16, device=self.device) if res else None
+            _lib.call("dfm_mfma_pack_fwd", _lib.ptr(dense), _lib.ptr(mb["fwd_hi"]), _lib.ptr(mb["fwd_lo"]), R, ks,
+                      kb, CP, mb["n_nt"], mb["n_kc"], st)
+            _lib.call("dfm_mfma_pack_adj", _lib.ptr(dense), _lib.ptr(mb["adj_hi"]), _lib.ptr(mb["adj_lo"]), R, ks,
+                      kb, CP, mb["n_kt"], mb["n_jc"], st)
+            self.mfma_batches.append(mb)
+        del dense
+        self.pf_storage = "bf16 fragments" + (" + residual" if res else "")
 
     def _pf_batch(self, ib):
         """The CSR pair of batch ib: stored, or evaluated now (generated mode; valid until the next call)."""
@@ -480,11 +519,14 @@ class SinglePhaseForwardOperator:
             n += sum(v.numel() * v.element_size() for v in sb.values() if isinstance(v, torch.Tensor))
         for d in self.dense_batches or []:
             n += d.numel() * 4
+        for mb in self.mfma_batches or []:
+            n += sum(v.numel() * v.element_size() for v in mb.values() if isinstance(v, torch.Tensor))
         return n
 
     def free_memory(self):
         self.sparse_batches = None
         self.dense_batches = None
+        self.mfma_batches = None
         self._scratch = None
 
     def _scratch_buffers(self, n_angles):
@@ -502,10 +544,8 @@ class SinglePhaseForwardOperator:
     def set_omega_subsets(self, subsets):
         """Subsets of the rotation angles for direct_cl / adjoint_cl(subset=m) (single_phase_forward_operator.py:
         1541-1599): an int M (interleaved, angle i in subset i % M) or a list of angle-index arrays; returns the
-        sorted index arrays; the same subsets again is a no-op."""
-        if self.pf_mode not in ("sparse", "generated"):
-            raise NotImplementedError(f"omega subsets need pf_mode 'sparse' or 'generated' (this operator: "
-                                      f"{self.pf_mode!r})")
+        sorted index arrays; the same subsets again is a no-op. Unlike the reference, the dense PF mode supports
+        subsets too (its products take the subset's angle list)."""
         R = self.N_Omega
         if np.isscalar(subsets):
             M = int(subsets)
@@ -606,10 +646,32 @@ class SinglePhaseForwardOperator:
                   _lib.ptr(sb["val_a"]), _lib.ptr(sino), _lib.ptr(om), n_ang, self.My, self.N_seg, b["kstride"],
                   b["K_batch"], alpha, 0, self._stream())
 
+    def _mfma_split(self, n_ang, mb):
+        """k split of the matrix-core forward: enough waves for 8 per CU on 256 CUs, at most 16 partial sums."""
+        tiles = -(-self.My // 16)
+        n_yg = -(-tiles // 8)
+        waves = n_ang * n_yg * (-(-mb["n_nt"] // self.mfma_variant["fwd"]))
+        return int(max(1, min(16, mb["n_kc"], -(-2048 // waves))))
+
+    def _mfma_part(self, n):
+        p = getattr(self, "_mfma_part_buf", None)
+        if p is None or p.numel() < n:
+            p = torch.empty(n, dtype=torch.float32, device=self.device)
+            self._mfma_part_buf = p
+        return p
+
     def _dense_forward(self, ib, sino, data, subset, accumulate):
         """Dense PF path: data[r] (+)= sino[r] (My x Kb) @ PF[r] (Kb x CP), batched over the angles."""
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
+        if self.mfma_batches is not None:
+            mb = self.mfma_batches[ib]
+            ks = self._mfma_split(n_ang, mb)
+            part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
+            _lib.call("dfm_mfma_forward", _lib.ptr(sino), b["kstride"], b["K_batch"], _lib.ptr(mb["fwd_hi"]),
+                      _lib.ptr(mb["fwd_lo"]), _lib.ptr(om), n_ang, self.My, self.N_seg, mb["n_nt"], mb["n_kc"],
+                      _lib.ptr(data), _lib.ptr(part), ks, int(accumulate), self.mfma_variant["fwd"], self._stream())
+            return
         pf = self.dense_batches[ib]
         if om is not None:
             pf = pf.index_select(0, om.long())
@@ -624,6 +686,12 @@ class SinglePhaseForwardOperator:
     def _dense_adjoint(self, ib, data, sino, subset, alpha):
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
+        if self.mfma_batches is not None:
+            mb = self.mfma_batches[ib]
+            _lib.call("dfm_mfma_adjoint", _lib.ptr(data), _lib.ptr(mb["adj_hi"]), _lib.ptr(mb["adj_lo"]),
+                      _lib.ptr(om), n_ang, self.My, self.N_seg, mb["n_kt"], mb["n_jc"], _lib.ptr(sino),
+                      b["kstride"], float(alpha), self.mfma_variant["adj"], self._stream())
+            return
         pf = self.dense_batches[ib]
         d = data
         if om is not None:

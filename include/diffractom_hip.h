@@ -119,6 +119,34 @@ int dfm_spmm_adjoint(const float* data, const int32_t* row_ptr, const uint16_t* 
 /* data rows omegas[s] (each row_len values) <- 0 (single_phase_forward_operator.py:126-132 zero_rows). */
 int dfm_zero_rows(float* data, const int32_t* omegas, int32_t n_sub, int64_t row_len, dfm_stream_t stream);
 
+/* ---------------------------------------------------------------- dense PF products on the matrix cores
+ * Replace the dense mode's CLBlast gemmStridedBatched calls (single_phase_forward_operator.py:2063-2127; PF batch
+ * from _eval_pf_batch 1131-1166) with BF16 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation.
+ * PF is stored as bf16 "hi" fragments and, for the residual correction, bf16 "lo" fragments (lo = bf16(PF - hi));
+ * lo NULL: hi only. Forward fragments (r, nt, kc) hold PF[r, 32 kc + 8 (l / 16) + i, 16 nt + l % 16] in lane l,
+ * adjoint fragments (r, kt, jc) hold PF[r, 16 kt + l % 16, 32 jc + 8 (l / 16) + i]; element offset
+ * ((r * n_tiles + tile) * n_chunks + chunk) * 512 + 8 l + i; entries beyond kb or cp are 0.
+ * pack: from a dense FP32 batch pf[r, k, j] (R x km x cp, rows k < kb used). Each fragment array holds
+ * R * n_tiles * n_chunks * 512 bf16 (forward: n_nt = ceil(cp / 16), n_kc = ceil(kb / 32); adjoint:
+ * n_kt = ceil(kb / 16) (at least kstride / 16), n_jc = ceil(cp / 32)). */
+int dfm_mfma_pack_fwd(const float* pf, void* hi, void* lo, int32_t R, int32_t km, int32_t kb, int32_t cp,
+                      int32_t n_nt, int32_t n_kc, dfm_stream_t stream);
+int dfm_mfma_pack_adj(const float* pf, void* hi, void* lo, int32_t R, int32_t km, int32_t kb, int32_t cp,
+                      int32_t n_kt, int32_t n_jc, dfm_stream_t stream);
+/* forward: data[omegas[a], y, j] (+)= sum_k PF[omegas[a], k, j] sino[a, y, k] (sino row stride sks, columns < kb
+ * read). ksplit > 1 splits k over that many partial sums in part (ksplit * n_ang * my * cp floats), summed in a
+ * fixed order. variant: j tiles per wave (2 or 4). The y-side operand is split into bf16 hi (+ lo when lo != NULL:
+ * hi*hi + hi*lo + lo*hi). */
+int dfm_mfma_forward(const float* sino, int32_t sks, int32_t kb, const void* hi, const void* lo,
+                     const int32_t* omegas, int32_t n_ang, int32_t my, int32_t cp, int32_t n_nt, int32_t n_kc,
+                     float* data, float* part, int32_t ksplit, int32_t accumulate, int32_t variant,
+                     dfm_stream_t stream);
+/* adjoint: sino[a, y, k] = alpha sum_j PF[omegas[a], k, j] data[omegas[a], y, j] for k < sks (columns >= kb are
+ * 0). variant: k tiles per wave (2 or 4). */
+int dfm_mfma_adjoint(const float* data, const void* hi, const void* lo, const int32_t* omegas, int32_t n_ang,
+                     int32_t my, int32_t cp, int32_t n_kt, int32_t n_jc, float* sino, int32_t sks, float alpha,
+                     int32_t variant, dfm_stream_t stream);
+
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
  * Dual update on the rows of a subset (spdhg_poisson.py:69-91 dual_update_sub), float64 arithmetic:

@@ -1,0 +1,113 @@
+"""Dense PF products on the matrix cores (pf_precision "bf16" / "bf16_residual") vs the FP64 oracle.
+
+Tolerances (relative to the largest entry of the oracle's result, and relative L2):
+- bf16 (8-bit significand operands, FP32 accumulation): each product carries the rounding of both operands (about
+  2.3e-3 standard deviation), an output summing n terms about 2.3e-3 / sqrt(n): L2 1e-3 where outputs sum many
+  terms (K = 300, sigma 6 deg), 3e-3 on the tiny problems (a handful of terms per output), max 1e-2;
+- bf16_residual (hi/lo operands, hi*hi + hi*lo + lo*hi): max 3e-5, L2 1e-5 (the FP32 path's bar is 2e-5 max).
+"""
+import numpy as np
+import pytest
+
+from pairs import build, cfg, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"bf16": (1e-2, 3e-3, 3e-3), "bf16_residual": (3e-5, 1e-5, 1e-5)}   # max, L2, adjointness
+
+
+def _inputs(op, seed=1):
+    rng = np.random.default_rng(seed)
+    x = (rng.random(op.coeff_shape) * op.support_mask()[None]).astype(np.float32)
+    y = rng.random(op.data_shape).astype(np.float32)
+    return x, y
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16_residual"])
+@pytest.mark.parametrize("shape", ["small", "split_k", "batches", "tall"])
+def test_forward_adjoint_vs_oracle(precision, shape):
+    print(f"case precision={precision} shape={shape}", flush=True)
+    if shape == "small":            # one k chunk: direct write, no split
+        op, orc, _ = build(cfg(), K=12, pf_mode="dense", pf_precision=precision)
+    elif shape == "split_k":        # K = 300 in one batch: the k split and its reduction
+        op, orc, _ = build(cfg(n_omega=12), K=300, sigma_deg=6.0, pf_mode="dense", pf_precision=precision)
+    elif shape == "batches":        # 3 orientation batches, accumulated forward; C*P = 72 (not a tile multiple)
+        c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
+        op, orc, _ = build(c, K=70, sigma_deg=4.0, pf_mode="dense", pf_precision=precision, max_gb=3e-4)
+        assert len(op.batches) > 1
+    else:                           # My = 140: 9 y tiles in 2 groups
+        c = cfg(nx=20, ny=20, my=140, n_omega=8, n_eta=20)
+        op, orc, _ = build(c, K=40, pf_mode="dense", pf_precision=precision)
+    assert op.mfma_batches is not None
+    tmax, tl2, tadj = TOL[precision]
+    if precision == "bf16" and shape == "split_k":
+        tl2 = 1e-3
+    x, y = _inputs(op)
+    d, d_ref = op.direct(x), orc.direct(x)
+    a, a_ref = op.adjoint(y), orc.adjoint(y)
+    print(f"  forward max {rel_max(d, d_ref):.2e} L2 {rel_l2(d, d_ref):.2e}; "
+          f"adjoint max {rel_max(a, a_ref):.2e} L2 {rel_l2(a, a_ref):.2e}", flush=True)
+    assert rel_max(d, d_ref) < tmax and rel_l2(d, d_ref) < tl2
+    assert rel_max(a, a_ref) < tmax and rel_l2(a, a_ref) < tl2
+    lhs = float(np.dot(x.ravel().astype(np.float64), a.ravel()))
+    rhs = float(np.dot(d.ravel().astype(np.float64), y.ravel()))
+    assert abs(lhs / rhs - 1) < tadj
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16_residual"])
+def test_subsets(precision):
+    """Subset rows equal the full rows, the other rows are untouched, subset adjoints of a partition sum to the
+    full adjoint (tests/test_subsets.py:49-86), on the matrix-core path."""
+    import torch
+    print(f"case precision={precision}", flush=True)
+    c = cfg(nx=20, ny=20, my=21, n_omega=30, angle_range=(0, 360), cor=0.5)
+    op, _, _ = build(c, K=90, pf_mode="dense", pf_precision=precision, max_gb=4e-4, oracle=False)
+    assert len(op.batches) > 1
+    subs = op.set_omega_subsets([np.arange(0, 7), np.arange(7, 20), np.arange(20, op.N_Omega)])
+    x, y = _inputs(op, 3)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.from_numpy(y).cuda()
+    full = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
+    op.direct_cl(xt, full)
+    adj_full = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
+    op.adjoint_cl(yt, adj_full)
+    acc = torch.zeros_like(adj_full)
+    tmp = torch.empty_like(adj_full)
+    for m, s in enumerate(subs):
+        d = torch.full(op.data_shape, -7.0, dtype=torch.float32, device="cuda")
+        op.direct_cl(xt, d, subset=m)
+        rows = torch.from_numpy(s).cuda()
+        assert torch.equal(d[rows], full[rows])
+        mask = torch.ones(op.N_Omega, dtype=torch.bool, device="cuda")
+        mask[rows] = False
+        assert bool((d[mask] == -7.0).all())
+        op.adjoint_cl(yt, tmp, subset=m)
+        acc += tmp
+    assert rel_max(acc.cpu().numpy(), adj_full.cpu().numpy()) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["bf16", "bf16_residual"])
+def test_spdhg_vs_oracle(precision):
+    """SPDHG epochs on the matrix-core path vs the FP64 oracle (same subset order from the same seed): with the
+    residual correction the iterate is within 1e-4 (relative L2) after 6 epochs, as the FP32 path; plain bf16
+    within 3e-3, objectives within 1e-3."""
+    from diffractom_amd import SPDHGPoisson
+    from oracle.model import OracleSPDHG
+    print(f"case spdhg {precision}", flush=True)
+    op, orc, _ = build(cfg(nx=16, ny=16, my=17, n_omega=24, cor=0.0), K=40, sigma_deg=4.0, pf_mode="dense",
+                       pf_precision=precision)
+    rng = np.random.default_rng(7)
+    xt = (rng.random(op.coeff_shape) * (rng.random(op.coeff_shape) < 0.2) * op.support_mask()[None]
+          ).astype(np.float32)
+    ax = orc.direct(xt)
+    b = rng.poisson(ax / ax.mean() * 50).astype(np.float32)
+    sp = SPDHGPoisson(op, subsets=4, seed=1)
+    x = sp.run(np.zeros(op.coeff_shape, np.float32), b, 6, diagnostics_interval=3)
+    ref = OracleSPDHG(orc, subsets=4, seed=1)
+    x_ref = ref.run(np.zeros(op.coeff_shape), b, 6, diagnostics_interval=3)
+    err = rel_l2(x, x_ref)
+    f = [s["f"] for s in sp.iter_stats if s["f"] is not None]
+    f_ref = [s["f"] for s in ref.iter_stats if s["f"] is not None]
+    print(f"  x_6 rel L2 {err:.2e}; f {f} vs {f_ref}", flush=True)
+    assert err < (1e-4 if precision == "bf16_residual" else 3e-3)
+    np.testing.assert_allclose(f, f_ref, rtol=1e-5 if precision == "bf16_residual" else 1e-3)
