@@ -515,16 +515,82 @@ def dual_prox(y, Ax, s, b, w, g, bg, eps):
     return np.where((w != 0) & (s != 0), yn, 0.0)
 
 
+TV_SAFETY = 0.99          # spdhg_poisson.py:277: the TV dual step is this fraction of its bound
+
+
+def _tv_free(T):
+    """Free forward differences (right, down) of (K, Ny, Nx) images: both ends have T > 0 (spdhg_poisson.py:110-113)."""
+    vr = np.zeros(T.shape, bool)
+    vd = np.zeros(T.shape, bool)
+    vr[:, :, :-1] = (T[:, :, :-1] > 0) & (T[:, :, 1:] > 0)
+    vd[:, :-1, :] = (T[:, :-1, :] > 0) & (T[:, 1:, :] > 0)
+    return vr, vd
+
+
+def _fwd_diff(x):
+    gx = np.zeros_like(x)
+    gy = np.zeros_like(x)
+    gx[:, :, :-1] = x[:, :, 1:] - x[:, :, :-1]
+    gy[:, :-1, :] = x[:, 1:, :] - x[:, :-1, :]
+    return gx, gy
+
+
+def tv_dual(x, T, qx, qy, lam, ps):
+    """The TV block's dual step (spdhg_poisson.py:110-129 tv_dual_core, 146-165 tv_dual_dense): per pixel
+    s = ps / (4 max over its free differences of (T_j + T_j')), q' = proj_{|q| <= lam}(q + s D x) on the free
+    differences (0 on the others, and where the pixel has none). Pinned by test_oracle_tv.py (an independent
+    Chambolle-Pock on the explicit matrix reaches the same objective)."""
+    vr, vd = _tv_free(T)
+    Tr = np.zeros_like(T)
+    Td = np.zeros_like(T)
+    Tr[:, :, :-1] = T[:, :, 1:]
+    Td[:, :-1, :] = T[:, 1:, :]
+    m = np.zeros_like(T)
+    m = np.where(vr, np.maximum(m, T + Tr), m)
+    m = np.where(vd, np.maximum(m, T + Td), m)
+    gx, gy = _fwd_diff(x)
+    s = np.where(m > 0, ps / (4.0 * np.where(m > 0, m, 1.0)), 0.0)
+    ax = np.where((m > 0) & vr, qx + s * gx, 0.0)
+    ay = np.where((m > 0) & vd, qy + s * gy, 0.0)
+    nrm = np.sqrt(ax * ax + ay * ay)
+    c = np.where(nrm > lam, lam / np.where(nrm > 0, nrm, 1.0), 1.0)
+    return ax * c, ay * c
+
+
+def tv_adjoint(dqx, dqy):
+    """D^T dq (spdhg_poisson.py:168-179 tv_adjoint_dense): -dqx - dqy + dqx(left) + dqy(up)."""
+    v = -dqx - dqy
+    v[:, :, 1:] += dqx[:, :, :-1]
+    v[:, 1:, :] += dqy[:, :-1, :]
+    return v
+
+
+def tv_value(x, T):
+    """sum_k TV(x_k) over the free differences (spdhg_poisson.py:131-137, 181-193)."""
+    vr, vd = _tv_free(T)
+    gx, gy = _fwd_diff(x)
+    return float(np.sqrt(np.where(vr, gx, 0.0) ** 2 + np.where(vd, gy, 0.0) ** 2).sum())
+
+
 class OracleSPDHG:
-    """SPDHGPoisson (no TV, dense layout, every array resident) in float64 (spdhg_poisson.py:298-751).
+    """SPDHGPoisson (dense layout, every array resident) in float64 (spdhg_poisson.py:298-751), with the total
+    variation block (tv > 0).
 
     The control flow, the random subset order (numpy default_rng(seed).permutation per epoch, the first subset of an
-    epoch never the last of the previous one) and the step sizes follow the reference; the arithmetic is float64."""
+    epoch never the last of the previous one; with tv the TV block inserted at rng.choice places), the step sizes
+    and the block probabilities follow the reference; the arithmetic is float64."""
 
-    def __init__(self, op, subsets=8, lam=0.0, prox_kind="nonneg", support="fov", eps_counts=1e-3, rho="auto", seed=0):
+    def __init__(self, op, subsets=8, lam=0.0, prox_kind="nonneg", support="fov", eps_counts=1e-3, rho="auto", seed=0,
+                 tv=0.0, tv_prob=None):
         self.op = op
         self.subsets = op.set_omega_subsets(subsets)
         self.M = len(self.subsets)
+        # spdhg_poisson.py:339-346: block probabilities
+        self.tv = float(tv)
+        self.tv_prob = (1.0 / (self.M + 1) if tv_prob is None else float(tv_prob)) if self.tv > 0 else 0.0
+        self._n_tv = max(1, int(round(self.M * self.tv_prob / (1 - self.tv_prob)))) if self.tv > 0 else 0
+        self._p_data = (1.0 - self.tv_prob) / self.M
+        self._inv_p_data = float(self.M) if self.tv == 0 else 1.0 / self._p_data
         self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
         self.eps = float(np.float32(eps_counts))
         self.rho = rho
@@ -546,6 +612,16 @@ class OracleSPDHG:
         self._last = int(perm[-1])
         return [int(m) for m in perm]
 
+    def _epoch_sequence(self):
+        """spdhg_poisson.py:372-381: with tv, the TV block (-1) at _n_tv random places."""
+        order = self._epoch_order()
+        if self.tv <= 0:
+            return order
+        n = len(order) + self._n_tv
+        tv_at = set(int(i) for i in self.rng.choice(n, self._n_tv, replace=False))
+        it = iter(order)
+        return [-1 if i in tv_at else next(it) for i in range(n)]
+
     def run(self, x0, b, n_epochs, weights=None, gain=None, background=None, diagnostics_interval=1, state=None,
             record=False):
         op = self.op
@@ -563,7 +639,7 @@ class OracleSPDHG:
             rho = float(self.rho)
         self.rho_used = rho
         gamma = np.sqrt(rho)
-        p = 1.0 / self.M
+        p = self._p_data
         sup = np.ones((op.Ny, op.Nx), bool) if self.supp is None else self.supp
         # S from the row sums of A 1_S (spdhg_poisson.py:621-631; dual_steps pdhg_poisson.py:79-84)
         ones = np.broadcast_to(sup, op.coeff_shape).astype(np.float64)
@@ -587,7 +663,12 @@ class OracleSPDHG:
         else:
             z = np.zeros(op.coeff_shape)
         ep0 = int(state["epoch"]) if state is not None else 0
-        inv_p = float(self.M)
+        inv_p = self._inv_p_data
+        if self.tv > 0:
+            qx = np.zeros(op.coeff_shape) if state is None or state.get("qx") is None else \
+                np.asarray(state["qx"], np.float64).reshape(op.coeff_shape).copy()
+            qy = np.zeros(op.coeff_shape) if state is None or state.get("qy") is None else \
+                np.asarray(state["qy"], np.float64).reshape(op.coeff_shape).copy()
         self.iter_stats = []
         self.iterates = []
         f = float("nan")
@@ -598,15 +679,17 @@ class OracleSPDHG:
             D[rows] = yn - y[rows]
             y[rows] = yn
 
-        def update(dz):
+        def update(dz, ip):
             nonlocal x, z
             z = z + dz
-            zb = z + inv_p * dz
+            zb = z + ip * dz
             x = np.where(T > 0, np.maximum(0.0, x - T * (zb + self.lam)), 0.0)
 
-        seq = self._epoch_order()
+        # spdhg_poisson.py:664-697
+        seq = self._epoch_sequence()
         D = np.zeros(op.data_shape)
-        D[self.subsets[seq[0]]] = op.direct_rows(x, self.subsets[seq[0]])
+        if seq[0] >= 0:
+            D[self.subsets[seq[0]]] = op.direct_rows(x, self.subsets[seq[0]])
         for e in range(n_epochs):
             last = e + 1 == n_epochs
             want_f = (e + 1) % max(1, diagnostics_interval) == 0 or last
@@ -616,27 +699,40 @@ class OracleSPDHG:
                 if not end:
                     nb = seq[j + 1]
                 elif not last:
-                    nseq = self._epoch_order()
+                    nseq = self._epoch_sequence()
                     nb = nseq[0]
                 else:
                     nb = None
-                dual(blk, D)
-                update(op.adjoint(D, subset=blk))
+                adj = blk if blk >= 0 else None
+                m_next = nb if nb is not None and nb >= 0 else None
+                if adj is not None:
+                    dual(adj, D)
+                    update(op.adjoint(D, subset=adj), inv_p)
+                else:
+                    nqx, nqy = tv_dual(x, T, qx, qy, self.tv, TV_SAFETY * self.tv_prob)
+                    dz = tv_adjoint(nqx - qx, nqy - qy)
+                    qx, qy = nqx, nqy
+                    update(dz, 1.0 / self.tv_prob)
                 if record:
                     self.iterates.append(x.copy())
-                if (end and want_f) or nb is None or nb == blk:
-                    if (end and want_f) or nb is not None:
+                if (end and want_f) or nb is None or (m_next is not None and m_next == adj):
+                    if (end and want_f) or m_next is not None:
                         D = op.direct(x)
                         if end and want_f:
                             f = deviance(D, b, w, g, bg, self.eps)
-                else:
-                    rows = self.subsets[nb]
+                            if self.tv > 0:
+                                f += self.tv * tv_value(x, T)
+                elif m_next is not None:
+                    rows = self.subsets[m_next]
                     D[rows] = op.direct_rows(x, rows)
             if nseq is not None:
                 seq = nseq
             self.iter_stats.append({"epoch": ep0 + e + 1, "f": f if want_f else None, "rho": rho})
         self.state = {"epoch": ep0 + n_epochs, "z": z, "y": y}
-        self.final_stats = {"epochs": n_epochs, "final_f": f, "rho": rho, "subsets": self.M}
+        if self.tv > 0:
+            self.state["qx"], self.state["qy"] = qx, qy
+        self.final_stats = {"epochs": n_epochs, "final_f": f, "rho": rho, "subsets": self.M, "tv": self.tv,
+                            "tv_prob": self.tv_prob}
         return x
 
 
