@@ -1,14 +1,20 @@
 #!/usr/bin/env python3
-"""SPDHG-Poisson subset steps per second on one MI355X (workload "al_c1": Al, 180 omega x 103 translations x 180 eta x
-5 rings, 99 x 99 pixels, K = 29713 orientations of a uniform 3-degree cubic fundamental-zone grid, sigma 2 deg, sparse
-CSR pole-figure matrix in float32, 10 omega subsets, Poisson data at a mean of 1000 counts).
+"""SPDHG-Poisson subset steps per second on one MI355X, BF16 matrix-core PF products against the FP32 path
+(workload "al_c1": Al, 180 omega x 103 translations x 180 eta x 5 rings, 99 x 99 pixels, K = 29713 orientations of
+a uniform 3-degree cubic fundamental-zone grid, sigma 2 deg, 10 omega subsets, Poisson data at a mean of 1000 counts).
 
-A step is one SPDHG subset iteration: the dual update of subset m, the CSR adjoint and the backprojection fused with
-the primal update, the forward projection and CSR product of the next subset. The warm-up and timed steps continue
+A step is one SPDHG subset iteration: the dual update of subset m, the PF adjoint and the backprojection fused with
+the primal update, the forward projection and PF product of the next subset. The warm-up and timed steps continue
 one run (the solver's own random subset order). Inputs and all solver arrays are resident in HBM before timing.
 
-Prints one JSON line (metric, value, roofline of the dominant kernel from HIP events over the timed steps,
-cpu_baseline from the C/OpenMP restatement on the host cores, forward/adjoint GB/s).
+Three PF-product precisions run on the same problem: "bf16" (dense PF as bf16 MFMA fragments, FP32 accumulation),
+"bf16_residual" (plus the hi/lo residual correction) and "fp32" (CSR SpMM; the tests pin it to the FP64 oracle
+within 2e-5). The headline (--precision, default bf16_residual: the BF16 path that stays
+within 1e-3 of FP32 on every output) reports its steps/s; every path reports its forward, adjoint and SPDHG-iterate
+relative L2 difference to the fp32 path, all solving the same counts.
+
+Prints one JSON line (metric, value, roofline of the headline path's dominant kernel from HIP events over the timed
+steps, cpu_baseline from the C/OpenMP restatement on the host cores, forward/adjoint GB/s, per-path figures).
 """
 from __future__ import annotations
 
@@ -35,6 +41,15 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512, help="orientations of the host-core baseline sample")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--precision", default="bf16_residual", choices=["bf16", "bf16_residual", "fp32"],
+                    help="PF product of the headline figure (the other precisions run too, for the comparison)")
+    ap.add_argument("--only", action="store_true", help="run the headline precision only")
+    ap.add_argument("--no-op-timing", action="store_true", help="skip the full forward / adjoint timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (the reference kernel of the FETCH_SIZE / WRITE_SIZE "
+                         "counter runs that give roofline.traffic)")
+    ap.add_argument("--graphs", action="store_true", help="replay subset steps from captured HIP graphs (pays off on "
+                    "small problems, where launches cost more than the kernels)")
     return ap.parse_args()
 
 
@@ -42,16 +57,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_problem(name, K=None, mean_counts=None):
+def build_problem(name, K=None, mean_counts=None, precision="fp32", counts=None):
+    """Operator, Poisson counts and the truth of workload `name` (dense PF on the matrix cores for the bf16
+    precisions, sparse CSR for fp32). counts: reuse these counts instead of drawing them from this operator's
+    forward projection (so that every precision solves the same problem)."""
     from diffractom_amd import SinglePhaseForwardOperator
     from diffractom_amd.workloads import CONFIGS, make_grid, poisson_counts, voronoi_truth
     p = CONFIGS[name]
     grid, mat, cfg = make_grid(name, K=K)
-    op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="sparse")
+    op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True,
+                                    pf_mode="sparse" if precision == "fp32" else "dense", pf_precision=precision)
     xt = voronoi_truth(op.K, p["n"], p["grains"], p["radius"], p["seed"], p["coeff"])
+    if counts is not None:
+        return op, counts, xt, p
     ax = op.direct(xt)
     b, _ = poisson_counts(ax, mean_counts or p["mean_counts"], p["seed"])
-    return op, b, p
+    return op, b, xt, p
 
 
 def kernel_times(solver, n_steps):
@@ -59,10 +80,10 @@ def kernel_times(solver, n_steps):
     are launched on (torch's current stream)."""
     import torch
     op = solver.op
-    names = ["dual_update", "spmm_adjoint", "radon_backward_spdhg", "radon_forward", "spmm_forward"]
+    fused = op.fused_width(max(len(s) for s in solver.subsets)) > 0
+    names = ["dual_update", "pf_adjoint"] + (["spdhg_fused"] if fused else
+                                             ["radon_backward_spdhg", "radon_forward"]) + ["pf_forward"]
     tot = {n: 0.0 for n in names}
-    cnt = {n: 0 for n in names}
-    from diffractom_amd import _lib
     evs = []
     for _ in range(n_steps):
         seq, pos = solver._seq, solver._pos
@@ -74,30 +95,114 @@ def kernel_times(solver, n_steps):
             nseq = solver._epoch_sequence()
             nb = nseq[0]
             solver._seq, solver._pos = nseq, 0
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
         e[0].record()
         solver._dual(blk)
         e[1].record()
         n_ang, _, _, geo = op._angle_set(blk)
-        sc = op._scratch_buffers(n_ang)
+        n2, _, geo_f, _ = op._angle_set(nb)
+        sc = op._scratch_buffers(max(n_ang, n2))
         bt = op.batches[0]
         op._pf_adjoint(0, solver.D, sc["sino"], blk)
         e[2].record()
-        op._radon_backward(sc["sino"], geo, op._tables(blk), n_ang, None, op.K_stride, bt["koff"], bt["kstride"],
-                           x=solver.xk, z=solver.zk, T=solver.Tk, inv_p=solver._inv_p_data, lam=solver.lam)
-        e[3].record()
-        n2, _, geo_f, _ = op._angle_set(nb)
-        op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2, op._tables(nb))
-        e[4].record()
+        if fused:
+            op._spdhg_fused(sc["sino"], blk, nb, solver.xk, solver.zk, solver.Tk, solver._inv_p_data, solver.lam,
+                            bt["koff"], bt["kstride"], op.fused_width(n_ang))
+        else:
+            op._radon_backward(sc["sino"], geo, op._tables(blk), n_ang, None, op.K_stride, bt["koff"],
+                               bt["kstride"], x=solver.xk, z=solver.zk, T=solver.Tk, inv_p=solver._inv_p_data,
+                               lam=solver.lam)
+            e[3].record()
+            op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2, op._tables(nb))
+        e[-2].record()
         op._pf_forward(0, sc["sino"], solver.D, nb, accumulate=False)
-        e[5].record()
+        e[-1].record()
         evs.append(e)
     torch.cuda.synchronize()
     for e in evs:
         for i, n in enumerate(names):
             tot[n] += e[i].elapsed_time(e[i + 1])
-            cnt[n] += 1
-    return {n: tot[n] / max(cnt[n], 1) for n in names}
+    return {n: tot[n] / max(n_steps, 1) for n in names}
+
+
+def kernel_bytes(op, M):
+    """Algorithmic HBM bytes per launch of each kernel class of one SPDHG step (subset of R/M angles)."""
+    npix = op.Nx * op.Ny
+    K, R, My, CP = op.K, op.N_Omega, op.My, op.N_seg
+    Rm = R / M
+    sino = 4.0 * Rm * My * K
+    rows = 4.0 * Rm * My * CP
+    if op.mfma_batches is not None:
+        mb = op.mfma_batches[0]
+        f = sum(mb[k].numel() * 2 for k in ("fwd_hi", "fwd_lo") if mb[k] is not None) * Rm / R
+        a = sum(mb[k].numel() * 2 for k in ("adj_hi", "adj_lo") if mb[k] is not None) * Rm / R
+    else:
+        f = a = 6.0 * op.sparse_nnz / M
+    return {"radon_backward_spdhg": 20.0 * K * npix + sino, "radon_forward": 4.0 * K * npix + sino,
+            "spdhg_fused": 20.0 * K * npix + 2.0 * sino,
+            "pf_adjoint": a + rows + sino, "pf_forward": f + sino + rows, "dual_update": 20.0 * rows}
+
+
+def rel_l2(a, b):
+    import torch
+    a, b = a.double(), b.double()
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def run_path(args, precision, counts=None):
+    """One precision: set-up, warm-up, timed SPDHG steps, per-kernel times, full forward / adjoint times; returns
+    the figures and the device results the precisions are compared on."""
+    import torch
+    from diffractom_amd import SPDHGPoisson
+    t0 = time.perf_counter()
+    op, b, xt, p = build_problem(args.config, precision=precision, counts=counts)
+    M = p["subsets"]
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    log(f"[bench] {precision}: K={op.K} PF {op.pf_mode} ({op.pf_storage}), {op.device_bytes() / 1e9:.1f} GB, "
+        f"set-up {setup:.1f} s")
+    solver = SPDHGPoisson(op, subsets=M, seed=0)
+    solver.graphs = args.graphs
+    solver.start(np.zeros(op.coeff_shape, np.float32), b)
+    torch.cuda.synchronize()
+    solver.advance(args.warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    ts = time.perf_counter()
+    e0.record()
+    solver.advance(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - ts
+    ms_step = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] {precision}: timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall:.3f} s)")
+    ktimes = None
+    if not args.no_kernel_timing:
+        ktimes = kernel_times(solver, args.steps)
+        log(f"[bench] {precision}: kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in ktimes.items()))
+    x_iter = solver.xk.clone()
+    solver.finish()
+    # full forward / adjoint (API layout, median of 10 after 2)
+    xg = torch.from_numpy(xt).cuda()
+    bg = torch.from_numpy(np.asarray(b, np.float32)).cuda()
+    data = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
+    out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
+    tf, ta = [float("nan")] * 3, [float("nan")] * 3
+    for _ in range(0 if args.no_op_timing else 12):
+        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a0.record()
+        op.direct_cl(xg, data)
+        a1.record()
+        op.adjoint_cl(bg, out)
+        a2.record()
+        torch.cuda.synchronize()
+        tf.append(a0.elapsed_time(a1))
+        ta.append(a1.elapsed_time(a2))
+    res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b, "ktimes": ktimes, "setup_s": setup,
+           "t_fwd": float(np.median(tf[2:])), "t_adj": float(np.median(ta[2:])),
+           "fwd_out": data, "adj_out": out, "x_iter": x_iter, "steps_total": args.warmup + 2 * args.steps}
+    return res
 
 
 def main():
@@ -108,107 +213,113 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
-    from diffractom_amd import SPDHGPoisson
-    t0 = time.perf_counter()
-    op, b, p = build_problem(args.config)
-    M = p["subsets"]
-    log(f"[bench] operator K={op.K} nnz={op.sparse_nnz} fill={op.sparse_fill:.4f} batches={len(op.batches)} "
-        f"setup {time.perf_counter() - t0:.1f} s")
-    solver = SPDHGPoisson(op, subsets=M, seed=0)
-    solver.start(np.zeros(op.coeff_shape, np.float32), b)
-    torch.cuda.synchronize()
-    solver.advance(args.warmup)
-    torch.cuda.synchronize()
-    log(f"[bench] warmup {args.warmup} steps done")
-    ts = time.perf_counter()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    solver.advance(args.steps)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - ts
-    ms_step = e0.elapsed_time(e1) / args.steps
-    log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall:.3f} s)")
-
+    # fp32 first: its counts are the problem every precision solves
+    precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]
+    results = {}
+    counts = None
+    for q in precisions:
+        r = run_path(args, q, counts)
+        counts = r["counts"]
+        results[q] = r
+        if q != args.precision:     # keep the headline path's operator (roofline) only
+            r["op"].free_memory()
+            r["op"] = None
+        torch.cuda.empty_cache()
+    head = results[args.precision]
+    op = head["op"]
+    M = head["M"]
     npix = op.Nx * op.Ny
     K, R, My, CP = op.K, op.N_Omega, op.My, op.N_seg
-    Rm = R / M
-    nnz_m = op.sparse_nnz / M
-    step_bytes = 24.0 * K * npix + 16.0 * Rm * My * CP + 12.0 * nnz_m
-
+    paths = {}
+    ref = results.get("fp32")
+    for q, r in results.items():
+        e = {"steps_per_s": round(1000.0 / r["ms_step"], 3), "ms_per_step": round(r["ms_step"], 4),
+             "forward_ms": round(r["t_fwd"], 4), "adjoint_ms": round(r["t_adj"], 4),
+             "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}}
+        if ref is not None and q != "fp32":
+            e["rel_l2_vs_fp32"] = {"forward": rel_l2(r["fwd_out"], ref["fwd_out"]),
+                                   "adjoint": rel_l2(r["adj_out"], ref["adj_out"]),
+                                   f"x_after_{r['steps_total']}_steps": rel_l2(r["x_iter"], ref["x_iter"])}
+        paths[q] = e
     roofline = None
-    ktimes = None
-    if not args.no_kernel_timing:
-        ktimes = kernel_times(solver, args.steps)
-        log("[bench] kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in ktimes.items()))
-        kbytes = {
-            "radon_backward_spdhg": 20.0 * K * npix + 4.0 * Rm * My * K,
-            "radon_forward": 4.0 * K * npix + 4.0 * Rm * My * K,
-            "spmm_adjoint": 6.0 * nnz_m + 4.0 * Rm * My * CP + 4.0 * Rm * My * K,
-            "spmm_forward": 6.0 * nnz_m + 4.0 * Rm * My * CP + 4.0 * Rm * My * K,
-            "dual_update": 20.0 * Rm * My * CP,
-        }
-        dom = max(ktimes, key=ktimes.get)
-        ach = kbytes[dom] / (ktimes[dom] * 1e-3) / 1e9
+    if head["ktimes"]:
+        kb = kernel_bytes(op, M)
+        kt = head["ktimes"]
+        dom = max(kt, key=kt.get)
+        ach = kb[dom] / (kt[dom] * 1e-3) / 1e9
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "avg_ms": round(ktimes[dom], 5),
-                    "algorithmic_bytes": kbytes[dom]}
-    # forward / adjoint GB/s of the whole operator (API layout, median of 10)
-    xd = solver.finish()
-    xt = torch.from_numpy(np.asarray(xd)).cuda()
-    data = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
-    out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
-    fb = 4.0 * K * npix + 4.0 * R * My * CP + 6.0 * op.sparse_nnz
-    tf, ta = [], []
-    for _ in range(12):
-        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        a0.record()
-        op.direct_cl(xt, data)
-        a1.record()
-        op.adjoint_cl(data, out)
-        a2.record()
-        torch.cuda.synchronize()
-        tf.append(a0.elapsed_time(a1))
-        ta.append(a1.elapsed_time(a2))
-    t_fwd, t_adj = float(np.median(tf[2:])), float(np.median(ta[2:]))
+                    "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(dom, args.config, args.precision), "avg_ms": round(kt[dom], 5),
+                    "algorithmic_bytes": kb[dom]}
 
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            sys.path.insert(0, ROOT)
             from oracle import cpu_baseline
             Ks = min(args.cpu_sample, op.K)
-            op_s, b_s, _ = build_problem(args.config, K=Ks)
+            op_s, b_s, _, _ = build_problem(args.config, K=Ks)
+            from diffractom_amd import SPDHGPoisson
             sol_s = SPDHGPoisson(op_s, subsets=M, seed=0)
             sol_s.start(np.zeros(op_s.coeff_shape, np.float32), b_s)
             rate, n, thr = cpu_baseline.time_steps(sol_s, budget_s=15.0)
             scaled = rate * Ks / op.K
             cpu = {"value": round(scaled, 4), "unit": "SPDHG steps/s", "cores": thr, "kind": "port",
-                   "sample": f"{n} steps of the C/OpenMP FP32 restatement on {Ks} of the {op.K} orientations "
+                   "sample": f"{n} steps of the C/OpenMP FP32 CSR restatement on {Ks} of the {op.K} orientations "
                              f"({rate:.3f} steps/s), scaled by {Ks}/{op.K} (the step's work is linear in K)"}
             log(f"[bench] cpu baseline {rate:.3f} steps/s on {Ks} orientations with {thr} threads")
         except Exception as exc:                                    # the GPU figure stands without it
             log(f"[bench] cpu baseline failed: {exc!r}")
 
+    ms_step = head["ms_step"]
     value = 1000.0 / ms_step
+    # forward / adjoint GB/s: x + data + one CSR of the PF matrix (its nonzeros from the fp32 run)
+    nnz_all = results["fp32"]["op_nnz"] if "fp32" in results else 0
+    fbytes = 4.0 * K * npix + 4.0 * R * My * CP + 6.0 * nnz_all
     line = {
-        "metric": "SPDHG-Poisson iters/s (subset steps/s), FP32 CSR path",
+        "metric": "SPDHG-Poisson iters/s + fwd/adj GB/s, BF16-MFMA vs FP32, <=1e-3 rel err",
         "value": round(value, 3), "unit": "SPDHG subset steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (Voronoi grains, Poisson counts)",
-        "config": {"workload": f"{args.config}: Al 5 rings, 180x103x180, 99x99 px, K={op.K}, sigma 2 deg, "
-                               f"M={M} subsets, sparse CSR PF (nnz {op.sparse_nnz})",
-                   "K": op.K, "nnz": op.sparse_nnz, "subsets": M},
+        "vs_baseline": None, "dtype": DTYPE[args.precision],
+        "rel_err_vs_fp32": paths[args.precision].get("rel_l2_vs_fp32"),
+        "data": "synthetic (Voronoi grains, Poisson counts at a mean of 1000)",
+        "config": {"workload": f"{args.config}: Al 5 rings, 180 omega x 103 x 180 eta, 99 x 99 px, K={op.K}, "
+                               f"sigma 2 deg, M={M} subsets; PF product {PF_DESC[args.precision]}",
+                   "K": op.K, "subsets": M, "precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "step_GBps_algorithmic": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
-        "forward_ms": round(t_fwd, 4), "adjoint_ms": round(t_adj, 4),
-        "forward_GBps": round(fb / (t_fwd * 1e-3) / 1e9, 1), "adjoint_GBps": round(fb / (t_adj * 1e-3) / 1e9, 1),
-        "kernel_ms": {k: round(v, 5) for k, v in (ktimes or {}).items()},
+        "forward_GBps": round(fbytes / (head["t_fwd"] * 1e-3) / 1e9, 1),
+        "adjoint_GBps": round(fbytes / (head["t_adj"] * 1e-3) / 1e9, 1),
+        "paths": paths,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 1 GiB read, 1 GiB written")
+
+
+def measured_traffic(kernel, config, precision):
+    """HBM bytes per launch of `kernel` on this workload from the committed counter summary
+    (profiles/traffic_<workload>.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy calibration), or
+    None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{config}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    e = t.get("kernels", {}).get(f"{precision}:{kernel}")
+    return None if e is None else float(e["bytes_per_launch"])
+
+
+DTYPE = {"bf16": "bf16", "bf16_residual": "bf16", "fp32": "f32"}
+PF_DESC = {"bf16": "dense BF16 MFMA, FP32 accumulation",
+           "bf16_residual": "dense BF16 MFMA with hi/lo residual correction, FP32 accumulation",
+           "fp32": "FP32 CSR SpMM"}
 
 
 if __name__ == "__main__":

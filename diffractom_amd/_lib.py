@@ -47,6 +47,11 @@ SIGNATURES = {
     "dfm_mean_counted_partial": [P, P, I64, I64, P, P, P],
     "dfm_dot_partial": [P, P, I64, P, P, P],
     "dfm_scale": [P, F32, I64, P],
+    "dfm_tv_dual_k": [P, P, P, P, P, P, I32, I32, I32, F32, F32, P],
+    "dfm_tv_adjoint_update_k": [P, P, P, P, P, I32, I32, I32, F32, F32, P],
+    "dfm_tv_value_partial": [P, P, I32, I32, I32, P, P, P],
+    "dfm_projector_lds_width": [I32, I32, I32],
+    "dfm_projector_lds": [I32, P, P, I32, P, P, P, F32, F32, P, P, I32, F32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_pack_fwd": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_pack_adj": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_forward": [P, I32, I32, P, P, P, I32, I32, I32, I32, I32, P, P, I32, I32, I32, P],

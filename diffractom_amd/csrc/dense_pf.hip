@@ -207,8 +207,8 @@ __global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
   auto load_b = [&](int c, bf16x8 (&h)[NT], bf16x8 (&l)[NT]) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      h[t] = tok[t] ? bh_p[t][(int64_t)c * 64] : zero8;
-      l[t] = (RES && tok[t]) ? bl_p[t][(int64_t)c * 64] : zero8;
+      h[t] = tok[t] ? __builtin_nontemporal_load(bh_p[t] + (int64_t)c * 64) : zero8;
+      l[t] = (RES && tok[t]) ? __builtin_nontemporal_load(bl_p[t] + (int64_t)c * 64) : zero8;
     }
   };
 

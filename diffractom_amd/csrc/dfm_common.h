@@ -29,6 +29,18 @@ void set_error(const char* fmt, ...);
     }                                                                            \
   } while (0)
 
+// Streaming (non-temporal) accesses for data a kernel touches once: they do not displace what the XCD's L2 holds for
+// reuse (sinogram slabs, image columns, geometry tables).
+typedef float f4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+  const f4_t v = __builtin_nontemporal_load(reinterpret_cast<const f4_t*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_stream(float4* p, const float4& v) {
+  const f4_t t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<f4_t*>(p));
+}
+
 inline unsigned grid_stride_blocks(int64_t n, int block, unsigned cap = 1u << 16) {
   int64_t b = (n + block - 1) / block;
   if (b < 1) b = 1;

@@ -202,18 +202,18 @@ __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __res
   }
   const int64_t o = pix * kstride4 + k;
   if (MODE == 0) {
-    out[o] = acc;
+    st_stream(out + o, acc);
   } else {
-    float4 zz = z[o], xx = x[o];
-    const float4 tt = T[o];
+    float4 zz = ld_stream(z + o), xx = ld_stream(x + o);
+    const float4 tt = ld_stream(T + o);
     float4 zn, xn;
 #define DFM_UPD(c)                                                                  \
   zn.c = zz.c + acc.c;                                                              \
   xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zn.c + inv_p * acc.c) + lam)) : 0.0f;
     DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
 #undef DFM_UPD
-    z[o] = zn;
-    x[o] = xn;
+    st_stream(z + o, zn);
+    st_stream(x + o, xn);
   }
 }
 
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* 
     acc.z += w * v.z;
     acc.w += w * v.w;
   }
-  sino[(int64_t)r * skstride4 + kq] = acc;
+  st_stream(sino + (int64_t)r * skstride4 + kq, acc);
 }
 
 // Backward from the tables, one 4 x 4 pixel tile x 16 channel quads per workgroup: the TB bins of every angle the
@@ -340,18 +340,18 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
   if (!live) return;
   const int64_t o = pix * kstride4 + kq + k4_off;
   if (MODE == 0) {
-    out[o] = acc;
+    st_stream(out + o, acc);
   } else {
-    float4 zz = z[o], xx = x[o];
-    const float4 tt = T[o];
+    float4 zz = ld_stream(z + o), xx = ld_stream(x + o);
+    const float4 tt = ld_stream(T + o);
     float4 zn, xn;
 #define DFM_UPD(c)                                                                  \
   zn.c = zz.c + acc.c;                                                              \
   xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zn.c + inv_p * acc.c) + lam)) : 0.0f;
     DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
 #undef DFM_UPD
-    z[o] = zn;
-    x[o] = xn;
+    st_stream(z + o, zn);
+    st_stream(x + o, xn);
   }
 }
 

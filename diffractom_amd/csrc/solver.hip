@@ -174,6 +174,116 @@ __global__ void __launch_bounds__(256) dot_kernel(const float* __restrict__ a, c
   block_sum<1>(s, partial, gridDim.x);
 }
 
+// ---- total-variation block (spdhg_poisson.py:108-193) on the channel-fastest layout: element i = p * k4 + q of a
+// float4 array (pixel p = ix + nx * iy, channel quad q); the right neighbour is i + k4, the lower one i + nx * k4.
+// A difference is free when T > 0 at both ends.
+__device__ __forceinline__ void tv_core(float tj, float tr, float td, bool vr0, bool vd0, float xj, float xr, float xd,
+                                        float ox, float oy, float lam, float ps, float& nx_, float& ny_) {
+  const bool vr = vr0 && tj > 0.0f && tr > 0.0f;
+  const bool vd = vd0 && tj > 0.0f && td > 0.0f;
+  float m = 0.0f;
+  if (vr) m = fmaxf(m, tj + tr);
+  if (vd) m = fmaxf(m, tj + td);
+  float ax = 0.0f, ay = 0.0f;
+  if (m > 0.0f) {
+    const float s = ps / (4.0f * m);
+    if (vr) ax = ox + s * (xr - xj);
+    if (vd) ay = oy + s * (xd - xj);
+    const float nrm = sqrtf(ax * ax + ay * ay);
+    if (nrm > lam) {
+      const float c = lam / nrm;
+      ax *= c;
+      ay *= c;
+    }
+  }
+  nx_ = ax;
+  ny_ = ay;
+}
+
+// q <- proj_{|q| <= lam}(q + s D x) on the free differences (0 elsewhere); dq = q' - q
+__global__ void __launch_bounds__(256) tv_dual_kernel(const float4* __restrict__ x, const float4* __restrict__ T,
+                                                      float4* __restrict__ qx, float4* __restrict__ qy,
+                                                      float4* __restrict__ dqx, float4* __restrict__ dqy, int nx,
+                                                      int ny, int k4, float lam, float ps) {
+  const int64_t n = (int64_t)nx * ny * k4;
+  const int64_t row = (int64_t)nx * k4;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / k4;
+    const int ix = (int)(p % nx), iy = (int)(p / nx);
+    const bool vr0 = ix + 1 < nx, vd0 = iy + 1 < ny;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 tj = T[i], tr = vr0 ? T[i + k4] : z4, td = vd0 ? T[i + row] : z4;
+    const float4 xj = x[i], xr = vr0 ? x[i + k4] : z4, xd = vd0 ? x[i + row] : z4;
+    const float4 ox = qx[i], oy = qy[i];
+    float4 nxv, nyv;
+    tv_core(tj.x, tr.x, td.x, vr0, vd0, xj.x, xr.x, xd.x, ox.x, oy.x, lam, ps, nxv.x, nyv.x);
+    tv_core(tj.y, tr.y, td.y, vr0, vd0, xj.y, xr.y, xd.y, ox.y, oy.y, lam, ps, nxv.y, nyv.y);
+    tv_core(tj.z, tr.z, td.z, vr0, vd0, xj.z, xr.z, xd.z, ox.z, oy.z, lam, ps, nxv.z, nyv.z);
+    tv_core(tj.w, tr.w, td.w, vr0, vd0, xj.w, xr.w, xd.w, ox.w, oy.w, lam, ps, nxv.w, nyv.w);
+    qx[i] = nxv;
+    qy[i] = nyv;
+    dqx[i] = make_float4(nxv.x - ox.x, nxv.y - ox.y, nxv.z - ox.z, nxv.w - ox.w);
+    dqy[i] = make_float4(nyv.x - oy.x, nyv.y - oy.y, nyv.z - oy.z, nyv.w - oy.w);
+  }
+}
+
+// dz = D^T dq = -dqx - dqy + dqx(left) + dqy(up), then the primal update z += dz, x = prox(x - T (z + inv_p dz + lam))
+__global__ void __launch_bounds__(256) tv_adjoint_update_kernel(const float4* __restrict__ dqx,
+                                                                const float4* __restrict__ dqy,
+                                                                float4* __restrict__ x, float4* __restrict__ z,
+                                                                const float4* __restrict__ T, int nx, int ny, int k4,
+                                                                float inv_p, float lam) {
+  const int64_t n = (int64_t)nx * ny * k4;
+  const int64_t row = (int64_t)nx * k4;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / k4;
+    const int ix = (int)(p % nx), iy = (int)(p / nx);
+    const float4 a = dqx[i], b = dqy[i];
+    float4 dz = make_float4(-a.x - b.x, -a.y - b.y, -a.z - b.z, -a.w - b.w);
+    if (ix > 0) {
+      const float4 l = dqx[i - k4];
+      dz.x += l.x; dz.y += l.y; dz.z += l.z; dz.w += l.w;
+    }
+    if (iy > 0) {
+      const float4 u = dqy[i - row];
+      dz.x += u.x; dz.y += u.y; dz.z += u.z; dz.w += u.w;
+    }
+    float4 zz = z[i], xx = x[i];
+    const float4 tt = T[i];
+#define DFM_TVUPD(c)                                                                           \
+  zz.c += dz.c;                                                                                \
+  xx.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zz.c + inv_p * dz.c) + lam)) : 0.0f;
+    DFM_TVUPD(x) DFM_TVUPD(y) DFM_TVUPD(z) DFM_TVUPD(w)
+#undef DFM_TVUPD
+    z[i] = zz;
+    x[i] = xx;
+  }
+}
+
+__global__ void __launch_bounds__(256) tv_value_kernel(const float4* __restrict__ x, const float4* __restrict__ T,
+                                                       int nx, int ny, int k4, double* __restrict__ partial) {
+  const int64_t n = (int64_t)nx * ny * k4;
+  const int64_t row = (int64_t)nx * k4;
+  double s[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / k4;
+    const int ix = (int)(p % nx), iy = (int)(p / nx);
+    const bool vr0 = ix + 1 < nx, vd0 = iy + 1 < ny;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 tj = T[i], tr = vr0 ? T[i + k4] : z4, td = vd0 ? T[i + row] : z4;
+    const float4 xj = x[i], xr = vr0 ? x[i + k4] : z4, xd = vd0 ? x[i + row] : z4;
+#define DFM_TVN(c)                                                                             \
+  {                                                                                            \
+    const float gx = (vr0 && tj.c > 0.0f && tr.c > 0.0f) ? xr.c - xj.c : 0.0f;                 \
+    const float gy = (vd0 && tj.c > 0.0f && td.c > 0.0f) ? xd.c - xj.c : 0.0f;                 \
+    s[0] += (double)sqrtf(gx * gx + gy * gy);                                                  \
+  }
+    DFM_TVN(x) DFM_TVN(y) DFM_TVN(z) DFM_TVN(w)
+#undef DFM_TVN
+  }
+  block_sum<1>(s, partial, gridDim.x);
+}
+
 static unsigned reduce_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b < 1) b = 1;
@@ -310,6 +420,43 @@ int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, 
   *n_partial = (int32_t)nb;
   hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, b, n, partial);
   DFM_LAUNCH_CHECK("dfm_dot_partial");
+  return DFM_OK;
+}
+
+int dfm_tv_dual_k(const float* x, const float* T, float* qx, float* qy, float* dqx, float* dqy, int32_t nx,
+                  int32_t ny, int32_t kstride, float lam, float ps, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && T && qx && qy && dqx && dqy, "dfm_tv_dual_k: null pointer");
+  DFM_CHECK_ARG(nx > 0 && ny > 0 && kstride > 0 && kstride % 4 == 0, "dfm_tv_dual_k: bad sizes");
+  const int64_t n = (int64_t)nx * ny * (kstride / 4);
+  hipLaunchKernelGGL(tv_dual_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)x, (const float4*)T, (float4*)qx, (float4*)qy, (float4*)dqx, (float4*)dqy, nx, ny,
+                     kstride / 4, lam, ps);
+  DFM_LAUNCH_CHECK("dfm_tv_dual_k");
+  return DFM_OK;
+}
+
+int dfm_tv_adjoint_update_k(const float* dqx, const float* dqy, float* x, float* z, const float* T, int32_t nx,
+                            int32_t ny, int32_t kstride, float inv_p, float lam, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && T && z && dqx && dqy, "dfm_tv_adjoint_update_k: null pointer");
+  DFM_CHECK_ARG(nx > 0 && ny > 0 && kstride > 0 && kstride % 4 == 0, "dfm_tv_adjoint_update_k: bad sizes");
+  const int64_t n = (int64_t)nx * ny * (kstride / 4);
+  hipLaunchKernelGGL(tv_adjoint_update_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)dqx, (const float4*)dqy, (float4*)x, (float4*)z, (const float4*)T, nx, ny,
+                     kstride / 4, inv_p, lam);
+  DFM_LAUNCH_CHECK("dfm_tv_adjoint_update_k");
+  return DFM_OK;
+}
+
+int dfm_tv_value_partial(const float* x, const float* T, int32_t nx, int32_t ny, int32_t kstride, double* partial,
+                         int32_t* n_partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && T && partial && n_partial, "dfm_tv_value_partial: null pointer");
+  DFM_CHECK_ARG(nx > 0 && ny > 0 && kstride > 0 && kstride % 4 == 0, "dfm_tv_value_partial: bad sizes");
+  const int64_t n = (int64_t)nx * ny * (kstride / 4);
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(tv_value_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const float4*)x,
+                     (const float4*)T, nx, ny, kstride / 4, partial);
+  DFM_LAUNCH_CHECK("dfm_tv_value_partial");
   return DFM_OK;
 }
 

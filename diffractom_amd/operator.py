@@ -31,6 +31,14 @@ def _round4(n: int) -> int:
     return -(-int(n) // 4) * 4
 
 
+CH_ALIGN = 64   # channel stride granule: 64 floats = 256 B, so every pixel row of a channel-fastest array and every
+                # 16-quad channel block start on a cache-line boundary (two 128 B lines per block, not three)
+
+
+def _round_ch(n: int) -> int:
+    return -(-int(n) // CH_ALIGN) * CH_ALIGN
+
+
 def _resolve_device(ctx):
     if ctx is None:
         if not torch.cuda.is_available():
@@ -334,7 +342,11 @@ class SinglePhaseForwardOperator:
         self.geo_fwd = torch.from_numpy(geo_f).to(self.device)
         self.geo_bwd = torch.from_numpy(geo_b).to(self.device)
         # kernel variants (identical results; set by autotune_projector)
+        # kernel choices (identical results up to float32 rounding): "lds" (the image of a channel group resident in
+        # LDS, when it fits), "table" (geometry tables, 2 taps), "arith" (footprint arithmetic, any taps)
         self.radon_variant = {"fwd": 8, "bwd": 8, "fwd_kernel": "table", "bwd_kernel": "table"}
+        self.fuse_projector = False
+        self.fused_channels = 4
         self._tables_full = self._radon_tables(np.arange(self.N_Omega)) if self.n_taps == 2 else None
 
     def _radon_tables(self, idx):
@@ -357,11 +369,11 @@ class SinglePhaseForwardOperator:
             k1 = min(k0 + kb, self.K)
             self.batches.append({"k_start": k0, "k_end": k1, "K_batch": k1 - k0})
             k0 = k1
-        self.K_batch_max = _round4(max(b["K_batch"] for b in self.batches))
-        # channel-fastest layout of a whole coefficient array: batch b at columns koff .. koff + round4(Kb)
+        self.K_batch_max = _round_ch(max(b["K_batch"] for b in self.batches))
+        # channel-fastest layout of a whole coefficient array: batch b at columns koff .. koff + round64(Kb)
         off = 0
         for b in self.batches:
-            b["kstride"] = _round4(b["K_batch"])
+            b["kstride"] = _round_ch(b["K_batch"])
             b["koff"] = off
             off += b["kstride"]
         self.K_stride = off
@@ -538,6 +550,7 @@ class SinglePhaseForwardOperator:
                   "sino": torch.empty(max(need_s, self.N_Omega * self.My * self.K_batch_max), dtype=torch.float32,
                                       device=self.device)}
             self._scratch = sc
+            self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
         return sc
 
     # ================================================================== subsets
@@ -595,7 +608,12 @@ class SinglePhaseForwardOperator:
         """img (mode 0, out_k) or the SPDHG update of x, z (mode 1) from the sinogram of n_ang angles."""
         st = self._stream()
         mode = 0 if x is None else 1
-        if tab is not None:
+        cw = self.lds_width() if self.radon_variant["bwd_kernel"] == "lds" else 0
+        if cw and koff % cw == 0 and kn % cw == 0:
+            self._lds_projector(mode, sino, geo, n_ang, out_k if mode == 0 else x, z, T, inv_p, lam, None, None, 0,
+                                kstride, koff, kn, cw)
+            return
+        if tab is not None and self.radon_variant["bwd_kernel"] != "arith":
             _lib.call("dfm_radon_backward_table", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(tab["ptab"]),
                       _lib.ptr(tab["tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff, kn, kn, _lib.ptr(x),
                       _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), mode, self.radon_variant["bwd"], st)
@@ -607,6 +625,35 @@ class SinglePhaseForwardOperator:
                       kstride, koff, kn, kn, self.n_taps, _lib.ptr(x), _lib.ptr(z), _lib.ptr(T), float(inv_p),
                       float(lam), st)
 
+    def lds_width(self):
+        """Channels per workgroup of the LDS-resident projector (4, 2, 1; at most fused_channels), or 0 when the
+        image does not fit the LDS or the projector needs more than 2 taps."""
+        if self.n_taps != 2:
+            return 0
+        cw = int(_lib.load().dfm_projector_lds_width(self.Nx, self.Ny, self.My))
+        while cw > self.fused_channels:
+            cw //= 2
+        return cw
+
+    def fused_width(self, na_b=None):
+        """Channels per workgroup of the fused projector step (0: not available or switched off with
+        fuse_projector = False)."""
+        return self.lds_width() if self.fuse_projector else 0
+
+    def _lds_projector(self, mode, sino_b, geo_b, na_b, x, z, T, inv_p, lam, sino_f, geo_f, na_f, kstride, koff, kn,
+                       cw):
+        _lib.call("dfm_projector_lds", int(mode), _lib.ptr(sino_b), _lib.ptr(geo_b), int(na_b), _lib.ptr(x),
+                  _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), _lib.ptr(sino_f), _lib.ptr(geo_f), int(na_f),
+                  float(self.radon_scale), self.Nx, self.Ny, self.My, int(kstride), int(koff), int(kn), int(kn),
+                  int(cw), self._stream())
+
+    def _spdhg_fused(self, sino, m, mn, x, z, T, inv_p, lam, koff, kn, cw):
+        """Backprojection of subset m's sinogram with the primal update of (x, z), then the forward projection of
+        subset mn from the updated x into the same sinogram scratch (dfm_projector_lds mode 3)."""
+        na_b, _, _, geo_b = self._angle_set(m)
+        na_f, _, geo_f, _ = self._angle_set(mn)
+        self._lds_projector(3, sino, geo_b, na_b, x, z, T, inv_p, lam, sino, geo_f, na_f, self.K_stride, koff, kn, cw)
+
     def zero_subset_rows(self, data, subset):
         m = self._subset(subset)
         _lib.call("dfm_zero_rows", _lib.ptr(data), _lib.ptr(self._sub_omegas[m]), len(self.omega_subsets[m]),
@@ -614,7 +661,12 @@ class SinglePhaseForwardOperator:
 
     # ================================================================== forward
     def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles, tab=None):
-        if tab is not None:
+        cw = self.lds_width() if self.radon_variant["fwd_kernel"] == "lds" else 0
+        if cw and koff % cw == 0 and kn % cw == 0:
+            self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
+                                kn, cw)
+            return
+        if tab is not None and self.radon_variant["fwd_kernel"] != "arith":
             _lib.call("dfm_radon_forward_table", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["row_ptr"]),
                       _lib.ptr(tab["pix"]), _lib.ptr(tab["w"]), self.My, n_angles, img_kstride, koff, kn, kn,
                       self.radon_variant["fwd"], self._stream())
@@ -658,6 +710,7 @@ class SinglePhaseForwardOperator:
         if p is None or p.numel() < n:
             p = torch.empty(n, dtype=torch.float32, device=self.device)
             self._mfma_part_buf = p
+            self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
         return p
 
     def _dense_forward(self, ib, sino, data, subset, accumulate):

@@ -6,6 +6,9 @@ Iteration, for the next subset m of a random permutation per epoch (p_m = 1/M):
     dz = A_m^T (y_m' - y_m); z += dz; x = max(0, x - T (z + dz/p_m + lam))   (fused into the backprojection)
     D_{m_next} = A_{m_next} x                       (forward of the next subset, into the same data-sized array)
 with S = gamma/(A 1_S), T = min_m p_m/(gamma A_m^T 1_counted), gamma = sqrt(rho), rho = 3e4/mean(b)^2 ("auto").
+With tv > 0 the total variation of every orientation image is one more block (probability p_tv, default 1/(M+1);
+the subsets (1 - p_tv)/M each): its dual q per pixel is projected onto |q| <= tv after a step
+0.99 p_tv / (4 max (T_j + T_j')), and x is updated with D^T dq (spdhg_poisson.py:108-193, 430-502).
 
 MI355X design: x, z and T stay on the GPU for the whole run in the orientation-fastest layout (Npix, K) the Radon
 kernels read, so a step is the subset's CSR adjoint, one backprojection kernel that also applies the primal update,
@@ -24,6 +27,8 @@ from .operator import as_host, check_device
 
 RHO_AUTO = 3e4            # pdhg_poisson.py:33
 NAMES = ("x", "z", "T")
+TV_NAMES = ("qx", "qy")
+TV_SAFETY = 0.99          # spdhg_poisson.py:277: the TV dual step is this fraction of its bound
 DFM_MAX_PARTIALS = 2048
 
 
@@ -60,8 +65,6 @@ class SPDHGPoisson:
             raise ValueError("tv must be >= 0")
         if tv_tree is not None:
             raise NotImplementedError("tv_tree needs an operator with the local coefficient layout")
-        if float(tv) > 0:
-            raise NotImplementedError("the total-variation block (tv > 0) is not implemented")
         self.op = operator
         self.subsets = operator.set_omega_subsets(subsets)
         self.M = len(self.subsets)
@@ -70,14 +73,25 @@ class SPDHGPoisson:
         self.rho = rho if rho == "auto" else float(rho)
         self.rng = np.random.default_rng(seed)
         self.resident = resident
-        self.tv = 0.0
-        self.tv_prob = 0.0
-        self._p_data = 1.0 / self.M
-        self._inv_p_data = float(self.M)
+        # block probabilities (spdhg_poisson.py:339-346): data subsets (1 - p_tv) / M each, the TV block p_tv
+        self.tv = float(tv)
+        self.tv_prob = (1.0 / (self.M + 1) if tv_prob is None else float(tv_prob)) if self.tv > 0 else 0.0
+        if self.tv > 0 and not 0 < self.tv_prob < 1:
+            raise ValueError("tv_prob must be in (0, 1)")
+        self._n_tv = max(1, int(round(self.M * self.tv_prob / (1 - self.tv_prob)))) if self.tv > 0 else 0
+        self._p_data = (1.0 - self.tv_prob) / self.M
+        self._inv_p_data = float(self.M) if self.tv == 0 else 1.0 / self._p_data
         self._last = None
         self.support_mask = self._support(support)
         self.state = None
+        # HIP graphs of the subset steps: the launches of a step of subset m followed by the forward of subset m' are
+        # captured the second time the pair (m, m') occurs and replayed from then on (every array of the step is
+        # allocated before; a reallocated operator scratch buffer drops the captured graphs). Only on the current
+        # stream (operator queue None).
         self.graphs = False
+        self._graph_cache = {}
+        self._graph_seen = set()
+        self._graph_gen = None
 
     def _support(self, support):
         """The support option as a (Ny, Nx) uint8 device mask or None (optimization/prox.py:75-99)."""
@@ -105,7 +119,15 @@ class SPDHGPoisson:
         return [int(m) for m in perm]
 
     def _epoch_sequence(self):
-        return self._epoch_order()
+        """The blocks of the next epoch (spdhg_poisson.py:372-381): the subsets and, with tv, the TV block (-1) at
+        _n_tv random places (without tv exactly the random numbers of _epoch_order)."""
+        order = self._epoch_order()
+        if self.tv <= 0:
+            return order
+        n = len(order) + self._n_tv
+        tv_at = set(int(i) for i in self.rng.choice(n, self._n_tv, replace=False))
+        it = iter(order)
+        return [-1 if i in tv_at else next(it) for i in range(n)]
 
     # ---------------------------------------------------------------- reductions
     def _partials(self):
@@ -149,6 +171,41 @@ class SPDHGPoisson:
         """D (rows of subset m, or all rows for m None) = A x."""
         self.op.direct_k(self.xk, self.D, subset=m)
 
+    def _tv_update(self):
+        """The TV block (spdhg_poisson.py:462-475): q <- proj(q + s D x), dz = D^T dq, z += dz, x updated with
+        1/p_tv."""
+        op = self.op
+        st = op._stream()
+        _lib.call("dfm_tv_dual_k", _lib.ptr(self.xk), _lib.ptr(self.Tk), _lib.ptr(self.qxk), _lib.ptr(self.qyk),
+                  _lib.ptr(self._dqx), _lib.ptr(self._dqy), op.Nx, op.Ny, op.K_stride, float(self.tv),
+                  float(np.float32(TV_SAFETY * self.tv_prob)), st)
+        _lib.call("dfm_tv_adjoint_update_k", _lib.ptr(self._dqx), _lib.ptr(self._dqy), _lib.ptr(self.xk),
+                  _lib.ptr(self.zk), _lib.ptr(self.Tk), op.Nx, op.Ny, op.K_stride, float(np.float32(1.0 / self.tv_prob)),
+                  float(self.lam), st)
+
+    def _tv_value(self):
+        op = self.op
+        n = ctypes.c_int32(0)
+        _lib.call("dfm_tv_value_partial", _lib.ptr(self.xk), _lib.ptr(self.Tk), op.Nx, op.Ny, op.K_stride,
+                  _lib.ptr(self._part), ctypes.byref(n), op._stream())
+        return float(self._part[: n.value].sum().item())
+
+    def _fused_step(self, m, mn):
+        """The adjoint update of subset m and the forward of subset mn with one projector launch per batch (the PF
+        adjoint, the fused backprojection + primal update + forward projection, the PF forward). False (nothing
+        done) when the fused projector does not apply to this operator."""
+        op = self.op
+        cw = op.fused_width(len(self.subsets[m]))
+        if cw == 0 or any(bt["koff"] % cw or bt["kstride"] % cw for bt in op.batches):
+            return False
+        sc = op._scratch_buffers(max(len(self.subsets[m]), len(self.subsets[mn])))
+        for ib, bt in enumerate(op.batches):
+            op._pf_adjoint(ib, self.D, sc["sino"], m)
+            op._spdhg_fused(sc["sino"], m, mn, self.xk, self.zk, self.Tk, self._inv_p_data, self.lam, bt["koff"],
+                            bt["kstride"], cw)
+            op._pf_forward(ib, sc["sino"], self.D, mn, accumulate=ib > 0)
+        return True
+
     # ---------------------------------------------------------------- run
     def start(self, x0, b, weights=None, gain=None, background=None, state=None):
         """Set-up of a run (spdhg_poisson.py:529-662): inputs on the device, rho, the step sizes S and T, x on the
@@ -157,6 +214,8 @@ class SPDHGPoisson:
         op = self.op
         dev = op.device
         cshape, dshape = tuple(op.coeff_shape), tuple(op.data_shape)
+        if self.tv > 0 and state is not None and state.get("z") is None:
+            raise ValueError("continuing a TV run needs the state's z")
         self._x_on_gpu = isinstance(x0, torch.Tensor)
         self._host_x = None if self._x_on_gpu else as_host(x0, cshape, "x0")
         self._x_dev = check_device(x0, cshape, "x0", dev) if self._x_on_gpu else \
@@ -224,24 +283,72 @@ class SPDHGPoisson:
                 op.to_k(zc, self.zk)
             else:
                 op.adjoint_k(self.y, self.zk)
+        if self.tv > 0:
+            self.qxk = torch.zeros(nk, dtype=torch.float32, device=dev)
+            self.qyk = torch.zeros(nk, dtype=torch.float32, device=dev)
+            for nm, qk in (("qx", self.qxk), ("qy", self.qyk)):
+                if state is not None and state.get(nm) is not None:
+                    op.to_k(torch.from_numpy(as_host(state[nm], cshape, f"state['{nm}']")).to(dev), qk)
+            self._dqx = torch.empty(nk, dtype=torch.float32, device=dev)
+            self._dqy = torch.empty(nk, dtype=torch.float32, device=dev)
+            self.resident_used = NAMES + TV_NAMES
         self._ep0 = int(state["epoch"]) if state is not None else 0
         self._seq = self._epoch_sequence()
         self._pos = 0
-        op.zero_subset_rows(self.D, self._seq[0])
-        self._forward(self._seq[0])
+        if self._seq[0] >= 0:
+            op.zero_subset_rows(self.D, self._seq[0])
+            self._forward(self._seq[0])
         self._t_start = t_start
         self.t_setup = time.perf_counter() - t_start
 
     def step(self, next_block=None, full_forward=False):
-        """One subset step of the current block: dual update, adjoint + primal update, and the forward of
-        next_block (or of every row with full_forward) into D."""
+        """One step of the current block (spdhg_poisson.py:430-484 _pass): a data subset's dual update, adjoint and
+        primal update, or the TV block (-1); then the forward of next_block when it is a data subset (or of every
+        row with full_forward) into D."""
         blk = self._seq[self._pos]
-        self._dual(blk)
-        self._adjoint_update(blk)
+        m_next = next_block if next_block is not None and next_block >= 0 else None
+        if self.graphs and self.op.queue is None and not full_forward and m_next is not None:
+            self._graph_step(blk, m_next)
+            return
+        self._step_launches(blk, m_next, full_forward)
+
+    def _graph_step(self, blk, m_next):
+        gen = getattr(self.op, "scratch_generation", 0)
+        if gen != self._graph_gen:
+            self._graph_cache.clear()
+            self._graph_seen.clear()
+            self._graph_gen = gen
+        key = (blk, m_next)
+        g = self._graph_cache.get(key)
+        if g is None and key in self._graph_seen:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._step_launches(blk, m_next, False)
+            if getattr(self.op, "scratch_generation", 0) != gen:    # a buffer was allocated while capturing
+                self._graph_cache.clear()
+                self._graph_seen.clear()
+                self._graph_gen = getattr(self.op, "scratch_generation", 0)
+                self._step_launches(blk, m_next, False)
+                return
+            self._graph_cache[key] = g
+        if g is None:
+            self._graph_seen.add(key)
+            self._step_launches(blk, m_next, False)
+        else:
+            g.replay()
+
+    def _step_launches(self, blk, m_next, full_forward):
+        if blk >= 0:
+            self._dual(blk)
+            if not full_forward and m_next is not None and self._fused_step(blk, m_next):
+                return
+            self._adjoint_update(blk)
+        else:
+            self._tv_update()
         if full_forward:
             self._forward(None)
-        elif next_block is not None:
-            self._forward(next_block)
+        elif m_next is not None:
+            self._forward(m_next)
 
     def advance(self, n_steps):
         """n subset steps in the epoch order (no objective evaluations): each step ends with the forward of the next
@@ -249,12 +356,12 @@ class SPDHGPoisson:
         for _ in range(int(n_steps)):
             if self._pos + 1 < len(self._seq):
                 nb = self._seq[self._pos + 1]
-                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self.step(nb, full_forward=nb >= 0 and nb == self._seq[self._pos])
                 self._pos += 1
             else:
                 nseq = self._epoch_sequence()
                 nb = nseq[0]
-                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self.step(nb, full_forward=nb >= 0 and nb == self._seq[self._pos])
                 self._seq, self._pos = nseq, 0
 
     def finish(self):
@@ -268,6 +375,11 @@ class SPDHGPoisson:
         torch.cuda.synchronize(dev)
         self.state = {"epoch": self._ep0 + len(getattr(self, "iter_stats", [])), "z": zc.cpu().numpy(),
                       "y": self.y.cpu().numpy()}
+        if self.tv > 0:
+            for nm, qk in (("qx", self.qxk), ("qy", self.qyk)):
+                op.from_k(qk, zc)
+                self.state[nm] = zc.cpu().numpy()
+            self.qxk = self.qyk = self._dqx = self._dqy = None
         if not self._x_on_gpu:
             self._host_x[...] = self._x_dev.cpu().numpy()
         out = self._x_dev if self._x_on_gpu else self._host_x
@@ -302,11 +414,15 @@ class SPDHGPoisson:
                     nb = nseq[0]
                 else:
                     nb = None
-                if (end and want_f) or nb is None or nb == blk:
-                    full = (end and want_f) or nb is not None
+                adj = blk if blk >= 0 else None
+                m_next = nb if nb is not None and nb >= 0 else None
+                if (end and want_f) or nb is None or (m_next is not None and m_next == adj):
+                    full = (end and want_f) or m_next is not None
                     self.step(None, full_forward=full)
                     if end and want_f:
                         f = self._deviance(self.D)
+                        if self.tv > 0:
+                            f += self.tv * self._tv_value()
                 else:
                     self.step(nb)
             if nseq is not None:
@@ -329,7 +445,8 @@ class SPDHGPoisson:
         out = self.finish()
         self.final_stats = {"epochs": len(self.iter_stats), "final_f": f, "setup_s": self.t_setup,
                             "rho": self.rho_used, "subsets": self.M, "resident": self.resident_used,
-                            "time": time.perf_counter() - self._t_start, "tv": 0.0, "tv_prob": 0.0, **self.scales}
+                            "time": time.perf_counter() - self._t_start, "tv": self.tv, "tv_prob": self.tv_prob,
+                            **self.scales}
         return out
 
     def _zero_padding(self, a_k):

@@ -160,6 +160,22 @@ int dfm_dual_update_sub(float* y, float* d, const float* sig, const float* b, co
 int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
                              int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap, float* x,
                              float* z, const float* T, float inv_p, float lam, dfm_stream_t stream);
+/* Projections with a channel group's whole image in LDS (radon_kernels.cl:44-141 radon_forward_k /
+ * radon_backward_k; spdhg_poisson.py:95-106 spdhg_update; spdhg_poisson.py:430-484 _pass for the fused step). A
+ * workgroup holds cw channels (4, 2 or 1) of all nx * ny pixels: nx ny cw <= 40960 and ns cw <= 40960;
+ * dfm_projector_lds_width returns the largest cw that fits (0: none). 2 taps only (pixels no larger than a bin).
+ * mode 0: x[p, k] = backward(sino_b) over the na_b angles of geo_b (backward records of dfm_radon_backward);
+ * mode 1: the SPDHG primal update of (x, z) with step T, as dfm_radon_backward_spdhg;
+ * mode 2: sino_f[a, s, k] = scale wscale_a sum_p K(t_a(p) - s) x[p, k] over the na_f angles of geo_f (forward records
+ *         of dfm_radon_forward);
+ * mode 3: mode 1, then mode 2 on the updated x (one SPDHG step's projector work). sino_f may be sino_b (each
+ *         workgroup reads its channels of sino_b before it writes them).
+ * x, z, T: (Npix, kstride) channels k_off .. k_off + k_n - 1; sinograms (angles, ns, skstride) channels 0 .. k_n - 1. */
+int dfm_projector_lds_width(int32_t nx, int32_t ny, int32_t ns);
+int dfm_projector_lds(int32_t mode, const float* sino_b, const void* geo_b, int32_t na_b, float* x, float* z,
+                      const float* T, float inv_p, float lam, float* sino_f, const void* geo_f, int32_t na_f,
+                      float scale, int32_t nx, int32_t ny, int32_t ns, int32_t kstride, int32_t k_off, int32_t k_n,
+                      int32_t skstride, int32_t cw, dfm_stream_t stream);
 /* The plain update (spdhg_poisson.py:95-106) on n elements: z += dz; x = T > 0 ? max(0, x - T (z + inv_p dz + lam)) : 0 */
 int dfm_spdhg_update(float* x, float* z, const float* T, const float* dz, float inv_p, float lam, int64_t n,
                      dfm_stream_t stream);
@@ -194,6 +210,19 @@ int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t
                              int32_t* n_partial, dfm_stream_t stream);
 int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
                     dfm_stream_t stream);
+/* Total-variation block of SPDHG (spdhg_poisson.py:108-193 tv_dual_dense / tv_adjoint_dense / tv_value_dense) on the
+ * channel-fastest arrays (Npix, kstride), pixel p = ix + nx iy; a forward difference (right, down) is free when T > 0
+ * at both ends.
+ * dual:  per pixel s = ps / (4 max over its free differences of (T_j + T_j')); q = proj_{|q| <= lam}(q + s D x) on
+ *        the free differences, 0 elsewhere; dq = q' - q.
+ * adjoint_update: dz = D^T dq; z += dz; x = T > 0 ? max(0, x - T (z + inv_p dz + lam)) : 0.
+ * value: float64 partial sums of |D x| over the free differences (n_partial <= DFM_MAX_PARTIALS). */
+int dfm_tv_dual_k(const float* x, const float* T, float* qx, float* qy, float* dqx, float* dqy, int32_t nx,
+                  int32_t ny, int32_t kstride, float lam, float ps, dfm_stream_t stream);
+int dfm_tv_adjoint_update_k(const float* dqx, const float* dqy, float* x, float* z, const float* T, int32_t nx,
+                            int32_t ny, int32_t kstride, float inv_p, float lam, dfm_stream_t stream);
+int dfm_tv_value_partial(const float* x, const float* T, int32_t nx, int32_t ny, int32_t kstride, double* partial,
+                         int32_t* n_partial, dfm_stream_t stream);
 /* a *= s (single_phase_forward_operator.py:2169, 2192 x *= 1/|x|). */
 int dfm_scale(float* a, float s, int64_t n, dfm_stream_t stream);
 

@@ -83,10 +83,21 @@ def test_subset_forward_and_adjoint(pf_mode, subsets):
                    orc.adjoint(y, subset=0)) < 2e-5
 
 
-def test_subsets_need_sparse():
+def test_dense_fp32_subsets():
+    """The reference raises NotImplementedError for subsets of a dense-PF operator
+    (single_phase_forward_operator.py:1564-1566); here the dense FP32 path supports them: subset rows equal the
+    full rows."""
+    import torch
     op, _, _ = build(pf_mode="dense", oracle=False)
-    with pytest.raises(NotImplementedError):
-        op.set_omega_subsets(2)
+    subs = op.set_omega_subsets(2)
+    x, _ = _inputs(op)
+    xt = torch.from_numpy(x).cuda()
+    full = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
+    op.direct_cl(xt, full)
+    d = torch.zeros_like(full)
+    op.direct_cl(xt, d, subset=1)
+    rows = torch.from_numpy(subs[1]).cuda()
+    assert torch.allclose(d[rows], full[rows], rtol=1e-6, atol=1e-6 * float(full.abs().max()))
 
 
 def test_power_iteration_matches_oracle():
@@ -110,3 +121,24 @@ def test_errors_raise_like_the_reference():
         op.set_omega_subsets(0)
     with pytest.raises(ValueError):
         op.direct_cl(torch.zeros(op.coeff_shape, device="cuda"), torch.zeros(op.data_shape, device="cuda"), subset=0)
+
+
+@pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("arith", 4)])
+@pytest.mark.parametrize("shape", ["square", "chunked"])
+def test_projector_kernels_match_oracle(kernel, channels, shape):
+    """Every projector kernel choice (LDS-resident image at 4, 2, 1 channels per workgroup; geometry tables;
+    footprint arithmetic) against the oracle; "chunked": 120 angles x 103 bins, so the LDS backprojection stages the
+    sinogram in two angle chunks at 4 channels."""
+    print(f"case kernel={kernel} channels={channels} shape={shape}", flush=True)
+    if shape == "square":
+        c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
+    else:
+        c = cfg(nx=16, ny=16, my=103, n_omega=120, angle_range=(0, 180), cor=0.3, n_eta=12)
+    op, orc, _ = build(c, K=8, pf_mode="sparse")
+    op.radon_variant["fwd_kernel"] = op.radon_variant["bwd_kernel"] = kernel
+    op.fused_channels = channels
+    if kernel == "lds":
+        assert op.lds_width() == channels
+    x, y = _inputs(op, 5)
+    assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
+    assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5

@@ -65,3 +65,79 @@ def test_spdhg_resume_and_torch_x0():
     three = SPDHGPoisson(op, subsets=3, seed=2)
     three.run(x.copy(), b, 1, state=st)
     assert three.state["epoch"] == 7
+
+
+@pytest.mark.parametrize("channels,batches", [(4, 1), (2, 1), (1, 1), (4, 3)])
+def test_fused_projector_step_matches_separate_kernels(channels, batches):
+    """The fused projector step (backprojection + primal update + next forward projection in one launch, at 4, 2
+    or 1 channels per workgroup, one or several orientation batches) gives the iterates of the separate table
+    kernels: 6 epochs, relative L2 within 1e-5; both paths are checked against the FP64 oracle above."""
+    from diffractom_amd import SPDHGPoisson
+    print(f"case channels={channels} batches={batches}", flush=True)
+    op, _, b = _problem(K=24)
+    if batches > 1:
+        op2, _, _ = build(cfg(nx=16, ny=16, my=17, n_omega=24, angle_range=(0, 180), cor=0.0), K=24, sigma_deg=4.0,
+                          pf_mode="sparse", max_gb=8.5 * 24 * 36 * 3 * 4 / 1024 ** 3, oracle=False)
+        assert len(op2.batches) == batches
+        op = op2
+    xs = []
+    for fuse in (True, False):
+        op.fuse_projector = fuse
+        op.fused_channels = channels
+        sp = SPDHGPoisson(op, subsets=4, seed=3)
+        xs.append(sp.run(np.zeros(op.coeff_shape, np.float32), b, 6, diagnostics_interval=3))
+        if fuse:
+            assert op.fused_width(6) == channels
+    op.fuse_projector = True
+    assert rel_l2(xs[0], xs[1]) < 1e-5
+
+
+@pytest.mark.parametrize("tv,tv_prob", [(0.5, None), (2.0, 0.4)])
+def test_spdhg_tv_matches_oracle(tv, tv_prob):
+    """The TV block (spdhg_poisson.py:108-193): the same block sequence from the same seed, iterates within 1e-4
+    (relative L2) of the FP64 oracle after 6 epochs (the oracle's TV is pinned against an independent
+    Chambolle-Pock in test_oracle_tv.py), objectives (deviance + tv TV) within 1e-5, TV duals within 1e-4; a run
+    continued from solver.state equals one run."""
+    from diffractom_amd import SPDHGPoisson
+    print(f"case tv={tv} tv_prob={tv_prob}", flush=True)
+    op, orc, b = _problem()
+    sp = SPDHGPoisson(op, subsets=3, seed=2, tv=tv, tv_prob=tv_prob)
+    x = sp.run(np.zeros(op.coeff_shape, np.float32), b, 6, diagnostics_interval=3)
+    ref = OracleSPDHG(orc, subsets=3, seed=2, tv=tv, tv_prob=tv_prob)
+    x_ref = ref.run(np.zeros(op.coeff_shape), b, 6, diagnostics_interval=3)
+    assert rel_l2(x, x_ref) < 1e-4
+    f = [s["f"] for s in sp.iter_stats if s["f"] is not None]
+    f_ref = [s["f"] for s in ref.iter_stats if s["f"] is not None]
+    np.testing.assert_allclose(f, f_ref, rtol=1e-5)
+    for nm in ("qx", "qy"):
+        assert rel_l2(sp.state[nm], ref.state[nm]) < 1e-4
+    # resume (tests/test_spdhg_tv.py): 4 + 2 epochs continued from solver.state (z, y, qx, qy) equal 6 epochs
+    two = SPDHGPoisson(op, subsets=3, seed=2, tv=tv, tv_prob=tv_prob)
+    x2 = two.run(np.zeros(op.coeff_shape, np.float32), b, 4, diagnostics_interval=4)
+    x2 = two.run(x2, b, 2, diagnostics_interval=2, state=two.state)
+    np.testing.assert_allclose(x2, x, rtol=1e-5, atol=1e-6 * np.abs(x).max())
+    with pytest.raises(ValueError):
+        SPDHGPoisson(op, subsets=3, tv=tv).run(x2, b, 1, state={"y": two.state["y"], "epoch": 6})
+
+
+@pytest.mark.parametrize("tv", [0.0, 0.5])
+def test_graph_replay_matches_eager(tv):
+    """Subset steps replayed from captured HIP graphs (captured at the second occurrence of a (subset, next subset)
+    pair, replayed from then on, eager first occurrences, epoch-end full forwards and objectives in between): the
+    iterate after every epoch equals the eager run's bit for bit."""
+    from diffractom_amd import SPDHGPoisson
+    print(f"case graphs tv={tv}", flush=True)
+    op, _, b = _problem(K=16)
+    xs = {}
+    for graphs in (False, True):
+        sp = SPDHGPoisson(op, subsets=4, seed=5, tv=tv)
+        sp.graphs = graphs
+        seen = []
+        sp.run(np.zeros(op.coeff_shape, np.float32), b, 12, diagnostics_interval=3,
+               callback=lambda e, x, st: seen.append(np.array(x, copy=True)) and False)
+        xs[graphs] = seen
+        if graphs:
+            assert len(sp._graph_cache) > 0
+    assert len(xs[True]) == len(xs[False]) == 12
+    for a, c in zip(xs[True], xs[False]):
+        assert np.array_equal(a, c)
