@@ -45,6 +45,7 @@ def parse():
                     help="PF product of the headline figure (the other precisions run too, for the comparison)")
     ap.add_argument("--only", action="store_true", help="run the headline precision only")
     ap.add_argument("--no-op-timing", action="store_true", help="skip the full forward / adjoint timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants (no timing pass)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (the reference kernel of the FETCH_SIZE / WRITE_SIZE "
                          "counter runs that give roofline.traffic)")
@@ -161,6 +162,9 @@ def run_path(args, precision, counts=None):
     setup = time.perf_counter() - t0
     log(f"[bench] {precision}: K={op.K} PF {op.pf_mode} ({op.pf_storage}), {op.device_bytes() / 1e9:.1f} GB, "
         f"set-up {setup:.1f} s")
+    if not args.no_autotune:
+        tuning = op.autotune(subset=0)
+        log(f"[bench] {precision}: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
     solver = SPDHGPoisson(op, subsets=M, seed=0)
     solver.graphs = args.graphs
     solver.start(np.zeros(op.coeff_shape, np.float32), b)
@@ -188,7 +192,7 @@ def run_path(args, precision, counts=None):
     bg = torch.from_numpy(np.asarray(b, np.float32)).cuda()
     data = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
     out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
-    tf, ta = [float("nan")] * 3, [float("nan")] * 3
+    tf, ta = [], []
     for _ in range(0 if args.no_op_timing else 12):
         a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         a0.record()
@@ -199,8 +203,9 @@ def run_path(args, precision, counts=None):
         torch.cuda.synchronize()
         tf.append(a0.elapsed_time(a1))
         ta.append(a1.elapsed_time(a2))
-    res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b, "ktimes": ktimes, "setup_s": setup,
-           "t_fwd": float(np.median(tf[2:])), "t_adj": float(np.median(ta[2:])),
+    res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b,
+           "tuning": {k: v[0] for k, v in (op.tuning or {}).items()}, "ktimes": ktimes, "setup_s": setup,
+           "t_fwd": float(np.median(tf[2:])) if tf else None, "t_adj": float(np.median(ta[2:])) if ta else None,
            "fwd_out": data, "adj_out": out, "x_iter": x_iter, "steps_total": args.warmup + 2 * args.steps}
     return res
 
@@ -234,8 +239,8 @@ def main():
     ref = results.get("fp32")
     for q, r in results.items():
         e = {"steps_per_s": round(1000.0 / r["ms_step"], 3), "ms_per_step": round(r["ms_step"], 4),
-             "forward_ms": round(r["t_fwd"], 4), "adjoint_ms": round(r["t_adj"], 4),
-             "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}}
+             "forward_ms": r["t_fwd"] and round(r["t_fwd"], 4), "adjoint_ms": r["t_adj"] and round(r["t_adj"], 4),
+             "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}, "tuned": r["tuning"]}
         if ref is not None and q != "fp32":
             e["rel_l2_vs_fp32"] = {"forward": rel_l2(r["fwd_out"], ref["fwd_out"]),
                                    "adjoint": rel_l2(r["adj_out"], ref["adj_out"]),
@@ -287,8 +292,8 @@ def main():
                    "K": op.K, "subsets": M, "precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "forward_GBps": round(fbytes / (head["t_fwd"] * 1e-3) / 1e9, 1),
-        "adjoint_GBps": round(fbytes / (head["t_adj"] * 1e-3) / 1e9, 1),
+        "forward_GBps": head["t_fwd"] and round(fbytes / (head["t_fwd"] * 1e-3) / 1e9, 1),
+        "adjoint_GBps": head["t_adj"] and round(fbytes / (head["t_adj"] * 1e-3) / 1e9, 1),
         "paths": paths,
     }
     print(json.dumps(line), flush=True)

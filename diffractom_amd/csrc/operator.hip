@@ -278,6 +278,159 @@ __global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* 
   st_stream(sino + (int64_t)r * skstride4 + kq, acc);
 }
 
+// The same product with the workgroup's table entries staged in LDS: the 16 bins of a workgroup are consecutive
+// rows of the CSR, i.e. one contiguous entry range, loaded once with coalesced loads (FT_CHUNK entries per pass) and
+// then read from LDS by all 16 channel quads of a bin (a broadcast); the vector memory path then carries the image
+// gathers only (with per-lane entry loads it also carries every entry 16 times).
+constexpr int FT_CHUNK = 2048;
+
+template <int UNR>
+__global__ void __launch_bounds__(256) radon_forward_table_lds_kernel(const float4* __restrict__ img,
+                                                                      float4* __restrict__ sino,
+                                                                      const int* __restrict__ row_ptr,
+                                                                      const int* __restrict__ ent_pix,
+                                                                      const float* __restrict__ ent_w, int ns, int na,
+                                                                      int kstride4, int k4_off, int k4_n,
+                                                                      int skstride4) {
+  __shared__ int lp[FT_CHUNK];
+  __shared__ float lw[FT_CHUNK];
+  const int sblocks = (ns + 15) / 16;
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, local;
+  if (!xcd_split(na * sblocks, n_blk, blk, local)) return;       // uniform over the workgroup
+  const int a = local / sblocks;
+  const int s0 = (local % sblocks) * 16;
+  const int s = s0 + (threadIdx.x >> 4);
+  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const bool live = kq < k4_n && s < ns;
+  const int g0 = row_ptr[a * ns + s0];
+  const int g1 = row_ptr[a * ns + min(s0 + 16, ns)];
+  const int r = a * ns + min(s, ns - 1);
+  const int e0 = live ? row_ptr[r] : 0;
+  const int e1 = live ? row_ptr[r + 1] : 0;
+  const float4* col = img + (live ? kq : 0) + k4_off;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int c0 = g0; c0 < g1; c0 += FT_CHUNK) {
+    const int c1 = min(c0 + FT_CHUNK, g1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < c1 - c0; i += 256) {
+      lp[i] = ent_pix[c0 + i];
+      lw[i] = ent_w[c0 + i];
+    }
+    __syncthreads();
+    const int en = min(e1, c1);
+    int e = max(e0, c0);
+    for (; e + UNR <= en; e += UNR) {
+      int p[UNR];
+      float w[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        p[u] = lp[e + u - c0];
+        w[u] = lw[e + u - c0];
+      }
+      float4 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) v[u] = col[(int64_t)p[u] * kstride4];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        acc.x += w[u] * v[u].x;
+        acc.y += w[u] * v[u].y;
+        acc.z += w[u] * v[u].z;
+        acc.w += w[u] * v[u].w;
+      }
+    }
+    for (; e < en; ++e) {
+      const float w = lw[e - c0];
+      const float4 v = col[(int64_t)lp[e - c0] * kstride4];
+      acc.x += w * v.x;
+      acc.y += w * v.y;
+      acc.z += w * v.z;
+      acc.w += w * v.w;
+    }
+  }
+  if (live) st_stream(sino + (int64_t)r * skstride4 + kq, acc);
+}
+
+// Forward from the bin-pair table: a thread per (channel quad, bin pair 2q, 2q + 1), a workgroup 16 quads x 16 pairs
+// of one angle, entries staged in LDS as above. A pixel whose two footprint taps fall in one pair is gathered once
+// for both bins, so a projection gathers about 1.5 instead of 2 image values per pixel and angle.
+template <int UNR>
+__global__ void __launch_bounds__(256) radon_forward_pairs_kernel(const float4* __restrict__ img,
+                                                                  float4* __restrict__ sino,
+                                                                  const int* __restrict__ pair_ptr,
+                                                                  const int* __restrict__ ent_pix,
+                                                                  const float2* __restrict__ ent_w, int ns, int na,
+                                                                  int kstride4, int k4_off, int k4_n, int skstride4) {
+  __shared__ int lp[FT_CHUNK];
+  __shared__ float2 lw[FT_CHUNK];
+  const int np2 = (ns + 1) / 2;
+  const int qblocks = (np2 + 15) / 16;
+  const int n_blk = (k4_n + QB - 1) / QB;
+  int blk, local;
+  if (!xcd_split(na * qblocks, n_blk, blk, local)) return;       // uniform over the workgroup
+  const int a = local / qblocks;
+  const int q0 = (local % qblocks) * 16;
+  const int q = q0 + (threadIdx.x >> 4);
+  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const bool live = kq < k4_n && q < np2;
+  const int g0 = pair_ptr[a * np2 + q0];
+  const int g1 = pair_ptr[a * np2 + min(q0 + 16, np2)];
+  const int r = a * np2 + min(q, np2 - 1);
+  const int e0 = live ? pair_ptr[r] : 0;
+  const int e1 = live ? pair_ptr[r + 1] : 0;
+  const float4* col = img + (live ? kq : 0) + k4_off;
+  float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
+  for (int c0 = g0; c0 < g1; c0 += FT_CHUNK) {
+    const int c1 = min(c0 + FT_CHUNK, g1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < c1 - c0; i += 256) {
+      lp[i] = ent_pix[c0 + i];
+      lw[i] = ent_w[c0 + i];
+    }
+    __syncthreads();
+    const int en = min(e1, c1);
+    int e = max(e0, c0);
+    for (; e + UNR <= en; e += UNR) {
+      int p[UNR];
+      float2 w[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        p[u] = lp[e + u - c0];
+        w[u] = lw[e + u - c0];
+      }
+      float4 v[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) v[u] = col[(int64_t)p[u] * kstride4];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        a0.x += w[u].x * v[u].x;
+        a0.y += w[u].x * v[u].y;
+        a0.z += w[u].x * v[u].z;
+        a0.w += w[u].x * v[u].w;
+        a1.x += w[u].y * v[u].x;
+        a1.y += w[u].y * v[u].y;
+        a1.z += w[u].y * v[u].z;
+        a1.w += w[u].y * v[u].w;
+      }
+    }
+    for (; e < en; ++e) {
+      const float2 w = lw[e - c0];
+      const float4 v = col[(int64_t)lp[e - c0] * kstride4];
+      a0.x += w.x * v.x;
+      a0.y += w.x * v.y;
+      a0.z += w.x * v.z;
+      a0.w += w.x * v.w;
+      a1.x += w.y * v.x;
+      a1.y += w.y * v.y;
+      a1.z += w.y * v.z;
+      a1.w += w.y * v.w;
+    }
+  }
+  if (!live) return;
+  st_stream(sino + ((int64_t)a * ns + 2 * q) * skstride4 + kq, a0);
+  if (2 * q + 1 < ns) st_stream(sino + ((int64_t)a * ns + 2 * q + 1) * skstride4 + kq, a1);
+}
+
 // Backward from the tables, one 4 x 4 pixel tile x 16 channel quads per workgroup: the TB bins of every angle the
 // tile reaches are staged in LDS (TB x 16 quads x 16 B per angle, angles in chunks of ACH), then every thread (pixel,
 // quad) reads its two taps from LDS. MODE 0 writes the image, MODE 1 applies the SPDHG primal update.
@@ -293,6 +446,8 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
                                                                   float4* __restrict__ z, const float4* __restrict__ T,
                                                                   float inv_p, float lam) {
   __shared__ float4 lds[ACH * TB * QB];   // ACH angles x TB bins x 16 quads
+  __shared__ int4 lpt[ACH * 16];          // the pixel table of the tile's 16 pixels for the ACH angles
+  __shared__ int lbase[ACH];
   const int tnx = (nx + 3) / 4, tny = (ny + 3) / 4;
   const int ntiles = tnx * tny;
   const int n_blk = (k4_n + QB - 1) / QB;
@@ -307,15 +462,32 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
   const int64_t pix = (int64_t)py * nx + px;
   const int64_t npix = (int64_t)nx * ny;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  // the update's operands are fetched first, so their HBM latency overlaps the sinogram work
+  const int64_t o = pix * kstride4 + kq + k4_off;
+  float4 zz = acc, xx = acc, tt = acc;
+  if (MODE == 1 && live) {
+    zz = ld_stream(z + o);
+    xx = ld_stream(x + o);
+    tt = ld_stream(T + o);
+  }
   for (int a0 = 0; a0 < na; a0 += ACH) {
     const int nan = min(ACH, na - a0);
+    __syncthreads();
+    // the chunk's tile bases and pixel-table rows, once per workgroup (not once per channel quad)
+    if (threadIdx.x < nan) lbase[threadIdx.x] = tbase[(int64_t)(a0 + threadIdx.x) * ntiles + tile];
+    for (int i = threadIdx.x; i < nan * 16; i += 256) {
+      const int al = i >> 4, l = i & 15;
+      const int ppx = (tile % tnx) * 4 + (l & 3), ppy = (tile / tnx) * 4 + (l >> 2);
+      lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)(a0 + al) * npix + (int64_t)ppy * nx + ppx]
+                                      : make_int4(0, 0, 0, 0);
+    }
     __syncthreads();
     // stage: element i = (angle al, bin b, quad qq)
     for (int i = threadIdx.x; i < nan * TB * QB; i += 256) {
       const int qq = i & (QB - 1);
       const int b = (i >> 4) % TB;
       const int al = i / (TB * QB);
-      const int s = tbase[(int64_t)(a0 + al) * ntiles + tile] + b;
+      const int s = lbase[al] + b;
       const int kk = blk * QB + qq;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (s >= 0 && s < ns && kk < k4_n) v = sino[((int64_t)(a0 + al) * ns + s) * skstride4 + kk];
@@ -325,8 +497,8 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
     if (live) {
 #pragma unroll 4
       for (int al = 0; al < nan; ++al) {
-        const int4 e = ptab[(int64_t)(a0 + al) * npix + pix];
-        const int base = tbase[(int64_t)(a0 + al) * ntiles + tile];
+        const int4 e = lpt[al * 16 + li];
+        const int base = lbase[al];
         const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
         const float4 s0 = lds[(al * TB + (e.x - base)) * QB + q];
         const float4 s1 = lds[(al * TB + (e.y - base)) * QB + q];
@@ -338,12 +510,9 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
     }
   }
   if (!live) return;
-  const int64_t o = pix * kstride4 + kq + k4_off;
   if (MODE == 0) {
     st_stream(out + o, acc);
   } else {
-    float4 zz = ld_stream(z + o), xx = ld_stream(x + o);
-    const float4 tt = ld_stream(T + o);
     float4 zn, xn;
 #define DFM_UPD(c)                                                                  \
   zn.c = zz.c + acc.c;                                                              \
@@ -703,22 +872,62 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
   DFM_CHECK_ARG(ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
                     k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
                 "dfm_radon_forward_table: bad sizes");
-  DFM_CHECK_ARG(variant == 1 || variant == 4 || variant == 8, "dfm_radon_forward_table: variant must be 1, 4 or 8");
+  // variant = unr + 16 * cap: entries fetched ahead (1, 4, 8) and, cap > 0, at most cap workgroups per CU (an
+  // otherwise unused LDS allocation of 160 KB / cap), so that fewer channel blocks share an XCD's L2
+  // + 256: the workgroup's entries staged in LDS
+  const int unr = variant % 16, cap = (variant / 16) % 16, staged = variant / 256;
+  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && (staged == 0 || staged == 1),
+                "dfm_radon_forward_table: variant %d: entries ahead 1, 4 or 8 (+ 16 * cap, cap 0 or 3 .. 15, + 256 "
+                "for LDS-staged entries)", variant);
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const unsigned grid = xcd_grid(n_angles * ((ns + 15) / 16), (k4n + QB - 1) / QB);
+  const size_t lds_cap = cap > 0 ? (size_t)(163840 / cap) & ~(size_t)255 : 0;
 #define DFM_FT(U)                                                                                                 \
-  hipLaunchKernelGGL(radon_forward_table_kernel<U>, dim3(grid), dim3(256), 0, (hipStream_t)stream,               \
-                     (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,      \
-                     k_off / 4, k4n, skstride / 4)
-  if (variant == 1)
+  if (staged)                                                                                                     \
+    hipLaunchKernelGGL(radon_forward_table_lds_kernel<U>, dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,   \
+                       (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,    \
+                       k_off / 4, k4n, skstride / 4);                                                             \
+  else                                                                                                            \
+    hipLaunchKernelGGL(radon_forward_table_kernel<U>, dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,       \
+                       (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,    \
+                       k_off / 4, k4n, skstride / 4)
+  if (unr == 1) {
     DFM_FT(1);
-  else if (variant == 4)
+  } else if (unr == 4) {
     DFM_FT(4);
-  else
+  } else {
     DFM_FT(8);
+  }
 #undef DFM_FT
   DFM_LAUNCH_CHECK("dfm_radon_forward_table");
+  return DFM_OK;
+}
+
+int dfm_radon_forward_pairs(const float* img_k, float* sino, const int32_t* pair_ptr, const int32_t* ent_pix,
+                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
+                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(img_k && sino && pair_ptr && ent_pix && ent_w, "dfm_radon_forward_pairs: null pointer");
+  DFM_CHECK_ARG(ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
+                    k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
+                "dfm_radon_forward_pairs: bad sizes");
+  DFM_CHECK_ARG(variant == 1 || variant == 4 || variant == 8, "dfm_radon_forward_pairs: variant must be 1, 4 or 8");
+  if (n_angles == 0 || k_n == 0) return DFM_OK;
+  const int k4n = k_n / 4;
+  const unsigned grid = xcd_grid(n_angles * (((ns + 1) / 2 + 15) / 16), (k4n + QB - 1) / QB);
+#define DFM_FP(U)                                                                                                 \
+  hipLaunchKernelGGL(radon_forward_pairs_kernel<U>, dim3(grid), dim3(256), 0, (hipStream_t)stream,               \
+                     (const float4*)img_k, (float4*)sino, pair_ptr, ent_pix, (const float2*)ent_w, ns, n_angles,  \
+                     kstride / 4, k_off / 4, k4n, skstride / 4)
+  if (variant == 1) {
+    DFM_FP(1);
+  } else if (variant == 4) {
+    DFM_FP(4);
+  } else {
+    DFM_FP(8);
+  }
+#undef DFM_FP
+  DFM_LAUNCH_CHECK("dfm_radon_forward_pairs");
   return DFM_OK;
 }
 

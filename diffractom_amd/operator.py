@@ -110,6 +110,18 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale):
     order = np.lexsort((pix, rows))
     row_ptr = np.zeros(na * ns + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=na * ns), out=row_ptr[1:])
+    # bin-pair table: rows (angle, q) for the bins 2q, 2q + 1; an entry (pixel, weight to 2q, weight to 2q + 1), a
+    # pixel whose two taps fall in one pair listed once
+    npairs = -(-ns // 2)
+    bins = np.concatenate([np.broadcast_to(s_, t.shape)[v_] for s_, v_ in zip(taps, valid)])
+    arow = rows // ns
+    key = (arow * npairs + bins // 2) * npix + pix
+    ukey, inv = np.unique(key, return_inverse=True)
+    pw = np.zeros((len(ukey), 2))
+    np.add.at(pw, (inv, bins % 2), wts)
+    prow = ukey // npix
+    pair_ptr = np.zeros(na * npairs + 1, np.int64)
+    np.cumsum(np.bincount(prow, minlength=na * npairs), out=pair_ptr[1:])
     sc = [np.clip(s_, 0, ns - 1) for s_ in taps]
     wb = [np.where(v_, bwd_scale * k_, 0.0) for k_, v_ in zip(kern, valid)]
     ptab = np.empty((na, npix, 4), np.int32)
@@ -127,7 +139,9 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale):
     if np.any(top_ - base >= 8):
         raise AssertionError("a 4 x 4 pixel tile reaches more than 8 bins at some angle")
     return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32),
-            "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32)}
+            "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
+            "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
+            "pair_w": pw.astype(np.float32)}
 
 
 class SinglePhaseForwardOperator:
@@ -344,7 +358,8 @@ class SinglePhaseForwardOperator:
         # kernel variants (identical results; set by autotune_projector)
         # kernel choices (identical results up to float32 rounding): "lds" (the image of a channel group resident in
         # LDS, when it fits), "table" (geometry tables, 2 taps), "arith" (footprint arithmetic, any taps)
-        self.radon_variant = {"fwd": 8, "bwd": 8, "fwd_kernel": "table", "bwd_kernel": "table"}
+        self.radon_variant = {"fwd": 8 + 256, "bwd": 4, "fwd_kernel": "table", "bwd_kernel": "table", "fwd_pairs": 8}
+        self.tuning = None
         self.fuse_projector = False
         self.fused_channels = 4
         self._tables_full = self._radon_tables(np.arange(self.N_Omega)) if self.n_taps == 2 else None
@@ -625,6 +640,98 @@ class SinglePhaseForwardOperator:
                       kstride, koff, kn, kn, self.n_taps, _lib.ptr(x), _lib.ptr(z), _lib.ptr(T), float(inv_p),
                       float(lam), st)
 
+    # ================================================================== autotuner
+    def autotune(self, subset=0, reps=5):
+        """Pick each product's fastest kernel variant by timing it on this operator's shapes (HIP events, median of
+        reps after 2 warm-up launches), as the reference times its tiled and plain projectors
+        (parallel_radon.py:240-281). Every candidate computes the same values (up to float32 summation order).
+        Candidates: forward projection (geometry table with 4 or 8 entries fetched ahead, staged in LDS or not,
+        capped occupancy; bin-pair table; LDS-resident image), backprojection with the SPDHG update (4, 8 or 16
+        angles staged per pass; LDS-resident image), the matrix-core PF products (2 or 4 output tiles per wave).
+        Returns {product: (choice, {candidate: ms})} and keeps it in self.tuning."""
+        if self.omega_subsets is None:
+            self.set_omega_subsets(10)
+        n_ang, _, geo_f, geo_b = self._angle_set(subset)
+        tab = self._tables(subset)
+        sc = self._scratch_buffers(n_ang)
+        b = self.batches[0]
+        dev = self.device
+        npix = self.Nx * self.Ny
+        g = torch.Generator(device=dev)
+        g.manual_seed(0)
+        x = torch.rand(npix * self.K_stride, generator=g, device=dev)
+        z = torch.zeros_like(x)
+        T = torch.zeros_like(x)
+
+        def timeit(fn):
+            ts = []
+            for i in range(reps + 2):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fn()
+                e1.record()
+                torch.cuda.synchronize(dev)
+                if i >= 2:
+                    ts.append(e0.elapsed_time(e1))
+            return float(np.median(ts))
+
+        out = {}
+        rv = self.radon_variant
+        saved = dict(rv)
+        if tab is not None:
+            cands = {"table:8": ("table", 8), "table:8+lds": ("table", 8 + 256), "table:4+lds": ("table", 4 + 256),
+                     "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "pairs:8": ("pairs", 8)}
+            if self.lds_width():
+                cands["lds"] = ("lds", None)
+            times = {}
+            for name, (kern, v) in cands.items():
+                rv["fwd_kernel"] = kern
+                if kern == "table":
+                    rv["fwd"] = v
+                elif kern == "pairs":
+                    rv["fwd_pairs"] = v
+                times[name] = timeit(lambda: self._radon_forward(x, self.K_stride, b["koff"], b["kstride"],
+                                                                 sc["sino"], geo_f, n_ang, tab))
+            best = min(times, key=times.get)
+            kern, v = cands[best]
+            saved["fwd_kernel"] = kern
+            if kern == "table":
+                saved["fwd"] = v
+            elif kern == "pairs":
+                saved["fwd_pairs"] = v
+            out["radon_forward"] = (best, times)
+            cands = {f"table:{a}": ("table", a) for a in (4, 8, 16)}
+            if self.lds_width():
+                cands["lds"] = ("lds", None)
+            times = {}
+            for name, (kern, v) in cands.items():
+                rv["bwd_kernel"] = kern
+                if v is not None:
+                    rv["bwd"] = v
+                times[name] = timeit(lambda: self._radon_backward(sc["sino"], geo_b, tab, n_ang, None, self.K_stride,
+                                                                  b["koff"], b["kstride"], x=x, z=z, T=T, inv_p=1.0))
+            best = min(times, key=times.get)
+            kern, v = cands[best]
+            saved["bwd_kernel"] = kern
+            if v is not None:
+                saved["bwd"] = v
+            out["radon_backward_spdhg"] = (best, times)
+        self.radon_variant.update(saved)
+        if self.mfma_batches is not None:
+            data = torch.rand(self.data_shape, generator=g, device=dev)
+            for key, fn in (("fwd", lambda: self._dense_forward(0, sc["sino"], data, subset, False)),
+                            ("adj", lambda: self._dense_adjoint(0, data, sc["sino"], subset, 1.0))):
+                times = {}
+                for v in (2, 4):
+                    self.mfma_variant[key] = v
+                    times[f"tiles_per_wave:{v}"] = timeit(fn)
+                best = min(times, key=times.get)
+                self.mfma_variant[key] = int(best.split(":")[1])
+                out[f"pf_{'forward' if key == 'fwd' else 'adjoint'}"] = (best, times)
+        del x, z, T
+        self.tuning = out
+        return out
+
     def lds_width(self):
         """Channels per workgroup of the LDS-resident projector (4, 2, 1; at most fused_channels), or 0 when the
         image does not fit the LDS or the projector needs more than 2 taps."""
@@ -661,6 +768,11 @@ class SinglePhaseForwardOperator:
 
     # ================================================================== forward
     def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles, tab=None):
+        if tab is not None and self.radon_variant["fwd_kernel"] == "pairs":
+            _lib.call("dfm_radon_forward_pairs", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["pair_ptr"]),
+                      _lib.ptr(tab["pair_pix"]), _lib.ptr(tab["pair_w"]), self.My, n_angles, img_kstride, koff, kn,
+                      kn, self.radon_variant["fwd_pairs"], self._stream())
+            return
         cw = self.lds_width() if self.radon_variant["fwd_kernel"] == "lds" else 0
         if cw and koff % cw == 0 and kn % cw == 0:
             self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,

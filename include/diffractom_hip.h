@@ -72,10 +72,17 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  *   weights, wscale_a folded in; 0 for taps off the detector);
  * tile bases tbase (n_angles, ceil(nx/4) * ceil(ny/4)): the first of the 8 bins every 4 x 4 pixel tile reaches.
  * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg.
- * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8; backward: angles
- * staged in LDS per pass, 4, 8, 16 or 32), picked by timing (the reference times its tiled / plain projectors the
- * same way, parallel_radon.py:240-281). */
+ * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8, plus 16 * cap
+ * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS;
+ * backward: angles staged in LDS per pass, 4, 8, 16 or 32), picked by timing (the reference times its tiled / plain
+ * projectors the same way, parallel_radon.py:240-281). */
 int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
+                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
+                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
+/* Forward from the bin-pair table (the same projection, radon_kernels.cl:44-101): rows (angle, q) for the bins 2q and
+ * 2q + 1, pair_ptr (n_angles * ceil(ns/2) + 1), entries (pixel, weight to 2q, weight to 2q + 1) with ent_w float2; a
+ * pixel whose two taps fall in one pair appears once. variant: entries fetched ahead, 1, 4 or 8. */
+int dfm_radon_forward_pairs(const float* img_k, float* sino, const int32_t* pair_ptr, const int32_t* ent_pix,
                             const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
                             int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
 int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,

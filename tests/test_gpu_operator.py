@@ -123,7 +123,8 @@ def test_errors_raise_like_the_reference():
         op.direct_cl(torch.zeros(op.coeff_shape, device="cuda"), torch.zeros(op.data_shape, device="cuda"), subset=0)
 
 
-@pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("arith", 4)])
+@pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("table_staged", 4),
+                                             ("table_capped", 4), ("table_pairs", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
     """Every projector kernel choice (LDS-resident image at 4, 2, 1 channels per workgroup; geometry tables;
@@ -135,10 +136,37 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
     else:
         c = cfg(nx=16, ny=16, my=103, n_omega=120, angle_range=(0, 180), cor=0.3, n_eta=12)
     op, orc, _ = build(c, K=8, pf_mode="sparse")
-    op.radon_variant["fwd_kernel"] = op.radon_variant["bwd_kernel"] = kernel
+    op.radon_variant["fwd_kernel"] = op.radon_variant["bwd_kernel"] = kernel.split("_")[0]
+    if kernel == "table_staged":                 # table entries staged in LDS (FT_CHUNK-entry passes)
+        op.radon_variant["fwd"] = 8 + 256
+    elif kernel == "table_capped":               # at most 6 workgroups per CU, 4 entries ahead
+        op.radon_variant["fwd"] = 4 + 16 * 6
+    elif kernel == "table_pairs":                # the bin-pair table forward
+        op.radon_variant["fwd_kernel"] = "pairs"
     op.fused_channels = channels
     if kernel == "lds":
         assert op.lds_width() == channels
     x, y = _inputs(op, 5)
     assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_residual"])
+def test_autotune_keeps_results(precision):
+    """The autotuner times every candidate of each product and keeps the fastest; the operator then still matches the
+    oracle (every candidate computes the same values)."""
+    print(f"case autotune {precision}", flush=True)
+    c = cfg(nx=20, ny=20, my=21, n_omega=30, angle_range=(0, 180), cor=0.5)
+    kw = {"pf_mode": "sparse"} if precision == "fp32" else {"pf_mode": "dense", "pf_precision": precision}
+    op, orc, _ = build(c, K=24, **kw)
+    op.set_omega_subsets(5)
+    tuning = op.autotune(subset=1, reps=2)
+    assert "radon_forward" in tuning and "radon_backward_spdhg" in tuning
+    if precision != "fp32":
+        assert "pf_forward" in tuning and "pf_adjoint" in tuning
+    for name, (choice, times) in tuning.items():
+        assert choice in times and all(t > 0 for t in times.values())
+    x, y = _inputs(op, 7)
+    tol = 2e-5 if precision == "fp32" else 3e-5
+    assert rel_max(op.direct(x), orc.direct(x)) < tol
+    assert rel_max(op.adjoint(y), orc.adjoint(y)) < tol
