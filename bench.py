@@ -37,7 +37,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="al_c1")
+    ap.add_argument("--config", default="al_c1", help="al_c1 (the headline line), al_c0, or al_c3 (the multi-slice "
+                    "batch: slices/s and the step-time percentiles)")
+    ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
+    ap.add_argument("--batch", type=int, default=32, help="al_c3: slices per batched run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512, help="orientations of the host-core baseline sample")
     ap.add_argument("--no-kernel-timing", action="store_true")
@@ -210,6 +213,69 @@ def run_path(args, precision, counts=None):
     return res
 
 
+def run_c3(args):
+    """The multi-slice workload: the slices' SPDHG steps batched (BatchedSPDHGPoisson: every kernel of a step launched
+    once for all slices of a group), plain BF16 PF products (the residual mode's fragments would exceed the 288 GB of
+    HBM). Groups of --batch slices run one after the other (64 slices of K ~ U(500, 8000) need about 260 GB at
+    once). Prints one JSON line: slices per second (a slice = its `steps` subset steps) and the p50 / p95 of the
+    batched step time."""
+    import torch
+    from diffractom_amd import SinglePhaseForwardOperator
+    from diffractom_amd.batched import BatchedSPDHGPoisson
+    from diffractom_amd.workloads import C3, c3_slices, poisson_counts
+    t0 = time.perf_counter()
+    cfg, mat, slices = c3_slices(args.slices)
+    n_all = len(slices)
+    ms_all, Ks = [], []
+    run_s = 0.0
+    for g0 in range(0, n_all, args.batch):
+        ops, bs, masks = [], [], []
+        for i in range(g0, min(g0 + args.batch, n_all)):
+            grid, mask, truth = slices[i]
+            op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="dense",
+                                            pf_precision="bf16")
+            b, _ = poisson_counts(op.direct(truth), C3["mean_counts"], C3["seed"] + i)
+            ops.append(op)
+            bs.append(b)
+            masks.append(mask)
+            slices[i] = None
+        Ks += [op.K for op in ops]
+        log(f"[bench] al_c3: slices {g0}..{g0 + len(ops) - 1} set up ({time.perf_counter() - t0:.0f} s, "
+            f"{torch.cuda.memory_allocated() / 1e9:.0f} GB)")
+        bat = BatchedSPDHGPoisson(ops, subsets=C3["subsets"], support=masks, seed=0)
+        bat.start(None, bs)
+        torch.cuda.synchronize()
+        bat.advance(args.warmup)
+        torch.cuda.synchronize()
+        evs = []
+        for _ in range(args.steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            bat.advance(1)
+            e1.record()
+            evs.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b in evs]
+        ms_all += ms
+        run_s += sum(ms) * 1e-3
+        log(f"[bench] al_c3: group of {len(ops)} slices (K sum {sum(op.K for op in ops)}): "
+            f"{np.mean(ms):.3f} ms per batched step")
+        del bat, ops, bs
+        torch.cuda.empty_cache()
+    ms = np.array(ms_all)
+    line = {"metric": "SPDHG multi-slice batch: slices/s", "value": round(n_all / run_s, 4),
+            "unit": f"slices/s ({args.steps} subset steps per slice)", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(float(ms.mean()), 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (jittered-grain grids, Voronoi grains in disc masks, Poisson counts)",
+            "config": {"workload": f"al_c3: {n_all} slices in groups of {args.batch}, 128x128 px, 180 omega x 131 "
+                                   f"x 180 eta x 5 rings, K {min(Ks)}..{max(Ks)}, disc masks r 15..60 px, M=10",
+                       "slices": n_all, "K_sum": int(sum(Ks))},
+            "step_ms_p50": round(float(np.percentile(ms, 50)), 4), "step_ms_p95": round(float(np.percentile(ms, 95)), 4),
+            "slice_steps_per_s": round(n_all * args.steps / run_s, 1)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -218,6 +284,9 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
+    if args.config == "al_c3":
+        run_c3(args)
+        return
     # fp32 first: its counts are the problem every precision solves
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]
     results = {}

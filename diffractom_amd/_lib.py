@@ -56,6 +56,7 @@ SIGNATURES = {
     "dfm_mfma_pack_fwd": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_pack_adj": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_forward": [P, I32, I32, P, P, P, I32, I32, I32, I32, I32, P, P, I32, I32, I32, P],
+    "dfm_mfma_grouped": [I32, P, P, I64, I32, I32, P],
     "dfm_mfma_adjoint": [P, P, P, P, I32, I32, I32, I32, I32, P, I32, F32, I32, P],
 }
 

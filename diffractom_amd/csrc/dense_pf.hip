@@ -113,6 +113,7 @@ __global__ void pack_adj_kernel(const float* __restrict__ dense, bf16x8* __restr
 // operand (mt_n 16-row tiles x 32, FP32 in HBM) is loaded once by the workgroup, split into bf16 hi (+ lo) and
 // stored in LDS in fragment order (double-buffered); each wave reads its A fragments from LDS (one ds_read_b128 per
 // lane) and streams its PF fragments from HBM one chunk ahead.
+// The same layout as dfm_mfma_desc of the C ABI (the grouped launch reads an array of them).
 struct MfmaArgs {
   const float* src;        // forward: sino (n_ang, my, sks); adjoint: data (R, my, cp)
   const bf16x8* hi;
@@ -125,15 +126,16 @@ struct MfmaArgs {
   int ksplit, ch_per;      // chunks split over ksplit workgroups (forward)
   int direct, accumulate;
   float alpha;
+  int ncols;               // adjoint: sinogram columns written (< ncols; the row stride is sks)
 };
+static_assert(sizeof(MfmaArgs) == sizeof(dfm_mfma_desc), "MfmaArgs and dfm_mfma_desc differ");
 
+// Workgroup w of one product (the decomposition below); lds_hi / lds_lo: 2 * MT_MAX * 64 fragments each.
 template <int NT, bool RES, bool FWD>
-__global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
-  __shared__ bf16x8 lds_hi[2][MT_MAX * 64];
-  __shared__ bf16x8 lds_lo[RES ? 2 : 1][RES ? MT_MAX * 64 : 1];
+__device__ __forceinline__ void mfma_pf_body(const MfmaArgs& p, int64_t w, bf16x8* __restrict__ lds_hi,
+                                             bf16x8* __restrict__ lds_lo) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  int64_t w = xcd_block();
   const int tg = (int)(w % p.n_tg);
   w /= p.n_tg;
   const int yg = (int)(w % p.n_yg);
@@ -198,8 +200,8 @@ __global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
       if (s < n_slots) {
         bf16x8 h, l;
         split8<RES>(raw[i], h, l);
-        lds_hi[buf][s] = h;
-        if (RES) lds_lo[buf][s] = l;
+        lds_hi[buf * MT_MAX * 64 + s] = h;
+        if (RES) lds_lo[buf * MT_MAX * 64 + s] = l;
       }
     }
   };
@@ -232,8 +234,8 @@ __global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
 #pragma unroll
     for (int mt = 0; mt < MT_MAX; ++mt) {
       if (mt < p.mt_n) {
-        const bf16x8 ah = lds_hi[cur][mt * 64 + lane];
-        const bf16x8 al = RES ? lds_lo[RES ? cur : 0][RES ? mt * 64 + lane : 0] : zero8;
+        const bf16x8 ah = lds_hi[cur * MT_MAX * 64 + mt * 64 + lane];
+        const bf16x8 al = RES ? lds_lo[cur * MT_MAX * 64 + mt * 64 + lane] : zero8;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           acc[mt][t] = mma(ah, bh[t], acc[mt][t]);
@@ -279,11 +281,33 @@ __global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
             p.out[(((int64_t)ks * p.n_ang + a) * p.my + y) * p.cp + col] = v;
           }
         } else {
-          if (col < p.sks) p.out[((int64_t)a * p.my + y) * p.sks + col] = p.alpha * v;
+          if (col < p.ncols) p.out[((int64_t)a * p.my + y) * p.sks + col] = p.alpha * v;
         }
       }
     }
   }
+}
+
+template <int NT, bool RES, bool FWD>
+__global__ __launch_bounds__(256, 2) void mfma_pf_kernel(MfmaArgs p) {
+  __shared__ bf16x8 lds_hi[2 * MT_MAX * 64];
+  __shared__ bf16x8 lds_lo[RES ? 2 * MT_MAX * 64 : 1];
+  mfma_pf_body<NT, RES, FWD>(p, xcd_block(), lds_hi, lds_lo);
+}
+
+// Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (products of different K, PF, column
+// offsets and data rows in one launch; entries with x < 0 pad the grid).
+template <int NT, bool RES, bool FWD>
+__global__ __launch_bounds__(256, 2) void mfma_pf_grouped_kernel(const MfmaArgs* __restrict__ descs,
+                                                                 const int2* __restrict__ wg_map, int64_t n_map) {
+  __shared__ bf16x8 lds_hi[2 * MT_MAX * 64];
+  __shared__ bf16x8 lds_lo[RES ? 2 * MT_MAX * 64 : 1];
+  const int64_t w = xcd_block();
+  if (w >= n_map) return;
+  const int2 m = wg_map[w];
+  if (m.x < 0) return;
+  const MfmaArgs p = descs[m.x];
+  mfma_pf_body<NT, RES, FWD>(p, m.y, lds_hi, lds_lo);
 }
 
 // Sum of the k-split partial results into the data rows of the angles (fixed order: deterministic).
@@ -300,6 +324,19 @@ __global__ void splitk_reduce_kernel(const float* __restrict__ part, float* __re
     const int r = omegas ? omegas[a] : a;
     float* d = data + (int64_t)r * row_len + rem;
     *d = accumulate ? *d + s : s;
+  }
+}
+
+template <bool FWD>
+void launch_grouped(const MfmaArgs* descs, const int2* wg_map, int64_t n_map, int variant, bool res,
+                    hipStream_t st) {
+  const dim3 grid((unsigned)((n_map + N_XCD - 1) / N_XCD * N_XCD)), block(256);
+  if (variant == 4) {
+    if (res) hipLaunchKernelGGL((mfma_pf_grouped_kernel<4, true, FWD>), grid, block, 0, st, descs, wg_map, n_map);
+    else hipLaunchKernelGGL((mfma_pf_grouped_kernel<4, false, FWD>), grid, block, 0, st, descs, wg_map, n_map);
+  } else {
+    if (res) hipLaunchKernelGGL((mfma_pf_grouped_kernel<2, true, FWD>), grid, block, 0, st, descs, wg_map, n_map);
+    else hipLaunchKernelGGL((mfma_pf_grouped_kernel<2, false, FWD>), grid, block, 0, st, descs, wg_map, n_map);
   }
 }
 
@@ -420,8 +457,24 @@ int dfm_mfma_adjoint(const float* data, const void* hi, const void* lo, const in
   p.ksplit = 1;
   p.ch_per = n_jc;
   p.alpha = alpha;
+  p.ncols = sks;
   launch_mfma<false>(p, variant, lo != nullptr, (int64_t)n_ang * p.n_yg * p.n_tg, (hipStream_t)stream);
   DFM_LAUNCH_CHECK("dfm_mfma_adjoint");
+  return DFM_OK;
+}
+
+int dfm_mfma_grouped(int32_t forward, const dfm_mfma_desc* descs, const int32_t* wg_map, int64_t n_wg,
+                     int32_t residual, int32_t variant, dfm_stream_t stream) {
+  DFM_CHECK_ARG(descs && wg_map && n_wg > 0, "dfm_mfma_grouped: null pointer or no workgroups");
+  DFM_CHECK_ARG(variant == 2 || variant == 4, "dfm_mfma_grouped: variant (tiles per wave) must be 2 or 4");
+  DFM_CHECK_ARG(n_wg <= ((int64_t)1 << 31) - 8, "dfm_mfma_grouped: %lld workgroups", (long long)n_wg);
+  if (forward)
+    launch_grouped<true>((const MfmaArgs*)descs, (const int2*)wg_map, n_wg, variant, residual != 0,
+                         (hipStream_t)stream);
+  else
+    launch_grouped<false>((const MfmaArgs*)descs, (const int2*)wg_map, n_wg, variant, residual != 0,
+                          (hipStream_t)stream);
+  DFM_LAUNCH_CHECK("dfm_mfma_grouped");
   return DFM_OK;
 }
 

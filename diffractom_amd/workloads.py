@@ -125,3 +125,47 @@ def make_grid(name, K=None):
             R = R[:K]
     return MatrixGrid(R, np.deg2rad(p["sigma_deg"])), al_material(), scan_cfg(p["n"], p["my"], p["n_omega"],
                                                                                 p["n_eta"])
+
+
+C3 = dict(n=128, my=131, n_omega=180, n_eta=180, sigma_deg=2.0, slices=64, K=(500, 8000), radius=(15.0, 60.0),
+          jitter_deg=1.0, grains_per=50, mean_counts=1000.0, seed=3, subsets=10)
+
+
+def jittered_grid(K, n_base, jitter_deg, symmetry, seed):
+    """K orientations: n_base random fundamental-zone orientations (the grains) and their copies rotated by random
+    axes through normally distributed angles of jitter_deg (the first n_base rows are the grains themselves)."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(seed)
+    base = random_fz(n_base, symmetry, int(rng.integers(1 << 31)))
+    reps = rng.integers(0, n_base, K - n_base)
+    axes = rng.standard_normal((K - n_base, 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    ang = np.deg2rad(jitter_deg) * rng.standard_normal(K - n_base)
+    J = Rotation.from_rotvec(axes * ang[:, None]).as_matrix()
+    return np.concatenate([base, np.einsum("kij,kjl->kil", J, base[reps])])
+
+
+def c3_slices(n_slices=None, seed=None):
+    """The multi-slice workload: slices of 128 x 128 pixels (Al, 180 omega x 131 translations x 180 eta x 5 rings),
+    slice i with K_i ~ U(500, 8000) orientations (grains with 1-degree jittered copies), a disc support of radius
+    r_i ~ U(15, 60) pixels and a Voronoi truth of its grains inside the disc. Returns (cfg, material, [(grid, support
+    mask, truth), ...])."""
+    p = C3
+    n_slices = p["slices"] if n_slices is None else int(n_slices)
+    rng = np.random.default_rng(p["seed"] if seed is None else seed)
+    cfg = scan_cfg(p["n"], p["my"], p["n_omega"], p["n_eta"])
+    n = p["n"]
+    yy, xx = np.mgrid[:n, :n]
+    c = (n - 1) / 2.0
+    out = []
+    for i in range(n_slices):
+        K = int(rng.integers(p["K"][0], p["K"][1] + 1))
+        r = float(rng.uniform(*p["radius"]))
+        n_base = max(1, K // p["grains_per"])
+        R = jittered_grid(K, n_base, p["jitter_deg"], "cubic", int(rng.integers(1 << 31)))
+        mask = ((xx - c) ** 2 + (yy - c) ** 2 <= r * r).astype(np.uint8)
+        n_grains = max(1, min(n_base, int(np.pi * r * r / 150)))
+        truth = voronoi_truth(K, n, n_grains, r, int(rng.integers(1 << 31)), coeff=(0.5, 1.5),
+                              orientations=rng.choice(n_base, n_grains, replace=False))
+        out.append((MatrixGrid(R, np.deg2rad(p["sigma_deg"])), mask, truth))
+    return cfg, al_material(), out

@@ -153,6 +153,32 @@ int dfm_mfma_forward(const float* sino, int32_t sks, int32_t kb, const void* hi,
 int dfm_mfma_adjoint(const float* data, const void* hi, const void* lo, const int32_t* omegas, int32_t n_ang,
                      int32_t my, int32_t cp, int32_t n_kt, int32_t n_jc, float* sino, int32_t sks, float alpha,
                      int32_t variant, dfm_stream_t stream);
+/* Grouped launch: several products of one direction (forward != 0: the forward) with different PF matrices, K,
+ * sinogram column offsets and data rows in one launch (slices of a multi-slice batch). descs: a device array of
+ * products; wg_map: a device array of n_wg int32 pairs (product, workgroup of that product); a product's workgroups
+ * are n_ang * n_yg * n_tg * ksplit, decomposed (tile group fastest, then y group, k split, angle). residual: the
+ * products have lo fragments. The fields (their meaning per direction as in dfm_mfma_forward / dfm_mfma_adjoint):
+ *   src: forward the sinogram (first column of the product), adjoint the data; hi, lo: fragments; omegas: the data
+ *   rows of the angles (NULL: 0..n_ang-1); out: forward data (direct = 1) or partial sums, adjoint the sinogram (first
+ *   column); sks: sinogram row stride; kb: forward columns read; n_tiles, n_chunks: as n_nt, n_kc / n_kt, n_jc;
+ *   mt_n, n_yg: 16-row y tiles per group and y groups (ceil(ceil(my/16)/8) groups); n_tg: groups of 4 * variant
+ *   output tiles; ch_per: chunks per k split; ncols: adjoint columns written. */
+typedef struct dfm_mfma_desc {
+  const float* src;
+  const void* hi;
+  const void* lo;
+  const int32_t* omegas;
+  float* out;
+  int32_t n_ang, my, cp, sks, kb;
+  int32_t n_tiles, n_chunks;
+  int32_t mt_n, n_yg, n_tg;
+  int32_t ksplit, ch_per;
+  int32_t direct, accumulate;
+  float alpha;
+  int32_t ncols;
+} dfm_mfma_desc;
+int dfm_mfma_grouped(int32_t forward, const dfm_mfma_desc* descs, const int32_t* wg_map, int64_t n_wg,
+                     int32_t residual, int32_t variant, dfm_stream_t stream);
 
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
