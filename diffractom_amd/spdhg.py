@@ -84,10 +84,11 @@ class SPDHGPoisson:
         self._last = None
         self.support_mask = self._support(support)
         self.state = None
-        # HIP graphs of the subset steps: the launches of a step of subset m followed by the forward of subset m' are
-        # captured the second time the pair (m, m') occurs and replayed from then on (every array of the step is
-        # allocated before; a reallocated operator scratch buffer drops the captured graphs). Only on the current
-        # stream (operator queue None).
+        # HIP graphs of the subset steps: the launches of a step are two graphs, ("update", m) = dual update,
+        # adjoint and primal update of subset m, and ("forward", m') = the forward of the next subset m' (with the
+        # fused projector the pair (m, m') is one graph); each is captured the second time it occurs and replayed
+        # from then on (every array of the step is allocated before; a reallocated operator scratch buffer drops the
+        # captured graphs). Only on the current stream (operator queue None).
         self.graphs = False
         self._graph_cache = {}
         self._graph_seen = set()
@@ -308,32 +309,36 @@ class SPDHGPoisson:
         blk = self._seq[self._pos]
         m_next = next_block if next_block is not None and next_block >= 0 else None
         if self.graphs and self.op.queue is None and not full_forward and m_next is not None:
-            self._graph_step(blk, m_next)
+            if blk >= 0 and self.op.fused_width(len(self.subsets[blk])):
+                self._graph_run(("step", blk, m_next), lambda: self._step_launches(blk, m_next, False))
+            else:
+                self._graph_run(("update", blk), lambda: self._step_launches(blk, None, False))
+                self._graph_run(("forward", m_next), lambda: self._forward(m_next))
             return
         self._step_launches(blk, m_next, full_forward)
 
-    def _graph_step(self, blk, m_next):
+    def _graph_run(self, key, launches):
+        """Run launches eagerly the first time key occurs, capture them the second time, replay them after."""
         gen = getattr(self.op, "scratch_generation", 0)
         if gen != self._graph_gen:
             self._graph_cache.clear()
             self._graph_seen.clear()
             self._graph_gen = gen
-        key = (blk, m_next)
         g = self._graph_cache.get(key)
         if g is None and key in self._graph_seen:
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._step_launches(blk, m_next, False)
+                launches()
             if getattr(self.op, "scratch_generation", 0) != gen:    # a buffer was allocated while capturing
                 self._graph_cache.clear()
                 self._graph_seen.clear()
                 self._graph_gen = getattr(self.op, "scratch_generation", 0)
-                self._step_launches(blk, m_next, False)
+                launches()
                 return
             self._graph_cache[key] = g
         if g is None:
             self._graph_seen.add(key)
-            self._step_launches(blk, m_next, False)
+            launches()
         else:
             g.replay()
 

@@ -170,3 +170,27 @@ def test_autotune_keeps_results(precision):
     tol = 2e-5 if precision == "fp32" else 3e-5
     assert rel_max(op.direct(x), orc.direct(x)) < tol
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < tol
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_residual"])
+def test_hexagonal_titanium_matches_oracle(precision):
+    """The C2 material: Ti (P6_3/mmc, a 2.95 A, c 4.68 A, |G| = 12), the 8 rings {10-10} ... {11-22} (planes as
+    three-index hkl), 360 degrees of omega, sigma 2.5 deg; forward and adjoint against the oracle (hexagonal pole
+    axes, the a* || x frame of the non-orthogonal cell)."""
+    from diffractom_amd import MatrixGrid, SinglePhaseForwardOperator
+    from diffractom_amd.workloads import TI_RINGS, random_fz, ti_material
+    from oracle.model import OracleOperator
+    print(f"case Ti {precision}", flush=True)
+    c = cfg(nx=16, ny=16, my=17, n_omega=24, angle_range=(0, 360), cor=0.4, n_eta=36, wavelength=0.2)
+    U = random_fz(10, "hexagonal", 4)
+    sigma = np.deg2rad(2.5)
+    kw = {"pf_mode": "sparse"} if precision == "fp32" else {"pf_mode": "dense", "pf_precision": precision}
+    op = SinglePhaseForwardOperator(cfg=c, material=ti_material(0.2), grid=MatrixGrid(U, sigma), max_gb=0.2,
+                                    normalized=True, **kw)
+    orc = OracleOperator(c, (2.95, 2.95, 4.68, 90, 90, 120), "hexagonal", TI_RINGS, 0.2, U, sigma,
+                         orthonormalize=False)
+    assert op.N_peaks == orc.P == 8 and op.data_shape == orc.data_shape
+    x, y = _inputs(op, 9)
+    tol = 2e-5 if precision == "fp32" else 3e-5
+    assert rel_max(op.direct(x), orc.direct(x)) < tol
+    assert rel_max(op.adjoint(y), orc.adjoint(y)) < tol
