@@ -37,8 +37,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="al_c1", help="al_c1 (the headline line), al_c0, or al_c3 (the multi-slice "
-                    "batch: slices/s and the step-time percentiles)")
+    ap.add_argument("--config", default="al_c1", help="al_c1 (the headline line), al_c0, al_c3 (the multi-slice "
+                    "batch: slices/s and the step-time percentiles), ti_c2 (hexagonal, 256^2 px, K 59,440, CSR), "
+                    "al_c4 (autograd loss + gradients, 8 slices)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
     ap.add_argument("--batch", type=int, default=32, help="al_c3: slices per batched run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -276,6 +277,117 @@ def run_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def run_c2(args):
+    """The large hexagonal workload (Ti, 256 x 256 px, 360 omega, K = 59,440, FP32 CSR PF of about 4 G non-zeros):
+    full forward / adjoint times and GB/s, then SPDHG subset steps per second. One JSON line."""
+    import torch
+    from diffractom_amd import SPDHGPoisson
+    from diffractom_amd.workloads import c2_problem
+    t0 = time.perf_counter()
+    op, xk, b, p = c2_problem()
+    torch.cuda.synchronize()
+    log(f"[bench] ti_c2: K={op.K} nnz={op.sparse_nnz} fill={op.sparse_fill:.4f} batches={len(op.batches)}, "
+        f"set-up {time.perf_counter() - t0:.1f} s, {torch.cuda.memory_allocated() / 1e9:.0f} GB")
+    npix = op.Nx * op.Ny
+    data = torch.empty(op.data_shape, dtype=torch.float32, device="cuda")
+    out = torch.empty_like(xk).view(-1)
+    tf, ta = [], []
+    for _ in range(3):
+        a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a0.record()
+        op.direct_k(xk.view(-1), data)
+        a1.record()
+        op.adjoint_k(data, out)
+        a2.record()
+        torch.cuda.synchronize()
+        tf.append(a0.elapsed_time(a1))
+        ta.append(a1.elapsed_time(a2))
+    del out
+    fbytes = 4.0 * op.K * npix + 4.0 * op.N_Omega * op.My * op.N_seg + 6.0 * op.sparse_nnz
+    log(f"[bench] ti_c2: forward {min(tf):.1f} ms, adjoint {min(ta):.1f} ms")
+    del xk
+    torch.cuda.empty_cache()
+    solver = SPDHGPoisson(op, subsets=p["subsets"], seed=0)
+    x0 = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
+    solver.start(x0, b)
+    torch.cuda.synchronize()
+    solver.advance(args.warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    solver.advance(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    line = {"metric": "SPDHG-Poisson iters/s (subset steps/s), FP32 CSR path, hexagonal Ti",
+            "value": round(1000.0 / ms, 4), "unit": "SPDHG subset steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (300 Voronoi grains + 10 % fibre background)",
+            "config": {"workload": f"ti_c2: Ti 8 rings, 360 omega x 261 x 360 eta, 256x256 px, K={op.K}, sigma 2.5 "
+                                   f"deg, M={p['subsets']}, FP32 CSR PF (nnz {op.sparse_nnz})",
+                       "K": op.K, "nnz": op.sparse_nnz},
+            "forward_ms": round(min(tf), 2), "adjoint_ms": round(min(ta), 2),
+            "forward_GBps": round(fbytes / (min(tf) * 1e-3) / 1e9, 1),
+            "adjoint_GBps": round(fbytes / (min(ta) * 1e-3) / 1e9, 1)}
+    print(json.dumps(line), flush=True)
+
+
+def run_c4(args):
+    """Differentiation at the C4 size: 8 slices of 128 x 128 px (180 omega x 131 x 180 eta x 5 rings, K = 5000,
+    sigma 2 deg, BF16 residual PF), loss = Poisson NLL(f_a * A c; b) + 1e-3 TV(c), gradients with respect to c and
+    the per-angle scales f_a ~ U(0.9, 1.1) through the autograd Function (backward = the HIP adjoint). One JSON line:
+    loss + gradient evaluations per second (all 8 slices)."""
+    import torch
+    from diffractom_amd import SinglePhaseForwardOperator
+    from diffractom_amd.differentiable import IntensityLayer, forward_project, poisson_nll, total_variation
+    from diffractom_amd.workloads import C3, jittered_grid, al_material, scan_cfg, poisson_counts, voronoi_truth
+    from diffractom_amd.grid import MatrixGrid
+    cfg = scan_cfg(128, 131, 180, 180)
+    rng = np.random.default_rng(4)
+    slices = []
+    for i in range(8):
+        R = jittered_grid(5000, 100, 1.0, "cubic", int(rng.integers(1 << 31)))
+        op = SinglePhaseForwardOperator(cfg, al_material(), MatrixGrid(R, np.deg2rad(2.0)), max_gb=1024.0,
+                                        normalized=True, pf_mode="dense", pf_precision="bf16_residual")
+        xt = voronoi_truth(op.K, 128, 60, 55.0, int(rng.integers(1 << 31)), coeff=(0.5, 1.5),
+                           orientations=rng.choice(100, 60, replace=False))
+        b, _ = poisson_counts(op.direct(xt), 1000.0, i)
+        lay = IntensityLayer(op.data_shape, op.N_peaks)
+        prm = lay.params0(device="cuda")
+        prm["angle_scale"] = torch.from_numpy(rng.uniform(0.9, 1.1, op.N_Omega)).cuda().requires_grad_(True)
+        c = torch.full(op.coeff_shape, 0.01, dtype=torch.float32, device="cuda").requires_grad_(True)
+        slices.append((op, lay, prm, c, torch.from_numpy(b).cuda()))
+    log("[bench] al_c4: 8 slices set up")
+
+    def evaluate():
+        total = 0.0
+        for op, lay, prm, c, bt in slices:
+            c.grad = None
+            prm["angle_scale"].grad = None
+            loss = poisson_nll(lay.forward(forward_project(c, op), prm), bt) + 1e-3 * total_variation(c)
+            loss.backward()
+            total = total + loss.detach()
+        return total
+
+    for _ in range(args.warmup):
+        evaluate()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        evaluate()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    line = {"metric": "loss + gradient evaluations/s (8 slices: Poisson NLL + 1e-3 TV, d/dc and d/df_a)",
+            "value": round(1000.0 / ms, 3), "unit": "evaluations/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16", "data": "synthetic (jittered-grain grids, Voronoi grains)",
+            "config": {"workload": "al_c4: 8 slices, 128x128 px, 180 omega x 131 x 180 eta x 5 rings, K=5000, "
+                                   "BF16 residual PF, autograd through the HIP adjoint"}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -284,8 +396,8 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
-    if args.config == "al_c3":
-        run_c3(args)
+    if args.config in ("al_c3", "ti_c2", "al_c4"):
+        {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4}[args.config](args)
         return
     # fp32 first: its counts are the problem every precision solves
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]

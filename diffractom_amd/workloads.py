@@ -169,3 +169,56 @@ def c3_slices(n_slices=None, seed=None):
                               orientations=rng.choice(n_base, n_grains, replace=False))
         out.append((MatrixGrid(R, np.deg2rad(p["sigma_deg"])), mask, truth))
     return cfg, al_material(), out
+
+
+C2 = dict(n=256, my=261, n_omega=360, n_eta=360, sigma_deg=2.5, grains=300, fibre=0.1, fibre_deg=10.0,
+          mean_counts=500.0, seed=2, subsets=10, radius=120.0)
+
+
+def c2_problem(device="cuda", max_gb=20.0):
+    """The large hexagonal workload: Ti (8 rings), 360 omega over 180 deg x 261 translations x 360 eta, 256 x 256
+    pixels, a uniform 3-degree hexagonal fundamental-zone grid (59,440 orientations), sigma 2.5 deg, sparse CSR PF
+    (the dense BF16 fragments would need about 250 GB). Truth, in the orientation-fastest layout on the device: 300
+    Voronoi grains in a disc of radius 120 px (coefficients U(0.5, 1.5)) plus a fibre background (orientations with
+    the c axis within 10 deg of the sample z axis) carrying 10 % of the coefficient sum. Returns (operator, truth
+    x_k, counts b as a device tensor, parameters)."""
+    import torch
+    from .operator import SinglePhaseForwardOperator
+    p = C2
+    R = uniform_fz(3.0, "hexagonal", seed=p["seed"], thin=0.6)
+    cfg = scan_cfg(p["n"], p["my"], p["n_omega"], p["n_eta"])
+    op = SinglePhaseForwardOperator(cfg, ti_material(), MatrixGrid(R, np.deg2rad(p["sigma_deg"])), max_gb=max_gb,
+                                    normalized=True, pf_mode="sparse")
+    K, n = op.K, p["n"]
+    rng = np.random.default_rng(p["seed"])
+    c = (n - 1) / 2.0
+    r = p["radius"] * np.sqrt(rng.random(p["grains"]))
+    t = rng.uniform(0, 2 * np.pi, p["grains"])
+    sx, sy = c + r * np.cos(t), c + r * np.sin(t)
+    ori = rng.choice(K, p["grains"], replace=False)
+    val = rng.uniform(0.5, 1.5, p["grains"])
+    yy, xx = np.mgrid[:n, :n]
+    inside = ((xx - c) ** 2 + (yy - c) ** 2 <= p["radius"] ** 2).ravel()
+    pix = np.nonzero(inside)[0]
+    cell = np.argmin((xx.ravel()[pix, None] - sx) ** 2 + (yy.ravel()[pix, None] - sy) ** 2, axis=1)
+    caxis = np.asarray(op.grid_rot_cpu)[:, :, 2]                      # crystal c axis in the sample frame
+    fibre = np.nonzero(np.abs(caxis[:, 2]) >= np.cos(np.deg2rad(p["fibre_deg"])))[0]
+    bg = p["fibre"] / (1 - p["fibre"]) * val.mean() / max(len(fibre), 1)
+    xk = torch.zeros((n * n, op.K_stride), dtype=torch.float32, device=device)
+    pt = torch.from_numpy(pix).to(device)
+    kcol = torch.from_numpy(ori[cell]).to(device)           # batch 0 starts at column 0; later batches offset below
+    off = np.zeros(K, np.int64)
+    for bt in op.batches:
+        off[bt["k_start"]:bt["k_end"]] = bt["koff"] - bt["k_start"]
+    col = lambda k: torch.from_numpy(np.asarray(k) + off[np.asarray(k)]).to(device)
+    xk[pt, col(ori[cell])] = torch.from_numpy(val[cell].astype(np.float32)).to(device)
+    fcol = col(fibre)
+    xk[pt[:, None], fcol[None, :]] += float(bg)
+    del kcol
+    data = torch.empty(op.data_shape, dtype=torch.float32, device=device)
+    op.direct_k(xk.view(-1), data)
+    ax = data.double()
+    s = p["mean_counts"] / float(ax.mean())
+    g = torch.Generator(device=device).manual_seed(p["seed"])
+    b = torch.poisson(torch.clamp(ax * s, min=0), generator=g).float()
+    return op, xk, b, p
