@@ -33,6 +33,10 @@ SIGNATURES = {
     "dfm_pf_fill_fwd": [P, P, P, P, I32, I32, I32, I32, P],
     "dfm_pf_fill_adj": [P, P, P, P, I32, I32, I32, I32, P],
     "dfm_spmm_forward": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
+    "dfm_spmm_forward_split": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P, I32, I32, P],
+    "dfm_spmm_t": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P],
+    "dfm_spmm_t_rows_per_block": [I32],
+    "dfm_transpose_sum": [P, P, P, I32, I32, I32, I32, I64, I64, I64, I64, I32, I64, I32, P],
     "dfm_spmm_adjoint": [P, P, P, P, P, P, I32, I32, I32, I32, I32, F32, I32, P],
     "dfm_zero_rows": [P, P, I32, I64, P],
     "dfm_dual_update_sub": [P, P, P, P, P, I64, P, I64, P, I64, F32, P, I32, I64, P],

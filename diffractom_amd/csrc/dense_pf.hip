@@ -361,6 +361,17 @@ void y_tiling(int my, int& mt_n, int& n_yg) {
 
 }  // namespace
 
+namespace dfm {
+int launch_splitk_reduce(const float* part, float* data, const int* omegas, int n_ang, int64_t row_len, int ksplit,
+                         int accumulate, hipStream_t st) {
+  const int64_t total = (int64_t)n_ang * row_len;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_stride_blocks(total, 256, 1u << 16)), dim3(256), 0, st, part, data,
+                     omegas, n_ang, row_len, ksplit, accumulate);
+  DFM_LAUNCH_CHECK("k-split reduce");
+  return DFM_OK;
+}
+}  // namespace dfm
+
 extern "C" {
 
 int dfm_mfma_pack_fwd(const float* dense, void* hi, void* lo, int R, int km, int kb, int cp, int n_nt, int n_kc,

@@ -41,6 +41,10 @@ __device__ __forceinline__ void st_stream(float4* p, const float4& v) {
   __builtin_nontemporal_store(t, reinterpret_cast<f4_t*>(p));
 }
 
+// data rows omegas[a] (row_len values each) (+)= sum over ks < ksplit of part[ks][a] (fixed order).
+int launch_splitk_reduce(const float* part, float* data, const int* omegas, int n_ang, int64_t row_len, int ksplit,
+                         int accumulate, hipStream_t st);
+
 inline unsigned grid_stride_blocks(int64_t n, int block, unsigned cap = 1u << 16) {
   int64_t b = (n + block - 1) / block;
   if (b < 1) b = 1;

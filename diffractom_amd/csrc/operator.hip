@@ -654,30 +654,45 @@ constexpr int SPMM_LDS_FLOATS = 12288;  // 48 KiB of staged sinogram / data rows
 
 // Forward: a thread per CSR row (angle, j); the sinogram rows of SPMM_TY detector positions staged in LDS chunk by
 // chunk of orientations (the CSR rows list k ascending, so each row advances through the chunks in order).
+// With ksplit > 1 the orientations are split into ksplit ranges of kpart (grid z = angle * ksplit + split), each
+// row's first entry of its range found by bisection (columns ascend), and the partial sums written to part
+// (ksplit, n_angles, my, cp) for a fixed-order reduction: ksplit times the waves of one launch.
 __global__ void __launch_bounds__(256) spmm_forward_kernel(const float* __restrict__ sino, const int* __restrict__ row_ptr,
                                                            const uint16_t* __restrict__ col_k,
                                                            const float* __restrict__ val, float* __restrict__ data,
                                                            const int* __restrict__ omegas, int my, int cp, int kstride,
-                                                           int kb, int kc, int compact, int accumulate) {
+                                                           int kb, int kc, int compact, int accumulate, int ksplit,
+                                                           int kpart, float* __restrict__ part, int n_angles) {
   extern __shared__ float lsino[];
   const int lid = threadIdx.x;
   const int j = blockIdx.x * blockDim.x + lid;
   const int y0 = blockIdx.y * SPMM_TY;
-  const int rs = blockIdx.z;
+  const int rs = blockIdx.z / ksplit;
+  const int ks = blockIdx.z % ksplit;
   const int r = omegas ? omegas[rs] : rs;
   const int ny = min(SPMM_TY, my - y0);
   const int rr = compact ? rs : r;
+  const int klo = ks * kpart, khi = min(kb, klo + kpart);
   int i = 0, e = 0;
   if (j < cp) {
     i = row_ptr[(int64_t)rr * cp + j];
     e = row_ptr[(int64_t)rr * cp + j + 1];
+    if (klo > 0) {                                   // first entry with column >= klo
+      int lo = i, hi = e;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)col_k[mid] < klo) lo = mid + 1;
+        else hi = mid;
+      }
+      i = lo;
+    }
   }
   float acc[SPMM_TY];
 #pragma unroll
   for (int t = 0; t < SPMM_TY; ++t) acc[t] = 0.0f;
   const float* sbase = sino + ((int64_t)rs * my + y0) * kstride;
-  for (int k0 = 0; k0 < kb; k0 += kc) {
-    const int kn = min(kc, kb - k0);
+  for (int k0 = klo; k0 < khi; k0 += kc) {
+    const int kn = min(kc, khi - k0);
     __syncthreads();
     for (int t = 0; t < ny; ++t)
       for (int kk = lid; kk < kn; kk += blockDim.x) lsino[t * kc + kk] = sbase[(int64_t)t * kstride + k0 + kk];
@@ -692,7 +707,10 @@ __global__ void __launch_bounds__(256) spmm_forward_kernel(const float* __restri
       for (int t = 0; t < SPMM_TY; ++t) acc[t] += v * ls[t * kc];
     }
   }
-  if (j < cp) {
+  if (j < cp && ksplit > 1) {
+    float* dst = part + (((int64_t)ks * n_angles + rs) * my + y0) * cp + j;
+    for (int t = 0; t < ny; ++t) dst[(int64_t)t * cp] = acc[t];
+  } else if (j < cp) {
     float* dst = data + ((int64_t)r * my + y0) * cp + j;
     if (accumulate) {
       for (int t = 0; t < ny; ++t) dst[(int64_t)t * cp] += acc[t];
@@ -1034,8 +1052,33 @@ int dfm_spmm_forward(const float* sino, const int32_t* row_ptr, const uint16_t* 
   const int kc = max(1, min(max(kb, 1), SPMM_LDS_FLOATS / SPMM_TY));
   dim3 grid((unsigned)((cp + 255) / 256), (unsigned)((my + SPMM_TY - 1) / SPMM_TY), (unsigned)n_angles);
   hipLaunchKernelGGL(spmm_forward_kernel, grid, dim3(256), (size_t)SPMM_TY * kc * sizeof(float), (hipStream_t)stream,
-                     sino, row_ptr, col_k, val, data, omegas, my, cp, kstride, kb, kc, compact, accumulate);
+                     sino, row_ptr, col_k, val, data, omegas, my, cp, kstride, kb, kc, compact, accumulate, 1, kb,
+                     (float*)nullptr, n_angles);
   DFM_LAUNCH_CHECK("dfm_spmm_forward");
+  return DFM_OK;
+}
+
+int dfm_spmm_forward_split(const float* sino, const int32_t* row_ptr, const uint16_t* col_k, const float* val,
+                           float* data, const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp,
+                           int32_t kstride, int32_t kb, int32_t compact, int32_t accumulate, float* part,
+                           int32_t ksplit, int32_t kc, dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && row_ptr && col_k && val && data, "dfm_spmm_forward_split: null pointer");
+  DFM_CHECK_ARG(n_angles >= 0 && my > 0 && cp > 0 && kb >= 0 && kb <= kstride && kb <= 65536,
+                "dfm_spmm_forward_split: bad sizes");
+  DFM_CHECK_ARG(ksplit >= 1 && ksplit <= 64 && (ksplit == 1 || part), "dfm_spmm_forward_split: k split %d needs "
+                "a workspace of ksplit * n_angles * my * cp floats (ksplit 1 .. 64)", ksplit);
+  DFM_CHECK_ARG(kc >= 32 && kc <= SPMM_LDS_FLOATS / SPMM_TY, "dfm_spmm_forward_split: chunk %d not in 32 .. %d", kc,
+                SPMM_LDS_FLOATS / SPMM_TY);
+  if (n_angles == 0) return DFM_OK;
+  const int kpart = (max(kb, 1) + ksplit - 1) / ksplit;
+  const int kcc = min(kc, max(kpart, 1));
+  dim3 grid((unsigned)((cp + 255) / 256), (unsigned)((my + SPMM_TY - 1) / SPMM_TY), (unsigned)(n_angles * ksplit));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(spmm_forward_kernel, grid, dim3(256), (size_t)SPMM_TY * kcc * sizeof(float), st, sino, row_ptr,
+                     col_k, val, data, omegas, my, cp, kstride, kb, kcc, compact, accumulate, ksplit, kpart, part,
+                     n_angles);
+  DFM_LAUNCH_CHECK("dfm_spmm_forward_split");
+  if (ksplit > 1) return dfm::launch_splitk_reduce(part, data, omegas, n_angles, (int64_t)my * cp, ksplit, accumulate, st);
   return DFM_OK;
 }
 

@@ -202,6 +202,7 @@ class SinglePhaseForwardOperator:
                              f"pf_mode must be 'dense' or 'auto', not {pf_mode!r}")
         self.pf_precision = pf_precision
         self.mfma_variant = {"fwd": 2, "adj": 2}
+        self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0}
 
         # ---- geometry (single_phase_forward_operator.py:376-441)
         c = self.cfg
@@ -717,6 +718,25 @@ class SinglePhaseForwardOperator:
                 saved["bwd"] = v
             out["radon_backward_spdhg"] = (best, times)
         self.radon_variant.update(saved)
+        if self.sparse_batches is not None:
+            data = torch.rand(self.data_shape, generator=g, device=dev)
+            times = {}
+            cands = {f"rows:ksplit:{k},kc:{c}": dict(fwd="rows", ksplit=k, kc=c) for k in (4, 16) for c in (256, 768)}
+            if _lib.load().dfm_spmm_t_rows_per_block(self.My) > 0:
+                cands.update({f"t:split:{k}": dict(fwd="t", tsplit=k) for k in (4, 8, 16, 32)})
+            for name, v in cands.items():
+                self.csr_variant.update(v)
+                times[name] = timeit(lambda: self._pf_forward(0, sc["sino"], data, subset, False))
+            best = min(times, key=times.get)
+            self.csr_variant.update(cands[best])
+            out["pf_forward"] = (best, times)
+            times = {}
+            for name in (("t", "rows") if _lib.load().dfm_spmm_t_rows_per_block(self.My) > 0 else ("rows",)):
+                self.csr_variant["adj"] = name
+                times[name] = timeit(lambda: self._pf_adjoint(0, data, sc["sino"], subset))
+            best = min(times, key=times.get)
+            self.csr_variant["adj"] = best
+            out["pf_adjoint"] = (best, times)
         if self.mfma_batches is not None:
             data = torch.rand(self.data_shape, generator=g, device=dev)
             for key, fn in (("fwd", lambda: self._dense_forward(0, sc["sino"], data, subset, False)),
@@ -794,9 +814,15 @@ class SinglePhaseForwardOperator:
             self._dense_forward(ib, sino, data, subset, accumulate)
             return
         sb = self._pf_batch(ib)
-        _lib.call("dfm_spmm_forward", _lib.ptr(sino), _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]),
+        if self._csr_kind("fwd") == "t":
+            self._spmm_t_forward(b, sb, om, n_ang, sino, data, accumulate)
+            return
+        ks = self._csr_split(n_ang, b["K_batch"])
+        part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
+        _lib.call("dfm_spmm_forward_split", _lib.ptr(sino), _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]),
                   _lib.ptr(sb["val_f"]), _lib.ptr(data), _lib.ptr(om), n_ang, self.My, self.N_seg, b["kstride"],
-                  b["K_batch"], 0, int(accumulate), self._stream())
+                  b["K_batch"], 0, int(accumulate), _lib.ptr(part), ks, int(self.csr_variant["kc"]),
+                  self._stream())
 
     def _pf_adjoint(self, ib, data, sino, subset):
         b = self.batches[ib]
@@ -806,9 +832,73 @@ class SinglePhaseForwardOperator:
             self._dense_adjoint(ib, data, sino, subset, alpha)
             return
         sb = self._pf_batch(ib)
+        if self._csr_kind("adj") == "t":
+            self._spmm_t_adjoint(b, sb, om, n_ang, data, sino, alpha)
+            return
         _lib.call("dfm_spmm_adjoint", _lib.ptr(data), _lib.ptr(sb["row_ptr_a"]), _lib.ptr(sb["col_j"]),
                   _lib.ptr(sb["val_a"]), _lib.ptr(sino), _lib.ptr(om), n_ang, self.My, self.N_seg, b["kstride"],
                   b["K_batch"], alpha, 0, self._stream())
+
+    # ------------------------------------------------------------------ CSR products on transposed operands
+    def _csr_kind(self, direction):
+        """"t" (wavefront-segmented kernel on transposed operands) or "rows" (a thread per CSR row)."""
+        v = self.csr_variant.get(direction, "auto")
+        if v == "auto":
+            return "t" if _lib.load().dfm_spmm_t_rows_per_block(self.My) > 0 else "rows"
+        return v
+
+    @property
+    def _ys(self):
+        return -(-self.My // 4) * 4
+
+    def _t_buf(self, *sizes):
+        """Views of one reusable workspace, each starting 256-byte aligned."""
+        offs, n = [], 0
+        for z in sizes:
+            offs.append(n)
+            n += -(-int(z) // 64) * 64
+        p = getattr(self, "_t_buf_t", None)
+        if p is None or p.numel() < n:
+            p = torch.empty(n, dtype=torch.float32, device=self.device)
+            self._t_buf_t = p
+            self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
+        return [p[o:o + int(z)] for o, z in zip(offs, sizes)]
+
+    def _tsplit(self, n_ang, kb):
+        """Orientation split of the transposed forward: enough workgroups for 4 per CU on 256 CUs."""
+        v = self.csr_variant.get("tsplit", "auto")
+        if v != "auto":
+            return int(max(1, min(int(v), max(kb, 1))))
+        rpb = _lib.load().dfm_spmm_t_rows_per_block(self.My)
+        wgs = -(-self.N_seg // rpb) * n_ang
+        return int(max(1, min(64, -(-1024 // wgs), max(kb // 128, 1))))
+
+    def _spmm_t_forward(self, b, sb, om, n_ang, sino, data, accumulate):
+        """sino (n_ang, my, kstride) -> sT (n_ang, kb, ys) -> split partials (ns, n_ang, cp, ys) -> data rows."""
+        my, ys, cp, kb = self.My, self._ys, self.N_seg, b["K_batch"]
+        ns = self._tsplit(n_ang, kb)
+        sT, part = self._t_buf(n_ang * kb * ys, ns * n_ang * cp * ys)
+        st = self._stream()
+        _lib.call("dfm_transpose_sum", _lib.ptr(sino), _lib.ptr(sT), None, 0, n_ang, my, kb, b["kstride"],
+                  my * b["kstride"], ys, kb * ys, 1, 0, 0, st)
+        _lib.call("dfm_spmm_t", _lib.ptr(sT), _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]), _lib.ptr(sb["val_f"]),
+                  _lib.ptr(part), _lib.ptr(om), n_ang, my, ys, kb, cp, cp, 0, 1.0, ns,
+                  int(self.csr_variant.get("tkc", 0)), st)
+        _lib.call("dfm_transpose_sum", _lib.ptr(part), _lib.ptr(data), _lib.ptr(om), 1, n_ang, cp, my, ys, cp * ys,
+                  cp, my * cp, ns, n_ang * cp * ys, int(accumulate), st)
+
+    def _spmm_t_adjoint(self, b, sb, om, n_ang, data, sino, alpha):
+        """data rows -> dT (n_ang, cp, ys) -> sT (n_ang, kstride, ys) -> sino (n_ang, my, kstride)."""
+        my, ys, cp, ks = self.My, self._ys, self.N_seg, b["kstride"]
+        dT, sT = self._t_buf(n_ang * cp * ys, n_ang * ks * ys)
+        st = self._stream()
+        _lib.call("dfm_transpose_sum", _lib.ptr(data), _lib.ptr(dT), _lib.ptr(om), 0, n_ang, my, cp, cp, my * cp, ys,
+                  cp * ys, 1, 0, 0, st)
+        _lib.call("dfm_spmm_t", _lib.ptr(dT), _lib.ptr(sb["row_ptr_a"]), _lib.ptr(sb["col_j"]), _lib.ptr(sb["val_a"]),
+                  _lib.ptr(sT), _lib.ptr(om), n_ang, my, ys, cp, b["K_batch"], ks, 0, alpha, 1,
+                  int(self.csr_variant.get("tkc", 0)), st)
+        _lib.call("dfm_transpose_sum", _lib.ptr(sT), _lib.ptr(sino), None, 0, n_ang, ks, my, ys, ks * ys, ks,
+                  my * ks, 1, 0, 0, st)
 
     def _mfma_split(self, n_ang, mb):
         """k split of the matrix-core forward: enough waves for 8 per CU on 256 CUs, at most 16 partial sums."""
@@ -816,6 +906,15 @@ class SinglePhaseForwardOperator:
         n_yg = -(-tiles // 8)
         waves = n_ang * n_yg * (-(-mb["n_nt"] // self.mfma_variant["fwd"]))
         return int(max(1, min(16, mb["n_kc"], -(-2048 // waves))))
+
+    def _csr_split(self, n_ang, kb):
+        """k split of the CSR forward: csr_variant["ksplit"], or "auto": enough waves for 32 per CU on 256 CUs (a
+        thread per (row, 16 detector positions) leaves too few otherwise), at most 16."""
+        v = self.csr_variant["ksplit"]
+        if v != "auto":
+            return int(max(1, min(int(v), max(kb, 1))))
+        lanes = -(-self.N_seg // 256) * 256 * -(-self.My // 16) * n_ang
+        return int(max(1, min(16, -(-(8192 * 64) // lanes), max(kb // 256, 1))))
 
     def _mfma_part(self, n):
         p = getattr(self, "_mfma_part_buf", None)

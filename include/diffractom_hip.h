@@ -120,6 +120,31 @@ int dfm_pf_fill_adj(const float* pf, const int32_t* row_ptr, uint16_t* col_j, fl
 int dfm_spmm_forward(const float* sino, const int32_t* row_ptr, const uint16_t* col_k, const float* val, float* data,
                      const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
                      int32_t compact, int32_t accumulate, dfm_stream_t stream);
+/* forward with the orientations split over ksplit workgroup ranges (ksplit times the parallelism of one angle set),
+ * partial sums in part (ksplit * n_angles * my * cp floats, unused when ksplit = 1), summed in a fixed order;
+ * kc: orientations staged in LDS per pass (32 .. 768). Same result as dfm_spmm_forward up to summation order. */
+int dfm_spmm_forward_split(const float* sino, const int32_t* row_ptr, const uint16_t* col_k, const float* val,
+                           float* data, const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp,
+                           int32_t kstride, int32_t kb, int32_t compact, int32_t accumulate, float* part,
+                           int32_t ksplit, int32_t kc, dfm_stream_t stream);
+/* The same products on transposed operands (wavefront-segmented SpMM: a wave per R CSR rows, lanes over y, column
+ * chunks of the operand staged in LDS), for either CSR form:
+ *   outT[s, a, row, y] = alpha sum_{e in row (rr, row), col(e) in split s} val[e] inT[a, col(e), y]
+ * with rr = a if compact else omegas[a]; inT (n_angles, ncols, ys), outT (nsplit, n_angles, rows_out, ys), ys >= my a
+ * multiple of 4, inT 16-byte aligned; split s covers columns [s cspan, (s + 1) cspan), cspan = ceil(ncols / nsplit);
+ * rows nrows .. rows_out - 1 are written 0; my <= 512; kc: columns per LDS chunk (0: the most that fit).
+ * forward: CSR (row_ptr_f, col_k), rows = cp, columns = kb, inT = the sinogram transposed;
+ * adjoint: CSR (row_ptr_a, col_j), rows = kb (rows_out = kstride), columns = cp, inT = the data rows transposed. */
+int dfm_spmm_t(const float* inT, const int32_t* row_ptr, const uint16_t* col, const float* val, float* outT,
+               const int32_t* omegas, int32_t n_angles, int32_t my, int32_t ys, int32_t ncols, int32_t nrows,
+               int32_t rows_out, int32_t compact, float alpha, int32_t nsplit, int32_t kc, dfm_stream_t stream);
+/* CSR rows per workgroup of dfm_spmm_t for a given my (0: my unsupported). */
+int dfm_spmm_t_rows_per_block(int32_t my);
+/* out[ao, c, r] (+)= sum_{s < nsplit} in[s split_stride + ai in_mat + r in_ld + c] for a = 0 .. n_mat-1, with
+ * ai = omegas[a] if omegas and omega_out == 0, ao = omegas[a] if omegas and omega_out == 1, else a. */
+int dfm_transpose_sum(const float* in, float* out, const int32_t* omegas, int32_t omega_out, int32_t n_mat,
+                      int32_t nr, int32_t nc, int64_t in_ld, int64_t in_mat, int64_t out_ld, int64_t out_mat,
+                      int32_t nsplit, int64_t split_stride, int32_t accumulate, dfm_stream_t stream);
 int dfm_spmm_adjoint(const float* data, const int32_t* row_ptr, const uint16_t* col_j, const float* val, float* sino,
                      const int32_t* omegas, int32_t n_angles, int32_t my, int32_t cp, int32_t kstride, int32_t kb,
                      float alpha, int32_t compact, dfm_stream_t stream);

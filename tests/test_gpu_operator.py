@@ -194,3 +194,23 @@ def test_hexagonal_titanium_matches_oracle(precision):
     tol = 2e-5 if precision == "fp32" else 3e-5
     assert rel_max(op.direct(x), orc.direct(x)) < tol
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < tol
+
+
+@pytest.mark.parametrize("ksplit,kc", [(1, 768), (3, 32), (8, 256)])
+def test_csr_forward_split_matches_oracle(ksplit, kc):
+    """The CSR forward with the orientations split over workgroup ranges (bisection to each range's first entry,
+    partial sums reduced in a fixed order) and LDS passes of kc orientations; several batches."""
+    print(f"case csr ksplit={ksplit} kc={kc}", flush=True)
+    c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
+    op, orc, _ = build(c, K=150, sigma_deg=5.0, pf_mode="sparse", max_gb=2.2e-4)
+    assert len(op.batches) > 1
+    op.csr_variant.update(ksplit=ksplit, kc=kc)
+    x, _ = _inputs(op, 4)
+    assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
+    op.set_omega_subsets(3)
+    import torch
+    xt = torch.from_numpy(x).cuda()
+    d = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
+    op.direct_cl(xt, d, subset=1)
+    rows = op.omega_subsets[1]
+    assert rel_max(d.cpu().numpy()[rows], orc.direct(x)[rows]) < 2e-5
