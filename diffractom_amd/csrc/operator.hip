@@ -228,20 +228,21 @@ __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __res
 // Forward from the table: a thread per (channel quad, bin); a workgroup covers 16 quads x 16 consecutive bins of one
 // angle; the channel blocks are split over the XCDs as above. UNR entries are fetched before they are used, so a
 // wave keeps UNR independent loads in flight instead of one dependent chain per entry.
-template <int UNR>
+template <int UNR, int QBT>
 __global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* __restrict__ img,
                                                                   float4* __restrict__ sino,
                                                                   const int* __restrict__ row_ptr,
                                                                   const int* __restrict__ ent_pix,
                                                                   const float* __restrict__ ent_w, int ns, int na,
                                                                   int kstride4, int k4_off, int k4_n, int skstride4) {
-  const int sblocks = (ns + 15) / 16;
-  const int n_blk = (k4_n + QB - 1) / QB;
+  constexpr int BINS = 256 / QBT;
+  const int sblocks = (ns + BINS - 1) / BINS;
+  const int n_blk = (k4_n + QBT - 1) / QBT;
   int blk, local;
   if (!xcd_split(na * sblocks, n_blk, blk, local)) return;
   const int a = local / sblocks;
-  const int s = (local % sblocks) * 16 + (threadIdx.x >> 4);
-  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const int s = (local % sblocks) * BINS + threadIdx.x / QBT;
+  const int kq = blk * QBT + (threadIdx.x & (QBT - 1));
   if (kq >= k4_n || s >= ns) return;
   const float4* col = img + kq + k4_off;
   const int r = a * ns + s;
@@ -284,7 +285,7 @@ __global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* 
 // gathers only (with per-lane entry loads it also carries every entry 16 times).
 constexpr int FT_CHUNK = 2048;
 
-template <int UNR>
+template <int UNR, int QBT>
 __global__ void __launch_bounds__(256) radon_forward_table_lds_kernel(const float4* __restrict__ img,
                                                                       float4* __restrict__ sino,
                                                                       const int* __restrict__ row_ptr,
@@ -294,17 +295,18 @@ __global__ void __launch_bounds__(256) radon_forward_table_lds_kernel(const floa
                                                                       int skstride4) {
   __shared__ int lp[FT_CHUNK];
   __shared__ float lw[FT_CHUNK];
-  const int sblocks = (ns + 15) / 16;
-  const int n_blk = (k4_n + QB - 1) / QB;
+  constexpr int BINS = 256 / QBT;
+  const int sblocks = (ns + BINS - 1) / BINS;
+  const int n_blk = (k4_n + QBT - 1) / QBT;
   int blk, local;
   if (!xcd_split(na * sblocks, n_blk, blk, local)) return;       // uniform over the workgroup
   const int a = local / sblocks;
-  const int s0 = (local % sblocks) * 16;
-  const int s = s0 + (threadIdx.x >> 4);
-  const int kq = blk * QB + (threadIdx.x & (QB - 1));
+  const int s0 = (local % sblocks) * BINS;
+  const int s = s0 + threadIdx.x / QBT;
+  const int kq = blk * QBT + (threadIdx.x & (QBT - 1));
   const bool live = kq < k4_n && s < ns;
   const int g0 = row_ptr[a * ns + s0];
-  const int g1 = row_ptr[a * ns + min(s0 + 16, ns)];
+  const int g1 = row_ptr[a * ns + min(s0 + BINS, ns)];
   const int r = a * ns + min(s, ns - 1);
   const int e0 = live ? row_ptr[r] : 0;
   const int e1 = live ? row_ptr[r + 1] : 0;
@@ -893,30 +895,44 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
   // variant = unr + 16 * cap: entries fetched ahead (1, 4, 8) and, cap > 0, at most cap workgroups per CU (an
   // otherwise unused LDS allocation of 160 KB / cap), so that fewer channel blocks share an XCD's L2
   // + 256: the workgroup's entries staged in LDS
-  const int unr = variant % 16, cap = (variant / 16) % 16, staged = variant / 256;
-  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && (staged == 0 || staged == 1),
+  // + 512 * q: channel block of 64 (q 0), 32 (q 1), 16 (q 2) or 128 (q 3) channels per workgroup (the image
+  //   slice an XCD streams for all angles at once: that many channels x the pixels)
+  const int unr = variant % 16, cap = (variant / 16) % 16, staged = (variant / 256) % 2, q = variant / 512;
+  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && q <= 3,
                 "dfm_radon_forward_table: variant %d: entries ahead 1, 4 or 8 (+ 16 * cap, cap 0 or 3 .. 15, + 256 "
-                "for LDS-staged entries)", variant);
+                "for LDS-staged entries, + 512 * q for 64, 32, 16 or 128 channels per block)", variant);
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
-  const unsigned grid = xcd_grid(n_angles * ((ns + 15) / 16), (k4n + QB - 1) / QB);
+  const int qb = q == 3 ? 2 * QB : QB >> q;
+  const unsigned grid = xcd_grid(n_angles * ((ns + 256 / qb - 1) / (256 / qb)), (k4n + qb - 1) / qb);
   const size_t lds_cap = cap > 0 ? (size_t)(163840 / cap) & ~(size_t)255 : 0;
-#define DFM_FT(U)                                                                                                 \
+#define DFM_FT(U, Q)                                                                                              \
   if (staged)                                                                                                     \
-    hipLaunchKernelGGL(radon_forward_table_lds_kernel<U>, dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,   \
-                       (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,    \
-                       k_off / 4, k4n, skstride / 4);                                                             \
+    hipLaunchKernelGGL((radon_forward_table_lds_kernel<U, Q>), dim3(grid), dim3(256), lds_cap,                   \
+                       (hipStream_t)stream, (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns,      \
+                       n_angles, kstride / 4, k_off / 4, k4n, skstride / 4);                                      \
   else                                                                                                            \
-    hipLaunchKernelGGL(radon_forward_table_kernel<U>, dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,       \
+    hipLaunchKernelGGL((radon_forward_table_kernel<U, Q>), dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,  \
                        (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,    \
                        k_off / 4, k4n, skstride / 4)
-  if (unr == 1) {
-    DFM_FT(1);
-  } else if (unr == 4) {
-    DFM_FT(4);
-  } else {
-    DFM_FT(8);
+#define DFM_FTQ(U)        \
+  if (qb == 16) {         \
+    DFM_FT(U, 16);        \
+  } else if (qb == 32) {  \
+    DFM_FT(U, 32);        \
+  } else if (qb == 8) {   \
+    DFM_FT(U, 8);         \
+  } else {                \
+    DFM_FT(U, 4);         \
   }
+  if (unr == 1) {
+    DFM_FTQ(1);
+  } else if (unr == 4) {
+    DFM_FTQ(4);
+  } else {
+    DFM_FTQ(8);
+  }
+#undef DFM_FTQ
 #undef DFM_FT
   DFM_LAUNCH_CHECK("dfm_radon_forward_table");
   return DFM_OK;

@@ -10,10 +10,10 @@
 // A workgroup of 16 waves owns 16 R consecutive CSR rows of one angle and one column range (split s). It walks the
 // range in chunks of kc columns: the chunk of `in` (kc rows of ys floats, contiguous) is staged in LDS, then every wave
 // consumes, for each of its R rows, the row's entries that fall in the chunk. The lanes of a wave run over the
-// detector positions y (YW groups of 64), so an entry is wave-uniform (read from the row's 64-entry window held in
-// registers with v_readlane) and its operand is one conflict-free LDS row read. Per row the window moves on only when
-// all 64 of its entries are consumed, so col/val are read once, coalesced. Summation follows the CSR order, as in the
-// row-per-thread kernels.
+// detector positions y (YW groups of 64), so an entry is wave-uniform (its column read from the row's 64-entry window
+// held in registers with v_readlane, its value by a scalar load) and its operand is one conflict-free LDS row read. Per
+// row the column window moves on only when all 64 of its entries are consumed. Summation follows the CSR order, as in
+// the row-per-thread kernels.
 #include "dfm_common.h"
 
 namespace {
@@ -21,7 +21,7 @@ namespace {
 constexpr int SPT_WAVES = 16;
 constexpr int SPT_LDS_FLOATS = 40960;   // 160 KiB
 
-template <int R, int YW>
+template <int R, int YW, int U>
 __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ inT, const int* __restrict__ row_ptr,
                                                       const uint16_t* __restrict__ col, const float* __restrict__ val,
                                                       float* __restrict__ outT, const int* __restrict__ omegas,
@@ -54,9 +54,8 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
       cur = lo;
     }
   }
-  // per row: window start we (entry index of lane 0), consumed count wo, end ee; window entries cb / vb per lane
+  // per row: window start we (entry index of lane 0), consumed count wo, end ee; window columns cb per lane
   int we[R], wo[R], ee[R], cb[R];
-  float vb[R];
   float acc[R][YW];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -65,11 +64,11 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
     wo[i] = 0;
     const int idx = we[i] + lane;
     cb[i] = idx < ee[i] ? (int)col[idx] : 0x7fffffff;
-    vb[i] = idx < ee[i] ? val[idx] : 0.f;
 #pragma unroll
     for (int h = 0; h < YW; ++h) acc[i][h] = 0.f;
   }
 
+  const float* lbase = lin + lane;
   const float* src = inT + (int64_t)a * ncols * ys;
   for (int c0 = c_lo; c0 < c_hi; c0 += kc) {
     const int kn = min(kc, c_hi - c0);
@@ -82,15 +81,31 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
     }
     __syncthreads();
     const int c1 = c0 + kn;
-    const float* lrow = lin + lane - c0 * ys;
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       while (true) {
         const int cnt = __popcll(__ballot(cb[i] < c1));     // entries of the window below c1 (a prefix)
-        for (int t = wo[i]; t < cnt; ++t) {
-          const int ct = __builtin_amdgcn_readlane(cb[i], t);
-          const float vt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vb[i]), t));
-          const float* lp = lrow + ct * ys;
+        const int off = (cb[i] - c0) * ys;                   // operand row offset of each window entry
+        const float* vrow = val + we[i];                      // entry values: wave-uniform (scalar) loads
+        int t = wo[i];
+        for (; t + U <= cnt; t += U) {                        // U operand rows in flight, summed in entry order
+          float x[U][YW], v[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const float* lp = lbase + __builtin_amdgcn_readlane(off, t + u);
+            v[u] = vrow[t + u];
+#pragma unroll
+            for (int h = 0; h < YW; ++h) x[u][h] = lp[64 * h];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int h = 0; h < YW; ++h) acc[i][h] += v[u] * x[u][h];
+        }
+        for (; t < cnt; ++t) {
+          const float* lp = lbase + __builtin_amdgcn_readlane(off, t);
+          const float vt = vrow[t];
 #pragma unroll
           for (int h = 0; h < YW; ++h) acc[i][h] += vt * lp[64 * h];
         }
@@ -100,7 +115,6 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
         wo[i] = 0;
         const int idx = we[i] + lane;
         cb[i] = idx < ee[i] ? (int)col[idx] : 0x7fffffff;
-        vb[i] = idx < ee[i] ? val[idx] : 0.f;
       }
     }
   }
@@ -158,7 +172,7 @@ template <int R, int YW>
 void launch_spmm_t(dim3 grid, size_t lds, hipStream_t st, const float* inT, const int* row_ptr, const uint16_t* col,
                    const float* val, float* outT, const int* omegas, int my, int ys, int ncols, int nrows,
                    int rows_out, int compact, float alpha, int cspan, int kc, int n_angles) {
-  hipLaunchKernelGGL((spmm_t_kernel<R, YW>), grid, dim3(SPT_WAVES * 64), lds, st, inT, row_ptr, col, val, outT,
+  hipLaunchKernelGGL((spmm_t_kernel<R, YW, (YW <= 4 ? 8 : 4)>), grid, dim3(SPT_WAVES * 64), lds, st, inT, row_ptr, col, val, outT,
                      omegas, my, ys, ncols, nrows, rows_out, compact, alpha, cspan, kc, n_angles);
 }
 

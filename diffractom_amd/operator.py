@@ -79,12 +79,14 @@ def as_host(a, shape, name):
     return out
 
 
-def radon_tables(geo64, idx, nx, ny, ns, radon_scale):
+def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
     """Geometry tables of the projector for the angles idx (compact order), in float64 NumPy: the forward CSR over
     (angle, bin) -> (pixel, weight), the pixel table (bin, bin, w0, w1) of the backprojection and the first bin of
     every 4 x 4 pixel tile (the C ABI's dfm_radon_forward_table / dfm_radon_backward_table). Weights are those of
     ParallelRadon (parallel_radon.py:92-124): forward scale/A K(t - s), backward (pi/N_Omega)/A K(t - s),
-    K(d) = clamp(hw - |d|, 0, top); pixels touch two bins (A <= 1)."""
+    K(d) = clamp(hw - |d|, 0, top); pixels touch two bins (A <= 1). band_ptr: the first entry of every (ray, band)
+    for n_bands bands of ceil(ny / n_bands) image rows (dfm_radon_forward_band); a ray's entries ascend in pixel
+    index, so each band is one contiguous range."""
     g = geo64
     idx = np.asarray(idx, np.int64)
     yy, xx = np.meshgrid(np.arange(ny, dtype=np.float64), np.arange(nx, dtype=np.float64), indexing="ij")
@@ -138,7 +140,10 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale):
         np.maximum.at(top_[a], tile, hi[a])
     if np.any(top_ - base >= 8):
         raise AssertionError("a 4 x 4 pixel tile reaches more than 8 bins at some angle")
-    return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32),
+    bh = -(-ny // n_bands)
+    band_ptr = np.zeros(na * ns * n_bands + 1, np.int64)
+    np.cumsum(np.bincount(rows * n_bands + (pix // nx) // bh, minlength=na * ns * n_bands), out=band_ptr[1:])
+    return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32), "band_ptr": band_ptr.astype(np.int32),
             "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
             "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
             "pair_w": pw.astype(np.float32)}
@@ -363,11 +368,15 @@ class SinglePhaseForwardOperator:
         self.tuning = None
         self.fuse_projector = False
         self.fused_channels = 4
+        # image bands of the banded forward projection: one XCD's band of 64 channels within about 2 MB of L2
+        self.radon_bands = int(min(8, max(1, -(-self.Nx * self.Ny // 8192))))
+        if self.radon_bands >= 4:
+            self.radon_variant["fwd_kernel"] = "band"
         self._tables_full = self._radon_tables(np.arange(self.N_Omega)) if self.n_taps == 2 else None
 
     def _radon_tables(self, idx):
         """Device copies of radon_tables() for the angles idx (compact order)."""
-        t = radon_tables(self._geo64, idx, self.Nx, self.Ny, self.My, self.radon_scale)
+        t = radon_tables(self._geo64, idx, self.Nx, self.Ny, self.My, self.radon_scale, self.radon_bands)
         return {k: torch.from_numpy(v).to(self.device) for k, v in t.items()}
 
     def _make_batches(self, max_gb, K_batch=None):
@@ -681,7 +690,10 @@ class SinglePhaseForwardOperator:
         saved = dict(rv)
         if tab is not None:
             cands = {"table:8": ("table", 8), "table:8+lds": ("table", 8 + 256), "table:4+lds": ("table", 4 + 256),
-                     "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "pairs:8": ("pairs", 8)}
+                     "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "table:8+lds+ch32": ("table", 8 + 256 + 512),
+                     "table:8+lds+ch128": ("table", 8 + 256 + 1536), "pairs:8": ("pairs", 8)}
+            if self.radon_bands > 1:
+                cands.update({"band:8": ("band", 8), "band:4": ("band", 4)})
             if self.lds_width():
                 cands["lds"] = ("lds", None)
             times = {}
@@ -691,6 +703,8 @@ class SinglePhaseForwardOperator:
                     rv["fwd"] = v
                 elif kern == "pairs":
                     rv["fwd_pairs"] = v
+                elif kern == "band":
+                    rv["fwd_band"] = v
                 times[name] = timeit(lambda: self._radon_forward(x, self.K_stride, b["koff"], b["kstride"],
                                                                  sc["sino"], geo_f, n_ang, tab))
             best = min(times, key=times.get)
@@ -700,6 +714,8 @@ class SinglePhaseForwardOperator:
                 saved["fwd"] = v
             elif kern == "pairs":
                 saved["fwd_pairs"] = v
+            elif kern == "band":
+                saved["fwd_band"] = v
             out["radon_forward"] = (best, times)
             cands = {f"table:{a}": ("table", a) for a in (4, 8, 16)}
             if self.lds_width():
@@ -797,6 +813,13 @@ class SinglePhaseForwardOperator:
         if cw and koff % cw == 0 and kn % cw == 0:
             self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
                                 kn, cw)
+            return
+        if tab is not None and self.radon_variant["fwd_kernel"] == "band":
+            nb = self.radon_bands
+            part = self._t_buf(nb * n_angles * self.My * kn)[0]
+            _lib.call("dfm_radon_forward_band", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(part),
+                      _lib.ptr(tab["band_ptr"]), _lib.ptr(tab["pix"]), _lib.ptr(tab["w"]), self.My, n_angles, nb,
+                      img_kstride, koff, kn, kn, self.radon_variant.get("fwd_band", 8), self._stream())
             return
         if tab is not None and self.radon_variant["fwd_kernel"] != "arith":
             _lib.call("dfm_radon_forward_table", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["row_ptr"]),

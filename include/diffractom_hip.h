@@ -79,6 +79,14 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
 int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
                             const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
                             int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
+/* The forward projection of dfm_radon_forward_table tiled by nb bands of image rows: band_ptr[(a ns + s) nb + b]
+ * is the first table entry of ray (a, s) in band b (entries ascend in pixel index), band_ptr[na ns nb] the total.
+ * Per band the partial sums go to part (nb * n_angles * ns * skstride floats), then sino = their sum in band order.
+ * variant: table entries fetched ahead (4 or 8). */
+int dfm_radon_forward_band(const float* img_k, float* sino, float* part, const int32_t* band_ptr,
+                           const int32_t* ent_pix, const float* ent_w, int32_t ns, int32_t n_angles, int32_t nb,
+                           int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t variant,
+                           dfm_stream_t stream);
 /* Forward from the bin-pair table (the same projection, radon_kernels.cl:44-101): rows (angle, q) for the bins 2q and
  * 2q + 1, pair_ptr (n_angles * ceil(ns/2) + 1), entries (pixel, weight to 2q, weight to 2q + 1) with ent_w float2; a
  * pixel whose two taps fall in one pair appears once. variant: entries fetched ahead, 1, 4 or 8. */

@@ -74,3 +74,22 @@ def test_pair_table_equals_projection_matrices(shape):
         assert not F[my:].any()
         np.testing.assert_allclose(F[:my], P, rtol=2e-6, atol=1e-7 * P.max())
     assert len(tab["pair_pix"]) < 0.8 * len(tab["pix"])
+
+@pytest.mark.parametrize("n_bands", [1, 3, 5])
+def test_band_pointers_split_each_ray_by_image_rows(n_bands):
+    """band_ptr cuts every ray's entries into n_bands contiguous ranges, range b holding exactly the entries whose
+    pixel lies in image rows [b h, (b + 1) h), h = ceil(ny / n_bands): the banded forward sums the same products."""
+    nx, ny, my = 14, 12, 17
+    op = O.OracleOperator(cfg(nx=nx, ny=ny, my=my, n_omega=16, angle_range=(0, 360), cor=0.7),
+                          (4.05,) * 3 + (90,) * 3, "cubic", AL3, 0.35, orientations(2, 1), np.deg2rad(4.0),
+                          orthonormalize=False)
+    idx = np.array([0, 3, 7, 11])
+    tab = radon_tables(_geo(op, "trapezoid"), idx, nx, ny, my, op.scale, n_bands)
+    h = -(-ny // n_bands)
+    bp, rp = tab["band_ptr"], tab["row_ptr"]
+    assert len(bp) == len(idx) * my * n_bands + 1
+    for r in range(len(idx) * my):
+        assert bp[r * n_bands] == rp[r] and bp[(r + 1) * n_bands] == rp[r + 1]
+        for b in range(n_bands):
+            rows = tab["pix"][bp[r * n_bands + b]:bp[r * n_bands + b + 1]] // nx
+            assert np.all((rows >= b * h) & (rows < (b + 1) * h))
