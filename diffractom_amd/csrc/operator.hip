@@ -484,17 +484,25 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
                                       : make_int4(0, 0, 0, 0);
     }
     __syncthreads();
-    // stage: element i = (angle al, bin b, quad qq)
-    for (int i = threadIdx.x; i < nan * TB * QB; i += 256) {
+    // stage: element i = (angle al, bin b, quad qq); all of the chunk's loads issued before the first LDS store
+    constexpr int PER = ACH * TB * QB / 256;
+    static_assert(ACH * TB * QB % 256 == 0, "whole staging rounds");
+    float4 v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = threadIdx.x + 256 * u;
       const int qq = i & (QB - 1);
       const int b = (i >> 4) % TB;
       const int al = i / (TB * QB);
-      const int s = lbase[al] + b;
-      const int kk = blk * QB + qq;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (s >= 0 && s < ns && kk < k4_n) v = sino[((int64_t)(a0 + al) * ns + s) * skstride4 + kk];
-      lds[i] = v;
+      v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (al < nan) {
+        const int s = lbase[al] + b;
+        const int kk = blk * QB + qq;
+        if (s >= 0 && s < ns && kk < k4_n) v[u] = sino[((int64_t)(a0 + al) * ns + s) * skstride4 + kk];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) lds[threadIdx.x + 256 * u] = v[u];
     __syncthreads();
     if (live) {
 #pragma unroll 4

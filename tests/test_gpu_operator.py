@@ -205,13 +205,13 @@ def test_hexagonal_titanium_matches_oracle(precision):
 
 @pytest.mark.parametrize("ksplit,kc", [(1, 768), (3, 32), (8, 256)])
 def test_csr_forward_split_matches_oracle(ksplit, kc):
-    """The CSR forward with the orientations split over workgroup ranges (bisection to each range's first entry,
-    partial sums reduced in a fixed order) and LDS passes of kc orientations; several batches."""
+    """The row-per-thread CSR forward with the orientations split over workgroup ranges (bisection to each range's
+    first entry, partial sums reduced in a fixed order) and LDS passes of kc orientations; several batches."""
     print(f"case csr ksplit={ksplit} kc={kc}", flush=True)
     c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
     op, orc, _ = build(c, K=150, sigma_deg=5.0, pf_mode="sparse", max_gb=2.2e-4)
     assert len(op.batches) > 1
-    op.csr_variant.update(ksplit=ksplit, kc=kc)
+    op.csr_variant.update(fwd="rows", ksplit=ksplit, kc=kc)
     x, _ = _inputs(op, 4)
     assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
     op.set_omega_subsets(3)
@@ -221,3 +221,30 @@ def test_csr_forward_split_matches_oracle(ksplit, kc):
     op.direct_cl(xt, d, subset=1)
     rows = op.omega_subsets[1]
     assert rel_max(d.cpu().numpy()[rows], orc.direct(x)[rows]) < 2e-5
+
+
+@pytest.mark.parametrize("my,tsplit,tkc", [(17, 1, 0), (17, 4, 8), (130, 3, 0), (200, 2, 16), (300, 5, 0)])
+def test_csr_transposed_kernels_match_oracle(my, tsplit, tkc):
+    """The wavefront-segmented CSR products on transposed operands, both directions: 1, 3 and 5 groups of 64 detector
+    positions per wave (16 or 8 rows per wave), orientation splits with partial sums, LDS chunks of tkc columns (0:
+    as many as fit; 8 and 16 force many chunk and window changes); several batches, full and subset rows."""
+    print(f"case csr transposed my={my} tsplit={tsplit} tkc={tkc}", flush=True)
+    c = cfg(nx=12, ny=12, my=my, n_omega=12, angle_range=(0, 360), cor=0.7, n_eta=24)
+    op, orc, _ = build(c, K=150, sigma_deg=5.0, pf_mode="sparse", max_gb=2.2e-4)
+    assert len(op.batches) > 1
+    op.csr_variant.update(fwd="t", adj="t", tsplit=tsplit, tkc=tkc)
+    x, y = _inputs(op, 4)
+    assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
+    assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5
+    op.set_omega_subsets(3)
+    import torch
+    xt = torch.from_numpy(x).cuda()
+    d = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
+    op.direct_cl(xt, d, subset=2)
+    rows = op.omega_subsets[2]
+    assert rel_max(d.cpu().numpy()[rows], orc.direct(x)[rows]) < 2e-5
+    ysub = np.zeros_like(y)
+    ysub[rows] = y[rows]
+    out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
+    op.adjoint_cl(torch.from_numpy(y).cuda(), out, subset=2)
+    assert rel_max(out.cpu().numpy(), orc.adjoint(ysub)) < 2e-5
