@@ -534,6 +534,126 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
   }
 }
 
+// The same backprojection with a workgroup covering one 4x4 tile for G consecutive channel blocks: the tile's bases
+// and pixel-table rows (channel-independent) are staged once per TCH angles and reused by every block, so the per-block
+// work is the sinogram staging (all of a pass's loads in flight) and the pixel sums. Blocks g*G .. g*G+G-1 run on XCD
+// g % 8, so their sinogram slabs stay in its L2.
+constexpr int TCH = 32;   // angles whose table rows are staged at once
+
+template <int MODE, int ACH, int G>
+__global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4* __restrict__ sino,
+                                                                   float4* __restrict__ out,
+                                                                   const int4* __restrict__ ptab,
+                                                                   const int* __restrict__ tbase, int nx, int ny,
+                                                                   int ns, int na, int kstride4, int k4_off, int k4_n,
+                                                                   int skstride4, float4* __restrict__ x,
+                                                                   float4* __restrict__ z,
+                                                                   const float4* __restrict__ T, float inv_p,
+                                                                   float lam) {
+  __shared__ float4 lds[ACH * TB * QB];   // ACH angles x TB bins x 16 quads
+  __shared__ int4 lpt[TCH * 16];          // pixel-table rows of the tile's 16 pixels for TCH angles
+  __shared__ int lbase[TCH];
+  static_assert(ACH * TB * QB % 256 == 0 && TCH % ACH == 0, "whole staging rounds");
+  const int tnx = (nx + 3) / 4, tny = (ny + 3) / 4;
+  const int ntiles = tnx * tny;
+  const int n_cb = (k4_n + QB - 1) / QB;
+  const int n_grp = (n_cb + G - 1) / G;
+  int grp, tile;
+  if (!xcd_split(ntiles, n_grp, grp, tile)) return;
+  const int q = threadIdx.x & (QB - 1);
+  const int li = threadIdx.x >> 4;
+  const int px = (tile % tnx) * 4 + (li & 3), py = (tile / tnx) * 4 + (li >> 2);
+  const bool inside = px < nx && py < ny;
+  const int64_t pix = (int64_t)py * nx + px;
+  const int64_t npix = (int64_t)nx * ny;
+  const bool tables_once = na <= TCH;
+  if (tables_once) {
+    if (threadIdx.x < na) lbase[threadIdx.x] = tbase[(int64_t)threadIdx.x * ntiles + tile];
+    for (int i = threadIdx.x; i < na * 16; i += 256) {
+      const int al = i >> 4, l = i & 15;
+      const int ppx = (tile % tnx) * 4 + (l & 3), ppy = (tile / tnx) * 4 + (l >> 2);
+      lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)al * npix + (int64_t)ppy * nx + ppx] : make_int4(0, 0, 0, 0);
+    }
+  }
+  for (int gi = 0; gi < G; ++gi) {
+    const int cb = grp * G + gi;
+    if (cb >= n_cb) break;                                // uniform over the workgroup
+    const int kq = cb * QB + q;
+    const bool live = inside && kq < k4_n;
+    const int64_t o = pix * kstride4 + kq + k4_off;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 zz = acc, xx = acc, tt = acc;
+    if (MODE == 1 && live) {
+      zz = ld_stream(z + o);
+      xx = ld_stream(x + o);
+      tt = ld_stream(T + o);
+    }
+    for (int t0 = 0; t0 < na; t0 += TCH) {
+      const int tn = min(TCH, na - t0);
+      if (!tables_once) {
+        __syncthreads();
+        if (threadIdx.x < tn) lbase[threadIdx.x] = tbase[(int64_t)(t0 + threadIdx.x) * ntiles + tile];
+        for (int i = threadIdx.x; i < tn * 16; i += 256) {
+          const int al = i >> 4, l = i & 15;
+          const int ppx = (tile % tnx) * 4 + (l & 3), ppy = (tile / tnx) * 4 + (l >> 2);
+          lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx]
+                                          : make_int4(0, 0, 0, 0);
+        }
+      }
+      for (int a0 = 0; a0 < tn; a0 += ACH) {
+        const int nan = min(ACH, tn - a0);
+        __syncthreads();
+        constexpr int PER = ACH * TB * QB / 256;
+        float4 v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int i = threadIdx.x + 256 * u;
+          const int qq = i & (QB - 1);
+          const int b = (i >> 4) % TB;
+          const int al = i / (TB * QB);
+          v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (al < nan) {
+            const int s = lbase[a0 + al] + b;
+            const int kk = cb * QB + qq;
+            if (s >= 0 && s < ns && kk < k4_n) v[u] = sino[((int64_t)(t0 + a0 + al) * ns + s) * skstride4 + kk];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) lds[threadIdx.x + 256 * u] = v[u];
+        __syncthreads();
+        if (live) {
+#pragma unroll 4
+          for (int al = 0; al < nan; ++al) {
+            const int4 e = lpt[(a0 + al) * 16 + li];
+            const int base = lbase[a0 + al];
+            const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
+            const float4 s0 = lds[(al * TB + (e.x - base)) * QB + q];
+            const float4 s1 = lds[(al * TB + (e.y - base)) * QB + q];
+            acc.x += w0 * s0.x + w1 * s1.x;
+            acc.y += w0 * s0.y + w1 * s1.y;
+            acc.z += w0 * s0.z + w1 * s1.z;
+            acc.w += w0 * s0.w + w1 * s1.w;
+          }
+        }
+      }
+    }
+    if (live) {
+      if (MODE == 0) {
+        st_stream(out + o, acc);
+      } else {
+        float4 zn, xn;
+#define DFM_UPD(c)                                                                  \
+  zn.c = zz.c + acc.c;                                                              \
+  xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * ((zn.c + inv_p * acc.c) + lam)) : 0.0f;
+        DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
+#undef DFM_UPD
+        st_stream(z + o, zn);
+        st_stream(x + o, xn);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------------------ dense PF
 // One thread per entry (r, k, c, p) of the batch; the sum over quadrature nodes and the ring's pole axes. Directions
 // and poles are float64 and so is the arithmetic: near a pole 1 - |n.v| is a difference of nearly equal numbers
@@ -983,21 +1103,60 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
                     skstride >= k_n && (mode == 0 || mode == 1),
                 "dfm_radon_backward_table: bad sizes");
-  DFM_CHECK_ARG(variant == 4 || variant == 8 || variant == 16 || variant == 32,
-                "dfm_radon_backward_table: variant (angles staged per pass) must be 4, 8, 16 or 32");
+  // variant = angles staged per pass (4, 8, 16, 32: a workgroup per tile and channel block), or ACH + 64 G with
+  // ACH 8 or 16 and G 2, 4 or 8: a workgroup per tile and G channel blocks, the tile's tables staged once
+  const int ach = variant % 64, grp = variant / 64;
+  DFM_CHECK_ARG((grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
+                    ((grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)),
+                "dfm_radon_backward_table: variant %d: angles staged per pass 4, 8, 16 or 32, or 4 / 8 / 16 + 64 G "
+                "for G = 2, 4, 8 channel blocks per workgroup", variant);
   if (k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int ntiles = ((nx + 3) / 4) * ((ny + 3) / 4);
+  if (grp > 0) {
+    const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
+    const unsigned ggrid = xcd_grid(ntiles, n_grp);
+#define DFM_BG(M, A, G)                                                                                           \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G>), dim3(ggrid), dim3(256), 0, (hipStream_t)stream,     \
+                     (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
+                     kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
+#define DFM_BGA(M, G)            \
+  if (ach == 4) {                \
+    DFM_BG(M, 4, G);             \
+  } else if (ach == 8) {         \
+    DFM_BG(M, 8, G);             \
+  } else {                       \
+    DFM_BG(M, 16, G);            \
+  }
+#define DFM_BGG(M)               \
+  if (grp == 2) {                \
+    DFM_BGA(M, 2)                \
+  } else if (grp == 4) {         \
+    DFM_BGA(M, 4)                \
+  } else {                       \
+    DFM_BGA(M, 8)                \
+  }
+    if (mode == 0) {
+      DFM_BGG(0)
+    } else {
+      DFM_BGG(1)
+    }
+#undef DFM_BGG
+#undef DFM_BGA
+#undef DFM_BG
+    DFM_LAUNCH_CHECK("dfm_radon_backward_table");
+    return DFM_OK;
+  }
   const unsigned grid = xcd_grid(ntiles, (k4n + QB - 1) / QB);
 #define DFM_BT(M, A)                                                                                              \
   hipLaunchKernelGGL((radon_backward_tile_kernel<M, A>), dim3(grid), dim3(256), 0, (hipStream_t)stream,          \
                      (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
   if (mode == 0) {
-    if (variant == 4) DFM_BT(0, 4); else if (variant == 8) DFM_BT(0, 8); else if (variant == 16) DFM_BT(0, 16);
+    if (ach == 4) DFM_BT(0, 4); else if (ach == 8) DFM_BT(0, 8); else if (ach == 16) DFM_BT(0, 16);
     else DFM_BT(0, 32);
   } else {
-    if (variant == 4) DFM_BT(1, 4); else if (variant == 8) DFM_BT(1, 8); else if (variant == 16) DFM_BT(1, 16);
+    if (ach == 4) DFM_BT(1, 4); else if (ach == 8) DFM_BT(1, 8); else if (ach == 16) DFM_BT(1, 16);
     else DFM_BT(1, 32);
   }
 #undef DFM_BT

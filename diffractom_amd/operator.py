@@ -718,6 +718,7 @@ class SinglePhaseForwardOperator:
                 saved["fwd_band"] = v
             out["radon_forward"] = (best, times)
             cands = {f"table:{a}": ("table", a) for a in (4, 8, 16)}
+            cands.update({f"table:{a}x{g}": ("table", a + 64 * g) for a, g in ((4, 4), (8, 2), (8, 4), (8, 8))})
             if self.lds_width():
                 cands["lds"] = ("lds", None)
             times = {}

@@ -125,7 +125,8 @@ def test_errors_raise_like_the_reference():
 
 @pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("table_staged", 4),
                                              ("table_capped", 4), ("table_pairs", 4), ("table_band", 4),
-                                             ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4), ("arith", 4)])
+                                             ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4),
+                                             ("table_group", 4), ("table_group16", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
     """Every projector kernel choice (LDS-resident image at 4, 2, 1 channels per workgroup; geometry tables;
@@ -148,6 +149,10 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_bands = 3
         op._tables_full = op._radon_tables(np.arange(op.N_Omega))
         op.radon_variant["fwd_kernel"] = "band"
+    elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
+        op.radon_variant["bwd"] = 8 + 64 * 4
+    elif kernel == "table_group16":              # ... 16 angles per pass, 2 channel blocks
+        op.radon_variant["bwd"] = 16 + 64 * 2
     elif kernel.startswith("table_ch"):          # channel blocks of 32, 16, 128 channels
         op.radon_variant["fwd"] = 8 + 256 + 512 * {"table_ch32": 1, "table_ch16": 2, "table_ch128": 3}[kernel]
     op.fused_channels = channels
