@@ -1023,15 +1023,15 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
   // variant = unr + 16 * cap: entries fetched ahead (1, 4, 8) and, cap > 0, at most cap workgroups per CU (an
   // otherwise unused LDS allocation of 160 KB / cap), so that fewer channel blocks share an XCD's L2
   // + 256: the workgroup's entries staged in LDS
-  // + 512 * q: channel block of 64 (q 0), 32 (q 1), 16 (q 2) or 128 (q 3) channels per workgroup (the image
-  //   slice an XCD streams for all angles at once: that many channels x the pixels)
+  // + 512 * q: channel block of 64 (q 0), 32 (q 1), 16 (q 2), 128 (q 3) or 256 (q 4) channels per workgroup (the
+  //   image slice an XCD streams for all angles at once: that many channels x the pixels)
   const int unr = variant % 16, cap = (variant / 16) % 16, staged = (variant / 256) % 2, q = variant / 512;
-  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && q <= 3,
+  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && q <= 4,
                 "dfm_radon_forward_table: variant %d: entries ahead 1, 4 or 8 (+ 16 * cap, cap 0 or 3 .. 15, + 256 "
-                "for LDS-staged entries, + 512 * q for 64, 32, 16 or 128 channels per block)", variant);
+                "for LDS-staged entries, + 512 * q for 64, 32, 16, 128 or 256 channels per block)", variant);
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
-  const int qb = q == 3 ? 2 * QB : QB >> q;
+  const int qb = q == 4 ? 4 * QB : q == 3 ? 2 * QB : QB >> q;
   const unsigned grid = xcd_grid(n_angles * ((ns + 256 / qb - 1) / (256 / qb)), (k4n + qb - 1) / qb);
   const size_t lds_cap = cap > 0 ? (size_t)(163840 / cap) & ~(size_t)255 : 0;
 #define DFM_FT(U, Q)                                                                                              \
@@ -1048,6 +1048,8 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
     DFM_FT(U, 16);        \
   } else if (qb == 32) {  \
     DFM_FT(U, 32);        \
+  } else if (qb == 64) {  \
+    DFM_FT(U, 64);        \
   } else if (qb == 8) {   \
     DFM_FT(U, 8);         \
   } else {                \

@@ -14,7 +14,6 @@
 namespace {
 
 constexpr int N_XCD = 8;
-constexpr int BQ = 16;     // channel quads per workgroup (64 channels)
 constexpr int BCH = 128;   // entries per bin staged per pass
 
 __device__ __forceinline__ bool xcd_split(int per_blk, int n_blk, int& blk, int& local) {
@@ -26,7 +25,7 @@ __device__ __forceinline__ bool xcd_split(int per_blk, int n_blk, int& blk, int&
   return blk < n_blk;
 }
 
-template <int UNR>
+template <int UNR, int BQ>
 __global__ void __launch_bounds__(256) radon_forward_band_kernel(const float4* __restrict__ img,
                                                                  float4* __restrict__ part,
                                                                  const int* __restrict__ band_ptr,
@@ -34,18 +33,19 @@ __global__ void __launch_bounds__(256) radon_forward_band_kernel(const float4* _
                                                                  const float* __restrict__ ent_w, int ns, int na,
                                                                  int nb, int kstride4, int k4_off, int k4_n,
                                                                  int pkstride4, int64_t pband4) {
-  __shared__ int lp[16][BCH + 1];
-  __shared__ float lw[16][BCH + 1];
+  constexpr int NBIN = 256 / BQ;                                   // bins per workgroup
+  __shared__ int lp[NBIN][BCH + 1];
+  __shared__ float lw[NBIN][BCH + 1];
   __shared__ int lmax[4];
-  const int sblocks = (ns + 15) / 16;
+  const int sblocks = (ns + NBIN - 1) / NBIN;
   const int n_cb = (k4_n + BQ - 1) / BQ;
   int blk, local;
   if (!xcd_split(na * sblocks, n_cb * nb, blk, local)) return;   // uniform over the workgroup
   const int cb = blk / nb, b = blk % nb;
   const int a = local / sblocks;
-  const int sl = threadIdx.x >> 4;                                 // bin of the workgroup
-  const int q = threadIdx.x & 15;                                  // channel quad of the block
-  const int s = (local % sblocks) * 16 + sl;
+  const int sl = threadIdx.x / BQ;                                 // bin of the workgroup
+  const int q = threadIdx.x & (BQ - 1);                            // channel quad of the block
+  const int s = (local % sblocks) * NBIN + sl;
   const int kq = cb * BQ + q;
   const bool live = kq < k4_n && s < ns;
   const int r = a * ns + min(s, ns - 1);
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) radon_forward_band_kernel(const float4* _
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   for (int c = 0; c < maxlen; c += BCH) {
     __syncthreads();
-    for (int j = q; j < BCH; j += 16) {                            // the 16 threads of bin sl copy its entries
+    for (int j = q; j < BCH; j += BQ) {                            // the BQ threads of bin sl copy its entries
       const int e = e0 + c + j;
       if (e < e1) {
         lp[sl][j] = ent_pix[e];
@@ -137,23 +137,38 @@ int dfm_radon_forward_band(const float* img_k, float* sino, float* part, const i
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
                     skstride >= k_n,
                 "dfm_radon_forward_band: bad sizes");
-  DFM_CHECK_ARG(variant == 4 || variant == 8, "dfm_radon_forward_band: variant %d: entries ahead 4 or 8", variant);
+  // variant = entries ahead (4, 8) + 16 * w: 64 (w 0), 128 (w 1) or 256 (w 2) channels per workgroup
+  const int unr = variant % 16, wsel = variant / 16;
+  DFM_CHECK_ARG((unr == 4 || unr == 8) && wsel <= 2,
+                "dfm_radon_forward_band: variant %d: entries ahead 4 or 8, + 16 w for 64 << w channels", variant);
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
-  const int per_blk = n_angles * ((ns + 15) / 16);
+  const int BQ = 16 << wsel;
+  const int per_blk = n_angles * ((ns + 256 / BQ - 1) / (256 / BQ));
   const int n_blk = (k4n + BQ - 1) / BQ * nb;
   const unsigned grid = (unsigned)((n_blk + N_XCD - 1) / N_XCD * N_XCD * per_blk);
   const int64_t n_rows = (int64_t)n_angles * ns;
   const int64_t pband4 = n_rows * (skstride / 4);
   const hipStream_t st = (hipStream_t)stream;
-  if (variant == 4)
-    hipLaunchKernelGGL(radon_forward_band_kernel<4>, dim3(grid), dim3(256), 0, st, (const float4*)img_k,
-                       (float4*)part, band_ptr, ent_pix, ent_w, ns, n_angles, nb, kstride / 4, k_off / 4, k4n,
-                       skstride / 4, pband4);
-  else
-    hipLaunchKernelGGL(radon_forward_band_kernel<8>, dim3(grid), dim3(256), 0, st, (const float4*)img_k,
-                       (float4*)part, band_ptr, ent_pix, ent_w, ns, n_angles, nb, kstride / 4, k_off / 4, k4n,
-                       skstride / 4, pband4);
+#define DFM_FB(U, Q)                                                                                              \
+  hipLaunchKernelGGL((radon_forward_band_kernel<U, Q>), dim3(grid), dim3(256), 0, st, (const float4*)img_k,      \
+                     (float4*)part, band_ptr, ent_pix, ent_w, ns, n_angles, nb, kstride / 4, k_off / 4, k4n,        \
+                     skstride / 4, pband4)
+#define DFM_FBQ(U)        \
+  if (BQ == 16) {         \
+    DFM_FB(U, 16);        \
+  } else if (BQ == 32) {  \
+    DFM_FB(U, 32);        \
+  } else {                \
+    DFM_FB(U, 64);        \
+  }
+  if (unr == 4) {
+    DFM_FBQ(4)
+  } else {
+    DFM_FBQ(8)
+  }
+#undef DFM_FBQ
+#undef DFM_FB
   DFM_LAUNCH_CHECK("dfm_radon_forward_band");
   const int64_t total = n_rows * k4n;
   hipLaunchKernelGGL(band_reduce_kernel, dim3(dfm::grid_stride_blocks(total, 256, 1u << 16)), dim3(256), 0, st,

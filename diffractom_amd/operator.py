@@ -691,9 +691,11 @@ class SinglePhaseForwardOperator:
         if tab is not None:
             cands = {"table:8": ("table", 8), "table:8+lds": ("table", 8 + 256), "table:4+lds": ("table", 4 + 256),
                      "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "table:8+lds+ch32": ("table", 8 + 256 + 512),
-                     "table:8+lds+ch128": ("table", 8 + 256 + 1536), "pairs:8": ("pairs", 8)}
+                     "table:8+lds+ch128": ("table", 8 + 256 + 1536), "table:8+lds+ch256": ("table", 8 + 256 + 2048),
+                     "pairs:8": ("pairs", 8)}
             if self.radon_bands > 1:
-                cands.update({"band:8": ("band", 8), "band:4": ("band", 4)})
+                cands.update({"band:8": ("band", 8), "band:4": ("band", 4), "band:8+ch128": ("band", 8 + 16),
+                              "band:8+ch256": ("band", 8 + 32)})
             if self.lds_width():
                 cands["lds"] = ("lds", None)
             times = {}

@@ -126,6 +126,7 @@ def test_errors_raise_like_the_reference():
 @pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("table_staged", 4),
                                              ("table_capped", 4), ("table_pairs", 4), ("table_band", 4),
                                              ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4),
+                                             ("table_ch256", 4), ("table_band128", 4), ("table_band256", 4),
                                              ("table_group", 4), ("table_group16", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
@@ -145,16 +146,18 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["fwd"] = 4 + 16 * 6
     elif kernel == "table_pairs":                # the bin-pair table forward
         op.radon_variant["fwd_kernel"] = "pairs"
-    elif kernel == "table_band":                 # forward tiled by 3 pixel bands, band partial sums
+    elif kernel.startswith("table_band"):        # forward tiled by 3 pixel bands, band partial sums
         op.radon_bands = 3
         op._tables_full = op._radon_tables(np.arange(op.N_Omega))
         op.radon_variant["fwd_kernel"] = "band"
+        op.radon_variant["fwd_band"] = 8 + 16 * {"table_band": 0, "table_band128": 1, "table_band256": 2}[kernel]
     elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
         op.radon_variant["bwd"] = 8 + 64 * 4
     elif kernel == "table_group16":              # ... 16 angles per pass, 2 channel blocks
         op.radon_variant["bwd"] = 16 + 64 * 2
-    elif kernel.startswith("table_ch"):          # channel blocks of 32, 16, 128 channels
-        op.radon_variant["fwd"] = 8 + 256 + 512 * {"table_ch32": 1, "table_ch16": 2, "table_ch128": 3}[kernel]
+    elif kernel.startswith("table_ch"):          # channel blocks of 32, 16, 128, 256 channels
+        op.radon_variant["fwd"] = 8 + 256 + 512 * {"table_ch32": 1, "table_ch16": 2, "table_ch128": 3,
+                                                    "table_ch256": 4}[kernel]
     op.fused_channels = channels
     if kernel == "lds":
         assert op.lds_width() == channels
