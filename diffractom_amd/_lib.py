@@ -34,6 +34,8 @@ SIGNATURES = {
     "dfm_pf_fill_adj": [P, P, P, P, I32, I32, I32, I32, P],
     "dfm_spmm_forward": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_spmm_forward_split": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P, I32, I32, P],
+    "dfm_radon_forward_sweep": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
+    "dfm_radon_sweep_lines": [I32, I32],
     "dfm_radon_forward_band": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_spmm_t": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P],
     "dfm_spmm_t_rows_per_block": [I32],

@@ -146,10 +146,28 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
         np.maximum.at(top_[a], tile, hi[a])
     if np.any(top_ - base >= 8):
         raise AssertionError("a 4 x 4 pixel tile reaches more than 8 bins at some angle")
+    # line sweep: per angle the sweep class (0: image rows, |X| >= |Y|; 1: image columns) and its float64 geometry
+    cls = (np.abs(g["X"][idx]) < np.abs(g["Y"][idx])).astype(np.int32)
     bh = -(-ny // n_bands)
     band_ptr = np.zeros(na * ns * n_bands + 1, np.int64)
     np.cumsum(np.bincount(rows * n_bands + (pix // nx) // bh, minlength=na * ns * n_bands), out=band_ptr[1:])
+    # angle groups of one sweep class, as many angles as 1024 (and 512) rays hold (dfm_radon_forward_sweep)
+    def sweep_groups(rays):
+        na_grp = max(1, rays // ns)
+        groups = [(c, grp) for c in (0, 1) for grp in np.array_split(np.flatnonzero(cls == c),
+                                                                      -(-int((cls == c).sum()) // na_grp))]
+        ga = np.full((max(len(groups), 1), na_grp), -1, np.int32)
+        gc = np.zeros(max(len(groups), 1), np.int32)
+        for i, (c, grp) in enumerate(groups):
+            ga[i, :len(grp)] = grp
+            gc[i] = c
+        return ga, gc
+    sweep_grp, sweep_gcls = sweep_groups(1024)
+    sweep_grp_h, sweep_gcls_h = sweep_groups(512)
     return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32), "band_ptr": band_ptr.astype(np.int32),
+            "sweep_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
+                                   radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
+            "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
             "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
             "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
             "pair_w": pw.astype(np.float32)}
@@ -696,6 +714,13 @@ class SinglePhaseForwardOperator:
         saved = dict(rv)
         if tab is not None:
             cands = {}
+            sl = _lib.load().dfm_radon_sweep_lines(self.Nx, self.Ny)
+            if sl and self.My <= 1024:         # 1024 rays per workgroup (512 for the 64-orientation form)
+                cands.update({f"sweep:{n}": ("sweep", n) for n in (8, 4, 2) if n <= sl})
+                ln = max(self.Nx, self.Ny)
+                for n, w, cq in ((2, 1, 8), (1, 1, 8), (2, 2 + 4, 16), (1, 2 + 4, 16)):
+                    if 2 * n * ln * (cq + 1) * 16 <= 163840 and (w < 4 or self.My <= 512):
+                        cands[f"sweep:{n}+ch{4 * cq}"] = ("sweep", n + 16 * w)
             if self.radon_bands > 1:
                 cands.update({"band:8+ch128": ("band", 8 + 16), "band:8": ("band", 8), "band:4": ("band", 4),
                               "band:8+ch256": ("band", 8 + 32)})
@@ -714,6 +739,8 @@ class SinglePhaseForwardOperator:
                     rv["fwd_pairs"] = v
                 elif kern == "band":
                     rv["fwd_band"] = v
+                elif kern == "sweep":
+                    rv["fwd_sweep"] = v
                 times[name] = timeit(lambda: self._radon_forward(x, self.K_stride, b["koff"], b["kstride"],
                                                                  sc["sino"], geo_f, n_ang, tab))
             best = _pick(times)
@@ -725,6 +752,8 @@ class SinglePhaseForwardOperator:
                 saved["fwd_pairs"] = v
             elif kern == "band":
                 saved["fwd_band"] = v
+            elif kern == "sweep":
+                saved["fwd_sweep"] = v
             out["radon_forward"] = (best, times)
             cands = {f"table:{a}x{g}": ("table", a + 64 * g) for a, g in ((4, 4), (4, 2), (8, 4), (8, 2), (8, 8))}
             cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})
@@ -825,6 +854,14 @@ class SinglePhaseForwardOperator:
         if cw and koff % cw == 0 and kn % cw == 0:
             self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
                                 kn, cw)
+            return
+        if tab is not None and self.radon_variant["fwd_kernel"] == "sweep":
+            lines = self.radon_variant.get("fwd_sweep", 0) or _lib.load().dfm_radon_sweep_lines(self.Nx, self.Ny)
+            h = "_h" if lines >= 64 else ""
+            g = tab["sweep_grp" + h]
+            _lib.call("dfm_radon_forward_sweep", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["sweep_geo"]),
+                      _lib.ptr(g), _lib.ptr(tab["sweep_gcls" + h]), g.shape[0], g.shape[1], self.My, self.Nx,
+                      self.Ny, img_kstride, koff, kn, kn, int(lines), self._stream())
             return
         if tab is not None and self.radon_variant["fwd_kernel"] == "band":
             nb = self.radon_bands

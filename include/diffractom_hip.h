@@ -79,6 +79,19 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
 int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
                             const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
                             int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
+/* The forward projection (radon_kernels.cl:44-101 radon_forward_k) by line sweep: per angle group (grp_ang: n_grp rows
+ * of na_grp angle indices, -1 for none; grp_cls: 0 = sweep image rows, for |X| >= |Y|, 1 = columns) and 16
+ * orientations, `lines` image lines are staged in LDS at a time and each ray adds its pixels on them; geo: per angle
+ * (X, Y, T0, hw, top, scale) in float64, the weight scale * min(hw - |X x + Y y + T0 - s|, top) where positive.
+ * Needs nx, ny <= 256; lines: image lines per LDS block (1, 2, 4 or 8, within the LDS: dfm_radon_sweep_lines)
+ * + 16 w for 16 << w orientations per workgroup (w 0, 1, 2) + 64 for workgroups of 512 threads (w = 2 only);
+ * na_grp * ns <= the workgroup's threads. */
+int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
+                            const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx, int32_t ny,
+                            int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t lines,
+                            dfm_stream_t stream);
+/* The most image lines per LDS block of dfm_radon_forward_sweep for an nx x ny image (0: unsupported). */
+int dfm_radon_sweep_lines(int32_t nx, int32_t ny);
 /* The forward projection of dfm_radon_forward_table tiled by nb bands of image rows: band_ptr[(a ns + s) nb + b]
  * is the first table entry of ray (a, s) in band b (entries ascend in pixel index), band_ptr[na ns nb] the total.
  * Per band the partial sums go to part (nb * n_angles * ns * skstride floats), then sino = their sum in band order.

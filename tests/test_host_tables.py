@@ -93,3 +93,41 @@ def test_band_pointers_split_each_ray_by_image_rows(n_bands):
         for b in range(n_bands):
             rows = tab["pix"][bp[r * n_bands + b]:bp[r * n_bands + b + 1]] // nx
             assert np.all((rows >= b * h) & (rows < (b + 1) * h))
+
+
+@pytest.mark.parametrize("shape", [(14, 12, 17, (0, 360)), (16, 16, 17, (0, 180))])
+def test_sweep_geometry_reproduces_the_table(shape):
+    """The line sweep's rule (pixels of line L with |coef p + cross L + T0 - s| < hw, weight scale * min(hw - |d|,
+    top); rows for |X| >= |Y|, columns otherwise) lists exactly the table's pixels and weights of every ray, and every
+    angle sits in one group of its class with at most 1024 rays per group."""
+    nx, ny, my, ar = shape
+    op = O.OracleOperator(cfg(nx=nx, ny=ny, my=my, n_omega=16, angle_range=ar, cor=0.7), (4.05,) * 3 + (90,) * 3,
+                          "cubic", AL3, 0.35, orientations(2, 1), np.deg2rad(4.0), orthonormalize=False)
+    idx = np.arange(16)
+    g = _geo(op, "trapezoid")
+    tab = radon_tables(g, idx, nx, ny, my, op.scale)
+    rp, geo = tab["row_ptr"], tab["sweep_geo"]
+    cls = (np.abs(g["X"][idx]) < np.abs(g["Y"][idx])).astype(int)
+    for r in range(len(idx) * my):
+        a, s = divmod(r, my)
+        X, Y, T0, hw, top, scale = geo[a]
+        coef, cross, ln, nl = (Y, X, ny, nx) if cls[a] else (X, Y, nx, ny)
+        pix, wts = [], []
+        for L in range(nl):
+            c = cross * L + T0 - s
+            e0, e1 = (-hw - c) / coef, (hw - c) / coef
+            for p in range(max(int(np.ceil(min(e0, e1))), 0), min(int(np.floor(max(e0, e1))), ln - 1) + 1):
+                k = hw - abs(coef * p + c)
+                if k > 0:
+                    pix.append(p * nx + L if cls[a] else L * nx + p)
+                    wts.append(np.float32(scale * min(k, top)))
+        ref = tab["pix"][rp[r]:rp[r + 1]]
+        o = np.argsort(pix)
+        np.testing.assert_array_equal(np.asarray(pix)[o], np.sort(ref))
+        np.testing.assert_allclose(np.asarray(wts, np.float32)[o], tab["w"][rp[r]:rp[r + 1]][np.argsort(ref)],
+                                   rtol=1e-6, atol=1e-7)
+    grp, gcls = tab["sweep_grp"], tab["sweep_gcls"]
+    assert grp.shape[1] * my <= 1024
+    np.testing.assert_array_equal(np.sort(grp[grp >= 0]), idx)
+    for row, c in zip(grp, gcls):
+        assert np.all(cls[row[row >= 0]] == c)
