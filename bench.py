@@ -245,6 +245,9 @@ def run_c3(args):
             f"{torch.cuda.memory_allocated() / 1e9:.0f} GB)")
         bat = BatchedSPDHGPoisson(ops, subsets=C3["subsets"], support=masks, seed=0)
         bat.start(None, bs)
+        if not args.no_autotune:
+            tuning = bat.autotune()
+            log("[bench] al_c3: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
         torch.cuda.synchronize()
         bat.advance(args.warmup)
         torch.cuda.synchronize()

@@ -186,6 +186,18 @@ class BatchedSPDHGPoisson:
         self._groups[key] = (buf, wgm, len(wg))
         return self._groups[key]
 
+    def autotune(self, reps=3):
+        """Time the projector variants of the first slice's operator on this batch's buffers (all slices'
+        orientations at once; subset 0) and keep the fastest, as SinglePhaseForwardOperator.autotune does for one
+        slice; the backprojection is timed on copies of x and z, so the solver state is untouched. Needs start()."""
+        from .operator import event_timer
+        o0 = self.ops[0]
+        xc, zc = self.xk.clone(), self.zk.clone()
+        self.tuning = o0.tune_projector(0, self.xk, self.Kc, 0, self.Kc, self.sino, xc, zc, self.Tk,
+                                        event_timer(o0.device, reps))
+        del xc, zc
+        return self.tuning
+
     def _pf(self, forward, m):
         buf, wgm, n_wg = self._group(forward, m)
         _lib.call("dfm_mfma_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg, int(self.residual),

@@ -79,6 +79,22 @@ def as_host(a, shape, name):
     return out
 
 
+def event_timer(device, reps):
+    """fn -> median HIP-event time (ms) of reps launches after 2 warm-up launches."""
+    def timeit(fn):
+        ts = []
+        for i in range(reps + 2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize(device)
+            if i >= 2:
+                ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+    return timeit
+
+
 def _pick(times, tol=0.02):
     """The fastest candidate, or the first-listed one within tol of it (timing noise does not flip the choice)."""
     t_min = min(times.values())
@@ -675,40 +691,12 @@ class SinglePhaseForwardOperator:
                       float(lam), st)
 
     # ================================================================== autotuner
-    def autotune(self, subset=0, reps=5):
-        """Pick each product's fastest kernel variant by timing it on this operator's shapes (HIP events, median of
-        reps after 2 warm-up launches), as the reference times its tiled and plain projectors
-        (parallel_radon.py:240-281). Every candidate computes the same values (up to float32 summation order).
-        Candidates: forward projection (geometry table with 4 or 8 entries fetched ahead, staged in LDS or not,
-        capped occupancy; bin-pair table; LDS-resident image), backprojection with the SPDHG update (4, 8 or 16
-        angles staged per pass; LDS-resident image), the matrix-core PF products (2 or 4 output tiles per wave).
-        Returns {product: (choice, {candidate: ms})} and keeps it in self.tuning."""
-        if self.omega_subsets is None:
-            self.set_omega_subsets(10)
+    def tune_projector(self, subset, img, img_kstride, koff, kn, sino, x, z, T, timeit):
+        """Time the projector candidates on the given buffers (img: orientation-fastest image(s) with stride
+        img_kstride, columns koff .. koff + kn; sino: scratch of the subset's angles; x, z, T: the backprojection's
+        SPDHG update operands, overwritten) and keep the fastest in radon_variant; {product: (choice, times)}."""
         n_ang, _, geo_f, geo_b = self._angle_set(subset)
         tab = self._tables(subset)
-        sc = self._scratch_buffers(n_ang)
-        b = self.batches[0]
-        dev = self.device
-        npix = self.Nx * self.Ny
-        g = torch.Generator(device=dev)
-        g.manual_seed(0)
-        x = torch.rand(npix * self.K_stride, generator=g, device=dev)
-        z = torch.zeros_like(x)
-        T = torch.zeros_like(x)
-
-        def timeit(fn):
-            ts = []
-            for i in range(reps + 2):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                fn()
-                e1.record()
-                torch.cuda.synchronize(dev)
-                if i >= 2:
-                    ts.append(e0.elapsed_time(e1))
-            return float(np.median(ts))
-
         out = {}
         rv = self.radon_variant
         saved = dict(rv)
@@ -741,8 +729,7 @@ class SinglePhaseForwardOperator:
                     rv["fwd_band"] = v
                 elif kern == "sweep":
                     rv["fwd_sweep"] = v
-                times[name] = timeit(lambda: self._radon_forward(x, self.K_stride, b["koff"], b["kstride"],
-                                                                 sc["sino"], geo_f, n_ang, tab))
+                times[name] = timeit(lambda: self._radon_forward(img, img_kstride, koff, kn, sino, geo_f, n_ang, tab))
             best = _pick(times)
             kern, v = cands[best]
             saved["fwd_kernel"] = kern
@@ -764,8 +751,8 @@ class SinglePhaseForwardOperator:
                 rv["bwd_kernel"] = kern
                 if v is not None:
                     rv["bwd"] = v
-                times[name] = timeit(lambda: self._radon_backward(sc["sino"], geo_b, tab, n_ang, None, self.K_stride,
-                                                                  b["koff"], b["kstride"], x=x, z=z, T=T, inv_p=1.0))
+                times[name] = timeit(lambda: self._radon_backward(sino, geo_b, tab, n_ang, None, img_kstride, koff, kn,
+                                                                  x=x, z=z, T=T, inv_p=1.0))
             best = _pick(times)
             kern, v = cands[best]
             saved["bwd_kernel"] = kern
@@ -773,6 +760,34 @@ class SinglePhaseForwardOperator:
                 saved["bwd"] = v
             out["radon_backward_spdhg"] = (best, times)
         self.radon_variant.update(saved)
+        return out
+
+    def autotune(self, subset=0, reps=5):
+        """Pick each product's fastest kernel variant by timing it on this operator's shapes (HIP events, median of
+        reps after 2 warm-up launches), as the reference times its tiled and plain projectors
+        (parallel_radon.py:240-281). Every candidate computes the same values (up to float32 summation order).
+        Candidates: forward projection (geometry table with 4 or 8 entries fetched ahead, staged in LDS or not,
+        capped occupancy; bin-pair table; LDS-resident image), backprojection with the SPDHG update (4, 8 or 16
+        angles staged per pass; LDS-resident image), the matrix-core PF products (2 or 4 output tiles per wave).
+        Returns {product: (choice, {candidate: ms})} and keeps it in self.tuning."""
+        if self.omega_subsets is None:
+            self.set_omega_subsets(10)
+        n_ang, _, geo_f, geo_b = self._angle_set(subset)
+        tab = self._tables(subset)
+        sc = self._scratch_buffers(n_ang)
+        b = self.batches[0]
+        dev = self.device
+        npix = self.Nx * self.Ny
+        g = torch.Generator(device=dev)
+        g.manual_seed(0)
+        x = torch.rand(npix * self.K_stride, generator=g, device=dev)
+        z = torch.zeros_like(x)
+        T = torch.zeros_like(x)
+
+        timeit = event_timer(dev, reps)
+
+        out = self.tune_projector(subset, x, self.K_stride, b["koff"], b["kstride"], sc["sino"], x.clone(), z, T,
+                                  timeit)
         if self.sparse_batches is not None:
             data = torch.rand(self.data_shape, generator=g, device=dev)
             times = {}

@@ -43,3 +43,33 @@ def test_batch_equals_single_slices(precision):
         tol_x, tol_f = (1e-5, 1e-6) if precision == "bf16_residual" else (1e-3, 1e-4)
         assert err < tol_x
         assert abs(f_b[i] / sol.iter_stats[-1]["f"] - 1) < tol_f
+
+
+def test_batch_autotune_keeps_the_iterates():
+    """The batch's projector autotune (times every forward / backward candidate on the batched buffers, the
+    backprojection on copies of x and z) leaves the solver state untouched, and the tuned steps give the same iterates
+    as the default kernels up to float32 summation order."""
+    import torch
+    from diffractom_amd.batched import BatchedSPDHGPoisson
+    print("case batched autotune", flush=True)
+    c = cfg(nx=20, ny=20, my=21, n_omega=24, angle_range=(0, 180), cor=0.3)
+    ops, bs = [], []
+    rng = np.random.default_rng(5)
+    for i, K in enumerate((40, 90)):
+        op, orc, _ = build(c, K=K, seed=7 + i, pf_mode="dense", pf_precision="bf16_residual")
+        xt = (rng.random(op.coeff_shape) * op.support_mask()[None]).astype(np.float32)
+        ax = orc.direct(xt)
+        bs.append(rng.poisson(ax / max(ax.mean(), 1e-9) * 40).astype(np.float32))
+        ops.append(op)
+    ref = BatchedSPDHGPoisson(ops, subsets=4, seed=2)
+    ref.start(None, bs)
+    ref.advance(10)
+    x_ref = ref.xk.clone()
+    bat = BatchedSPDHGPoisson(ops, subsets=4, seed=2)
+    bat.start(None, bs)
+    x0, z0 = bat.xk.clone(), bat.zk.clone()
+    tuning = bat.autotune(reps=1)
+    assert "radon_forward" in tuning and "radon_backward_spdhg" in tuning
+    assert torch.equal(bat.xk, x0) and torch.equal(bat.zk, z0)
+    bat.advance(10)
+    assert rel_l2(bat.xk.cpu().numpy(), x_ref.cpu().numpy()) < 1e-5
