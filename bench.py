@@ -356,6 +356,8 @@ def run_c4(args):
         R = jittered_grid(5000, 100, 1.0, "cubic", int(rng.integers(1 << 31)))
         op = SinglePhaseForwardOperator(cfg, al_material(), MatrixGrid(R, np.deg2rad(2.0)), max_gb=1024.0,
                                         normalized=True, pf_mode="dense", pf_precision="bf16_residual")
+        if not args.no_autotune:
+            op.autotune(subset=0, reps=3)
         xt = voronoi_truth(op.K, 128, 60, 55.0, int(rng.integers(1 << 31)), coeff=(0.5, 1.5),
                            orientations=rng.choice(100, 60, replace=False))
         b, _ = poisson_counts(op.direct(xt), 1000.0, i)

@@ -34,6 +34,7 @@ SIGNATURES = {
     "dfm_pf_fill_adj": [P, P, P, P, I32, I32, I32, I32, P],
     "dfm_spmm_forward": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_spmm_forward_split": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P, I32, I32, P],
+    "dfm_smooth_tv": [P, P, I32, I32, I32, F32, P, P, P],
     "dfm_radon_forward_sweep": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_sweep_lines": [I32, I32],
     "dfm_radon_forward_band": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P],
@@ -113,6 +114,9 @@ def call(name, *args):
             raise MemoryError(msg)
         raise RuntimeError(msg)
     return st
+
+
+MAX_PARTIALS = 2048   # DFM_MAX_PARTIALS of the header: partial sums a reduction entry point writes at most
 
 
 def ptr(t):

@@ -284,6 +284,37 @@ __global__ void __launch_bounds__(256) tv_value_kernel(const float4* __restrict_
   block_sum<1>(s, partial, gridDim.x);
 }
 
+// Smoothed isotropic TV of coefficient images (K, ny, nx) (a differentiable loss term, differentiable/ loss of the
+// C4 refinement): value sum_{k,y,x} sqrt(dx^2 + dy^2 + eps) with forward differences (0 at the last column / row),
+// and its gradient, in one pass: grad(y, x) = -(dx + dy)/s(y, x) + dx(y, x-1)/s(y, x-1) + dy(y-1, x)/s(y-1, x).
+__global__ void __launch_bounds__(256) smooth_tv_kernel(const float* __restrict__ c, float* __restrict__ grad,
+                                                        int nx, int ny, int64_t n, float eps,
+                                                        double* __restrict__ partial) {
+  double s[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % nx);
+    const int y = (int)((i / nx) % ny);
+    const float v = c[i];
+    const bool r = x + 1 < nx, d = y + 1 < ny;
+    const float dx = r ? c[i + 1] - v : 0.0f, dy = d ? c[i + nx] - v : 0.0f;
+    const float sv = sqrtf(dx * dx + dy * dy + eps);
+    s[0] += (double)sv;
+    float g = -(dx + dy) / sv;
+    if (x > 0) {
+      const float l = c[i - 1];
+      const float dxl = v - l, dyl = d ? c[i + nx - 1] - l : 0.0f;
+      g += dxl / sqrtf(dxl * dxl + dyl * dyl + eps);
+    }
+    if (y > 0) {
+      const float u = c[i - nx];
+      const float dyu = v - u, dxu = r ? c[i - nx + 1] - u : 0.0f;
+      g += dyu / sqrtf(dxu * dxu + dyu * dyu + eps);
+    }
+    grad[i] = g;
+  }
+  block_sum<1>(s, partial, gridDim.x);
+}
+
 static unsigned reduce_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b < 1) b = 1;
@@ -457,6 +488,18 @@ int dfm_tv_value_partial(const float* x, const float* T, int32_t nx, int32_t ny,
   hipLaunchKernelGGL(tv_value_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const float4*)x,
                      (const float4*)T, nx, ny, kstride / 4, partial);
   DFM_LAUNCH_CHECK("dfm_tv_value_partial");
+  return DFM_OK;
+}
+
+int dfm_smooth_tv(const float* c, float* grad, int32_t K, int32_t ny, int32_t nx, float eps, double* partial,
+                  int32_t* n_partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(c && grad && partial && n_partial, "dfm_smooth_tv: null pointer");
+  DFM_CHECK_ARG(K >= 0 && ny > 0 && nx > 0 && eps >= 0.0f, "dfm_smooth_tv: bad sizes");
+  const int64_t n = (int64_t)K * ny * nx;
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(smooth_tv_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, c, grad, nx, ny, n, eps, partial);
+  DFM_LAUNCH_CHECK("dfm_smooth_tv");
   return DFM_OK;
 }
 

@@ -9,6 +9,8 @@ parameters it is the reduction of (dL/dd) * u over the other axes (data_model.py
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 GROUPS = ("angle_scale", "trans_scale", "ring_scale", "background")
@@ -100,9 +102,35 @@ def poisson_nll(mu, b, weights=None, eps=1e-3):
     return term.sum()
 
 
+class _SmoothTV(torch.autograd.Function):
+    """total_variation of float32 device coefficients by the HIP kernel (value with float64 partial sums, gradient
+    in the same pass)."""
+
+    @staticmethod
+    def forward(ctx, coeffs, eps):
+        from . import _lib
+        c = coeffs.detach().contiguous()
+        grad = torch.empty_like(c)
+        part = torch.empty(_lib.MAX_PARTIALS, dtype=torch.float64, device=c.device)
+        n = ctypes.c_int32(0)
+        K, ny, nx = c.shape
+        _lib.call("dfm_smooth_tv", _lib.ptr(c), _lib.ptr(grad), K, ny, nx, float(eps), _lib.ptr(part),
+                  ctypes.byref(n), _lib.stream_handle())
+        ctx.save_for_backward(grad)
+        return part[:n.value].sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return grad * g.to(grad.dtype), None
+
+
 def total_variation(coeffs, eps=1e-8):
     """Isotropic forward-difference TV of every orientation image (K, Ny, Nx): sum sqrt(dx^2 + dy^2 + eps), the
-    last row / column differences zero."""
+    last row / column differences zero. float32 device tensors: one HIP pass (value summed in float64, gradient kept
+    for the backward); otherwise in float64 with torch."""
+    if coeffs.is_cuda and coeffs.dtype == torch.float32 and coeffs.dim() == 3:
+        return _SmoothTV.apply(coeffs, eps)
     c = coeffs.to(torch.float64)
     dx = torch.zeros_like(c)
     dy = torch.zeros_like(c)

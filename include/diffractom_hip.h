@@ -289,6 +289,12 @@ int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t
                              int32_t* n_partial, dfm_stream_t stream);
 int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
                     dfm_stream_t stream);
+/* Smoothed isotropic TV of coefficient images c (K, ny, nx) float32, a loss term of the differentiable refinement
+ * (differentiable/data_model.py:20-64 is the intensity model it is used with): partial[b] (b < *n_partial <=
+ * DFM_MAX_PARTIALS, float64) sum to sum sqrt(dx^2 + dy^2 + eps) over forward differences (0 at the last column and
+ * row); grad (K, ny, nx) = its gradient with respect to c. */
+int dfm_smooth_tv(const float* c, float* grad, int32_t K, int32_t ny, int32_t nx, float eps, double* partial,
+                  int32_t* n_partial, dfm_stream_t stream);
 /* Total-variation block of SPDHG (spdhg_poisson.py:108-193 tv_dual_dense / tv_adjoint_dense / tv_value_dense) on the
  * channel-fastest arrays (Npix, kstride), pixel p = ix + nx iy; a forward difference (right, down) is free when T > 0
  * at both ends.

@@ -103,3 +103,22 @@ def test_loss_gradients_vs_fp64_finite_differences():
     err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
     print(f"directional derivatives: relative L2 {err:.2e}", flush=True)
     assert err < 1e-4
+
+
+def test_total_variation_kernel_matches_float64():
+    """The HIP smooth-TV pass (float32 device coefficients: value with float64 partial sums, gradient in the same
+    pass) against the float64 torch formula and its autograd gradient on the CPU."""
+    import torch
+    from diffractom_amd.differentiable import total_variation
+    g = torch.Generator().manual_seed(2)
+    c = torch.rand((3, 9, 11), generator=g, dtype=torch.float64)
+    c[1, 2:5, 3:7] = 0.25                                  # flat patches: the eps branch
+    cd = c.float().cuda().requires_grad_(True)
+    v = total_variation(cd)
+    v.backward()
+    cc = c.float().double().requires_grad_(True)
+    v64 = total_variation(cc)
+    v64.backward()
+    assert abs(float(v) / float(v64) - 1) < 1e-6
+    err = (cd.grad.double().cpu() - cc.grad).abs().max() / cc.grad.abs().max()
+    assert float(err) < 1e-5, float(err)
