@@ -246,7 +246,7 @@ class SinglePhaseForwardOperator:
             raise ValueError(f"pf_precision {pf_precision!r} runs the dense PF product on the matrix cores; "
                              f"pf_mode must be 'dense' or 'auto', not {pf_mode!r}")
         self.pf_precision = pf_precision
-        self.mfma_variant = {"fwd": 2, "adj": 2}
+        self.mfma_variant = {"fwd": 2, "adj": 2, "fwd_waves": 2048}
         self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0}
 
         # ---- geometry (single_phase_forward_operator.py:376-441)
@@ -811,15 +811,24 @@ class SinglePhaseForwardOperator:
             out["pf_adjoint"] = (best, times)
         if self.mfma_batches is not None:
             data = torch.rand(self.data_shape, generator=g, device=dev)
-            for key, fn in (("fwd", lambda: self._dense_forward(0, sc["sino"], data, subset, False)),
-                            ("adj", lambda: self._dense_adjoint(0, data, sc["sino"], subset, 1.0))):
-                times = {}
-                for v in (2, 4):
-                    self.mfma_variant[key] = v
-                    times[f"tiles_per_wave:{v}"] = timeit(fn)
-                best = _pick(times)
-                self.mfma_variant[key] = int(best.split(":")[1])
-                out[f"pf_{'forward' if key == 'fwd' else 'adjoint'}"] = (best, times)
+            fwd = lambda: self._dense_forward(0, sc["sino"], data, subset, False)
+            times, cands = {}, {}
+            for v in (2, 4):
+                for w in (2048, 4096, 8192, 16384):
+                    cands[f"tiles_per_wave:{v}" + ("" if w == 2048 else f"+waves:{w}")] = (v, w)
+            for name, (v, w) in cands.items():
+                self.mfma_variant.update(fwd=v, fwd_waves=w)
+                times[name] = timeit(fwd)
+            best = _pick(times)
+            self.mfma_variant.update(fwd=cands[best][0], fwd_waves=cands[best][1])
+            out["pf_forward"] = (best, times)
+            times = {}
+            for v in (2, 4):
+                self.mfma_variant["adj"] = v
+                times[f"tiles_per_wave:{v}"] = timeit(lambda: self._dense_adjoint(0, data, sc["sino"], subset, 1.0))
+            best = _pick(times)
+            self.mfma_variant["adj"] = int(best.split(":")[1])
+            out["pf_adjoint"] = (best, times)
         del x, z, T
         self.tuning = out
         return out
@@ -988,11 +997,12 @@ class SinglePhaseForwardOperator:
                   my * ks, 1, 0, 0, st)
 
     def _mfma_split(self, n_ang, mb):
-        """k split of the matrix-core forward: enough waves for 8 per CU on 256 CUs, at most 16 partial sums."""
+        """k split of the matrix-core forward: enough waves for mfma_variant["fwd_waves"] (2048: 8 per CU on 256
+        CUs), at most 32 partial sums."""
         tiles = -(-self.My // 16)
         n_yg = -(-tiles // 8)
         waves = n_ang * n_yg * (-(-mb["n_nt"] // self.mfma_variant["fwd"]))
-        return int(max(1, min(16, mb["n_kc"], -(-2048 // waves))))
+        return int(max(1, min(32, mb["n_kc"], -(-int(self.mfma_variant.get("fwd_waves", 2048)) // waves))))
 
     def _csr_split(self, n_ang, kb):
         """k split of the CSR forward: csr_variant["ksplit"], or "auto": enough waves for 32 per CU on 256 CUs (a
