@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
     top = gg.top;
     scale = gg.scale;
   }
-  const double inv = 1.0 / coef;
+  const double inv = 1.0 / coef, hinv = fabs(hw * inv);
   float4 acc[SW_CQ];
 #pragma unroll
   for (int qq = 0; qq < SW_CQ; ++qq) acc[qq] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -102,16 +102,16 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
     __syncthreads();
     if (L0 + RB < nlines) stage_load(L0 + RB);
     if (live) {
+      const double cb0 = cross * (double)L0 + off0;
       for (int l = 0; l < RB && L0 + l < nlines; ++l) {
-        // positions p with |coef p + c| < hw on line L0 + l
-        const double c = cross * (double)(L0 + l) + off0;
-        const double e0 = (-hw - c) * inv, e1 = (hw - c) * inv;
-        int p0 = (int)ceil(fmin(e0, e1)), p1 = (int)floor(fmax(e0, e1));
-        p0 = max(p0, 0);
-        p1 = min(p1, len - 1);
+        // positions p with |coef p + c| < hw on line L0 + l: p in (-c / coef -+ hw / |coef|)
+        const double c = cb0 + cross * (double)l;
+        const double mid = -c * inv;
+        const int p0 = max((int)ceil(mid - hinv), 0), p1 = min((int)floor(mid + hinv), len - 1);
         const float4* row = lb + l * len * SW_PS;
-        for (int p = p0; p <= p1; ++p) {
-          const double k = hw - fabs(coef * (double)p + c);
+        double d = coef * (double)p0 + c;                      // the footprint offset, stepped by coef
+        for (int p = p0; p <= p1; ++p, d += coef) {
+          const double k = hw - fabs(d);
           if (k > 0.0) {
             const float w = (float)(scale * fmin(k, top));
             const float4* px = row + p * SW_PS;
