@@ -129,10 +129,13 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
     }
   }
   if (live) {
+    // plain stores: a thread's CQ quads are one 64-128 B run of its ray's row, the lanes' runs lie a row apart, so
+    // each store instruction touches 64 lines; L2 merges the runs into whole lines (non-temporal partial-line
+    // stores went to HBM as 4x the bytes)
     const int64_t r = (int64_t)a * ns + s;
 #pragma unroll
     for (int qq = 0; qq < SW_CQ; ++qq)
-      if (q0 + qq < k4_n) dfm::st_stream(sino + r * skstride4 + q0 + qq, acc[qq]);
+      if (q0 + qq < k4_n) sino[r * skstride4 + q0 + qq] = acc[qq];
   }
 }
 

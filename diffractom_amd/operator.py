@@ -95,7 +95,7 @@ def event_timer(device, reps):
     return timeit
 
 
-def _pick(times, tol=0.02):
+def _pick(times, tol=0.04):
     """The fastest candidate, or the first-listed one within tol of it (timing noise does not flip the choice)."""
     t_min = min(times.values())
     return next(name for name, t in times.items() if t <= t_min * (1.0 + tol))
