@@ -161,13 +161,13 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
                     k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
                 "dfm_radon_forward_sweep: bad sizes (na_grp * ns <= 1024, image <= 256 x 256)");
   // variant = lines per LDS block (1, 2, 4, 8) + 16 w: 16 (w 0), 32 (w 1) or 64 (w 2) orientations per workgroup,
-  // + 64 for workgroups of 512 threads (na_grp * ns <= 512; 64 orientations only); instantiated: 16 x 2/4/8 and
-  // 32 x 1/2/4 at 1024 threads, 64 x 1/2 at 512 (the others spill registers)
+  // + 64 for workgroups of 512 threads (na_grp * ns <= 512; 32 or 64 orientations); instantiated: 16 x 2/4/8 and
+  // 32 x 1/2/4 at 1024 threads, 32 x 2 and 64 x 1/2 at 512 (the others spill registers)
   const int len = nx > ny ? nx : ny;
   const int rb = lines % 16, w = (lines / 16) % 4, half = lines / 64;
   const int cq = 4 << (w > 2 ? 0 : w);
   const int th = half ? 512 : SW_THREADS;
-  const bool inst = half ? (w == 2 && rb <= 2) : ((w == 0 && rb >= 2) || (w == 1 && rb <= 4));
+  const bool inst = half ? ((w == 2 && rb <= 2) || (w == 1 && rb == 2)) : ((w == 0 && rb >= 2) || (w == 1 && rb <= 4));
   DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4 || rb == 8) && w <= 2 && half <= 1 && inst && na_grp * ns <= th &&
                     (size_t)2 * rb * len * (cq + 1) * sizeof(float4) <= 163840,
                 "dfm_radon_forward_sweep: variant %d: %d lines of %d pixels x %d orientations, %d threads not "
@@ -183,7 +183,8 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
                      (const float4*)img_k, (float4*)sino, (const SweepGeo*)geo, grp_ang, grp_cls, n_grp, na_grp, ns,  \
                      nx, ny, kstride / 4, k_off / 4, k4n, skstride / 4)
   if (half) {
-    if (rb == 1) DFM_SW(1, 16, 512);
+    if (cq == 8) DFM_SW(2, 8, 512);
+    else if (rb == 1) DFM_SW(1, 16, 512);
     else DFM_SW(2, 16, 512);
   } else if (cq == 4) {
     if (rb == 2) DFM_SW(2, 4, 1024);

@@ -128,7 +128,7 @@ def test_errors_raise_like_the_reference():
                                              ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4),
                                              ("table_ch256", 4), ("table_band128", 4), ("table_band256", 4),
                                              ("table_group", 4), ("table_group16", 4), ("sweep", 4), ("sweep2", 4),
-                                             ("sweep32", 4), ("sweep64", 4), ("sweep64_1", 4),
+                                             ("sweep32", 4), ("sweep32h", 4), ("sweep64", 4), ("sweep64_1", 4),
                                              ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
@@ -155,8 +155,8 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["fwd_band"] = 8 + 16 * {"table_band": 0, "table_band128": 1, "table_band256": 2}[kernel]
     elif kernel.startswith("sweep"):             # forward by line sweep (4 or 2 lines per LDS block)
         op.radon_variant["fwd_kernel"] = "sweep"
-        op.radon_variant["fwd_sweep"] = {"sweep": 4, "sweep2": 2, "sweep32": 2 + 16, "sweep64": 2 + 96,
-                                         "sweep64_1": 1 + 96}[kernel]
+        op.radon_variant["fwd_sweep"] = {"sweep": 4, "sweep2": 2, "sweep32": 2 + 16, "sweep32h": 2 + 80,
+                                         "sweep64": 2 + 96, "sweep64_1": 1 + 96}[kernel]
         op.radon_variant["bwd_kernel"] = "table"
     elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
         op.radon_variant["bwd"] = 8 + 64 * 4
