@@ -540,8 +540,8 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
 // g % 8, so their sinogram slabs stay in its L2.
 constexpr int TCH = 32;   // angles whose table rows are staged at once
 
-template <int MODE, int ACH, int G>
-__global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4* __restrict__ sino,
+template <int MODE, int ACH, int G, int TS>
+__global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(const float4* __restrict__ sino,
                                                                    float4* __restrict__ out,
                                                                    const int4* __restrict__ ptab,
                                                                    const int* __restrict__ tbase, int nx, int ny,
@@ -550,28 +550,30 @@ __global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4*
                                                                    float4* __restrict__ z,
                                                                    const float4* __restrict__ T, float inv_p,
                                                                    float lam) {
-  __shared__ float4 lds[ACH * TB * QB];   // ACH angles x TB bins x 16 quads
-  __shared__ int4 lpt[TCH * 16];          // pixel-table rows of the tile's 16 pixels for TCH angles
+  constexpr int NPX = TS * TS, NTH = NPX * 16;
+  constexpr int TBB = TS == 4 ? TB : 16;   // bins a tile reaches at one angle (host-checked)
+  __shared__ float4 lds[ACH * TBB * QB];   // ACH angles x TBB bins x 16 quads
+  __shared__ int4 lpt[TCH * NPX];          // pixel-table rows of the tile's pixels for TCH angles
   __shared__ int lbase[TCH];
-  static_assert(ACH * TB * QB % 256 == 0 && TCH % ACH == 0, "whole staging rounds");
-  const int tnx = (nx + 3) / 4, tny = (ny + 3) / 4;
+  static_assert(ACH * TBB * QB % NTH == 0 && TCH % ACH == 0, "whole staging rounds");
+  const int tnx = (nx + TS - 1) / TS, tny = (ny + TS - 1) / TS;
   const int ntiles = tnx * tny;
   const int n_cb = (k4_n + QB - 1) / QB;
   const int n_grp = (n_cb + G - 1) / G;
   int grp, tile;
   if (!xcd_split(ntiles, n_grp, grp, tile)) return;
   const int q = threadIdx.x & (QB - 1);
-  const int li = threadIdx.x >> 4;
-  const int px = (tile % tnx) * 4 + (li & 3), py = (tile / tnx) * 4 + (li >> 2);
+  const int li = threadIdx.x >> 4;                       // pixel of the tile, row-major TS x TS
+  const int px = (tile % tnx) * TS + li % TS, py = (tile / tnx) * TS + li / TS;
   const bool inside = px < nx && py < ny;
   const int64_t pix = (int64_t)py * nx + px;
   const int64_t npix = (int64_t)nx * ny;
   const bool tables_once = na <= TCH;
   if (tables_once) {
     if (threadIdx.x < na) lbase[threadIdx.x] = tbase[(int64_t)threadIdx.x * ntiles + tile];
-    for (int i = threadIdx.x; i < na * 16; i += 256) {
-      const int al = i >> 4, l = i & 15;
-      const int ppx = (tile % tnx) * 4 + (l & 3), ppy = (tile / tnx) * 4 + (l >> 2);
+    for (int i = threadIdx.x; i < na * NPX; i += NTH) {
+      const int al = i / NPX, l = i % NPX;
+      const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
       lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)al * npix + (int64_t)ppy * nx + ppx] : make_int4(0, 0, 0, 0);
     }
   }
@@ -593,9 +595,9 @@ __global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4*
       if (!tables_once) {
         __syncthreads();
         if (threadIdx.x < tn) lbase[threadIdx.x] = tbase[(int64_t)(t0 + threadIdx.x) * ntiles + tile];
-        for (int i = threadIdx.x; i < tn * 16; i += 256) {
-          const int al = i >> 4, l = i & 15;
-          const int ppx = (tile % tnx) * 4 + (l & 3), ppy = (tile / tnx) * 4 + (l >> 2);
+        for (int i = threadIdx.x; i < tn * NPX; i += NTH) {
+          const int al = i / NPX, l = i % NPX;
+          const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
           lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx]
                                           : make_int4(0, 0, 0, 0);
         }
@@ -603,14 +605,14 @@ __global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4*
       for (int a0 = 0; a0 < tn; a0 += ACH) {
         const int nan = min(ACH, tn - a0);
         __syncthreads();
-        constexpr int PER = ACH * TB * QB / 256;
+        constexpr int PER = ACH * TBB * QB / NTH;
         float4 v[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-          const int i = threadIdx.x + 256 * u;
+          const int i = threadIdx.x + NTH * u;
           const int qq = i & (QB - 1);
-          const int b = (i >> 4) % TB;
-          const int al = i / (TB * QB);
+          const int b = (i >> 4) % TBB;
+          const int al = i / (TBB * QB);
           v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
           if (al < nan) {
             const int s = lbase[a0 + al] + b;
@@ -619,16 +621,16 @@ __global__ void __launch_bounds__(256) radon_backward_group_kernel(const float4*
           }
         }
 #pragma unroll
-        for (int u = 0; u < PER; ++u) lds[threadIdx.x + 256 * u] = v[u];
+        for (int u = 0; u < PER; ++u) lds[threadIdx.x + NTH * u] = v[u];
         __syncthreads();
         if (live) {
 #pragma unroll 4
           for (int al = 0; al < nan; ++al) {
-            const int4 e = lpt[(a0 + al) * 16 + li];
+            const int4 e = lpt[(a0 + al) * NPX + li];
             const int base = lbase[a0 + al];
             const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
-            const float4 s0 = lds[(al * TB + (e.x - base)) * QB + q];
-            const float4 s1 = lds[(al * TB + (e.y - base)) * QB + q];
+            const float4 s0 = lds[(al * TBB + (e.x - base)) * QB + q];
+            const float4 s1 = lds[(al * TBB + (e.y - base)) * QB + q];
             acc.x += w0 * s0.x + w1 * s1.x;
             acc.y += w0 * s0.y + w1 * s1.y;
             acc.z += w0 * s0.z + w1 * s1.z;
@@ -1105,21 +1107,50 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
                     skstride >= k_n && (mode == 0 || mode == 1),
                 "dfm_radon_backward_table: bad sizes");
-  // variant = angles staged per pass (4, 8, 16, 32: a workgroup per tile and channel block), or ACH + 64 G with
-  // ACH 8 or 16 and G 2, 4 or 8: a workgroup per tile and G channel blocks, the tile's tables staged once
-  const int ach = variant % 64, grp = variant / 64;
-  DFM_CHECK_ARG((grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
-                    ((grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)),
+  // variant = angles staged per pass (4, 8, 16, 32: a workgroup per 4x4 tile and channel block), or ACH + 64 G
+  // with ACH 4, 8 or 16 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged
+  // once; + 1024: 8x8 tiles (tbase then holds the 8x8 tiles' first bins), ACH 4 or 8, G 1, 2 or 4
+  const int ach = variant % 64, grp = (variant / 64) % 16, t8 = variant / 1024;
+  DFM_CHECK_ARG((t8 == 0 && grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
+                    (t8 == 0 && (grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)) ||
+                    (t8 == 1 && (grp == 1 || grp == 2 || grp == 4) && (ach == 4 || ach == 8)),
                 "dfm_radon_backward_table: variant %d: angles staged per pass 4, 8, 16 or 32, or 4 / 8 / 16 + 64 G "
-                "for G = 2, 4, 8 channel blocks per workgroup", variant);
+                "for G = 2, 4, 8 channel blocks per workgroup, or 4 / 8 + 64 G + 1024 (8x8 tiles, G = 1, 2, 4)",
+                variant);
   if (k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int ntiles = ((nx + 3) / 4) * ((ny + 3) / 4);
+  if (t8) {
+    const int ntiles8 = ((nx + 7) / 8) * ((ny + 7) / 8);
+    const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
+    const unsigned ggrid = xcd_grid(ntiles8, n_grp);
+#define DFM_B8(M, A, G)                                                                                           \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 8>), dim3(ggrid), dim3(1024), 0, (hipStream_t)stream, \
+                     (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
+                     kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
+#define DFM_B8G(M, A)        \
+  if (grp == 1) {            \
+    DFM_B8(M, A, 1);         \
+  } else if (grp == 2) {     \
+    DFM_B8(M, A, 2);         \
+  } else {                   \
+    DFM_B8(M, A, 4);         \
+  }
+    if (mode == 0) {
+      if (ach == 4) { DFM_B8G(0, 4) } else { DFM_B8G(0, 8) }
+    } else {
+      if (ach == 4) { DFM_B8G(1, 4) } else { DFM_B8G(1, 8) }
+    }
+#undef DFM_B8G
+#undef DFM_B8
+    DFM_LAUNCH_CHECK("dfm_radon_backward_table");
+    return DFM_OK;
+  }
   if (grp > 0) {
     const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
     const unsigned ggrid = xcd_grid(ntiles, n_grp);
 #define DFM_BG(M, A, G)                                                                                           \
-  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G>), dim3(ggrid), dim3(256), 0, (hipStream_t)stream,     \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4>), dim3(ggrid), dim3(256), 0, (hipStream_t)stream,  \
                      (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
 #define DFM_BGA(M, G)            \

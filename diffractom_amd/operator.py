@@ -162,6 +162,15 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
         np.maximum.at(top_[a], tile, hi[a])
     if np.any(top_ - base >= 8):
         raise AssertionError("a 4 x 4 pixel tile reaches more than 8 bins at some angle")
+    # the same for 8 x 8 tiles (at most 16 bins; the 8 x 8 backprojection is unavailable otherwise)
+    tnx8, tny8 = -(-nx // 8), -(-ny // 8)
+    tile8 = ((py // 8) * tnx8 + (px // 8)).astype(np.int64)
+    base8 = np.full((na, tnx8 * tny8), np.iinfo(np.int64).max)
+    top8 = np.full((na, tnx8 * tny8), np.iinfo(np.int64).min)
+    for a in range(na):
+        np.minimum.at(base8[a], tile8, lo[a])
+        np.maximum.at(top8[a], tile8, hi[a])
+    tbase8 = base8.astype(np.int32) if not np.any(top8 - base8 >= 16) else np.full((1, 1), -1, np.int32)
     # line sweep: per angle the sweep class (0: image rows, |X| >= |Y|; 1: image columns) and its float64 geometry
     cls = (np.abs(g["X"][idx]) < np.abs(g["Y"][idx])).astype(np.int32)
     bh = -(-ny // n_bands)
@@ -181,6 +190,7 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
     sweep_grp, sweep_gcls = sweep_groups(1024)
     sweep_grp_h, sweep_gcls_h = sweep_groups(512)
     return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32), "band_ptr": band_ptr.astype(np.int32),
+            "tbase8": tbase8,
             "sweep_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
                                    radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
@@ -679,8 +689,12 @@ class SinglePhaseForwardOperator:
                                 kstride, koff, kn, cw)
             return
         if tab is not None and self.radon_variant["bwd_kernel"] != "arith":
+            t8 = self.radon_variant["bwd"] >= 1024                  # 8x8 tiles: their first-bin table
+            if t8 and tab["tbase8"].numel() == 1:
+                raise ValueError("8x8 backprojection tiles reach more than 16 bins for this geometry")
             _lib.call("dfm_radon_backward_table", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(tab["ptab"]),
-                      _lib.ptr(tab["tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff, kn, kn, _lib.ptr(x),
+                      _lib.ptr(tab["tbase8" if t8 else "tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff,
+                      kn, kn, _lib.ptr(x),
                       _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), mode, self.radon_variant["bwd"], st)
         elif mode == 0:
             _lib.call("dfm_radon_backward", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(geo), self.Nx, self.Ny,
@@ -745,6 +759,9 @@ class SinglePhaseForwardOperator:
                 saved["fwd_sweep"] = v
             out["radon_forward"] = (best, times)
             cands = {f"table:{a}x{g}": ("table", a + 64 * g) for a, g in ((4, 4), (4, 2), (8, 4), (8, 2), (8, 8))}
+            if tab["tbase8"].numel() > 1:
+                cands.update({f"table8:{a}x{g}": ("table", a + 64 * g + 1024)
+                              for a, g in ((4, 1), (4, 2), (8, 1), (8, 2), (4, 4))})
             cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})
             if self.lds_width():
                 cands["lds"] = ("lds", None)

@@ -127,7 +127,8 @@ def test_errors_raise_like_the_reference():
                                              ("table_capped", 4), ("table_pairs", 4), ("table_band", 4),
                                              ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4),
                                              ("table_ch256", 4), ("table_band128", 4), ("table_band256", 4),
-                                             ("table_group", 4), ("table_group16", 4), ("sweep", 4), ("sweep2", 4),
+                                             ("table_group", 4), ("table_group16", 4), ("table_tile8", 4),
+                                             ("table_tile8g2", 4), ("sweep", 4), ("sweep2", 4),
                                              ("sweep32", 4), ("sweep32h", 4), ("sweep64", 4), ("sweep64_1", 4),
                                              ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
@@ -160,6 +161,10 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["bwd_kernel"] = "table"
     elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
         op.radon_variant["bwd"] = 8 + 64 * 4
+    elif kernel == "table_tile8":                # backprojection over 8x8 tiles, 4 angles per pass
+        op.radon_variant["bwd"] = 4 + 64 * 1 + 1024
+    elif kernel == "table_tile8g2":              # ... 8 angles per pass, 2 channel blocks
+        op.radon_variant["bwd"] = 8 + 64 * 2 + 1024
     elif kernel == "table_group16":              # ... 16 angles per pass, 2 channel blocks
         op.radon_variant["bwd"] = 16 + 64 * 2
     elif kernel.startswith("table_ch"):          # channel blocks of 32, 16, 128, 256 channels

@@ -49,6 +49,9 @@ def test_tables_equal_projection_matrices(kernel, shape):
         tile = (yy.ravel() // 4) * tnx + xx.ravel() // 4
         base = tab["tbase"][i][tile]
         assert np.all((pt[:, 0] - base >= 0) & (pt[:, 0] - base < 8) & (pt[:, 1] - base >= 0) & (pt[:, 1] - base < 8))
+        tile8 = (yy.ravel() // 8) * -(-nx // 8) + xx.ravel() // 8       # 8 x 8 tiles: 16 bins from their base
+        base8 = tab["tbase8"][i][tile8]
+        assert np.all((pt[:, :2] - base8[:, None] >= 0) & (pt[:, :2] - base8[:, None] < 16))
 
 
 @pytest.mark.parametrize("shape", [(16, 16, 17, (0, 180)), (14, 12, 16, (0, 360)), (99, 99, 103, (0, 180))])
