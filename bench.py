@@ -400,8 +400,8 @@ def run_c4(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: every GPU box here has one GPU; the multi-GPU "
-                                   "sharding over orientations is a design in DESIGN.md, not a measured path",
+        print(json.dumps({"error": "this benchmark measures one GPU; sharding the orientations over several GPUs "
+                                   "is a design, not a measured path",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
