@@ -40,6 +40,7 @@ SIGNATURES = {
     "dfm_radon_forward_band": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_spmm_t": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P],
     "dfm_spmm_t_rows_per_block": [I32],
+    "dfm_spmm_t_direct": [P, I64, I32, P, P, P, P, I64, I32, P, I32, I32, I32, I32, I32, I32, F32, I32, I32, P],
     "dfm_transpose_sum": [P, P, P, I32, I32, I32, I32, I64, I64, I64, I64, I32, I64, I32, P],
     "dfm_spmm_adjoint": [P, P, P, P, P, P, I32, I32, I32, I32, I32, F32, I32, P],
     "dfm_zero_rows": [P, P, I32, I64, P],

@@ -14,6 +14,8 @@
 // held in registers with v_readlane, its value by a scalar load) and its operand is one conflict-free LDS row read. Per
 // row the column window moves on only when all 64 of its entries are consumed. Summation follows the CSR order, as in
 // the row-per-thread kernels.
+#include <algorithm>
+
 #include "dfm_common.h"
 
 namespace {
@@ -26,7 +28,8 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
                                                       const uint16_t* __restrict__ col, const float* __restrict__ val,
                                                       float* __restrict__ outT, const int* __restrict__ omegas,
                                                       int my, int ys, int ncols, int nrows, int rows_out, int compact,
-                                                      float alpha, int cspan, int kc, int n_angles) {
+                                                      float alpha, int cspan, int kc, int n_angles, int64_t in_ld,
+                                                      int64_t out_ld, int ys_out) {
   extern __shared__ float lin[];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -69,15 +72,61 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
   }
 
   const float* lbase = lin + lane;
-  const float* src = inT + (int64_t)a * ncols * ys;
+  // in_ld > 0: the operand untransposed, (my, in_ld) per angle, transposed while it is staged (ys odd: the
+  // transposing LDS writes, a row of ys floats apart, hit distinct banks); in_ld == 0: transposed, (ncols, ys)
+  const float* src = in_ld ? inT + (int64_t)a * my * in_ld : inT + (int64_t)a * ncols * ys;
   for (int c0 = c_lo; c0 < c_hi; c0 += kc) {
     const int kn = min(kc, c_hi - c0);
     __syncthreads();
-    {
+    constexpr int SU = YW >= 7 ? 4 : 8;                      // staging loads in flight per lane
+    if (in_ld) {
+      // an item is 4 rows x 64 columns: wave w copies row quads g = w + 16 m, lane (cq_lo + 8 yq + 32 cq_hi) the
+      // float4 of columns 4 cq .. 4 cq + 3 (cq = cq_lo + 8 cq_hi) of row 4 g + yq. c0, in_ld and the base are
+      // multiples of 4 floats, and the columns up to kn rounded up to 4 lie in the row and in the LDS chunk; with
+      // ys odd the 4 transposing writes of a float4 hit distinct banks over every 32 lanes (4 (cq ys mod 8) + yq).
+      const int yq = (lane >> 3) & 3, cq = (lane & 7) + ((lane >> 5) << 3);
+      const int nj = (kn + 63) / 64, ng = (my + 3) / 4;
+      const int items = (ng - w + SPT_WAVES - 1) / SPT_WAVES * nj;
+      for (int it0 = 0; it0 < items; it0 += SU) {
+        float4 v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          const int it = it0 + u, m = it / nj, kk = 64 * (it - m * nj) + 4 * cq;
+          const int y = 4 * (w + SPT_WAVES * m) + yq;
+          v[u] = (it < items && y < my && kk < kn)
+                     ? *reinterpret_cast<const float4*>(src + (int64_t)y * in_ld + c0 + kk)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          const int it = it0 + u, m = it / nj, kk = 64 * (it - m * nj) + 4 * cq;
+          const int y = 4 * (w + SPT_WAVES * m) + yq;
+          if (it < items && y < my && kk < kn) {
+            float* l = lin + kk * ys + y;
+            l[0] = v[u].x;
+            l[ys] = v[u].y;
+            l[2 * ys] = v[u].z;
+            l[3 * ys] = v[u].w;
+          }
+        }
+      }
+    } else {
       const float4* g = reinterpret_cast<const float4*>(src + (int64_t)c0 * ys);
       float4* l4 = reinterpret_cast<float4*>(lin);
       const int n4 = kn * ys / 4;
-      for (int i = threadIdx.x; i < n4; i += SPT_WAVES * 64) l4[i] = g[i];
+      for (int i0 = threadIdx.x; i0 < n4; i0 += SPT_WAVES * 64 * SU) {
+        float4 v[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          const int i = i0 + SPT_WAVES * 64 * u;
+          v[u] = i < n4 ? g[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          const int i = i0 + SPT_WAVES * 64 * u;
+          if (i < n4) l4[i] = v[u];
+        }
+      }
     }
     __syncthreads();
     const int c1 = c0 + kn;
@@ -119,7 +168,25 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
     }
   }
 
-  float* dst = outT + ((int64_t)s * n_angles + a) * rows_out * ys;
+  if (out_ld) {
+    // out (n_angles, my, out_ld): per group of 64 y, the workgroup's RB = 16 R rows go through LDS as (y, row) with
+    // the odd row stride RB + 1 (64 (RB + 1) floats), then out one y row of RB consecutive floats at a time
+    constexpr int RB = SPT_WAVES * R, RW = RB + 1;
+    const int r = threadIdx.x % RB, row = blockIdx.x * RB + r;
+    float* dst = outT + (int64_t)a * my * out_ld;
+#pragma unroll
+    for (int h = 0; h < YW; ++h) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < R; ++i) lin[lane * RW + w * R + i] = alpha * acc[i][h];
+      __syncthreads();
+      if (row < rows_out)
+        for (int yl = threadIdx.x / RB; yl < 64 && 64 * h + yl < my; yl += SPT_WAVES * 64 / RB)
+          dst[(int64_t)(64 * h + yl) * out_ld + row] = lin[yl * RW + r];
+    }
+    return;
+  }
+  float* dst = outT + ((int64_t)s * n_angles + a) * rows_out * ys_out;
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int row = row0 + i;
@@ -127,7 +194,7 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
 #pragma unroll
       for (int h = 0; h < YW; ++h) {
         const int y = 64 * h + lane;
-        if (y < my) dst[(int64_t)row * ys + y] = alpha * acc[i][h];
+        if (y < my) dst[(int64_t)row * ys_out + y] = alpha * acc[i][h];
       }
     }
   }
@@ -171,9 +238,47 @@ __global__ void __launch_bounds__(256) transpose_sum_kernel(const float* __restr
 template <int R, int YW>
 void launch_spmm_t(dim3 grid, size_t lds, hipStream_t st, const float* inT, const int* row_ptr, const uint16_t* col,
                    const float* val, float* outT, const int* omegas, int my, int ys, int ncols, int nrows,
-                   int rows_out, int compact, float alpha, int cspan, int kc, int n_angles) {
-  hipLaunchKernelGGL((spmm_t_kernel<R, YW, (YW <= 4 ? 8 : 4)>), grid, dim3(SPT_WAVES * 64), lds, st, inT, row_ptr, col, val, outT,
-                     omegas, my, ys, ncols, nrows, rows_out, compact, alpha, cspan, kc, n_angles);
+                   int rows_out, int compact, float alpha, int cspan, int kc, int n_angles, int64_t in_ld,
+                   int64_t out_ld, int ys_out) {
+  hipLaunchKernelGGL((spmm_t_kernel<R, YW, (YW <= 4 ? 8 : 4)>), grid, dim3(SPT_WAVES * 64), lds, st, inT, row_ptr,
+                     col, val, outT, omegas, my, ys, ncols, nrows, rows_out, compact, alpha, cspan, kc, n_angles,
+                     in_ld, out_ld, ys_out);
+}
+
+// ys: the LDS row stride (the transposed input's row stride when in_ld == 0); ys_out: the transposed output's
+int spmm_t_impl(const float* in, int64_t in_ld, const int32_t* row_ptr, const uint16_t* col, const float* val,
+                float* out, int64_t out_ld, int32_t ys_out, const int32_t* omegas, int32_t n_angles, int32_t my,
+                int32_t ys, int32_t ncols, int32_t nrows, int32_t rows_out, int32_t compact, float alpha,
+                int32_t nsplit, int32_t kc, hipStream_t st) {
+  const int kc_max = (SPT_LDS_FLOATS - 64) / ys;
+  if (kc <= 0 || kc > kc_max) kc = kc_max;
+  int cspan = (ncols + nsplit - 1) / nsplit;
+  if (in_ld) {                       // float4 staging of the untransposed operand: chunks start at multiples of 4
+    kc &= ~3;
+    cspan = (cspan + 3) & ~3;
+  }
+  if (n_angles == 0 || rows_out == 0) return DFM_OK;
+  const int yw = (my + 63) / 64;
+  const int R = my <= 192 ? 16 : 8;
+  const dim3 grid((unsigned)((rows_out + SPT_WAVES * R - 1) / (SPT_WAVES * R)), (unsigned)n_angles,
+                  (unsigned)nsplit);
+  size_t lds = ((size_t)kc * ys + 64) * sizeof(float);
+  if (out_ld) lds = std::max(lds, (size_t)64 * (SPT_WAVES * R + 1) * sizeof(float));
+#define DFM_SPT(RR, YY)                                                                                      \
+  launch_spmm_t<RR, YY>(grid, lds, st, in, row_ptr, col, val, out, omegas, my, ys, ncols, nrows, rows_out,  \
+                        compact, alpha, cspan, kc, n_angles, in_ld, out_ld, ys_out)
+  switch (yw) {
+    case 1: DFM_SPT(16, 1); break;
+    case 2: DFM_SPT(16, 2); break;
+    case 3: DFM_SPT(16, 3); break;
+    case 4: DFM_SPT(8, 4); break;
+    case 5: DFM_SPT(8, 5); break;
+    case 6: DFM_SPT(8, 6); break;
+    case 7: DFM_SPT(8, 7); break;
+    default: DFM_SPT(8, 8); break;
+  }
+#undef DFM_SPT
+  return DFM_OK;
 }
 
 }  // namespace
@@ -197,31 +302,32 @@ int dfm_spmm_t(const float* inT, const int32_t* row_ptr, const uint16_t* col, co
     dfm::set_error("dfm_spmm_t: the transposed operand must be 16-byte aligned");
     return DFM_ERR_ARG;
   }
-  const int kc_max = (SPT_LDS_FLOATS - 64) / ys;
-  if (kc <= 0 || kc > kc_max) kc = kc_max;
-  if (n_angles == 0 || rows_out == 0) return DFM_OK;
-  const int yw = (my + 63) / 64;
-  const int R = my <= 192 ? 16 : 8;
-  const int cspan = (ncols + nsplit - 1) / nsplit;
-  const dim3 grid((unsigned)((rows_out + SPT_WAVES * R - 1) / (SPT_WAVES * R)), (unsigned)n_angles,
-                  (unsigned)nsplit);
-  const size_t lds = ((size_t)kc * ys + 64) * sizeof(float);
-  const hipStream_t st = (hipStream_t)stream;
-#define DFM_SPT(RR, YY)                                                                                      \
-  launch_spmm_t<RR, YY>(grid, lds, st, inT, row_ptr, col, val, outT, omegas, my, ys, ncols, nrows, rows_out, \
-                        compact, alpha, cspan, kc, n_angles)
-  switch (yw) {
-    case 1: DFM_SPT(16, 1); break;
-    case 2: DFM_SPT(16, 2); break;
-    case 3: DFM_SPT(16, 3); break;
-    case 4: DFM_SPT(8, 4); break;
-    case 5: DFM_SPT(8, 5); break;
-    case 6: DFM_SPT(8, 6); break;
-    case 7: DFM_SPT(8, 7); break;
-    default: DFM_SPT(8, 8); break;
-  }
-#undef DFM_SPT
+  spmm_t_impl(inT, 0, row_ptr, col, val, outT, 0, ys, omegas, n_angles, my, ys, ncols, nrows, rows_out, compact,
+              alpha, nsplit, kc, (hipStream_t)stream);
   DFM_LAUNCH_CHECK("dfm_spmm_t");
+  return DFM_OK;
+}
+
+int dfm_spmm_t_direct(const float* in, int64_t in_ld, int32_t in_ys, const int32_t* row_ptr, const uint16_t* col,
+                      const float* val, float* out, int64_t out_ld, int32_t out_ys, const int32_t* omegas,
+                      int32_t n_angles, int32_t my, int32_t ncols, int32_t nrows, int32_t rows_out, int32_t compact,
+                      float alpha, int32_t nsplit, int32_t kc, dfm_stream_t stream) {
+  DFM_CHECK_ARG(in && row_ptr && col && val && out, "dfm_spmm_t_direct: null pointer");
+  DFM_CHECK_ARG(n_angles >= 0 && my > 0 && my <= 512 && ncols > 0 && ncols <= 65536 && nrows >= 0 &&
+                    rows_out >= nrows && nsplit >= 1 && nsplit <= ncols && n_angles <= 65535 &&
+                    (in_ld ? in_ld >= ncols : (in_ys >= my && in_ys % 4 == 0)) &&
+                    (out_ld ? (out_ld >= rows_out && nsplit == 1) : out_ys >= my),
+                "dfm_spmm_t_direct: bad sizes (my %d ncols %d nrows %d rows_out %d nsplit %d in_ld %lld in_ys %d "
+                "out_ld %lld out_ys %d)", my, ncols, nrows, rows_out, nsplit, (long long)in_ld, in_ys,
+                (long long)out_ld, out_ys);
+  if (((uintptr_t)in) % 16 || in_ld % 4) {
+    dfm::set_error("dfm_spmm_t_direct: the input must be 16-byte aligned, in_ld a multiple of 4");
+    return DFM_ERR_ARG;
+  }
+  const int ys = in_ld ? (my | 1) : in_ys;     // odd LDS row stride for the transposing staging
+  spmm_t_impl(in, in_ld, row_ptr, col, val, out, out_ld, out_ys, omegas, n_angles, my, ys, ncols, nrows, rows_out,
+              compact, alpha, nsplit, kc, (hipStream_t)stream);
+  DFM_LAUNCH_CHECK("dfm_spmm_t_direct");
   return DFM_OK;
 }
 

@@ -257,7 +257,8 @@ class SinglePhaseForwardOperator:
                              f"pf_mode must be 'dense' or 'auto', not {pf_mode!r}")
         self.pf_precision = pf_precision
         self.mfma_variant = {"fwd": 2, "adj": 2, "fwd_waves": 2048}
-        self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0}
+        self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0,
+                            "tdirect": 3}
 
         # ---- geometry (single_phase_forward_operator.py:376-441)
         c = self.cfg
@@ -989,26 +990,43 @@ class SinglePhaseForwardOperator:
         return int(max(1, min(64, -(-1024 // wgs), max(kb // 128, 1))))
 
     def _spmm_t_forward(self, b, sb, om, n_ang, sino, data, accumulate):
-        """sino (n_ang, my, kstride) -> sT (n_ang, kb, ys) -> split partials (ns, n_ang, cp, ys) -> data rows."""
+        """sino (n_ang, my, kstride) -> split partials (ns, n_ang, cp, ys) -> data rows; the sinogram is transposed
+        while the kernel stages it in LDS (bit 2 of csr_variant["tdirect"]) or first into sT (n_ang, kb, ys)."""
         my, ys, cp, kb = self.My, self._ys, self.N_seg, b["K_batch"]
         ns = self._tsplit(n_ang, kb)
-        sT, part = self._t_buf(n_ang * kb * ys, ns * n_ang * cp * ys)
+        direct = int(self.csr_variant.get("tdirect", 3)) & 2
         st = self._stream()
-        _lib.call("dfm_transpose_sum", _lib.ptr(sino), _lib.ptr(sT), None, 0, n_ang, my, kb, b["kstride"],
-                  my * b["kstride"], ys, kb * ys, 1, 0, 0, st)
-        _lib.call("dfm_spmm_t", _lib.ptr(sT), _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]), _lib.ptr(sb["val_f"]),
-                  _lib.ptr(part), _lib.ptr(om), n_ang, my, ys, kb, cp, cp, 0, 1.0, ns,
+        if direct:
+            (part,) = self._t_buf(ns * n_ang * cp * ys)
+            src, in_ld = sino, b["kstride"]
+        else:
+            src, part = self._t_buf(n_ang * kb * ys, ns * n_ang * cp * ys)
+            in_ld = 0
+            _lib.call("dfm_transpose_sum", _lib.ptr(sino), _lib.ptr(src), None, 0, n_ang, my, kb, b["kstride"],
+                      my * b["kstride"], ys, kb * ys, 1, 0, 0, st)
+        _lib.call("dfm_spmm_t_direct", _lib.ptr(src), in_ld, ys, _lib.ptr(sb["row_ptr_f"]), _lib.ptr(sb["col_k"]),
+                  _lib.ptr(sb["val_f"]), _lib.ptr(part), 0, ys, _lib.ptr(om), n_ang, my, kb, cp, cp, 0, 1.0, ns,
                   int(self.csr_variant.get("tkc", 0)), st)
         _lib.call("dfm_transpose_sum", _lib.ptr(part), _lib.ptr(data), _lib.ptr(om), 1, n_ang, cp, my, ys, cp * ys,
                   cp, my * cp, ns, n_ang * cp * ys, int(accumulate), st)
 
     def _spmm_t_adjoint(self, b, sb, om, n_ang, data, sino, alpha):
-        """data rows -> dT (n_ang, cp, ys) -> sT (n_ang, kstride, ys) -> sino (n_ang, my, kstride)."""
+        """data rows -> dT (n_ang, cp, ys) -> sino (n_ang, my, kstride), written through the kernel's LDS transpose
+        (bit 1 of csr_variant["tdirect"]) or as sT (n_ang, kstride, ys) and transposed."""
         my, ys, cp, ks = self.My, self._ys, self.N_seg, b["kstride"]
-        dT, sT = self._t_buf(n_ang * cp * ys, n_ang * ks * ys)
+        direct = int(self.csr_variant.get("tdirect", 3)) & 1
         st = self._stream()
+        if direct:
+            (dT,) = self._t_buf(n_ang * cp * ys)
+        else:
+            dT, sT = self._t_buf(n_ang * cp * ys, n_ang * ks * ys)
         _lib.call("dfm_transpose_sum", _lib.ptr(data), _lib.ptr(dT), _lib.ptr(om), 0, n_ang, my, cp, cp, my * cp, ys,
                   cp * ys, 1, 0, 0, st)
+        if direct:
+            _lib.call("dfm_spmm_t_direct", _lib.ptr(dT), 0, ys, _lib.ptr(sb["row_ptr_a"]), _lib.ptr(sb["col_j"]),
+                      _lib.ptr(sb["val_a"]), _lib.ptr(sino), ks, 0, _lib.ptr(om), n_ang, my, cp, b["K_batch"], ks, 0,
+                      alpha, 1, int(self.csr_variant.get("tkc", 0)), st)
+            return
         _lib.call("dfm_spmm_t", _lib.ptr(dT), _lib.ptr(sb["row_ptr_a"]), _lib.ptr(sb["col_j"]), _lib.ptr(sb["val_a"]),
                   _lib.ptr(sT), _lib.ptr(om), n_ang, my, ys, cp, b["K_batch"], ks, 0, alpha, 1,
                   int(self.csr_variant.get("tkc", 0)), st)

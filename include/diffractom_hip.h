@@ -159,6 +159,15 @@ int dfm_spmm_forward_split(const float* sino, const int32_t* row_ptr, const uint
 int dfm_spmm_t(const float* inT, const int32_t* row_ptr, const uint16_t* col, const float* val, float* outT,
                const int32_t* omegas, int32_t n_angles, int32_t my, int32_t ys, int32_t ncols, int32_t nrows,
                int32_t rows_out, int32_t compact, float alpha, int32_t nsplit, int32_t kc, dfm_stream_t stream);
+/* dfm_spmm_t with either operand untransposed: in_ld > 0: the input is (n_angles, my, in_ld) (the sinogram for the
+ * forward), transposed while it is staged in LDS (in_ld a multiple of 4; the column splits and chunks then start at
+ * multiples of 4); in_ld == 0: transposed (n_angles, ncols, in_ys). The input is 16-byte aligned.
+ * out_ld > 0 (nsplit 1): the output is written as (n_angles, my, out_ld) (the sinogram for the adjoint), through an
+ * LDS transpose of each workgroup's rows; out_ld == 0: as dfm_spmm_t's outT with row stride out_ys. */
+int dfm_spmm_t_direct(const float* in, int64_t in_ld, int32_t in_ys, const int32_t* row_ptr, const uint16_t* col,
+                      const float* val, float* out, int64_t out_ld, int32_t out_ys, const int32_t* omegas,
+                      int32_t n_angles, int32_t my, int32_t ncols, int32_t nrows, int32_t rows_out, int32_t compact,
+                      float alpha, int32_t nsplit, int32_t kc, dfm_stream_t stream);
 /* CSR rows per workgroup of dfm_spmm_t for a given my (0: my unsupported). */
 int dfm_spmm_t_rows_per_block(int32_t my);
 /* out[ao, c, r] (+)= sum_{s < nsplit} in[s split_stride + ai in_mat + r in_ld + c] for a = 0 .. n_mat-1, with

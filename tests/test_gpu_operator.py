@@ -243,16 +243,19 @@ def test_csr_forward_split_matches_oracle(ksplit, kc):
     assert rel_max(d.cpu().numpy()[rows], orc.direct(x)[rows]) < 2e-5
 
 
-@pytest.mark.parametrize("my,tsplit,tkc", [(17, 1, 0), (17, 4, 8), (130, 3, 0), (200, 2, 16), (300, 5, 0)])
-def test_csr_transposed_kernels_match_oracle(my, tsplit, tkc):
+@pytest.mark.parametrize("my,tsplit,tkc,direct", [(17, 1, 0, 3), (17, 4, 8, 3), (130, 3, 0, 3), (200, 2, 16, 3),
+                                                  (300, 5, 0, 3), (17, 4, 8, 0), (130, 3, 0, 0), (300, 5, 0, 0)])
+def test_csr_transposed_kernels_match_oracle(my, tsplit, tkc, direct):
     """The wavefront-segmented CSR products on transposed operands, both directions: 1, 3 and 5 groups of 64 detector
     positions per wave (16 or 8 rows per wave), orientation splits with partial sums, LDS chunks of tkc columns (0:
-    as many as fit; 8 and 16 force many chunk and window changes); several batches, full and subset rows."""
-    print(f"case csr transposed my={my} tsplit={tsplit} tkc={tkc}", flush=True)
+    as many as fit; 8 and 16 force many chunk and window changes); the sinogram transposed in the kernels (direct 3:
+    staged through LDS by the forward, written through LDS by the adjoint) or by separate transposes (0); several
+    batches, full and subset rows."""
+    print(f"case csr transposed my={my} tsplit={tsplit} tkc={tkc} direct={direct}", flush=True)
     c = cfg(nx=12, ny=12, my=my, n_omega=12, angle_range=(0, 360), cor=0.7, n_eta=24)
     op, orc, _ = build(c, K=150, sigma_deg=5.0, pf_mode="sparse", max_gb=2.2e-4)
     assert len(op.batches) > 1
-    op.csr_variant.update(fwd="t", adj="t", tsplit=tsplit, tkc=tkc)
+    op.csr_variant.update(fwd="t", adj="t", tsplit=tsplit, tkc=tkc, tdirect=direct)
     x, y = _inputs(op, 4)
     assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5
