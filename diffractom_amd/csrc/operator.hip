@@ -534,14 +534,16 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
   }
 }
 
-// The same backprojection with a workgroup covering one 4x4 tile for G consecutive channel blocks: the tile's bases
-// and pixel-table rows (channel-independent) are staged once per TCH angles and reused by every block, so the per-block
-// work is the sinogram staging (all of a pass's loads in flight) and the pixel sums. Blocks g*G .. g*G+G-1 run on XCD
-// g % 8, so their sinogram slabs stay in its L2.
+// The same backprojection with a workgroup covering one TS x TS tile for G consecutive channel blocks of QW quads: the
+// tile's bases and pixel-table rows (channel-independent) are staged once per TCH angles and reused by every block, so
+// the per-block work is the sinogram staging (all of a pass's loads in flight) and the pixel sums. Blocks
+// g*G .. g*G+G-1 run on XCD g % 8, so their sinogram slabs stay in its L2. The blocks of a workgroup run one after
+// another; wider blocks (QW 32 or 64) instead put a pixel's 128 or 256 channels in flight at once, x, z, T loads and
+// each pass's sinogram loads of all of them.
 constexpr int TCH = 32;   // angles whose table rows are staged at once
 
-template <int MODE, int ACH, int G, int TS>
-__global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(const float4* __restrict__ sino,
+template <int MODE, int ACH, int G, int TS, int QW>
+__global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(const float4* __restrict__ sino,
                                                                    float4* __restrict__ out,
                                                                    const int4* __restrict__ ptab,
                                                                    const int* __restrict__ tbase, int nx, int ny,
@@ -550,20 +552,20 @@ __global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(cons
                                                                    float4* __restrict__ z,
                                                                    const float4* __restrict__ T, float inv_p,
                                                                    float lam) {
-  constexpr int NPX = TS * TS, NTH = NPX * 16;
+  constexpr int NPX = TS * TS, NTH = NPX * QW;    // QW float4 quads of every pixel per channel block
   constexpr int TBB = TS == 4 ? TB : 16;   // bins a tile reaches at one angle (host-checked)
-  __shared__ float4 lds[ACH * TBB * QB];   // ACH angles x TBB bins x 16 quads
+  __shared__ float4 lds[ACH * TBB * QW];   // ACH angles x TBB bins x QW quads
   __shared__ int4 lpt[TCH * NPX];          // pixel-table rows of the tile's pixels for TCH angles
   __shared__ int lbase[TCH];
-  static_assert(ACH * TBB * QB % NTH == 0 && TCH % ACH == 0, "whole staging rounds");
+  static_assert(ACH * TBB * QW % NTH == 0 && TCH % ACH == 0, "whole staging rounds");
   const int tnx = (nx + TS - 1) / TS, tny = (ny + TS - 1) / TS;
   const int ntiles = tnx * tny;
-  const int n_cb = (k4_n + QB - 1) / QB;
+  const int n_cb = (k4_n + QW - 1) / QW;
   const int n_grp = (n_cb + G - 1) / G;
   int grp, tile;
   if (!xcd_split(ntiles, n_grp, grp, tile)) return;
-  const int q = threadIdx.x & (QB - 1);
-  const int li = threadIdx.x >> 4;                       // pixel of the tile, row-major TS x TS
+  const int q = threadIdx.x & (QW - 1);
+  const int li = threadIdx.x / QW;                       // pixel of the tile, row-major TS x TS
   const int px = (tile % tnx) * TS + li % TS, py = (tile / tnx) * TS + li / TS;
   const bool inside = px < nx && py < ny;
   const int64_t pix = (int64_t)py * nx + px;
@@ -580,7 +582,7 @@ __global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(cons
   for (int gi = 0; gi < G; ++gi) {
     const int cb = grp * G + gi;
     if (cb >= n_cb) break;                                // uniform over the workgroup
-    const int kq = cb * QB + q;
+    const int kq = cb * QW + q;
     const bool live = inside && kq < k4_n;
     const int64_t o = pix * kstride4 + kq + k4_off;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -605,18 +607,18 @@ __global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(cons
       for (int a0 = 0; a0 < tn; a0 += ACH) {
         const int nan = min(ACH, tn - a0);
         __syncthreads();
-        constexpr int PER = ACH * TBB * QB / NTH;
+        constexpr int PER = ACH * TBB * QW / NTH;
         float4 v[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
           const int i = threadIdx.x + NTH * u;
-          const int qq = i & (QB - 1);
-          const int b = (i >> 4) % TBB;
-          const int al = i / (TBB * QB);
+          const int qq = i & (QW - 1);
+          const int b = (i / QW) % TBB;
+          const int al = i / (TBB * QW);
           v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
           if (al < nan) {
             const int s = lbase[a0 + al] + b;
-            const int kk = cb * QB + qq;
+            const int kk = cb * QW + qq;
             if (s >= 0 && s < ns && kk < k4_n) v[u] = sino[((int64_t)(t0 + a0 + al) * ns + s) * skstride4 + kk];
           }
         }
@@ -629,8 +631,8 @@ __global__ void __launch_bounds__(TS * TS * 16) radon_backward_group_kernel(cons
             const int4 e = lpt[(a0 + al) * NPX + li];
             const int base = lbase[a0 + al];
             const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
-            const float4 s0 = lds[(al * TBB + (e.x - base)) * QB + q];
-            const float4 s1 = lds[(al * TBB + (e.y - base)) * QB + q];
+            const float4 s0 = lds[(al * TBB + (e.x - base)) * QW + q];
+            const float4 s1 = lds[(al * TBB + (e.y - base)) * QW + q];
             acc.x += w0 * s0.x + w1 * s1.x;
             acc.y += w0 * s0.y + w1 * s1.y;
             acc.z += w0 * s0.z + w1 * s1.z;
@@ -1110,22 +1112,59 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
   // variant = angles staged per pass (4, 8, 16, 32: a workgroup per 4x4 tile and channel block), or ACH + 64 G
   // with ACH 4, 8 or 16 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged
   // once; + 1024: 8x8 tiles (tbase then holds the 8x8 tiles' first bins), ACH 4 or 8, G 1, 2 or 4
-  const int ach = variant % 64, grp = (variant / 64) % 16, t8 = variant / 1024;
-  DFM_CHECK_ARG((t8 == 0 && grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
-                    (t8 == 0 && (grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)) ||
-                    (t8 == 1 && (grp == 1 || grp == 2 || grp == 4) && (ach == 4 || ach == 8)),
+  // + 4096 w (w 1, 2): 4x4 tiles with channel blocks of 16 << w quads (128 or 256 channels; 512 or 1024 threads),
+  // ACH 4 or 8, G 1 or 2
+  const int ach = variant % 64, grp = (variant / 64) % 16, t8 = (variant / 1024) % 4, wq = variant / 4096;
+  DFM_CHECK_ARG((wq == 0 && t8 == 0 && grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
+                    (wq == 0 && t8 == 0 && (grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)) ||
+                    (wq == 0 && t8 == 1 && (grp == 1 || grp == 2 || grp == 4) && (ach == 4 || ach == 8)) ||
+                    ((wq == 1 || wq == 2) && t8 == 0 && (grp == 1 || grp == 2) && (ach == 4 || ach == 8)),
                 "dfm_radon_backward_table: variant %d: angles staged per pass 4, 8, 16 or 32, or 4 / 8 / 16 + 64 G "
-                "for G = 2, 4, 8 channel blocks per workgroup, or 4 / 8 + 64 G + 1024 (8x8 tiles, G = 1, 2, 4)",
+                "for G = 2, 4, 8 channel blocks per workgroup, or 4 / 8 + 64 G + 1024 (8x8 tiles, G = 1, 2, 4), or "
+                "4 / 8 + 64 G + 4096 w (G = 1, 2; blocks of 16 << w quads, w = 1, 2)",
                 variant);
   if (k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int ntiles = ((nx + 3) / 4) * ((ny + 3) / 4);
+  if (wq) {
+    const int qw = QB << wq;
+    const int n_grp = ((k4n + qw - 1) / qw + grp - 1) / grp;
+    const unsigned ggrid = xcd_grid(ntiles, n_grp);
+#define DFM_BW(M, A, G, W)                                                                                        \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4, W>), dim3(ggrid), dim3(16 * W), 0,                  \
+                     (hipStream_t)stream, (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny,  \
+                     ns, n_angles, kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z,             \
+                     (const float4*)T, inv_p, lam)
+#define DFM_BWG(M, A, W)     \
+  if (grp == 1) {            \
+    DFM_BW(M, A, 1, W);      \
+  } else {                   \
+    DFM_BW(M, A, 2, W);      \
+  }
+#define DFM_BWA(M, W)                 \
+  if (ach == 4) {                     \
+    DFM_BWG(M, 4, W)                  \
+  } else {                            \
+    DFM_BWG(M, 8, W)                  \
+  }
+    if (mode == 0) {
+      if (wq == 1) { DFM_BWA(0, 32) } else { DFM_BWA(0, 64) }
+    } else {
+      if (wq == 1) { DFM_BWA(1, 32) } else { DFM_BWA(1, 64) }
+    }
+#undef DFM_BWA
+#undef DFM_BWG
+#undef DFM_BW
+    DFM_LAUNCH_CHECK("dfm_radon_backward_table");
+    return DFM_OK;
+  }
   if (t8) {
     const int ntiles8 = ((nx + 7) / 8) * ((ny + 7) / 8);
     const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
     const unsigned ggrid = xcd_grid(ntiles8, n_grp);
 #define DFM_B8(M, A, G)                                                                                           \
-  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 8>), dim3(ggrid), dim3(1024), 0, (hipStream_t)stream, \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 8, QB>), dim3(ggrid), dim3(1024), 0,                   \
+                     (hipStream_t)stream,                                                                         \
                      (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
 #define DFM_B8G(M, A)        \
@@ -1150,7 +1189,8 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
     const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
     const unsigned ggrid = xcd_grid(ntiles, n_grp);
 #define DFM_BG(M, A, G)                                                                                           \
-  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4>), dim3(ggrid), dim3(256), 0, (hipStream_t)stream,  \
+  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4, QB>), dim3(ggrid), dim3(256), 0,                    \
+                     (hipStream_t)stream,                                                                         \
                      (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
 #define DFM_BGA(M, G)            \

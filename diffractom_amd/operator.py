@@ -690,7 +690,7 @@ class SinglePhaseForwardOperator:
                                 kstride, koff, kn, cw)
             return
         if tab is not None and self.radon_variant["bwd_kernel"] != "arith":
-            t8 = self.radon_variant["bwd"] >= 1024                  # 8x8 tiles: their first-bin table
+            t8 = (self.radon_variant["bwd"] // 1024) % 4 == 1       # 8x8 tiles: their first-bin table
             if t8 and tab["tbase8"].numel() == 1:
                 raise ValueError("8x8 backprojection tiles reach more than 16 bins for this geometry")
             _lib.call("dfm_radon_backward_table", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(tab["ptab"]),
@@ -763,6 +763,8 @@ class SinglePhaseForwardOperator:
             if tab["tbase8"].numel() > 1:
                 cands.update({f"table8:{a}x{g}": ("table", a + 64 * g + 1024)
                               for a, g in ((4, 1), (4, 2), (8, 1), (8, 2), (4, 4))})
+            cands.update({f"table:{a}x{g}:ch{64 << w}": ("table", a + 64 * g + 4096 * w)
+                          for w in (1, 2) for a, g in ((4, 1), (8, 1), (4, 2))})
             cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})
             if self.lds_width():
                 cands["lds"] = ("lds", None)

@@ -73,9 +73,13 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  * tile bases tbase (n_angles, ceil(nx/4) * ceil(ny/4)): the first of the 8 bins every 4 x 4 pixel tile reaches.
  * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg.
  * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8, plus 16 * cap
- * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS;
- * backward: angles staged in LDS per pass, 4, 8, 16 or 32), picked by timing (the reference times its tiled / plain
- * projectors the same way, parallel_radon.py:240-281). */
+ * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS,
+ * plus 512 q for channel blocks of 32, 16, 128 or 256 channels (q 1 .. 4; 64 for q 0);
+ * backward: angles staged in LDS per pass, 4, 8, 16 or 32, a workgroup per 4 x 4 tile and 64 channels; or ACH + 64 G
+ * (ACH 4, 8 or 16; G 2, 4 or 8): a workgroup per tile and G channel blocks, the tile's table rows staged once; + 1024
+ * (ACH 4 or 8, G 1, 2 or 4): 8 x 8 tiles, tbase then holding each 8 x 8 tile's first of 16 bins; + 4096 w (ACH 4 or 8,
+ * G 1 or 2): 4 x 4 tiles with blocks of 64 << w channels, w 1 or 2), picked by timing (the reference times its tiled /
+ * plain projectors the same way, parallel_radon.py:240-281). */
 int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
                             const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
                             int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
