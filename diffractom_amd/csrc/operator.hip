@@ -604,11 +604,13 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
                                           : make_int4(0, 0, 0, 0);
         }
       }
-      for (int a0 = 0; a0 < tn; a0 += ACH) {
+      // PIPE: software-pipelined passes, the sinogram loads of pass a0 + ACH in flight while pass a0 is summed (for
+      // the 512 / 1024-thread forms; in the 256-thread ones the 8 more VGPRs cost more in occupancy than they gain)
+      constexpr bool PIPE = NTH >= 512;
+      constexpr int PER = ACH * TBB * QW / NTH;
+      float4 v[PER];
+      auto load_pass = [&](int a0) {
         const int nan = min(ACH, tn - a0);
-        __syncthreads();
-        constexpr int PER = ACH * TBB * QW / NTH;
-        float4 v[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
           const int i = threadIdx.x + NTH * u;
@@ -622,9 +624,19 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
             if (s >= 0 && s < ns && kk < k4_n) v[u] = sino[((int64_t)(t0 + a0 + al) * ns + s) * skstride4 + kk];
           }
         }
+      };
+      if (PIPE) {
+        __syncthreads();                                   // the chunk's bases staged
+        load_pass(0);
+      }
+      for (int a0 = 0; a0 < tn; a0 += ACH) {
+        const int nan = min(ACH, tn - a0);
+        __syncthreads();                                   // every wave done with the previous pass's LDS
+        if (!PIPE) load_pass(a0);
 #pragma unroll
         for (int u = 0; u < PER; ++u) lds[threadIdx.x + NTH * u] = v[u];
         __syncthreads();
+        if (PIPE && a0 + ACH < tn) load_pass(a0 + ACH);
         if (live) {
 #pragma unroll 4
           for (int al = 0; al < nan; ++al) {
