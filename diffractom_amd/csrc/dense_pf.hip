@@ -205,7 +205,10 @@ __device__ __forceinline__ void mfma_pf_body(const MfmaArgs& p, int64_t w, bf16x
       }
     }
   };
-  bf16x8 bh[NT], bl[NT], nh[NT], nl[NT];
+  // PF fragments of chunk c (bh, bl), c + 1 (nh, nl) and, with AHEAD2, c + 2 (mh, ml) in flight: at 2 output tiles
+  // per wave (146 VGPRs, 3 waves per SIMD) the second chunk ahead keeps the occupancy and doubles the bytes in flight
+  constexpr bool AHEAD2 = NT <= 2;
+  bf16x8 bh[NT], bl[NT], nh[NT], nl[NT], mh[NT], ml[NT];
   auto load_b = [&](int c, bf16x8 (&h)[NT], bf16x8 (&l)[NT]) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -223,13 +226,18 @@ __device__ __forceinline__ void mfma_pf_body(const MfmaArgs& p, int64_t w, bf16x
   load_raw(c0);
   load_b(c0, bh, bl);
   store_lds(0);
+  if (AHEAD2 && c0 + 1 < c1) load_b(c0 + 1, nh, nl);
   __syncthreads();
   int cur = 0;
   for (int c = c0; c < c1; ++c) {
     const bool more = c + 1 < c1;
     if (more) {
       load_raw(c + 1);
-      load_b(c + 1, nh, nl);
+      if (AHEAD2) {
+        if (c + 2 < c1) load_b(c + 2, mh, ml);
+      } else {
+        load_b(c + 1, nh, nl);
+      }
     }
 #pragma unroll
     for (int mt = 0; mt < MT_MAX; ++mt) {
@@ -253,6 +261,10 @@ __device__ __forceinline__ void mfma_pf_body(const MfmaArgs& p, int64_t w, bf16x
       for (int t = 0; t < NT; ++t) {
         bh[t] = nh[t];
         bl[t] = nl[t];
+        if (AHEAD2) {
+          nh[t] = mh[t];
+          nl[t] = ml[t];
+        }
       }
     }
     cur ^= 1;
