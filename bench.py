@@ -55,6 +55,8 @@ def parse():
                          "counter runs that give roofline.traffic)")
     ap.add_argument("--graphs", action="store_true", help="replay subset steps from captured HIP graphs (pays off on "
                     "small problems, where launches cost more than the kernels)")
+    ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem"],
+                    help="al_c0 / al_c1: the solver timed (osem: ordered-subsets EM subset iterations, one precision)")
     return ap.parse_args()
 
 
@@ -339,6 +341,48 @@ def run_c2(args):
     print(json.dumps(line), flush=True)
 
 
+def run_osem(args):
+    """OSEMPoisson on al_c0 / al_c1 (the same counts and operator as the SPDHG line, --precision's PF product):
+    warm-up, then `steps` timed subset iterations (forward of the subset, EM ratio, subset adjoint, multiplicative
+    update) with x, the back-projection and the M sensitivities resident in HBM. One JSON line: subset iterations/s,
+    and the deviance before and after."""
+    import torch
+    from diffractom_amd import OSEMPoisson
+    op, b, _, p = build_problem(args.config, precision=args.precision)
+    M = p["subsets"]
+    if not args.no_autotune:
+        tuning = op.autotune(subset=0)
+        log(f"[bench] osem {args.precision}: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    em = OSEMPoisson(op, subsets=M, seed=0)
+    x = em.start(np.zeros(op.coeff_shape, np.float32), b)
+    op.direct_cl(x, em.pred)
+    f0 = em._deviance(em.pred)
+    order = []
+    while len(order) < args.warmup + args.steps:
+        order += [int(m) for m in em.rng.permutation(M)]
+    for m in order[: args.warmup]:
+        em.iterate(m)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for m in order[args.warmup: args.warmup + args.steps]:
+        em.iterate(m)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    op.direct_cl(x, em.pred)
+    f1 = em._deviance(em.pred)
+    log(f"[bench] osem {args.precision}: {ms:.3f} ms per subset iteration; deviance {f0:.6g} -> {f1:.6g}")
+    line = {"metric": "OSEM-Poisson subset iterations/s", "value": round(1000.0 / ms, 3),
+            "unit": "OSEM subset iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": DTYPE[args.precision], "data": "synthetic (Voronoi grains, Poisson counts at a mean of 1000)",
+            "config": {"workload": f"{args.config}: K={op.K}, M={M} subsets, PF product {PF_DESC[args.precision]}",
+                       "K": op.K, "subsets": M, "precision": args.precision},
+            "deviance": {"start": f0, "after": f1, "iterations": args.warmup + args.steps}}
+    print(json.dumps(line), flush=True)
+
+
 def run_c4(args):
     """Differentiation at the C4 size: 8 slices of 128 x 128 px (180 omega x 131 x 180 eta x 5 rings, K = 5000,
     sigma 2 deg, BF16 residual PF), loss = Poisson NLL(f_a * A c; b) + 1e-3 TV(c), gradients with respect to c and
@@ -407,6 +451,9 @@ def main():
     import torch
     if args.config in ("al_c3", "ti_c2", "al_c4"):
         {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4}[args.config](args)
+        return
+    if args.solver == "osem":
+        run_osem(args)
         return
     # fp32 first: its counts are the problem every precision solves
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]

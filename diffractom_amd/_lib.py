@@ -55,6 +55,9 @@ SIGNATURES = {
     "dfm_project_support_k": [P, P, I64, I32, I32, P],
     "dfm_deviance_partial": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P, P],
     "dfm_mean_counted_partial": [P, P, I64, I64, P, P, P],
+    "dfm_poisson_em_ratio": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P],
+    "dfm_poisson_weight_over_gain": [P, I64, P, I64, I64, P, P],
+    "dfm_em_update": [P, P, P, P, I64, F32, I64, P],
     "dfm_dot_partial": [P, P, I64, P, P, P],
     "dfm_scale": [P, F32, I64, P],
     "dfm_tv_dual_k": [P, P, P, P, P, P, I32, I32, I32, F32, F32, P],

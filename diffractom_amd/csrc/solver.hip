@@ -322,6 +322,38 @@ static unsigned reduce_blocks(int64_t n) {
   return (unsigned)b;
 }
 
+// The EM pieces of the Poisson term (optimization/poisson_term.py:39-47 pt_ratio / pt_wg; osem_poisson.py:69-73
+// osem_update), float32 arithmetic in the reference's order. A zero weight marks a point outside the fit.
+//   ratio: out = w / g * b / mu, mu = max(pred, 0) + bg + eps g        weight over gain: out = w / g
+__global__ void __launch_bounds__(256) em_ratio_kernel(float* __restrict__ out, const float* __restrict__ pred,
+                                                       const float* __restrict__ b, const float* __restrict__ w,
+                                                       int64_t pw, const float* __restrict__ g, int64_t pg,
+                                                       const float* __restrict__ bg, int64_t pbg, float eps,
+                                                       int64_t n, int ratio) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = pval(w, pw, i, 1.0f);
+    const float gi = pval(g, pg, i, 1.0f);
+    if (!ratio) {
+      out[i] = wi / gi;
+      continue;
+    }
+    const float mu = fmaxf(pred[i], 0.0f) + pval(bg, pbg, i, 0.0f) + eps * gi;
+    out[i] = wi == 0.0f ? 0.0f : wi / gi * b[i] / mu;
+  }
+}
+
+// x = 0 outside the mask (periodic over pm points; pm 0: no mask), else x back / (sens + lam) where sens > 0
+__global__ void __launch_bounds__(256) em_update_kernel(float* __restrict__ x, const float* __restrict__ back,
+                                                        const float* __restrict__ sens,
+                                                        const uint8_t* __restrict__ mask, int64_t pm, float lam,
+                                                        int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const bool ok = pm ? mask[i % pm] != 0 : true;
+    const float s = sens[i];
+    x[i] = !ok ? 0.0f : (s > 0.0f ? x[i] * back[i] / (s + lam) : x[i]);
+  }
+}
+
 static dim3 pix_grid(int64_t npix, int kstride) {
   int64_t gy = npix < 32768 ? npix : 32768;
   return dim3((unsigned)((kstride + 255) / 256), (unsigned)gy);
@@ -441,6 +473,38 @@ int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t
   *n_partial = (int32_t)nb;
   hipLaunchKernelGGL(mean_counted_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, b, w, pw, n, partial);
   DFM_LAUNCH_CHECK("dfm_mean_counted_partial");
+  return DFM_OK;
+}
+
+int dfm_poisson_em_ratio(const float* pred, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                         const float* bg, int64_t pbg, float eps, int64_t n, float* out, dfm_stream_t stream) {
+  DFM_CHECK_ARG(pred && b && out && (pw == 0 || w) && (pg == 0 || g) && (pbg == 0 || bg) && n >= 0,
+                "dfm_poisson_em_ratio: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(em_ratio_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, out, pred,
+                     b, w, pw, g, pg, bg, pbg, eps, n, 1);
+  DFM_LAUNCH_CHECK("dfm_poisson_em_ratio");
+  return DFM_OK;
+}
+
+int dfm_poisson_weight_over_gain(const float* w, int64_t pw, const float* g, int64_t pg, int64_t n, float* out,
+                                 dfm_stream_t stream) {
+  DFM_CHECK_ARG(out && (pw == 0 || w) && (pg == 0 || g) && n >= 0, "dfm_poisson_weight_over_gain: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(em_ratio_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, out,
+                     nullptr, nullptr, w, pw, g, pg, nullptr, 0, 0.0f, n, 0);
+  DFM_LAUNCH_CHECK("dfm_poisson_weight_over_gain");
+  return DFM_OK;
+}
+
+int dfm_em_update(float* x, const float* back, const float* sens, const uint8_t* mask, int64_t pm, float lam,
+                  int64_t n, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && back && sens && (pm == 0 || mask) && pm >= 0 && n >= 0 && lam >= 0.0f,
+                "dfm_em_update: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(em_update_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, x, back,
+                     sens, mask, pm, lam, n);
+  DFM_LAUNCH_CHECK("dfm_em_update");
   return DFM_OK;
 }
 

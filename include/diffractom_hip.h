@@ -300,6 +300,19 @@ int dfm_deviance_partial(const float* mu, const float* b, const float* w, int64_
                          dfm_stream_t stream);
 int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t n, double* partial,
                              int32_t* n_partial, dfm_stream_t stream);
+/* The EM pieces of the Poisson term for OSEM / MLEM (optimization/poisson_term.py:39-47 pt_ratio, pt_wg;
+ * osem_poisson.py:69-73 osem_update), over n data points or n coefficients; w, g, bg periodic as in
+ * dfm_deviance_partial (a zero weight: the point is not fitted, e.g. its angle is in no subset):
+ *   dfm_poisson_em_ratio:         out = w / g * b / mu, mu = max(pred, 0) + bg + eps g  (0 where w = 0)
+ *   dfm_poisson_weight_over_gain: out = w / g
+ *   dfm_em_update:                x = 0 where mask[i % pm] == 0 (pm 0: no mask), else x back / (sens + lam) where
+ *                                 sens > 0, unchanged where sens <= 0 (lam >= 0) */
+int dfm_poisson_em_ratio(const float* pred, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                         const float* bg, int64_t pbg, float eps, int64_t n, float* out, dfm_stream_t stream);
+int dfm_poisson_weight_over_gain(const float* w, int64_t pw, const float* g, int64_t pg, int64_t n, float* out,
+                                 dfm_stream_t stream);
+int dfm_em_update(float* x, const float* back, const float* sens, const uint8_t* mask, int64_t pm, float lam,
+                  int64_t n, dfm_stream_t stream);
 int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
                     dfm_stream_t stream);
 /* Smoothed isotropic TV of coefficient images c (K, ny, nx) float32, a loss term of the differentiable refinement

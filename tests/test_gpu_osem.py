@@ -1,0 +1,83 @@
+"""OSEMPoisson on the GPU vs the FP64 oracle restatement of osem_poisson.py (same start, same subset order from the
+same seed): iterates, objectives, the start floor; held-out angles ignored; MLEM; weights and background; a torch x0
+updated in place. Tolerance: FP32 iterates within 1e-4 (relative L2) of the FP64 iterates, objectives within 1e-5."""
+import numpy as np
+import pytest
+
+from oracle.model import OracleOSEM
+from pairs import build, cfg, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(K=12, seed=7, gain=0.05):
+    op, orc, _ = build(cfg(nx=16, ny=16, my=17, n_omega=24, angle_range=(0, 180), cor=0.0), K=K, sigma_deg=4.0,
+                       pf_mode="sparse")
+    rng = np.random.default_rng(seed)
+    xt = (rng.random(op.coeff_shape) * (rng.random(op.coeff_shape) < 0.2) * op.support_mask()[None]
+          ).astype(np.float32)
+    ax = orc.direct(xt)
+    b = (rng.poisson(ax / ax.mean() * 50 / gain) * gain).astype(np.float32)
+    return op, orc, b
+
+
+@pytest.mark.parametrize("subsets,lam,extras", [(4, 0.0, False), (None, 0.0, False), (3, 20.0, True)])
+def test_osem_matches_oracle(subsets, lam, extras):
+    from diffractom_amd import OSEMPoisson
+    print(f"case subsets={subsets} lam={lam} extras={extras}", flush=True)
+    op, orc, b = _problem()
+    kw = {"gain": 0.05}
+    if extras:
+        rng = np.random.default_rng(3)
+        kw["weights"] = (rng.random(op.N_seg) > 0.2).astype(np.float32)
+        kw["background"] = np.full(op.N_seg, 0.3, np.float32)
+    em = OSEMPoisson(op, subsets=subsets, seed=1, lam=lam)
+    x = em.run(np.zeros(op.coeff_shape, np.float32), b, 8, diagnostics_interval=2, **kw)
+    ref = OracleOSEM(orc, subsets=subsets, seed=1, lam=lam)
+    x_ref = ref.run(np.zeros(op.coeff_shape), b, 8, diagnostics_interval=2, **kw)
+    assert abs(em.start_floor_used / ref.start_floor_used - 1) < 1e-5
+    assert rel_l2(x, x_ref) < 1e-4
+    f = [s["f"] for s in em.iter_stats if "f" in s]
+    f_ref = [s["f"] for s in ref.iter_stats if "f" in s]
+    np.testing.assert_allclose(f, f_ref, rtol=1e-5)
+    assert x.min() >= 0 and (x * ~op.support_mask()[None]).max() == 0
+
+
+def test_osem_ignores_held_out_angles():
+    """tests/test_rock_pipeline.py:86-103: angles in no subset do not change the fit (bit-identical), and the
+    deviance counts the fitted angles only."""
+    from diffractom_amd import OSEMPoisson
+    op, orc, b = _problem(seed=2)
+    fit = np.ones(op.data_shape[0], bool)
+    fit[[1, 2, 7, 13]] = False
+    subs = [np.flatnonzero(fit)[i::3] for i in range(3)]
+    b2 = b.copy()
+    b2[~fit] = np.random.default_rng(0).uniform(0, 10, b2[~fit].shape).astype(np.float32)
+    xs = []
+    for bb in (b, b2):
+        em = OSEMPoisson(op, subsets=subs, seed=4)
+        xs.append(em.run(np.zeros(op.coeff_shape, np.float32), bb, 5, gain=0.05, diagnostics_interval=0))
+    np.testing.assert_array_equal(xs[0], xs[1])
+    ref = OracleOSEM(orc, subsets=subs, seed=4)
+    ref.run(np.zeros(op.coeff_shape), b2, 5, gain=0.05, diagnostics_interval=0)
+    assert abs(em.final_f / ref.final_f - 1) < 1e-5
+
+
+def test_osem_torch_x0_and_recomputed_sensitivities():
+    """A torch x0 is updated in place and returned; store_sensitivities=False (the sensitivity recomputed every
+    iteration) gives the stored run's iterates; a support array is honoured."""
+    import torch
+    from diffractom_amd import OSEMPoisson
+    op, _, b = _problem(seed=5)
+    supp = op.support_mask().copy()
+    supp[:, :3] = False
+    one = OSEMPoisson(op, subsets=4, seed=2, support=supp)
+    x_one = one.run(np.zeros(op.coeff_shape, np.float32), b, 4, gain=0.05, diagnostics_interval=0)
+    xt = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
+    two = OSEMPoisson(op, subsets=4, seed=2, support=supp, store_sensitivities=False)
+    out = two.run(xt, b, 4, gain=0.05, diagnostics_interval=0)
+    assert out is xt
+    np.testing.assert_array_equal(xt.cpu().numpy(), x_one)
+    assert (x_one * ~supp[None]).max() == 0 and x_one[:, supp].max() > 0
+    with pytest.raises(ValueError):
+        OSEMPoisson(op, subsets=4, lam=-1.0)
