@@ -343,9 +343,9 @@ def run_c2(args):
 
 def run_osem(args):
     """OSEMPoisson on al_c0 / al_c1 (the same counts and operator as the SPDHG line, --precision's PF product):
-    warm-up, then `steps` timed subset iterations (forward of the subset, EM ratio, subset adjoint, multiplicative
-    update) with x, the back-projection and the M sensitivities resident in HBM. One JSON line: subset iterations/s,
-    and the deviance before and after."""
+    warm-up, then `steps` timed subset iterations (forward of the subset, EM ratio, subset PF adjoint, the
+    backprojection with the multiplicative update) with x and the M sensitivities resident in HBM in the
+    orientation-fastest layout. One JSON line: subset iterations/s, and the deviance before and after."""
     import torch
     from diffractom_amd import OSEMPoisson
     op, b, _, p = build_problem(args.config, precision=args.precision)
@@ -354,9 +354,8 @@ def run_osem(args):
         tuning = op.autotune(subset=0)
         log(f"[bench] osem {args.precision}: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
     em = OSEMPoisson(op, subsets=M, seed=0)
-    x = em.start(np.zeros(op.coeff_shape, np.float32), b)
-    op.direct_cl(x, em.pred)
-    f0 = em._deviance(em.pred)
+    em.start(np.zeros(op.coeff_shape, np.float32), b)
+    f0 = em.deviance()
     order = []
     while len(order) < args.warmup + args.steps:
         order += [int(m) for m in em.rng.permutation(M)]
@@ -370,9 +369,9 @@ def run_osem(args):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
-    op.direct_cl(x, em.pred)
-    f1 = em._deviance(em.pred)
-    log(f"[bench] osem {args.precision}: {ms:.3f} ms per subset iteration; deviance {f0:.6g} -> {f1:.6g}")
+    f1 = em.deviance()
+    log(f"[bench] osem {args.precision}: {ms:.3f} ms per subset iteration (EM update fused into the backprojection: "
+        f"{op.em_fused(0)}); deviance {f0:.6g} -> {f1:.6g}")
     line = {"metric": "OSEM-Poisson subset iterations/s", "value": round(1000.0 / ms, 3),
             "unit": "OSEM subset iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

@@ -587,10 +587,10 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
     const int64_t o = pix * kstride4 + kq + k4_off;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 zz = acc, xx = acc, tt = acc;
-    if (MODE == 1 && live) {
-      zz = ld_stream(z + o);
+    if (MODE >= 1 && live) {
+      if (MODE == 1) zz = ld_stream(z + o);
       xx = ld_stream(x + o);
-      tt = ld_stream(T + o);
+      tt = ld_stream(T + o);                              // MODE 2: the EM sensitivity
     }
     for (int t0 = 0; t0 < na; t0 += TCH) {
       const int tn = min(TCH, na - t0);
@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
     if (live) {
       if (MODE == 0) {
         st_stream(out + o, acc);
-      } else {
+      } else if (MODE == 1) {
         float4 zn, xn;
 #define DFM_UPD(c)                                                                  \
   zn.c = zz.c + acc.c;                                                              \
@@ -664,6 +664,14 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
         DFM_UPD(x) DFM_UPD(y) DFM_UPD(z) DFM_UPD(w)
 #undef DFM_UPD
         st_stream(z + o, zn);
+        st_stream(x + o, xn);
+      } else {
+        // the EM update (osem_poisson.py:69-73): x = x A^T r / (sens + lam) where sens > 0
+        float4 xn;
+        xn.x = tt.x > 0.0f ? xx.x * acc.x / (tt.x + lam) : xx.x;
+        xn.y = tt.y > 0.0f ? xx.y * acc.y / (tt.y + lam) : xx.y;
+        xn.z = tt.z > 0.0f ? xx.z * acc.z / (tt.z + lam) : xx.z;
+        xn.w = tt.w > 0.0f ? xx.w * acc.w / (tt.w + lam) : xx.w;
         st_stream(x + o, xn);
       }
     }
@@ -1115,11 +1123,12 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                              int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
                              int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
                              int32_t mode, int32_t variant, dfm_stream_t stream) {
-  DFM_CHECK_ARG(sino && ptab && tbase && (mode == 0 ? img_k != nullptr : (x && z && T)),
+  DFM_CHECK_ARG(sino && ptab && tbase &&
+                    (mode == 0 ? img_k != nullptr : mode == 1 ? (x && z && T) : (x && T)),
                 "dfm_radon_backward_table: null pointer");
   DFM_CHECK_ARG(nx > 0 && ny > 0 && ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 &&
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
-                    skstride >= k_n && (mode == 0 || mode == 1),
+                    skstride >= k_n && mode >= 0 && mode <= 2 && (lam >= 0.0f || mode != 2),
                 "dfm_radon_backward_table: bad sizes");
   // variant = angles staged per pass (4, 8, 16, 32: a workgroup per 4x4 tile and channel block), or ACH + 64 G
   // with ACH 4, 8 or 16 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged
@@ -1161,8 +1170,10 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
   }
     if (mode == 0) {
       if (wq == 1) { DFM_BWA(0, 32) } else { DFM_BWA(0, 64) }
-    } else {
+    } else if (mode == 1) {
       if (wq == 1) { DFM_BWA(1, 32) } else { DFM_BWA(1, 64) }
+    } else {
+      if (wq == 1) { DFM_BWA(2, 32) } else { DFM_BWA(2, 64) }
     }
 #undef DFM_BWA
 #undef DFM_BWG
@@ -1189,8 +1200,10 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
   }
     if (mode == 0) {
       if (ach == 4) { DFM_B8G(0, 4) } else { DFM_B8G(0, 8) }
-    } else {
+    } else if (mode == 1) {
       if (ach == 4) { DFM_B8G(1, 4) } else { DFM_B8G(1, 8) }
+    } else {
+      if (ach == 4) { DFM_B8G(2, 4) } else { DFM_B8G(2, 8) }
     }
 #undef DFM_B8G
 #undef DFM_B8
@@ -1223,14 +1236,21 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
   }
     if (mode == 0) {
       DFM_BGG(0)
-    } else {
+    } else if (mode == 1) {
       DFM_BGG(1)
+    } else {
+      DFM_BGG(2)
     }
 #undef DFM_BGG
 #undef DFM_BGA
 #undef DFM_BG
     DFM_LAUNCH_CHECK("dfm_radon_backward_table");
     return DFM_OK;
+  }
+  if (mode == 2) {
+    dfm::set_error("dfm_radon_backward_table: mode 2 (the EM update) needs a grouped variant (+ 64 G, + 1024 or "
+                   "+ 4096 w), not %d", variant);
+    return DFM_ERR_UNSUPPORTED;
   }
   const unsigned grid = xcd_grid(ntiles, (k4n + QB - 1) / QB);
 #define DFM_BT(M, A)                                                                                              \

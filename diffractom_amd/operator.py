@@ -415,7 +415,8 @@ class SinglePhaseForwardOperator:
         # kernel variants (identical results; set by autotune_projector)
         # kernel choices (identical results up to float32 rounding): "lds" (the image of a channel group resident in
         # LDS, when it fits), "table" (geometry tables, 2 taps), "arith" (footprint arithmetic, any taps)
-        self.radon_variant = {"fwd": 8 + 256, "bwd": 4, "fwd_kernel": "table", "bwd_kernel": "table", "fwd_pairs": 8}
+        self.radon_variant = {"fwd": 8 + 256, "bwd": 4 + 64 * 4, "fwd_kernel": "table", "bwd_kernel": "table",
+                              "fwd_pairs": 8}
         self.tuning = None
         self.fuse_projector = False
         self.fused_channels = 4
@@ -679,11 +680,20 @@ class SinglePhaseForwardOperator:
             return self._tables_full
         return None if self._sub_tables is None else self._sub_tables[self._subset(subset)]
 
+    def em_fused(self, subset=None):
+        """True when the backprojection choice applies the EM update itself (a grouped table variant, mode 2)."""
+        v = int(self.radon_variant["bwd"]) if self.radon_variant["bwd"] is not None else 0
+        grouped = (v // 64) % 16 > 0 or (v // 1024) % 4 == 1 or v // 4096 > 0
+        return self.radon_variant["bwd_kernel"] == "table" and grouped and self._tables(subset) is not None
+
     def _radon_backward(self, sino, geo, tab, n_ang, out_k, kstride, koff, kn, x=None, z=None, T=None, inv_p=0.0,
-                        lam=0.0):
-        """img (mode 0, out_k) or the SPDHG update of x, z (mode 1) from the sinogram of n_ang angles."""
+                        lam=0.0, em=False):
+        """img (mode 0, out_k), the SPDHG update of x, z (mode 1) or, with em (grouped table variants only), the EM
+        update x = x img / (T + lam) where T > 0 (mode 2) from the sinogram of n_ang angles."""
         st = self._stream()
-        mode = 0 if x is None else 1
+        mode = 0 if x is None else (2 if em else 1)
+        if em and (tab is None or self.radon_variant["bwd_kernel"] != "table"):
+            raise ValueError("the fused EM update needs a grouped table backprojection (em_fused())")
         cw = self.lds_width() if self.radon_variant["bwd_kernel"] == "lds" else 0
         if cw and koff % cw == 0 and kn % cw == 0:
             self._lds_projector(mode, sino, geo, n_ang, out_k if mode == 0 else x, z, T, inv_p, lam, None, None, 0,

@@ -6,10 +6,15 @@ Loss on the fitted angles (angles in some subset), mu = max(A x, 0) + bg + eps_c
 For the next subset m of a random permutation per epoch (numpy default_rng(seed)):
     x <- x A_m^T(w b / (g mu)) / (A_m^T(w / g) + lam)      (where the sensitivity A_m^T(w / g) > 0; 0 off the support)
 
-MI355X design: x, the back-projection and the per-subset sensitivities stay on the GPU in the operator's coefficient
-layout (K, Ny, Nx); the EM ratio and the multiplicative update are one element-wise kernel each
-(dfm_poisson_em_ratio, dfm_em_update), the deviance a float64 reduction (dfm_deviance_partial). Angles outside every
-subset are folded into the weights (w = 0), so every kernel sees one weight array.
+MI355X design: x and the per-subset sensitivities stay on the GPU for the whole run in the orientation-fastest layout
+(Npix, K) the Radon kernels read (as SPDHGPoisson's x, z, T), so an iteration is the subset's forward projection and
+PF product, the EM ratio (one element-wise kernel, dfm_poisson_em_ratio), the subset's PF adjoint and one
+backprojection kernel that also applies the multiplicative update (mode 2 of dfm_radon_backward_table): no
+gather / scatter passes and no back-projection array. Backprojection choices without that mode (the plain tile,
+LDS-resident and arithmetic kernels) backproject into a scratch image and update with dfm_em_update. The support is
+applied at the start: a zero coefficient stays zero under the multiplicative update. The deviance is a float64
+reduction (dfm_deviance_partial). Angles outside every subset are folded into the weights (w = 0), so every kernel
+sees one weight array.
 """
 from __future__ import annotations
 
@@ -68,11 +73,22 @@ class OSEMPoisson:
             raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
         return torch.from_numpy((mask > 0).astype(np.uint8)).to(op.device)
 
-    def _fwd(self, x, out, m):
-        self.op.direct_cl(x, out, subset=None if self.subsets is None else m)
+    def _sub(self, m):
+        return None if self.subsets is None else m
 
-    def _adj(self, r, out, m):
-        self.op.adjoint_cl(r, out, subset=None if self.subsets is None else m)
+    def _adj(self, r, out_k, m):
+        """out_k = A_m^T r in the orientation-fastest layout."""
+        self.op.adjoint_k(r, out_k, subset=self._sub(m))
+
+    def deviance(self):
+        """The deviance of the current iterate on the fitted angles (a full forward)."""
+        self.op.direct_k(self.xk, self.pred)
+        return self._deviance(self.pred)
+
+    def coefficients(self):
+        """The current iterate in the (K, Ny, Nx) layout (the x that start returned, updated)."""
+        self.op.from_k(self.xk, self.x)
+        return self.x
 
     def _deviance(self, pred):
         n = ctypes.c_int32(0)
@@ -117,9 +133,10 @@ class OSEMPoisson:
         _lib.call("dfm_poisson_weight_over_gain", _lib.ptr(self.w), self.pw, _lib.ptr(self.g), self.pg, nd,
                   _lib.ptr(wg), st)
         self.wg = wg
-        self.back = torch.empty(cshape, dtype=torch.float32, device=dev)
+        nk = op.Nx * op.Ny * op.K_stride
+        self.back = None
         n_sens = self.M if self.store_sensitivities else 1
-        self.sens = [torch.empty(cshape, dtype=torch.float32, device=dev) for _ in range(n_sens)]
+        self.sens = [torch.empty(nk, dtype=torch.float32, device=dev) for _ in range(n_sens)]
         if self.store_sensitivities:
             for m in range(self.M):
                 self._adj(wg, self.sens[m], m)
@@ -135,11 +152,12 @@ class OSEMPoisson:
                 floor = float(start_floor) * float(np.median(pos))
             else:
                 stot = 0.0
+                pix = allowed[: op.Nx * op.Ny]                    # the (Ny, Nx) mask; the same for every k
                 for m in range(self.M):
                     if not self.store_sensitivities:
                         self._adj(wg, self.sens[0], m)
-                    s = self.sens[m if self.store_sensitivities else 0]
-                    stot += float(s.cpu().numpy().ravel()[allowed].sum(dtype=np.float64))
+                    s = self.sens[m if self.store_sensitivities else 0].view(op.Nx * op.Ny, op.K_stride)
+                    stot += float(s[torch.from_numpy(pix).to(dev)].sum(dtype=torch.float64).item())
                 wgh = wg.cpu().numpy().ravel().astype(np.float64)
                 tot = float(wgh @ self.b.cpu().numpy().ravel().astype(np.float64))
                 if self.pbg:
@@ -149,24 +167,36 @@ class OSEMPoisson:
             self.start_floor_used = floor
         x.copy_(torch.from_numpy(xh.reshape(cshape)).to(dev))
         self.x = x
+        self.xk = op.to_k(x)
         return x
 
     def iterate(self, m):
         """One update with subset m (osem_poisson.py:155-166): x <- x A_m^T(w b / (g mu)) / (A_m^T(w / g) + lam)."""
         op, st = self.op, self.op._stream()
-        self._fwd(self.x, self.pred, m)
+        sub = self._sub(m)
+        op.direct_k(self.xk, self.pred, subset=sub)
         _lib.call("dfm_poisson_em_ratio", _lib.ptr(self.pred), _lib.ptr(self.b), _lib.ptr(self.w), self.pw,
                   _lib.ptr(self.g), self.pg, _lib.ptr(self.bg), self.pbg, float(self.eps_counts), self.pred.numel(),
                   _lib.ptr(self.r), st)
-        self._adj(self.r, self.back, m)
         if self.store_sensitivities:
             s = self.sens[m]
         else:
             self._adj(self.wg, self.sens[0], m)
             s = self.sens[0]
-        _lib.call("dfm_em_update", _lib.ptr(self.x), _lib.ptr(self.back), _lib.ptr(s),
-                  _lib.ptr(self.mask) if self.mask is not None else None, self.pmask, float(self.lam),
-                  self.x.numel(), st)
+        if op.em_fused(sub):
+            n_ang, _, _, geo = op._angle_set(sub)
+            tab = op._tables(sub)
+            sc = op._scratch_buffers(n_ang)
+            for ib, bt in enumerate(op.batches):
+                op._pf_adjoint(ib, self.r, sc["sino"], sub)
+                op._radon_backward(sc["sino"], geo, tab, n_ang, None, op.K_stride, bt["koff"], bt["kstride"],
+                                   x=self.xk, T=s, lam=self.lam, em=True)
+            return
+        if self.back is None:
+            self.back = torch.empty_like(self.xk)
+        op.adjoint_k(self.r, self.back, subset=sub)
+        _lib.call("dfm_em_update", _lib.ptr(self.xk), _lib.ptr(self.back), _lib.ptr(s), None, 0, float(self.lam),
+                  self.xk.numel(), st)
 
     def epoch(self):
         """All subsets once, in a random order (osem_poisson.py:154)."""
@@ -187,7 +217,6 @@ class OSEMPoisson:
         the end, in iter_stats (epoch, f, time) and final_f; callback(epoch, x, stats) after every epoch, a true
         return value stops the run."""
         t0 = time.perf_counter()
-        op = self.op
         x = self.start(x0, b, weights, gain, background, start_floor)
         self.iter_stats = []
         f = None
@@ -196,16 +225,17 @@ class OSEMPoisson:
             stats = {"epoch": ep + 1, "time": time.perf_counter() - t0}
             f = None
             if diagnostics_interval and ((ep + 1) % int(diagnostics_interval) == 0 or ep + 1 == int(n_epochs)):
-                op.direct_cl(x, self.pred)
-                f = self._deviance(self.pred)
+                f = self.deviance()
                 stats["f"] = f
                 if verbose:
                     print(f"OSEM epoch {ep + 1}: deviance {f:.6g} ({stats['time']:.1f} s)", flush=True)
             self.iter_stats.append(stats)
-            if callback is not None and callback(ep + 1, x if self.on_gpu else x.cpu().numpy(), stats):
-                break
+            if callback is not None:
+                self.coefficients()
+                if callback(ep + 1, x if self.on_gpu else x.cpu().numpy(), stats):
+                    break
         if f is None:
-            op.direct_cl(x, self.pred)
-            f = self._deviance(self.pred)
+            f = self.deviance()
         self.final_f = f
+        self.coefficients()
         return x if self.on_gpu else x.cpu().numpy()

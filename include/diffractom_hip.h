@@ -71,7 +71,10 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  * pixel table ptab (n_angles, Npix, 4) int32: bin s0, bin s1 (>= 0, < ns), float bits of w0, w1 (the backprojection
  *   weights, wscale_a folded in; 0 for taps off the detector);
  * tile bases tbase (n_angles, ceil(nx/4) * ceil(ny/4)): the first of the 8 bins every 4 x 4 pixel tile reaches.
- * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg.
+ * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg;
+ * mode 2 (grouped variants: + 64 G, + 1024, + 4096 w): the EM update x = T > 0 ? x backward(sino) / (T + lam) : x with
+ * T the EM sensitivity, lam >= 0, z unused (osem_poisson.py:69-73 osem_update applied to the output of
+ * radon_backward_k); the plain tile variants return DFM_ERR_UNSUPPORTED for it.
  * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8, plus 16 * cap
  * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS,
  * plus 512 q for channel blocks of 32, 16, 128 or 256 channels (q 1 .. 4; 64 for q 0);

@@ -21,17 +21,27 @@ def _problem(K=12, seed=7, gain=0.05):
     return op, orc, b
 
 
-@pytest.mark.parametrize("subsets,lam,extras", [(4, 0.0, False), (None, 0.0, False), (3, 20.0, True)])
-def test_osem_matches_oracle(subsets, lam, extras):
+@pytest.mark.parametrize("subsets,lam,extras,bwd", [(4, 0.0, False, None), (None, 0.0, False, None),
+                                                    (3, 20.0, True, None), (4, 5.0, True, 4),
+                                                    (3, 0.0, False, 4 + 64 + 1024), (4, 0.0, False, 8 + 64 + 8192)])
+def test_osem_matches_oracle(subsets, lam, extras, bwd):
+    """bwd: the backprojection variant (None: the operator's default, a grouped kernel applying the EM update
+    itself; 4: the plain tile kernel, backprojection then the element-wise update; 8x8 tiles; 256-channel blocks)."""
     from diffractom_amd import OSEMPoisson
-    print(f"case subsets={subsets} lam={lam} extras={extras}", flush=True)
+    print(f"case subsets={subsets} lam={lam} extras={extras} bwd={bwd}", flush=True)
     op, orc, b = _problem()
+    if bwd is not None:
+        op.radon_variant["bwd"] = bwd
     kw = {"gain": 0.05}
     if extras:
         rng = np.random.default_rng(3)
         kw["weights"] = (rng.random(op.N_seg) > 0.2).astype(np.float32)
         kw["background"] = np.full(op.N_seg, 0.3, np.float32)
     em = OSEMPoisson(op, subsets=subsets, seed=1, lam=lam)
+    sub = 0 if subsets else None
+    if bwd is not None and (bwd // 1024) % 4 == 1 and op._tables(sub)["tbase8"].numel() == 1:
+        pytest.skip("8x8 tiles reach more than 16 bins for this geometry")
+    assert op.em_fused(sub) == (bwd != 4)
     x = em.run(np.zeros(op.coeff_shape, np.float32), b, 8, diagnostics_interval=2, **kw)
     ref = OracleOSEM(orc, subsets=subsets, seed=1, lam=lam)
     x_ref = ref.run(np.zeros(op.coeff_shape), b, 8, diagnostics_interval=2, **kw)
