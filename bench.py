@@ -55,8 +55,9 @@ def parse():
                          "counter runs that give roofline.traffic)")
     ap.add_argument("--graphs", action="store_true", help="replay subset steps from captured HIP graphs (pays off on "
                     "small problems, where launches cost more than the kernels)")
-    ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem"],
-                    help="al_c0 / al_c1: the solver timed (osem: ordered-subsets EM subset iterations, one precision)")
+    ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem", "pdhg"],
+                    help="al_c0 / al_c1: the solver timed (osem: ordered-subsets EM subset iterations; pdhg: "
+                         "Chambolle-Pock iterations over all angles; one precision)")
     return ap.parse_args()
 
 
@@ -382,6 +383,44 @@ def run_osem(args):
     print(json.dumps(line), flush=True)
 
 
+def run_pdhg(args):
+    """PDHGPoisson on al_c0 / al_c1 (the SPDHG line's counts and operator, --precision's PF product): warm-up, then
+    `steps` timed iterations over all angles (dual update, PF adjoint, the backprojection with the primal step and
+    the extrapolation, forward of x_bar, the A x average, the float64 deviance) with x, x_bar and T resident in HBM.
+    One JSON line: iterations/s and the deviance before and after."""
+    import torch
+    from diffractom_amd import PDHGPoisson
+    op, b, _, p = build_problem(args.config, precision=args.precision)
+    if not args.no_autotune:
+        tuning = op.autotune(subset=0)
+        log(f"[bench] pdhg {args.precision}: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    pd = PDHGPoisson(op)
+    pd.start(np.zeros(op.coeff_shape, np.float32), b)
+    f0 = pd._deviance(pd.Ax)
+    for _ in range(args.warmup):
+        pd.iterate()
+        pd._deviance(pd.Ax)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        pd.iterate()
+        f1 = pd._deviance(pd.Ax)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    log(f"[bench] pdhg {args.precision}: {ms:.3f} ms per iteration (primal step fused into the backprojection: "
+        f"{op.em_fused(None)}); deviance {f0:.6g} -> {f1:.6g}")
+    line = {"metric": "PDHG-Poisson iterations/s", "value": round(1000.0 / ms, 3), "unit": "PDHG iterations/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.precision],
+            "data": "synthetic (Voronoi grains, Poisson counts at a mean of 1000)",
+            "config": {"workload": f"{args.config}: K={op.K}, all {op.N_Omega} angles per iteration, PF product "
+                                   f"{PF_DESC[args.precision]}", "K": op.K, "precision": args.precision},
+            "deviance": {"start": f0, "after": f1, "iterations": args.warmup + args.steps}}
+    print(json.dumps(line), flush=True)
+
+
 def run_c4(args):
     """Differentiation at the C4 size: 8 slices of 128 x 128 px (180 omega x 131 x 180 eta x 5 rings, K = 5000,
     sigma 2 deg, BF16 residual PF), loss = Poisson NLL(f_a * A c; b) + 1e-3 TV(c), gradients with respect to c and
@@ -451,8 +490,8 @@ def main():
     if args.config in ("al_c3", "ti_c2", "al_c4"):
         {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4}[args.config](args)
         return
-    if args.solver == "osem":
-        run_osem(args)
+    if args.solver in ("osem", "pdhg"):
+        {"osem": run_osem, "pdhg": run_pdhg}[args.solver](args)
         return
     # fp32 first: its counts are the problem every precision solves
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]

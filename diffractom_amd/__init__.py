@@ -5,10 +5,12 @@ from .crystallography import Material, group_reflections_into_rings
 from .grid import Grid, MatrixGrid
 from .operator import SinglePhaseForwardOperator, estimate_L_power
 from .osem import OSEMPoisson
+from .pdhg import PDHGPoisson
 from .quadrature import QuadratureRule
 from .spdhg import SPDHGPoisson
 from .support import fov_support_mask
 
 __all__ = ["Material", "Grid", "MatrixGrid", "SinglePhaseForwardOperator", "SPDHGPoisson", "OSEMPoisson",
+           "PDHGPoisson",
            "estimate_L_power",
            "QuadratureRule", "fov_support_mask", "group_reflections_into_rings"]

@@ -58,6 +58,8 @@ SIGNATURES = {
     "dfm_poisson_em_ratio": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P],
     "dfm_poisson_weight_over_gain": [P, I64, P, I64, I64, P, P],
     "dfm_em_update": [P, P, P, P, I64, F32, I64, P],
+    "dfm_pdhg_primal_update": [P, P, P, P, F32, I64, P],
+    "dfm_average": [P, P, I64, P],
     "dfm_dot_partial": [P, P, I64, P, P, P],
     "dfm_scale": [P, F32, I64, P],
     "dfm_tv_dual_k": [P, P, P, P, P, P, I32, I32, I32, F32, F32, P],

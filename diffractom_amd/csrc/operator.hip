@@ -590,7 +590,7 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
     if (MODE >= 1 && live) {
       if (MODE == 1) zz = ld_stream(z + o);
       xx = ld_stream(x + o);
-      tt = ld_stream(T + o);                              // MODE 2: the EM sensitivity
+      tt = ld_stream(T + o);                              // MODE 2: the EM sensitivity; 3: the PDHG step
     }
     for (int t0 = 0; t0 < na; t0 += TCH) {
       const int tn = min(TCH, na - t0);
@@ -665,13 +665,24 @@ __global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(cons
 #undef DFM_UPD
         st_stream(z + o, zn);
         st_stream(x + o, xn);
-      } else {
+      } else if (MODE == 2) {
         // the EM update (osem_poisson.py:69-73): x = x A^T r / (sens + lam) where sens > 0
         float4 xn;
         xn.x = tt.x > 0.0f ? xx.x * acc.x / (tt.x + lam) : xx.x;
         xn.y = tt.y > 0.0f ? xx.y * acc.y / (tt.y + lam) : xx.y;
         xn.z = tt.z > 0.0f ? xx.z * acc.z / (tt.z + lam) : xx.z;
         xn.w = tt.w > 0.0f ? xx.w * acc.w / (tt.w + lam) : xx.w;
+        st_stream(x + o, xn);
+      } else {
+        // the PDHG primal step and extrapolation (pdhg_poisson.py:69-80 primal_update): x' = max(0, x - T (A^T y +
+        // lam)) where T > 0, else 0; x_bar = 2 x' - x (into z)
+        float4 xn, xb;
+#define DFM_PD(c)                                                        \
+  xn.c = tt.c > 0.0f ? fmaxf(0.0f, xx.c - tt.c * (acc.c + lam)) : 0.0f;  \
+  xb.c = 2.0f * xn.c - xx.c;
+        DFM_PD(x) DFM_PD(y) DFM_PD(z) DFM_PD(w)
+#undef DFM_PD
+        st_stream(z + o, xb);
         st_stream(x + o, xn);
       }
     }
@@ -1124,11 +1135,11 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                              int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
                              int32_t mode, int32_t variant, dfm_stream_t stream) {
   DFM_CHECK_ARG(sino && ptab && tbase &&
-                    (mode == 0 ? img_k != nullptr : mode == 1 ? (x && z && T) : (x && T)),
+                    (mode == 0 ? img_k != nullptr : mode == 2 ? (x && T) : (x && z && T)),
                 "dfm_radon_backward_table: null pointer");
   DFM_CHECK_ARG(nx > 0 && ny > 0 && ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 &&
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
-                    skstride >= k_n && mode >= 0 && mode <= 2 && (lam >= 0.0f || mode != 2),
+                    skstride >= k_n && mode >= 0 && mode <= 3 && (lam >= 0.0f || mode != 2),
                 "dfm_radon_backward_table: bad sizes");
   // variant = angles staged per pass (4, 8, 16, 32: a workgroup per 4x4 tile and channel block), or ACH + 64 G
   // with ACH 4, 8 or 16 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged
@@ -1172,8 +1183,10 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
       if (wq == 1) { DFM_BWA(0, 32) } else { DFM_BWA(0, 64) }
     } else if (mode == 1) {
       if (wq == 1) { DFM_BWA(1, 32) } else { DFM_BWA(1, 64) }
-    } else {
+    } else if (mode == 2) {
       if (wq == 1) { DFM_BWA(2, 32) } else { DFM_BWA(2, 64) }
+    } else {
+      if (wq == 1) { DFM_BWA(3, 32) } else { DFM_BWA(3, 64) }
     }
 #undef DFM_BWA
 #undef DFM_BWG
@@ -1202,8 +1215,10 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
       if (ach == 4) { DFM_B8G(0, 4) } else { DFM_B8G(0, 8) }
     } else if (mode == 1) {
       if (ach == 4) { DFM_B8G(1, 4) } else { DFM_B8G(1, 8) }
-    } else {
+    } else if (mode == 2) {
       if (ach == 4) { DFM_B8G(2, 4) } else { DFM_B8G(2, 8) }
+    } else {
+      if (ach == 4) { DFM_B8G(3, 4) } else { DFM_B8G(3, 8) }
     }
 #undef DFM_B8G
 #undef DFM_B8
@@ -1238,8 +1253,10 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
       DFM_BGG(0)
     } else if (mode == 1) {
       DFM_BGG(1)
-    } else {
+    } else if (mode == 2) {
       DFM_BGG(2)
+    } else {
+      DFM_BGG(3)
     }
 #undef DFM_BGG
 #undef DFM_BGA
@@ -1247,9 +1264,9 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
     DFM_LAUNCH_CHECK("dfm_radon_backward_table");
     return DFM_OK;
   }
-  if (mode == 2) {
-    dfm::set_error("dfm_radon_backward_table: mode 2 (the EM update) needs a grouped variant (+ 64 G, + 1024 or "
-                   "+ 4096 w), not %d", variant);
+  if (mode >= 2) {
+    dfm::set_error("dfm_radon_backward_table: mode %d (the EM / PDHG update) needs a grouped variant (+ 64 G, "
+                   "+ 1024 or + 4096 w), not %d", mode, variant);
     return DFM_ERR_UNSUPPORTED;
   }
   const unsigned grid = xcd_grid(ntiles, (k4n + QB - 1) / QB);

@@ -354,6 +354,26 @@ __global__ void __launch_bounds__(256) em_update_kernel(float* __restrict__ x, c
   }
 }
 
+// PDHG (pdhg_poisson.py:69-86 primal_update / average): x' = max(0, x - T (grad + lam)) where T > 0, else 0;
+// x_bar = 2 x' - x; and A x_k = (A x_bar_k + A x_(k-1)) / 2
+__global__ void __launch_bounds__(256) pdhg_primal_kernel(float* __restrict__ x, float* __restrict__ xb,
+                                                          const float* __restrict__ grad,
+                                                          const float* __restrict__ T, float lam, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float t = T[i];
+    const float xo = x[i];
+    const float xn = t > 0.0f ? fmaxf(0.0f, xo - t * (grad[i] + lam)) : 0.0f;
+    xb[i] = 2.0f * xn - xo;
+    x[i] = xn;
+  }
+}
+
+__global__ void __launch_bounds__(256) average_kernel(float* __restrict__ ax, const float* __restrict__ axb,
+                                                      int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    ax[i] = 0.5f * (axb[i] + ax[i]);
+}
+
 static dim3 pix_grid(int64_t npix, int kstride) {
   int64_t gy = npix < 32768 ? npix : 32768;
   return dim3((unsigned)((kstride + 255) / 256), (unsigned)gy);
@@ -505,6 +525,24 @@ int dfm_em_update(float* x, const float* back, const float* sens, const uint8_t*
   hipLaunchKernelGGL(em_update_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, x, back,
                      sens, mask, pm, lam, n);
   DFM_LAUNCH_CHECK("dfm_em_update");
+  return DFM_OK;
+}
+
+int dfm_pdhg_primal_update(float* x, float* xb, const float* grad, const float* T, float lam, int64_t n,
+                           dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && xb && grad && T && n >= 0, "dfm_pdhg_primal_update: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(pdhg_primal_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, x, xb,
+                     grad, T, lam, n);
+  DFM_LAUNCH_CHECK("dfm_pdhg_primal_update");
+  return DFM_OK;
+}
+
+int dfm_average(float* ax, const float* axb, int64_t n, dfm_stream_t stream) {
+  DFM_CHECK_ARG(ax && axb && n >= 0, "dfm_average: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(average_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, ax, axb, n);
+  DFM_LAUNCH_CHECK("dfm_average");
   return DFM_OK;
 }
 

@@ -681,19 +681,21 @@ class SinglePhaseForwardOperator:
         return None if self._sub_tables is None else self._sub_tables[self._subset(subset)]
 
     def em_fused(self, subset=None):
-        """True when the backprojection choice applies the EM update itself (a grouped table variant, mode 2)."""
+        """True when the backprojection choice applies the EM or PDHG update itself (a grouped table variant, modes 2
+        and 3 of dfm_radon_backward_table)."""
         v = int(self.radon_variant["bwd"]) if self.radon_variant["bwd"] is not None else 0
         grouped = (v // 64) % 16 > 0 or (v // 1024) % 4 == 1 or v // 4096 > 0
         return self.radon_variant["bwd_kernel"] == "table" and grouped and self._tables(subset) is not None
 
     def _radon_backward(self, sino, geo, tab, n_ang, out_k, kstride, koff, kn, x=None, z=None, T=None, inv_p=0.0,
-                        lam=0.0, em=False):
-        """img (mode 0, out_k), the SPDHG update of x, z (mode 1) or, with em (grouped table variants only), the EM
-        update x = x img / (T + lam) where T > 0 (mode 2) from the sinogram of n_ang angles."""
+                        lam=0.0, em=False, pdhg=False):
+        """img (mode 0, out_k), the SPDHG update of x, z (mode 1) or, grouped table variants only (em_fused()), the
+        EM update x = x img / (T + lam) where T > 0 (em: mode 2) or the PDHG primal step x' = max(0, x - T (img +
+        lam)), z = 2 x' - x (pdhg: mode 3) from the sinogram of n_ang angles."""
         st = self._stream()
-        mode = 0 if x is None else (2 if em else 1)
-        if em and (tab is None or self.radon_variant["bwd_kernel"] != "table"):
-            raise ValueError("the fused EM update needs a grouped table backprojection (em_fused())")
+        mode = 0 if x is None else (2 if em else 3 if pdhg else 1)
+        if mode >= 2 and (tab is None or self.radon_variant["bwd_kernel"] != "table"):
+            raise ValueError("the fused EM / PDHG updates need a grouped table backprojection (em_fused())")
         cw = self.lds_width() if self.radon_variant["bwd_kernel"] == "lds" else 0
         if cw and koff % cw == 0 and kn % cw == 0:
             self._lds_projector(mode, sino, geo, n_ang, out_k if mode == 0 else x, z, T, inv_p, lam, None, None, 0,

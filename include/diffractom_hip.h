@@ -74,7 +74,9 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg;
  * mode 2 (grouped variants: + 64 G, + 1024, + 4096 w): the EM update x = T > 0 ? x backward(sino) / (T + lam) : x with
  * T the EM sensitivity, lam >= 0, z unused (osem_poisson.py:69-73 osem_update applied to the output of
- * radon_backward_k); the plain tile variants return DFM_ERR_UNSUPPORTED for it.
+ * radon_backward_k); mode 3 (grouped variants): the PDHG primal step and extrapolation x' = T > 0 ? max(0, x - T
+ * (backward(sino) + lam)) : 0, z = 2 x' - x (x_bar), x = x' (pdhg_poisson.py:69-80 primal_update applied to the
+ * output of radon_backward_k); the plain tile variants return DFM_ERR_UNSUPPORTED for modes 2 and 3.
  * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8, plus 16 * cap
  * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS,
  * plus 512 q for channel blocks of 32, 16, 128 or 256 channels (q 1 .. 4; 64 for q 0);
@@ -316,6 +318,12 @@ int dfm_poisson_weight_over_gain(const float* w, int64_t pw, const float* g, int
                                  dfm_stream_t stream);
 int dfm_em_update(float* x, const float* back, const float* sens, const uint8_t* mask, int64_t pm, float lam,
                   int64_t n, dfm_stream_t stream);
+/* PDHG pieces (pdhg_poisson.py:69-86 primal_update, average): x' = T > 0 ? max(0, x - T (grad + lam)) : 0,
+ * xb = 2 x' - x, x = x' over n coefficients; and ax = (axb + ax) / 2 over n data points (A x of the new iterate from
+ * A x_bar). */
+int dfm_pdhg_primal_update(float* x, float* xb, const float* grad, const float* T, float lam, int64_t n,
+                           dfm_stream_t stream);
+int dfm_average(float* ax, const float* axb, int64_t n, dfm_stream_t stream);
 int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
                     dfm_stream_t stream);
 /* Smoothed isotropic TV of coefficient images c (K, ny, nx) float32, a loss term of the differentiable refinement
