@@ -93,7 +93,8 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
  * orientations, `lines` image lines are staged in LDS at a time and each ray adds its pixels on them; geo: per angle
  * (X, Y, T0, hw, top, scale) in float64, the weight scale * min(hw - |X x + Y y + T0 - s|, top) where positive.
  * Needs nx, ny <= 256; lines: image lines per LDS block (1, 2, 4 or 8, within the LDS: dfm_radon_sweep_lines)
- * + 16 w for 16 << w orientations per workgroup (w 0, 1, 2) + 64 for workgroups of 512 threads (w = 2 only);
+ * + 16 w for 16 << w orientations per workgroup (w 0, 1, 2) + 64 for workgroups of 512 threads; instantiated: 16
+ * orientations x 2, 4 or 8 lines and 32 x 1, 2 or 4 at 1024 threads, 32 x 2 and 64 x 1 or 2 at 512 threads;
  * na_grp * ns <= the workgroup's threads. */
 int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
                             const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx, int32_t ny,
