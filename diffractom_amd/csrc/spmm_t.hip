@@ -11,9 +11,9 @@
 // range in chunks of kc columns: the chunk of `in` (kc rows of ys floats, contiguous) is staged in LDS, then every wave
 // consumes, for each of its R rows, the row's entries that fall in the chunk. The lanes of a wave run over the
 // detector positions y (YW groups of 64), so an entry is wave-uniform (its column read from the row's 64-entry window
-// held in registers with v_readlane, its value by a scalar load) and its operand is one conflict-free LDS row read. Per
-// row the column window moves on only when all 64 of its entries are consumed. Summation follows the CSR order, as in
-// the row-per-thread kernels.
+// held in registers with v_readlane, its value likewise from the window's values) and its operand is one
+// conflict-free LDS row read. Per row the column window moves on only when all 64 of its entries are consumed.
+// Summation follows the CSR order, as in the row-per-thread kernels.
 #include <algorithm>
 
 #include "dfm_common.h"
@@ -57,8 +57,10 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
       cur = lo;
     }
   }
-  // per row: window start we (entry index of lane 0), consumed count wo, end ee; window columns cb per lane
+  // per row: window start we (entry index of lane 0), consumed count wo, end ee; window columns cb and values vw
+  // per lane (one coalesced vector load each; an entry's column and value are then read with v_readlane)
   int we[R], wo[R], ee[R], cb[R];
+  float vw[R];
   float acc[R][YW];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
@@ -67,6 +69,7 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
     wo[i] = 0;
     const int idx = we[i] + lane;
     cb[i] = idx < ee[i] ? (int)col[idx] : 0x7fffffff;
+    vw[i] = idx < ee[i] ? val[idx] : 0.0f;
 #pragma unroll
     for (int h = 0; h < YW; ++h) acc[i][h] = 0.f;
   }
@@ -135,14 +138,14 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
       while (true) {
         const int cnt = __popcll(__ballot(cb[i] < c1));     // entries of the window below c1 (a prefix)
         const int off = (cb[i] - c0) * ys;                   // operand row offset of each window entry
-        const float* vrow = val + we[i];                      // entry values: wave-uniform (scalar) loads
+        const int vbits = __float_as_int(vw[i]);
         int t = wo[i];
         for (; t + U <= cnt; t += U) {                        // U operand rows in flight, summed in entry order
           float x[U][YW], v[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const float* lp = lbase + __builtin_amdgcn_readlane(off, t + u);
-            v[u] = vrow[t + u];
+            v[u] = __int_as_float(__builtin_amdgcn_readlane(vbits, t + u));
 #pragma unroll
             for (int h = 0; h < YW; ++h) x[u][h] = lp[64 * h];
           }
@@ -154,7 +157,7 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
         }
         for (; t < cnt; ++t) {
           const float* lp = lbase + __builtin_amdgcn_readlane(off, t);
-          const float vt = vrow[t];
+          const float vt = __int_as_float(__builtin_amdgcn_readlane(vbits, t));
 #pragma unroll
           for (int h = 0; h < YW; ++h) acc[i][h] += vt * lp[64 * h];
         }
@@ -164,6 +167,7 @@ __global__ void __launch_bounds__(1024) spmm_t_kernel(const float* __restrict__ 
         wo[i] = 0;
         const int idx = we[i] + lane;
         cb[i] = idx < ee[i] ? (int)col[idx] : 0x7fffffff;
+        vw[i] = idx < ee[i] ? val[idx] : 0.0f;
       }
     }
   }
@@ -259,7 +263,7 @@ int spmm_t_impl(const float* in, int64_t in_ld, const int32_t* row_ptr, const ui
   }
   if (n_angles == 0 || rows_out == 0) return DFM_OK;
   const int yw = (my + 63) / 64;
-  const int R = my <= 192 ? 16 : 8;
+  const int R = my <= 128 ? 16 : 8;   // 16 rows of 3 y groups would spill with the value windows
   const dim3 grid((unsigned)((rows_out + SPT_WAVES * R - 1) / (SPT_WAVES * R)), (unsigned)n_angles,
                   (unsigned)nsplit);
   size_t lds = ((size_t)kc * ys + 64) * sizeof(float);
@@ -270,7 +274,7 @@ int spmm_t_impl(const float* in, int64_t in_ld, const int32_t* row_ptr, const ui
   switch (yw) {
     case 1: DFM_SPT(16, 1); break;
     case 2: DFM_SPT(16, 2); break;
-    case 3: DFM_SPT(16, 3); break;
+    case 3: DFM_SPT(8, 3); break;
     case 4: DFM_SPT(8, 4); break;
     case 5: DFM_SPT(8, 5); break;
     case 6: DFM_SPT(8, 6); break;
@@ -287,7 +291,7 @@ extern "C" {
 
 int dfm_spmm_t_rows_per_block(int32_t my) {
   if (my <= 0 || my > 512) return 0;
-  return SPT_WAVES * (my <= 192 ? 16 : 8);
+  return SPT_WAVES * (my <= 128 ? 16 : 8);
 }
 
 int dfm_spmm_t(const float* inT, const int32_t* row_ptr, const uint16_t* col, const float* val, float* outT,
