@@ -543,7 +543,7 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
 constexpr int TCH = 32;   // angles whose table rows are staged at once
 
 template <int MODE, int ACH, int G, int TS, int QW>
-__global__ void __launch_bounds__(TS * TS * QW) radon_backward_group_kernel(const float4* __restrict__ sino,
+__global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_group_kernel(const float4* __restrict__ sino,
                                                                    float4* __restrict__ out,
                                                                    const int4* __restrict__ ptab,
                                                                    const int* __restrict__ tbase, int nx, int ny,
