@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from .operator import as_host, check_device
-from .spdhg import DFM_MAX_PARTIALS, _periodic
+from .spdhg import DFM_MAX_PARTIALS, _periodic, support_to_device
 
 
 class OSEMPoisson:
@@ -51,27 +51,12 @@ class OSEMPoisson:
         self.eps_counts = float(eps_counts)
         self.rng = np.random.default_rng(seed)
         self.store_sensitivities = bool(store_sensitivities)
-        self.mask = self._support(support)
+        self.mask = support_to_device(operator, support)
         self.pmask = int(self.mask.numel()) if self.mask is not None else 0
         self.iter_stats = []
         self.final_f = None
         self.start_floor_used = None
         self._part = torch.zeros(2 * DFM_MAX_PARTIALS, dtype=torch.float64, device=operator.device)
-
-    def _support(self, support):
-        """(Ny, Nx) uint8 device mask or None (optimization/prox.py:75-99)."""
-        op = self.op
-        if support is None or support is False:
-            return None
-        if isinstance(support, str):
-            if support != "fov":
-                raise ValueError(f"support must be 'fov', None or an (Ny, Nx) array, not {support!r}")
-            mask = op.support_mask()
-        else:
-            mask = np.asarray(support)
-        if mask.shape != (op.Ny, op.Nx):
-            raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
-        return torch.from_numpy((mask > 0).astype(np.uint8)).to(op.device)
 
     def _sub(self, m):
         return None if self.subsets is None else m

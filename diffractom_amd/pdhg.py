@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from .operator import as_host, check_device
-from .spdhg import DFM_MAX_PARTIALS, RHO_AUTO, _periodic
+from .spdhg import DFM_MAX_PARTIALS, RHO_AUTO, _periodic, support_to_device
 
 
 class PDHGPoisson:
@@ -40,26 +40,11 @@ class PDHGPoisson:
         self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
         self.eps_counts = float(eps_counts)
         self.rho = rho if rho == "auto" else float(rho)
-        self.support_mask = self._support(support)
+        self.support_mask = support_to_device(operator, support)
         self.state = None
         self.iter_stats = []
         self.final_stats = None
         self._part = torch.zeros(2 * DFM_MAX_PARTIALS, dtype=torch.float64, device=operator.device)
-
-    def _support(self, support):
-        """(Ny, Nx) uint8 device mask or None (optimization/prox.py:75-99)."""
-        op = self.op
-        if support is None or support is False:
-            return None
-        if isinstance(support, str):
-            if support != "fov":
-                raise ValueError(f"support must be 'fov', None or an (Ny, Nx) array, not {support!r}")
-            mask = op.support_mask()
-        else:
-            mask = np.asarray(support)
-        if mask.shape != (op.Ny, op.Nx):
-            raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
-        return torch.from_numpy((mask > 0).astype(np.uint8)).to(op.device)
 
     def _deviance(self, mu):
         n = ctypes.c_int32(0)

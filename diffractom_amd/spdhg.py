@@ -52,6 +52,22 @@ def _periodic(a, dshape, name, device):
     return torch.from_numpy(h.ravel().copy()).to(device), int(h.size)
 
 
+def support_to_device(op, support):
+    """The support option of the solvers as a flat (Ny * Nx) uint8 device mask or None (optimization/prox.py:75-99):
+    "fov" (the operator's field of view), None / False (no support constraint) or an (Ny, Nx) array."""
+    if support is None or support is False:
+        return None
+    if isinstance(support, str):
+        if support != "fov":
+            raise ValueError(f"support must be 'fov', None or an (Ny, Nx) array, not {support!r}")
+        mask = op.support_mask()
+    else:
+        mask = np.asarray(support)
+    if mask.shape != (op.Ny, op.Nx):
+        raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
+    return torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8).ravel()).to(op.device)
+
+
 class SPDHGPoisson:
     """SPDHG over omega subsets for the Poisson deviance."""
 
@@ -95,19 +111,7 @@ class SPDHGPoisson:
         self._graph_gen = None
 
     def _support(self, support):
-        """The support option as a (Ny, Nx) uint8 device mask or None (optimization/prox.py:75-99)."""
-        op = self.op
-        if support is None or support is False:
-            return None
-        if isinstance(support, str):
-            if support != "fov":
-                raise ValueError(f"support must be 'fov', None or an (Ny, Nx) array, not {support!r}")
-            mask = op.support_mask()
-        else:
-            mask = np.asarray(support)
-        if mask.shape != (op.Ny, op.Nx):
-            raise ValueError(f"support mask shape {mask.shape} != (Ny, Nx) = {(op.Ny, op.Nx)}")
-        return torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8).ravel()).to(op.device)
+        return support_to_device(self.op, support)
 
     # ---------------------------------------------------------------- subset order
     def _epoch_order(self):
