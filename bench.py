@@ -55,9 +55,10 @@ def parse():
                          "counter runs that give roofline.traffic)")
     ap.add_argument("--graphs", action="store_true", help="replay subset steps from captured HIP graphs (pays off on "
                     "small problems, where launches cost more than the kernels)")
-    ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem", "pdhg"],
+    ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem", "pdhg", "fista"],
                     help="al_c0 / al_c1: the solver timed (osem: ordered-subsets EM subset iterations; pdhg: "
-                         "Chambolle-Pock iterations over all angles; one precision)")
+                         "Chambolle-Pock iterations over all angles; fista: SQS-metric FISTA iterations over all "
+                         "angles; one precision)")
     return ap.parse_args()
 
 
@@ -421,6 +422,38 @@ def run_pdhg(args):
     print(json.dumps(line), flush=True)
 
 
+def run_fista(args):
+    """FISTAPoisson on al_c0 / al_c1 (the SPDHG line's counts and operator, --precision's PF product): `warmup`
+    iterations (the weighted-least-squares start phase included), then `steps` more timed as one continued run (the
+    metric at the resume point, as the reference). One JSON line: iterations/s and the objective."""
+    import torch
+    from diffractom_amd import FISTAPoisson
+    op, b, _, p = build_problem(args.config, precision=args.precision)
+    if not args.no_autotune:
+        tuning = op.autotune(subset=0)
+        log(f"[bench] fista {args.precision}: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    fi = FISTAPoisson(op)
+    x = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
+    fi.run(x, b, args.warmup)
+    f0 = fi.final_stats["final_obj"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fi.run(x, b, args.steps, state=fi.state)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    bt = sum(s["backtracks"] for s in fi.iter_stats)
+    log(f"[bench] fista {args.precision}: {ms:.3f} ms per iteration (wall clock, the metric set-up of the continued "
+        f"run included; {bt} backtracks); objective {f0:.6g} -> {fi.final_stats['final_obj']:.6g}")
+    line = {"metric": "FISTA-Poisson iterations/s", "value": round(1000.0 / ms, 3), "unit": "FISTA iterations/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.precision],
+            "data": "synthetic (Voronoi grains, Poisson counts at a mean of 1000)",
+            "config": {"workload": f"{args.config}: K={op.K}, all {op.N_Omega} angles per iteration, PF product "
+                                   f"{PF_DESC[args.precision]}", "K": op.K, "precision": args.precision},
+            "objective": {"after_warmup": f0, "after": fi.final_stats["final_obj"], "backtracks": bt}}
+    print(json.dumps(line), flush=True)
+
+
 def run_c4(args):
     """Differentiation at the C4 size: 8 slices of 128 x 128 px (180 omega x 131 x 180 eta x 5 rings, K = 5000,
     sigma 2 deg, BF16 residual PF), loss = Poisson NLL(f_a * A c; b) + 1e-3 TV(c), gradients with respect to c and
@@ -490,8 +523,8 @@ def main():
     if args.config in ("al_c3", "ti_c2", "al_c4"):
         {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4}[args.config](args)
         return
-    if args.solver in ("osem", "pdhg"):
-        {"osem": run_osem, "pdhg": run_pdhg}[args.solver](args)
+    if args.solver in ("osem", "pdhg", "fista"):
+        {"osem": run_osem, "pdhg": run_pdhg, "fista": run_fista}[args.solver](args)
         return
     # fp32 first: its counts are the problem every precision solves
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]

@@ -4,6 +4,7 @@ reference's Python call signatures."""
 from .crystallography import Material, group_reflections_into_rings
 from .grid import Grid, MatrixGrid
 from .operator import SinglePhaseForwardOperator, estimate_L_power
+from .fista import FISTAPoisson
 from .osem import OSEMPoisson
 from .pdhg import PDHGPoisson
 from .quadrature import QuadratureRule
@@ -11,6 +12,6 @@ from .spdhg import SPDHGPoisson
 from .support import fov_support_mask
 
 __all__ = ["Material", "Grid", "MatrixGrid", "SinglePhaseForwardOperator", "SPDHGPoisson", "OSEMPoisson",
-           "PDHGPoisson",
+           "PDHGPoisson", "FISTAPoisson",
            "estimate_L_power",
            "QuadratureRule", "fov_support_mask", "group_reflections_into_rings"]

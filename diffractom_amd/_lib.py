@@ -60,6 +60,11 @@ SIGNATURES = {
     "dfm_em_update": [P, P, P, P, I64, F32, I64, P],
     "dfm_pdhg_primal_update": [P, P, P, P, F32, I64, P],
     "dfm_average": [P, P, I64, P],
+    "dfm_fista_value_grad_partial": [P, P, P, I64, P, I64, P, I64, F32, I32, I32, F32, I64, P, P, P],
+    "dfm_fista_curvature_partial": [P, P, P, P, I64, P, I64, P, I64, F32, I32, I64, P, P, P],
+    "dfm_fista_sums_partial": [P, P, I64, P, P, I64, I32, I64, P, P, P],
+    "dfm_fista_elementwise": [I32, P, P, P, P, P, I64, F32, F32, I64, P],
+    "dfm_fista_step_sums_partial": [P, P, P, P, I64, P, P, P],
     "dfm_dot_partial": [P, P, I64, P, P, P],
     "dfm_scale": [P, F32, I64, P],
     "dfm_tv_dual_k": [P, P, P, P, P, P, I32, I32, I32, F32, F32, P],

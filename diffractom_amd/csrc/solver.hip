@@ -315,6 +315,128 @@ __global__ void __launch_bounds__(256) smooth_tv_kernel(const float* __restrict_
   block_sum<1>(s, partial, gridDim.x);
 }
 
+// FISTA in the SQS metric (optimization/fista_poisson.py:61-282), over n data points or n coefficients (API layout
+// (K, Ny, Nx), the support periodic over npix pixels; npix 0: no support).
+// value (+ gradient in place of mu): mode 0 the deviance, mode 1 its quadratic approximation at the data
+// (weights w / (g max(b, floor g)), halved squares)
+__global__ void __launch_bounds__(256) fista_value_grad_kernel(float* __restrict__ mu, const float* __restrict__ b,
+                                                               const float* __restrict__ w, int64_t pw,
+                                                               const float* __restrict__ g, int64_t pg,
+                                                               const float* __restrict__ bg, int64_t pbg, float eps,
+                                                               int write_grad, int mode, float floor_counts,
+                                                               int64_t n, double* __restrict__ partial) {
+  double s[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = pval(w, pw, i, 1.0f);
+    float gr = 0.0f;
+    if (wi != 0.0f) {
+      const float gi = pval(g, pg, i, 1.0f);
+      const double bi = (double)b[i];
+      const double sc = (double)wi / (double)gi;
+      if (mode == 0) {
+        const double m = (double)fmaxf(mu[i], 0.0f) + (double)pval(bg, pbg, i, 0.0f) + (double)eps * gi;
+        double v = m - bi;
+        if (bi > 0.0) v += bi * log(bi / m);
+        s[0] += sc * v;
+        gr = (float)(sc * (1.0 - bi / m));
+      } else {
+        const double r = (double)mu[i] + (double)pval(bg, pbg, i, 0.0f) - bi;
+        const double c = sc / fmax(bi, (double)floor_counts * gi);
+        s[0] += 0.5 * c * r * r;
+        gr = (float)(c * r);
+      }
+    }
+    if (write_grad) mu[i] = gr;
+  }
+  block_sum<1>(s, partial, gridDim.x);
+}
+
+// u <- u c with c the Poisson curvature w / g max(b, m) / m^2, m = max(max(mu, 0) + bg, floor g) (mode 1: the
+// start's w / (g max(b, floor g))); partial sums of c u^2
+__global__ void __launch_bounds__(256) fista_curvature_kernel(float* __restrict__ u, const float* __restrict__ mu,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ w, int64_t pw,
+                                                              const float* __restrict__ g, int64_t pg,
+                                                              const float* __restrict__ bg, int64_t pbg,
+                                                              float floor_counts, int mode, int64_t n,
+                                                              double* __restrict__ partial) {
+  double s[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float wi = pval(w, pw, i, 1.0f);
+    const float gi = pval(g, pg, i, 1.0f);
+    const float r = u[i];
+    float c;
+    if (mode == 0) {
+      const float m = fmaxf(fmaxf(mu[i], 0.0f) + pval(bg, pbg, i, 0.0f), floor_counts * gi);
+      c = wi != 0.0f ? wi / gi * fmaxf(b[i], m) / (m * m) : 0.0f;
+    } else {
+      c = wi != 0.0f ? wi / (gi * fmaxf(b[i], floor_counts * gi)) : 0.0f;
+    }
+    u[i] = r * c;
+    s[0] += (double)c * (double)r * (double)r;
+  }
+  block_sum<1>(s, partial, gridDim.x);
+}
+
+// partial sums: what 0: x over the support; 1: w b and w x (two sums; x a data-sized prediction)
+__global__ void __launch_bounds__(256) fista_sums_kernel(const float* __restrict__ x, const uint8_t* __restrict__ supp,
+                                                         int64_t npix, const float* __restrict__ b,
+                                                         const float* __restrict__ w, int64_t pw, int what,
+                                                         int64_t n, double* __restrict__ partial) {
+  double s[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (what == 0) {
+      if (!npix || supp[i % npix]) s[0] += (double)x[i];
+    } else {
+      const float wi = pval(w, pw, i, 1.0f);
+      s[0] += (double)wi * b[i];
+      s[1] += (double)wi * x[i];
+    }
+  }
+  block_sum<2>(s, partial, gridDim.x);
+}
+
+// element-wise: op 0 D = 1/d where d > 0 in the support, else 0; op 1 ones on the support; op 2 x = 0 outside the
+// support or where x < 0; op 3 xn = max(0, y - th D (grad + lam)) where D > 0, else 0; op 4 out = a + beta (a - c)
+__global__ void __launch_bounds__(256) fista_elementwise_kernel(int op, float* __restrict__ out,
+                                                                const float* __restrict__ a,
+                                                                const float* __restrict__ c,
+                                                                const float* __restrict__ D,
+                                                                const uint8_t* __restrict__ supp, int64_t npix,
+                                                                float th, float lam, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const bool inside = npix ? supp[i % npix] != 0 : true;
+    if (op == 0) {
+      out[i] = (inside && a[i] > 0.0f) ? 1.0f / a[i] : 0.0f;
+    } else if (op == 1) {
+      out[i] = inside ? 1.0f : 0.0f;
+    } else if (op == 2) {
+      if (!inside || out[i] < 0.0f) out[i] = 0.0f;
+    } else if (op == 3) {
+      out[i] = D[i] > 0.0f ? fmaxf(0.0f, a[i] - th * D[i] * (c[i] + lam)) : 0.0f;
+    } else {
+      const float ai = a[i];
+      out[i] = ai + th * (ai - c[i]);
+    }
+  }
+}
+
+// partial sums of <grad, xn - y>, sum (xn - y)^2 / D over D > 0, and sum xn
+__global__ void __launch_bounds__(256) fista_step_sums_kernel(const float* __restrict__ xn,
+                                                              const float* __restrict__ y,
+                                                              const float* __restrict__ grad,
+                                                              const float* __restrict__ D, int64_t n,
+                                                              double* __restrict__ partial) {
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = (double)xn[i] - (double)y[i];
+    s[0] += (double)grad[i] * d;
+    if (D[i] > 0.0f) s[1] += d * d / (double)D[i];
+    s[2] += (double)xn[i];
+  }
+  block_sum<3>(s, partial, gridDim.x);
+}
+
 static unsigned reduce_blocks(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b < 1) b = 1;
@@ -543,6 +665,71 @@ int dfm_average(float* ax, const float* axb, int64_t n, dfm_stream_t stream) {
   if (n == 0) return DFM_OK;
   hipLaunchKernelGGL(average_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, ax, axb, n);
   DFM_LAUNCH_CHECK("dfm_average");
+  return DFM_OK;
+}
+
+int dfm_fista_value_grad_partial(float* mu, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                                 const float* bg, int64_t pbg, float eps, int32_t write_grad, int32_t mode,
+                                 float floor_counts, int64_t n, double* partial, int32_t* n_partial,
+                                 dfm_stream_t stream) {
+  DFM_CHECK_ARG(mu && b && partial && n_partial && (pw == 0 || w) && (pg == 0 || g) && (pbg == 0 || bg) && n >= 0 &&
+                    (mode == 0 || mode == 1),
+                "dfm_fista_value_grad_partial: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(fista_value_grad_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, mu, b, w, pw, g, pg, bg,
+                     pbg, eps, write_grad, mode, floor_counts, n, partial);
+  DFM_LAUNCH_CHECK("dfm_fista_value_grad_partial");
+  return DFM_OK;
+}
+
+int dfm_fista_curvature_partial(float* u, const float* mu, const float* b, const float* w, int64_t pw, const float* g,
+                                int64_t pg, const float* bg, int64_t pbg, float floor_counts, int32_t mode, int64_t n,
+                                double* partial, int32_t* n_partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(u && mu && b && partial && n_partial && (pw == 0 || w) && (pg == 0 || g) && (pbg == 0 || bg) &&
+                    n >= 0 && (mode == 0 || mode == 1),
+                "dfm_fista_curvature_partial: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(fista_curvature_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, u, mu, b, w, pw, g, pg, bg,
+                     pbg, floor_counts, mode, n, partial);
+  DFM_LAUNCH_CHECK("dfm_fista_curvature_partial");
+  return DFM_OK;
+}
+
+int dfm_fista_sums_partial(const float* x, const uint8_t* supp, int64_t npix, const float* b, const float* w,
+                           int64_t pw, int32_t what, int64_t n, double* partial, int32_t* n_partial,
+                           dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && partial && n_partial && (npix == 0 || supp) && npix >= 0 && n >= 0 &&
+                    (what == 0 || (what == 1 && b && (pw == 0 || w))),
+                "dfm_fista_sums_partial: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(fista_sums_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, x, supp, npix, b, w, pw, what,
+                     n, partial);
+  DFM_LAUNCH_CHECK("dfm_fista_sums_partial");
+  return DFM_OK;
+}
+
+int dfm_fista_elementwise(int32_t op, float* out, const float* a, const float* c, const float* D,
+                          const uint8_t* supp, int64_t npix, float th, float lam, int64_t n, dfm_stream_t stream) {
+  DFM_CHECK_ARG(out && op >= 0 && op <= 4 && (npix == 0 || supp) && npix >= 0 && n >= 0 &&
+                    (op != 0 || a) && (op != 3 || (a && c && D)) && (op != 4 || (a && c)),
+                "dfm_fista_elementwise: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(fista_elementwise_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     op, out, a, c, D, supp, npix, th, lam, n);
+  DFM_LAUNCH_CHECK("dfm_fista_elementwise");
+  return DFM_OK;
+}
+
+int dfm_fista_step_sums_partial(const float* xn, const float* y, const float* grad, const float* D, int64_t n,
+                                double* partial, int32_t* n_partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(xn && y && grad && D && partial && n_partial && n >= 0, "dfm_fista_step_sums_partial: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(fista_step_sums_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, xn, y, grad, D, n, partial);
+  DFM_LAUNCH_CHECK("dfm_fista_step_sums_partial");
   return DFM_OK;
 }
 

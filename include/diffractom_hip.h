@@ -325,6 +325,32 @@ int dfm_em_update(float* x, const float* back, const float* sens, const uint8_t*
 int dfm_pdhg_primal_update(float* x, float* xb, const float* grad, const float* T, float lam, int64_t n,
                            dfm_stream_t stream);
 int dfm_average(float* ax, const float* axb, int64_t n, dfm_stream_t stream);
+/* FISTA in the SQS metric (optimization/fista_poisson.py:61-282 kernels, the API coefficient layout; supp periodic
+ * over npix pixels, npix 0: no support; float64 partial sums as dfm_deviance_partial, NV sums stored NV-major):
+ *   dfm_fista_value_grad_partial (poisson_value_grad): mode 0 the deviance, mode 1 its quadratic approximation at the
+ *     data; write_grad: mu <- df/dmu;
+ *   dfm_fista_curvature_partial (times_curvature): u <- c u with the Poisson (mode 0) or start (mode 1) curvature;
+ *     sum c u^2;
+ *   dfm_fista_sums_partial: what 0, sum of x over the support (support_sum); what 1, sum w b and sum w x
+ *     (weighted_sums);
+ *   dfm_fista_elementwise: op 0 invert_metric (out = 1/a where a > 0 in the support), 1 support_ones, 2
+ *     project_support (out in place), 3 metric_step (out = max(0, a - th D (c + lam)) where D > 0), 4 extrapolate
+ *     (out = a + th (a - c));
+ *   dfm_fista_step_sums_partial (step_sums): <grad, xn - y>, sum (xn - y)^2 / D over D > 0, sum xn. */
+int dfm_fista_value_grad_partial(float* mu, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                                 const float* bg, int64_t pbg, float eps, int32_t write_grad, int32_t mode,
+                                 float floor_counts, int64_t n, double* partial, int32_t* n_partial,
+                                 dfm_stream_t stream);
+int dfm_fista_curvature_partial(float* u, const float* mu, const float* b, const float* w, int64_t pw, const float* g,
+                                int64_t pg, const float* bg, int64_t pbg, float floor_counts, int32_t mode, int64_t n,
+                                double* partial, int32_t* n_partial, dfm_stream_t stream);
+int dfm_fista_sums_partial(const float* x, const uint8_t* supp, int64_t npix, const float* b, const float* w,
+                           int64_t pw, int32_t what, int64_t n, double* partial, int32_t* n_partial,
+                           dfm_stream_t stream);
+int dfm_fista_elementwise(int32_t op, float* out, const float* a, const float* c, const float* D,
+                          const uint8_t* supp, int64_t npix, float th, float lam, int64_t n, dfm_stream_t stream);
+int dfm_fista_step_sums_partial(const float* xn, const float* y, const float* grad, const float* D, int64_t n,
+                                double* partial, int32_t* n_partial, dfm_stream_t stream);
 int dfm_dot_partial(const float* a, const float* b, int64_t n, double* partial, int32_t* n_partial,
                     dfm_stream_t stream);
 /* Smoothed isotropic TV of coefficient images c (K, ny, nx) float32, a loss term of the differentiable refinement

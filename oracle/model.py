@@ -913,3 +913,151 @@ class OraclePDHG:
         self.state = {"it": it0 + int(niter), "x_bar": xb, "y": y}
         self.final_f = f
         return x
+
+
+# ============================================================================ FISTA
+FISTA_REFRESH = (5, 10, 20, 40, 80, 160, 320, 640, 1280, 2560, 5120)      # fista_poisson.py:284
+
+
+class OracleFISTA:
+    """FISTA in the SQS metric with backtracking, in float64 (optimization/fista_poisson.py:311-617, the
+    GPU-resident path; its kernels 61-282): the weighted-least-squares start phase (mode 1: the quadratic
+    approximation of the deviance at the data, weights w / (g max(b, floor g))), the metric D = 1 / A^T(c A 1_S),
+    the descent test, the metric at y on the first rejection then halving, function-value restarts, the refresh
+    schedule, the adjoint-scale estimate. Pinned by tests/test_oracle_fista.py: the L-BFGS-B minimiser reached (the
+    reference's bar for its Poisson solvers), the descent of every accepted step, a continued run equal to one run."""
+
+    def __init__(self, op, prox_kind="nonneg", lam=0.0, support="fov", eps_counts=1e-3, floor_counts=1.0,
+                 refresh=FISTA_REFRESH, refresh_on_failure=True, max_backtracks=60, start="wls", n_start=20):
+        self.op = op
+        self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
+        self.eps = float(np.float32(eps_counts))
+        self.floor = float(np.float32(floor_counts))
+        self.refresh = tuple(int(r) for r in refresh)
+        self.refresh_on_failure = bool(refresh_on_failure)
+        self.max_backtracks = int(max_backtracks)
+        self.start = start
+        self.n_start = int(n_start)
+        if support is None:
+            self.supp = None
+        elif isinstance(support, str):
+            self.supp = np.asarray(op.support_mask(), bool)
+        else:
+            self.supp = np.asarray(support) > 0
+        self.adjoint_scale = 1.0
+
+    def _value_grad(self, mu, mode):
+        b, w, g, bg = self.b, self.w, self.g, self.bg
+        cnt = w != 0
+        sc = np.where(cnt, w / g, 0.0)
+        if mode == 0:
+            m = np.maximum(mu, 0.0) + bg + self.eps * g
+            v = m - b + np.where(b > 0, b * np.log(np.where(b > 0, b, 1.0) / m), 0.0)
+            return float(np.sum(sc * v)), np.where(cnt, sc * (1.0 - b / m), 0.0)
+        r = mu + bg - b
+        c = sc / np.maximum(b, self.floor * g)
+        return float(np.sum(0.5 * c * r * r)), np.where(cnt, c * r, 0.0)
+
+    def _metric(self, mu, mode):
+        op, b, w, g, bg = self.op, self.b, self.w, self.g, self.bg
+        cnt = w != 0
+        ones = np.broadcast_to(self.sup, op.coeff_shape).astype(np.float64)
+        r = op.direct(ones)
+        if mode == 0:
+            m = np.maximum(np.maximum(mu, 0.0) + bg, self.floor * g)
+            c = np.where(cnt, w / g * np.maximum(b, m) / (m * m), 0.0)
+        else:
+            c = np.where(cnt, w / (g * np.maximum(b, self.floor * g)), 0.0)
+        u_r = float(np.sum(c * r * r))
+        atu = op.adjoint(c * r)
+        at_u = float(atu[np.broadcast_to(self.sup, op.coeff_shape)].sum())
+        if u_r > 0 and at_u > 0:
+            self.adjoint_scale = at_u / u_r
+        return np.where(np.broadcast_to(self.sup, op.coeff_shape) & (atu > 0), 1.0 / np.where(atu > 0, atu, 1.0), 0.0)
+
+    def run(self, x0, b, niter, weights=None, gain=None, background=None, state=None):
+        op = self.op
+        dshape, cshape = tuple(op.data_shape), tuple(op.coeff_shape)
+        n = int(np.prod(dshape))
+        self.b = np.asarray(b, np.float64).reshape(dshape)
+        self.w = (np.ones(n) if weights is None else periodic(weights, n)).reshape(dshape)
+        self.g = (np.ones(n) if gain is None else periodic(gain, n)).reshape(dshape)
+        self.bg = (np.zeros(n) if background is None else periodic(background, n)).reshape(dshape)
+        self.sup = np.ones((op.Ny, op.Nx), bool) if self.supp is None else self.supp
+        x = np.where(np.broadcast_to(self.sup, cshape), np.maximum(np.asarray(x0, np.float64).reshape(cshape), 0.0),
+                     0.0)
+        zero_start = state is None and x.max() == 0.0
+        wls_until = 0
+        if self.start == "wls" and self.n_start > 0:
+            if zero_start:
+                wls_until = self.n_start
+            elif state is not None and state.get("wls_until", 0) > int(state["it"]):
+                wls_until = int(state["wls_until"])
+        mode = 1 if wls_until else 0
+        if zero_start and self.start == "uniform":
+            ones = np.broadcast_to(self.sup, cshape).astype(np.float64)
+            r = op.direct(ones)
+            sb, sa = float((self.w * self.b).sum()), float((self.w * r).sum())
+            if sa > 0 and sb > 0:
+                x = ones * (sb / sa)
+        it0, t_mom, theta = 0, 1.0, 1.0
+        if state is not None:
+            it0, t_mom, theta = int(state["it"]), float(state["t"]), float(state["theta"])
+            y = np.asarray(state["y"], np.float64).reshape(cshape).copy()
+        else:
+            y = x.copy()
+        Ax = op.direct(x)
+        Ay = op.direct(y)
+        D = self._metric(Ax, mode)
+        f_x = self._value_grad(Ax, mode)[0]
+        F_x = f_x + self.lam * x.sum()
+        self.iter_stats = []
+        for k in range(int(niter)):
+            it = it0 + k + 1
+            f_y, gmu = self._value_grad(Ay, mode)
+            grad = op.adjoint(gmu)
+            theta = min(1.0, 2.0 * theta)
+            n_bt, local = 0, False
+            while True:
+                xn = np.where(D > 0, np.maximum(0.0, y - theta * D * (grad + self.lam)), 0.0)
+                Axn = op.direct(xn)
+                f_n = self._value_grad(Axn, mode)[0]
+                d = xn - y
+                lin = float(np.sum(grad * d)) / self.adjoint_scale
+                quad = float(np.sum(np.where(D > 0, d * d / np.where(D > 0, D, 1.0), 0.0))) / (theta * self.adjoint_scale)
+                if f_n <= f_y + lin + 0.5 * quad + 1e-6 * abs(f_y) or quad == 0.0:
+                    break
+                n_bt += 1
+                if not local and self.refresh_on_failure:
+                    D = self._metric(op.direct(y), mode)
+                    local, theta = True, 1.0
+                    continue
+                theta *= 0.5
+                if n_bt > self.max_backtracks:
+                    raise RuntimeError("no sufficient decrease")
+            F_n = f_n + self.lam * xn.sum()
+            restarted = F_n > F_x
+            if restarted:
+                t_mom = 1.0
+            t_new = 0.5 * (1.0 + np.sqrt(1.0 + 4.0 * t_mom * t_mom))
+            beta = (t_mom - 1.0) / t_new
+            t_mom = t_new
+            y = xn + beta * (xn - x)
+            Ay = Axn + beta * (Axn - Ax)
+            x, Ax = xn, Axn
+            f_x, F_x = f_n, F_n
+            switch = mode == 1 and it >= wls_until
+            if switch:
+                mode = 0
+                f_x = self._value_grad(Ax, 0)[0]
+                F_x = f_x + self.lam * x.sum()
+            refreshed = (it in self.refresh or switch) and k + 1 < niter and mode == 0
+            if refreshed:
+                D = self._metric(Ax, 0)
+                t_mom, theta = 1.0, 1.0
+                y, Ay = x.copy(), Ax.copy()
+            self.iter_stats.append({"iter": it, "f": f_n, "obj": F_n, "backtracks": n_bt, "restart": bool(restarted),
+                                    "metric_refresh": bool(refreshed), "phase": "wls" if it <= wls_until else "poisson"})
+        self.state = {"it": it0 + int(niter), "t": t_mom, "theta": theta, "y": y, "wls_until": wls_until}
+        self.final_f, self.final_obj = f_x, F_x
+        return x
