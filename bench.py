@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0     # dense bf16 / fp16 matrix-core peak of one MI355X
 
 
 def parse():
@@ -42,8 +43,10 @@ def parse():
                     "al_c4 (autograd loss + gradients, 8 slices)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
     ap.add_argument("--batch", type=int, default=32, help="al_c3: slices per batched run")
+    ap.add_argument("--compare-steps", type=int, default=300, help="the precisions' SPDHG iterates are compared "
+                    "after this many steps of the run (the untimed steps after the timed ones)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=512, help="orientations of the host-core baseline sample")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of host-core baseline steps")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--precision", default="bf16_residual", choices=["bf16", "bf16_residual", "fp32"],
                     help="PF product of the headline figure (the other precisions run too, for the comparison)")
@@ -84,72 +87,79 @@ def build_problem(name, K=None, mean_counts=None, precision="fp32", counts=None)
     return op, b, xt, p
 
 
-def kernel_times(solver, n_steps):
-    """Average duration (ms) of every kernel class over n_steps, from HIP events recorded on the stream the kernels
-    are launched on (torch's current stream)."""
+def timed_steps(solver, n_steps, per_kernel=True):
+    """Run n_steps subset steps of the solver (advance()) between two HIP events; with per_kernel the solver's own
+    launches record an event after every kernel class (op.marks), so the per-kernel durations come from the same
+    timed steps. Returns (ms per step, {kernel class: average ms per step} or None)."""
     import torch
     op = solver.op
-    fused = op.fused_width(max(len(s) for s in solver.subsets)) > 0
-    names = ["dual_update", "pf_adjoint"] + (["spdhg_fused"] if fused else
-                                             ["radon_backward_spdhg", "radon_forward"]) + ["pf_forward"]
-    tot = {n: 0.0 for n in names}
-    evs = []
-    for _ in range(n_steps):
-        seq, pos = solver._seq, solver._pos
-        blk = seq[pos]
-        if pos + 1 < len(seq):
-            nb = seq[pos + 1]
-            solver._pos += 1
-        else:
-            nseq = solver._epoch_sequence()
-            nb = nseq[0]
-            solver._seq, solver._pos = nseq, 0
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
-        e[0].record()
-        solver._dual(blk)
-        e[1].record()
-        n_ang, _, _, geo = op._angle_set(blk)
-        n2, _, geo_f, _ = op._angle_set(nb)
-        sc = op._scratch_buffers(max(n_ang, n2))
-        bt = op.batches[0]
-        op._pf_adjoint(0, solver.D, sc["sino"], blk)
-        e[2].record()
-        if fused:
-            op._spdhg_fused(sc["sino"], blk, nb, solver.xk, solver.zk, solver.Tk, solver._inv_p_data, solver.lam,
-                            bt["koff"], bt["kstride"], op.fused_width(n_ang))
-        else:
-            op._radon_backward(sc["sino"], geo, op._tables(blk), n_ang, None, op.K_stride, bt["koff"],
-                               bt["kstride"], x=solver.xk, z=solver.zk, T=solver.Tk, inv_p=solver._inv_p_data,
-                               lam=solver.lam)
-            e[3].record()
-            op._radon_forward(solver.xk, op.K_stride, 0, bt["kstride"], sc["sino"], geo_f, n2, op._tables(nb))
-        e[-2].record()
-        op._pf_forward(0, sc["sino"], solver.D, nb, accumulate=False)
-        e[-1].record()
-        evs.append(e)
+    op.marks = [] if per_kernel else None
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    solver.advance(n_steps)
+    e1.record()
     torch.cuda.synchronize()
-    for e in evs:
-        for i, n in enumerate(names):
-            tot[n] += e[i].elapsed_time(e[i + 1])
-    return {n: tot[n] / max(n_steps, 1) for n in names}
+    marks, op.marks = op.marks, None
+    ms = e0.elapsed_time(e1) / n_steps
+    if not per_kernel:
+        return ms, None
+    tot = {}
+    prev = e0
+    for name, e in marks:
+        tot[name] = tot.get(name, 0.0) + prev.elapsed_time(e)
+        prev = e
+    return ms, {k: v / n_steps for k, v in tot.items()}
+
+
+def pf_nnz(op):
+    """Non-zeros of the PF matrix (all angles, all orientations): the CSR's count, or the count the dense / fragment
+    forms recorded when they were built."""
+    return int(op.sparse_nnz)
 
 
 def kernel_bytes(op, M):
-    """Algorithmic HBM bytes per launch of each kernel class of one SPDHG step (subset of R/M angles)."""
+    """Algorithmic HBM bytes per launch of each kernel class of one SPDHG step (subset of R/M angles): what the
+    step must move whatever the storage form. The PF products: 6 bytes per non-zero of the subset's PF (one CSR
+    form: a 4-byte value, a 2-byte index) + the sinogram + the data rows, also for the matrix-core path (its
+    zero-padded fragment stream is reported beside it as `fragment_bytes`, not counted as algorithmic)."""
     npix = op.Nx * op.Ny
     K, R, My, CP = op.K, op.N_Omega, op.My, op.N_seg
     Rm = R / M
     sino = 4.0 * Rm * My * K
     rows = 4.0 * Rm * My * CP
-    if op.mfma_batches is not None:
-        mb = op.mfma_batches[0]
-        f = sum(mb[k].numel() * 2 for k in ("fwd_hi", "fwd_lo") if mb[k] is not None) * Rm / R
-        a = sum(mb[k].numel() * 2 for k in ("adj_hi", "adj_lo") if mb[k] is not None) * Rm / R
-    else:
-        f = a = 6.0 * op.sparse_nnz / M
+    pf = 6.0 * pf_nnz(op) / M
     return {"radon_backward_spdhg": 20.0 * K * npix + sino, "radon_forward": 4.0 * K * npix + sino,
             "spdhg_fused": 20.0 * K * npix + 2.0 * sino,
-            "pf_adjoint": a + rows + sino, "pf_forward": f + sino + rows, "dual_update": 6.0 * rows}
+            "pf_adjoint": pf + rows + sino, "pf_forward": pf + sino + rows, "dual_update": 6.0 * rows}
+
+
+def mfma_figures(op, M, kt):
+    """The matrix-core PF products' fragment stream (bytes per launch) and MFMA utilisation: FLOPs the MFMAs issue
+    per launch (2 Rm My_p CP_p K_p per product pass, the padded tile sizes; 3 passes with the residual) and the
+    dense-algorithmic FLOPs 2 Rm My CP K, both over the kernel's HIP-event time against the dense BF16 peak."""
+    if op.mfma_batches is None:
+        return None
+    R, My, CP, K = op.N_Omega, op.My, op.N_seg, op.K
+    Rm = R / M
+    out = {}
+    for name, keys in (("pf_forward", ("fwd_hi", "fwd_lo")), ("pf_adjoint", ("adj_hi", "adj_lo"))):
+        if name not in kt:
+            continue
+        frag = sum(mb[k].numel() * 2 for mb in op.mfma_batches for k in keys if mb[k] is not None) * Rm / R
+        passes = 3 if op.pf_precision.endswith("residual") else 1
+        myp = -(-My // 16) * 16
+        cpp = -(-CP // (16 if name == "pf_forward" else 32)) * (16 if name == "pf_forward" else 32)
+        kp = sum(-(-b["K_batch"] // (32 if name == "pf_forward" else 16)) * (32 if name == "pf_forward" else 16)
+                 for b in op.batches)
+        issued = 2.0 * Rm * myp * cpp * kp * passes
+        dense = 2.0 * Rm * My * CP * K
+        t = kt[name] * 1e-3
+        out[name] = {"fragment_bytes": frag, "fragment_GBps": round(frag / t / 1e9, 1),
+                     "mfma_flops_issued": issued, "mfma_TFLOPs_issued": round(issued / t / 1e12, 2),
+                     "mfma_frac_issued": round(issued / t / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                     "dense_flops": dense, "dense_TFLOPs": round(dense / t / 1e12, 2),
+                     "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS}
+    return out
 
 
 def rel_l2(a, b):
@@ -158,7 +168,80 @@ def rel_l2(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
-def run_path(args, precision, counts=None):
+SAMPLE_ROWS = (0, 45, 90, 135)      # angle rows of the FP64-oracle sample (both sweep classes)
+SAMPLE_K0, SAMPLE_KN = 12288, 1024   # orientation block of the sample
+
+
+def cpu_baseline_leg(args, op, b, M):
+    """The host-side leg of the benchmark, the only part that runs code under oracle/ (as the CPU baseline and as
+    the checker):
+    (1) the C/OpenMP FP32 restatement of one SPDHG subset step (oracle/c/cpu_step.c: dual update, CSR adjoint,
+        backprojection with the primal update, forward projection, CSR forward) timed on the full workload, all K
+        orientations, on the state of a solver started on this FP32-CSR operator;
+    (2) the FP64 oracle (oracle/model.py) on a sample of the workload: the forward rows of SAMPLE_ROWS for
+        coefficients supported on the orientation block SAMPLE_K0 .. +SAMPLE_KN (seeded U(0, 1) on the support),
+        and the adjoint of the counts on those rows restricted to the block. Every precision's GPU forward / adjoint
+        of the same inputs is compared with these values (fp64_errors).
+    Returns (cpu_baseline dict or None, oracle sample dict or None)."""
+    import torch
+    from diffractom_amd import SPDHGPoisson
+    cpu = smp = None
+    try:
+        from oracle import cpu_baseline
+        sol = SPDHGPoisson(op, subsets=M, seed=0)
+        sol.start(np.zeros(op.coeff_shape, np.float32), b)
+        torch.cuda.synchronize()
+        rate, n, thr = cpu_baseline.time_steps(sol, budget_s=args.cpu_budget)
+        sol.finish()
+        cpu = {"value": round(rate, 4), "unit": "SPDHG steps/s", "cores": thr, "kind": "port",
+               "sample": f"{n} consecutive SPDHG subset steps of the C/OpenMP FP32 CSR restatement on the full "
+                         f"workload ({op.K} orientations, {op.N_Omega // M} angles per subset), {thr} threads"}
+        log(f"[bench] cpu baseline: {rate:.4f} steps/s ({n} full-size steps, {thr} threads)")
+    except Exception as exc:                                        # the GPU figures stand without it
+        log(f"[bench] cpu baseline failed: {exc!r}")
+    try:
+        from oracle.model import OracleOperator
+        from diffractom_amd.workloads import AL_RINGS
+        t0 = time.perf_counter()
+        ks = np.arange(SAMPLE_K0, min(SAMPLE_K0 + SAMPLE_KN, op.K))
+        rows = np.array([r for r in SAMPLE_ROWS if r < op.N_Omega])
+        orc = OracleOperator(op.cfg, (4.05, 4.05, 4.05, 90, 90, 90), "cubic", AL_RINGS, 0.2, op.grid_rot_cpu[ks],
+                             float(op.sigma_cpu[0]), orthonormalize=False)
+        rng = np.random.default_rng(11)
+        xb = rng.random((len(ks), op.Ny, op.Nx)) * op.support_mask()[None]
+        fwd = orc.direct_rows(xb, rows)
+        orc.set_omega_subsets([rows])
+        yb = np.zeros(op.data_shape)
+        yb[rows] = np.asarray(b, np.float64)[rows]
+        adj = orc.adjoint(yb, subset=0)
+        t_orc = time.perf_counter() - t0
+        smp = {"rows": rows, "ks": ks, "x": xb.astype(np.float32), "y_rows": yb[rows].astype(np.float32),
+               "fwd": fwd, "adj": adj, "seconds": t_orc}
+        log(f"[bench] FP64 oracle sample ({len(rows)} angle rows x {len(ks)} orientations): {t_orc:.1f} s")
+    except Exception as exc:
+        log(f"[bench] FP64 oracle sample failed: {exc!r}")
+    return cpu, smp
+
+
+def fp64_errors(op, smp):
+    """Relative L2 errors of this operator's forward rows and block adjoint against the FP64 oracle sample."""
+    import torch
+    if smp is None:
+        return None
+    ks, rows = smp["ks"], smp["rows"]
+    x = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
+    x[ks[0]:ks[-1] + 1] = torch.from_numpy(smp["x"]).cuda()
+    fwd = op.direct(x)[torch.from_numpy(rows).cuda()].double().cpu().numpy()
+    y = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
+    y[torch.from_numpy(rows).cuda()] = torch.from_numpy(smp["y_rows"]).cuda()
+    adj = op.adjoint(y)[ks[0]:ks[-1] + 1].double().cpu().numpy()
+    del x, y
+    e = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    return {"forward": e(fwd, smp["fwd"]), "adjoint": e(adj, smp["adj"]),
+            "sample": f"angle rows {list(map(int, rows))} x orientations {int(ks[0])}..{int(ks[-1])}"}
+
+
+def run_path(args, precision, counts=None, leg=None):
     """One precision: set-up, warm-up, timed SPDHG steps, per-kernel times, full forward / adjoint times; returns
     the figures and the device results the precisions are compared on."""
     import torch
@@ -179,20 +262,17 @@ def run_path(args, precision, counts=None):
     torch.cuda.synchronize()
     solver.advance(args.warmup)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
     ts = time.perf_counter()
-    e0.record()
-    solver.advance(args.steps)
-    e1.record()
-    torch.cuda.synchronize()
+    ms_step, ktimes = timed_steps(solver, args.steps, per_kernel=not args.no_kernel_timing)
     wall = time.perf_counter() - ts
-    ms_step = e0.elapsed_time(e1) / args.steps
     log(f"[bench] {precision}: timed {args.steps} steps: {ms_step:.3f} ms/step (wall {wall:.3f} s)")
-    ktimes = None
-    if not args.no_kernel_timing:
-        ktimes = kernel_times(solver, args.steps)
+    if ktimes:
         log(f"[bench] {precision}: kernel ms: " + ", ".join(f"{k}={v:.4f}" for k, v in ktimes.items()))
+    # continue the same run (untimed) to the step count the precisions' iterates are compared at
+    n_done = args.warmup + args.steps
+    if args.compare_steps > n_done:
+        solver.advance(args.compare_steps - n_done)
+        n_done = args.compare_steps
     x_iter = solver.xk.clone()
     solver.finish()
     # full forward / adjoint (API layout, median of 10 after 2)
@@ -211,10 +291,13 @@ def run_path(args, precision, counts=None):
         torch.cuda.synchronize()
         tf.append(a0.elapsed_time(a1))
         ta.append(a1.elapsed_time(a2))
-    res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b,
+    if leg is not None and precision == "fp32" and not args.no_cpu_baseline:
+        leg["cpu"], leg["sample"] = cpu_baseline_leg(args, op, b, M)
+    fp64 = fp64_errors(op, leg.get("sample")) if leg is not None else None
+    res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b, "fp64": fp64,
            "tuning": {k: v[0] for k, v in (op.tuning or {}).items()}, "ktimes": ktimes, "setup_s": setup,
            "t_fwd": float(np.median(tf[2:])) if tf else None, "t_adj": float(np.median(ta[2:])) if ta else None,
-           "fwd_out": data, "adj_out": out, "x_iter": x_iter, "steps_total": args.warmup + 2 * args.steps}
+           "fwd_out": data, "adj_out": out, "x_iter": x_iter, "steps_total": n_done}
     return res
 
 
@@ -532,8 +615,9 @@ def main():
     precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]
     results = {}
     counts = None
+    leg = {}
     for q in precisions:
-        r = run_path(args, q, counts)
+        r = run_path(args, q, counts, leg)
         counts = r["counts"]
         results[q] = r
         if q != args.precision:     # keep the headline path's operator (roofline) only
@@ -550,7 +634,8 @@ def main():
     for q, r in results.items():
         e = {"steps_per_s": round(1000.0 / r["ms_step"], 3), "ms_per_step": round(r["ms_step"], 4),
              "forward_ms": r["t_fwd"] and round(r["t_fwd"], 4), "adjoint_ms": r["t_adj"] and round(r["t_adj"], 4),
-             "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}, "tuned": r["tuning"]}
+             "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}, "tuned": r["tuning"],
+             "rel_l2_vs_fp64_oracle": r["fp64"]}
         if ref is not None and q != "fp32":
             e["rel_l2_vs_fp32"] = {"forward": rel_l2(r["fwd_out"], ref["fwd_out"]),
                                    "adjoint": rel_l2(r["adj_out"], ref["adj_out"]),
@@ -574,29 +659,16 @@ def main():
                             "frac": round(kb[k] / (kt[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                             "traffic": measured_traffic(k, args.config, args.precision, tuned.get(k))}
                         for k in kt if k in kb}
+        for k, v in (mfma_figures(op, M, kt) or {}).items():
+            roofline_all[k]["matrix_cores"] = v
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            from oracle import cpu_baseline
-            Ks = min(args.cpu_sample, op.K)
-            op_s, b_s, _, _ = build_problem(args.config, K=Ks)
-            from diffractom_amd import SPDHGPoisson
-            sol_s = SPDHGPoisson(op_s, subsets=M, seed=0)
-            sol_s.start(np.zeros(op_s.coeff_shape, np.float32), b_s)
-            rate, n, thr = cpu_baseline.time_steps(sol_s, budget_s=15.0)
-            scaled = rate * Ks / op.K
-            cpu = {"value": round(scaled, 4), "unit": "SPDHG steps/s", "cores": thr, "kind": "port",
-                   "sample": f"{n} steps of the C/OpenMP FP32 CSR restatement on {Ks} of the {op.K} orientations "
-                             f"({rate:.3f} steps/s), scaled by {Ks}/{op.K} (the step's work is linear in K)"}
-            log(f"[bench] cpu baseline {rate:.3f} steps/s on {Ks} orientations with {thr} threads")
-        except Exception as exc:                                    # the GPU figure stands without it
-            log(f"[bench] cpu baseline failed: {exc!r}")
+    cpu = leg.get("cpu")
+    smp = leg.get("sample")
 
     ms_step = head["ms_step"]
     value = 1000.0 / ms_step
-    # forward / adjoint GB/s: x + data + one CSR of the PF matrix (its nonzeros from the fp32 run)
-    nnz_all = results["fp32"]["op_nnz"] if "fp32" in results else 0
+    # forward / adjoint GB/s: x + data + one CSR of the PF matrix (its non-zeros, counted when the PF was built)
+    nnz_all = pf_nnz(op)
     fbytes = 4.0 * K * npix + 4.0 * R * My * CP + 6.0 * nnz_all
     line = {
         "metric": "SPDHG-Poisson iters/s + fwd/adj GB/s, BF16-MFMA vs FP32, <=1e-3 rel err",
@@ -604,6 +676,12 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": DTYPE[args.precision],
         "rel_err_vs_fp32": paths[args.precision].get("rel_l2_vs_fp32"),
+        "rel_err_vs_fp64_oracle": head["fp64"],
+        "fp64_oracle": smp and {"seconds": round(smp["seconds"], 2), "kind": "FP64 NumPy/SciPy restatement",
+                                "sample": f"forward rows of angles {list(map(int, smp['rows']))} for coefficients "
+                                          f"on orientations {int(smp['ks'][0])}..{int(smp['ks'][-1])}, and the "
+                                          f"adjoint of the counts on those rows over the same orientations"},
+        "pf_nnz": nnz_all,
         "data": "synthetic (Voronoi grains, Poisson counts at a mean of 1000)",
         "config": {"workload": f"{args.config}: Al 5 rings, 180 omega x 103 x 180 eta, 99 x 99 px, K={op.K}, "
                                f"sigma 2 deg, M={M} subsets; PF product {PF_DESC[args.precision]}",

@@ -58,8 +58,10 @@ enum { BWD_IMG = 0, BWD_SPDHG = 1, FWD = 2, FUSED = 3 };
 
 template <int CW, int MODE>
 __global__ void __launch_bounds__(THREADS) projector_lds_kernel(
-    const float* __restrict__ sino_b, const AngleGeo* __restrict__ geo_b, int na_b, float* __restrict__ x,
-    float* __restrict__ z, const float* __restrict__ T, float inv_p, float lam, float* __restrict__ sino_f,
+    // sino_b (read) and sino_f (written) are not __restrict__: the fused step (MODE 3) reads the backprojection's
+    // sinogram from the same buffer its forward projection then writes
+    const float* sino_b, const AngleGeo* __restrict__ geo_b, int na_b, float* __restrict__ x,
+    float* __restrict__ z, const float* __restrict__ T, float inv_p, float lam, float* sino_f,
     const AngleGeo* __restrict__ geo_f, int na_f, float scale, int nx, int ny, int ns, int kstride, int k_off,
     int k_n, int skstride) {
   constexpr int PPT = LDS_FLOATS / CW / THREADS;      // pixels per thread at most (npix * CW <= LDS_FLOATS)

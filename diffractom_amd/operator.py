@@ -418,6 +418,7 @@ class SinglePhaseForwardOperator:
         self.radon_variant = {"fwd": 8 + 256, "bwd": 4 + 64 * 4, "fwd_kernel": "table", "bwd_kernel": "table",
                               "fwd_pairs": 8}
         self.tuning = None
+        self.marks = None                       # a list while a profiling pass records per-launch events (mark())
         self.fuse_projector = False
         self.fused_channels = 4
         # image bands of the banded forward projection: one XCD's band of 64 channels within about 2 MB of L2
@@ -553,18 +554,21 @@ class SinglePhaseForwardOperator:
             self.dense_batches = [self._eval_dense_batch(b).reshape(self.N_Omega, b["kstride"], self.N_seg)
                                   for b in self.batches]
             self.pf_storage = "dense"
+            self._count_nnz(self.dense_batches)
             return
         R, CP = self.N_Omega, self.N_seg
         res = self.pf_precision == "bf16_residual"
         st = self._stream()
         self.mfma_batches = []
         dense = None
+        nnz = 0
         for b in self.batches:
             if dense is None or dense.shape[1] != b["kstride"]:
                 dense = None
                 dense = self._eval_dense_batch(b)
             else:
                 self._eval_dense_batch(b, out=dense)
+            nnz += int(torch.count_nonzero(dense).item())
             kb, ks = b["K_batch"], b["kstride"]
             mb = {"n_nt": -(-CP // 16), "n_kc": -(-kb // 32), "n_kt": -(-ks // 16), "n_jc": -(-CP // 32)}
             nf = R * mb["n_nt"] * mb["n_kc"] * 512
@@ -579,6 +583,13 @@ class SinglePhaseForwardOperator:
             self.mfma_batches.append(mb)
         del dense
         self.pf_storage = "bf16 fragments" + (" + residual" if res else "")
+        self.sparse_nnz = nnz
+        self.sparse_fill = nnz / (R * self.K * CP)
+
+    def _count_nnz(self, dense_batches):
+        """sparse_nnz / sparse_fill of dense batches (the PF's non-zeros, as the CSR forms would hold them)."""
+        self.sparse_nnz = int(sum(int(torch.count_nonzero(d).item()) for d in dense_batches))
+        self.sparse_fill = self.sparse_nnz / (self.N_Omega * self.K * self.N_seg)
 
     def _pf_batch(self, ib):
         """The CSR pair of batch ib: stored, or evaluated now (generated mode; valid until the next call)."""
@@ -1154,7 +1165,17 @@ class SinglePhaseForwardOperator:
         sc = self._scratch_buffers(n_ang)
         for ib, b in enumerate(self.batches):
             self._radon_forward(x_k, self.K_stride, b["koff"], b["kstride"], sc["sino"], geo, n_ang, tab)
+            self.mark("radon_forward")
             self._pf_forward(ib, sc["sino"], data, subset, accumulate=ib > 0)
+            self.mark("pf_forward")
+
+    def mark(self, name):
+        """With self.marks a list (a profiling pass), record a HIP event on the stream the kernels run on and append
+        (name, event): the launches since the previous mark belong to `name`."""
+        if self.marks is not None:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(self.queue)
+            self.marks.append((name, e))
 
     def adjoint_k(self, data, out_k, subset=None):
         """out_k = A^T data (channel-fastest)."""

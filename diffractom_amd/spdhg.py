@@ -169,8 +169,10 @@ class SPDHGPoisson:
         sc = op._scratch_buffers(n_ang)
         for ib, bt in enumerate(op.batches):
             op._pf_adjoint(ib, self.D, sc["sino"], m)
+            op.mark("pf_adjoint")
             op._radon_backward(sc["sino"], geo, tab, n_ang, None, op.K_stride, bt["koff"], bt["kstride"],
                                x=self.xk, z=self.zk, T=self.Tk, inv_p=self._inv_p_data, lam=self.lam)
+            op.mark("radon_backward_spdhg")
 
     def _forward(self, m):
         """D (rows of subset m, or all rows for m None) = A x."""
@@ -206,9 +208,12 @@ class SPDHGPoisson:
         sc = op._scratch_buffers(max(len(self.subsets[m]), len(self.subsets[mn])))
         for ib, bt in enumerate(op.batches):
             op._pf_adjoint(ib, self.D, sc["sino"], m)
+            op.mark("pf_adjoint")
             op._spdhg_fused(sc["sino"], m, mn, self.xk, self.zk, self.Tk, self._inv_p_data, self.lam, bt["koff"],
                             bt["kstride"], cw)
+            op.mark("spdhg_fused")
             op._pf_forward(ib, sc["sino"], self.D, mn, accumulate=ib > 0)
+            op.mark("pf_forward")
         return True
 
     # ---------------------------------------------------------------- run
@@ -219,6 +224,11 @@ class SPDHGPoisson:
         op = self.op
         dev = op.device
         cshape, dshape = tuple(op.coeff_shape), tuple(op.data_shape)
+        # graphs captured in an earlier run hold that run's buffer addresses (y, sig, D, b, w, x, z, T are
+        # allocated anew below): none of them may be replayed in this run
+        self._graph_cache.clear()
+        self._graph_seen.clear()
+        self._graph_gen = None
         if self.tv > 0 and state is not None and state.get("z") is None:
             raise ValueError("continuing a TV run needs the state's z")
         self._x_on_gpu = isinstance(x0, torch.Tensor)
@@ -312,7 +322,8 @@ class SPDHGPoisson:
         row with full_forward) into D."""
         blk = self._seq[self._pos]
         m_next = next_block if next_block is not None and next_block >= 0 else None
-        if self.graphs and self.op.queue is None and not full_forward and m_next is not None:
+        if self.graphs and self.op.queue is None and not full_forward and m_next is not None and \
+                self.op.marks is None:
             if blk >= 0 and self.op.fused_width(len(self.subsets[blk])):
                 self._graph_run(("step", blk, m_next), lambda: self._step_launches(blk, m_next, False))
             else:
@@ -349,11 +360,13 @@ class SPDHGPoisson:
     def _step_launches(self, blk, m_next, full_forward):
         if blk >= 0:
             self._dual(blk)
+            self.op.mark("dual_update")
             if not full_forward and m_next is not None and self._fused_step(blk, m_next):
                 return
             self._adjoint_update(blk)
         else:
             self._tv_update()
+            self.op.mark("tv")
         if full_forward:
             self._forward(None)
         elif m_next is not None:

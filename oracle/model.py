@@ -572,6 +572,57 @@ def tv_value(x, T):
     return float(np.sqrt(np.where(vr, gx, 0.0) ** 2 + np.where(vd, gy, 0.0) ** 2).sum())
 
 
+# ============================================================================ differentiable model pieces
+def intensity_forward(u, angle_scale, trans_scale, ring_scale, background, n_rings):
+    """The intensity layer d[a, s, j] = f_a t_s r_{p(j)} u[a, s, j] + b_j for data (R, My, C P), segment j = c P + p,
+    so p(j) = j mod P (differentiable/data_model.py:40-43; the segment order of utils/arrays.py:3-8). Written over
+    the flat segment axis with the ring scale tiled C times. Pinned by tests/test_oracle_intensity.py: a one-hot
+    angle / translation / ring scale touches exactly its rows / points / segments, the sum over unit inputs has the
+    closed form (sum f)(sum t)(C sum r) + R My sum b, and the vjp equals the exact directional derivatives."""
+    u = np.asarray(u, np.float64)
+    R, My, CP = u.shape
+    C = CP // int(n_rings)
+    rs = np.tile(np.asarray(ring_scale, np.float64), C)                   # r_{j mod P}
+    f = np.asarray(angle_scale, np.float64)
+    t = np.asarray(trans_scale, np.float64)
+    return f[:, None, None] * t[None, :, None] * rs[None, None, :] * u + np.asarray(background, np.float64)
+
+
+def intensity_vjp(u, angle_scale, trans_scale, ring_scale, r, n_rings):
+    """Gradients of <r, d> of intensity_forward (data_model.py:44-64): d/df_a = sum_{s,j} r t_s r_p u,
+    d/dt_s = sum_{a,j} r f_a r_p u, d/dr_p = sum over a, s and the segments of ring p of r f_a t_s u,
+    d/db_j = sum_{a,s} r, and d/du = r f_a t_s r_p."""
+    u = np.asarray(u, np.float64)
+    r = np.asarray(r, np.float64)
+    R, My, CP = u.shape
+    P = int(n_rings)
+    C = CP // P
+    f = np.asarray(angle_scale, np.float64)
+    t = np.asarray(trans_scale, np.float64)
+    rp = np.asarray(ring_scale, np.float64)
+    rs = np.tile(rp, C)
+    ru = r * u
+    g_f = np.array([np.sum(ru[a] * t[:, None] * rs[None, :]) for a in range(R)])
+    g_t = np.array([np.sum(ru[:, s] * f[:, None] * rs[None, :]) for s in range(My)])
+    g_r = np.array([np.sum(ru[:, :, p::P] * f[:, None, None] * t[None, :, None]) for p in range(P)])
+    g_b = r.sum(axis=(0, 1))
+    g_u = r * f[:, None, None] * t[None, :, None] * rs[None, None, :]
+    return {"angle_scale": g_f, "trans_scale": g_t, "ring_scale": g_r, "background": g_b, "u": g_u}
+
+
+def smooth_tv(c, eps=1e-8):
+    """Smoothed isotropic TV of every orientation image of c (K, Ny, Nx): sum over pixels of
+    sqrt(dx^2 + dy^2 + eps) with forward differences, zero beyond the last row / column (the C4 loss term; the
+    reference's TV, spdhg_poisson.py:131-137, without the free-difference mask and smoothed so that it has a
+    gradient everywhere). Pinned by tests/test_oracle_intensity.py (closed forms on ramps and constants)."""
+    c = np.asarray(c, np.float64)
+    dx = np.zeros_like(c)
+    dy = np.zeros_like(c)
+    dx[:, :, :-1] = np.diff(c, axis=2)
+    dy[:, :-1, :] = np.diff(c, axis=1)
+    return float(np.sqrt(dx * dx + dy * dy + eps).sum())
+
+
 class OracleSPDHG:
     """SPDHGPoisson (dense layout, every array resident) in float64 (spdhg_poisson.py:298-751), with the total
     variation block (tv > 0).

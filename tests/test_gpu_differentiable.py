@@ -1,7 +1,8 @@
 """The forward projection as a torch.autograd.Function (backward = the HIP adjoint) and the intensity layer
 (differentiable/data_model.py:20-64), checked by the dot-product test, by the layer's explicit vjp, and by central
 finite differences in FP64 of a Poisson + TV loss evaluated with the FP64 oracle's operator (64 random directions,
-h = 1e-4; tolerance 1e-4 relative on the directional derivatives, the FP32 operator's own error being ~1e-5)."""
+h = 1e-4; tolerance 1e-4 relative on the directional derivatives, the FP32 operator's own error being ~1e-5); the
+finite-difference loss is built only from the oracle's pieces (operator, intensity layer, deviance, smoothed TV)."""
 import numpy as np
 import pytest
 
@@ -46,6 +47,32 @@ def test_dot_product_and_subset_rows():
     assert torch.allclose(x.grad, torch.from_numpy(op.adjoint(ysub.cpu().numpy())).cuda(), rtol=1e-6, atol=1e-6)
 
 
+def test_intensity_layer_matches_oracle():
+    """The layer's forward and vjp (float64 and float32 parameters) against the FP64 oracle's independent
+    restatement (oracle.model.intensity_forward / intensity_vjp, itself pinned by closed forms)."""
+    import torch
+    from diffractom_amd.differentiable import IntensityLayer
+    from oracle.model import intensity_forward, intensity_vjp
+    op, _ = _problem()
+    lay = IntensityLayer(op.data_shape, op.N_peaks)
+    rng = np.random.default_rng(5)
+    prm_np = {"angle_scale": rng.uniform(0.9, 1.1, op.N_Omega), "trans_scale": rng.uniform(0.8, 1.2, op.My),
+              "ring_scale": rng.uniform(0.5, 2.0, op.N_peaks), "background": rng.uniform(0, 3, op.N_seg)}
+    u = rng.random(op.data_shape)
+    r = rng.standard_normal(op.data_shape)
+    args = (prm_np["angle_scale"], prm_np["trans_scale"], prm_np["ring_scale"])
+    d_ref = intensity_forward(u, *args, prm_np["background"], op.N_peaks)
+    g_ref = intensity_vjp(u, *args, r, op.N_peaks)
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
+        prm = {k: torch.from_numpy(v).to("cuda", dt) for k, v in prm_np.items()}
+        d = lay.forward(torch.from_numpy(u).to("cuda", torch.float32 if dt == torch.float32 else dt), prm)
+        assert np.abs(d.double().cpu().numpy() - d_ref).max() <= tol * np.abs(d_ref).max(), dt
+        g = lay.vjp(torch.from_numpy(u).to("cuda", dt), prm, torch.from_numpy(r).to("cuda", dt))
+        for k, v in g_ref.items():
+            gv = g[k].double().cpu().numpy()
+            assert np.abs(gv - v).max() <= tol * max(np.abs(v).max(), 1.0) * 50, (dt, k)
+
+
 def test_intensity_layer_vjp_matches_autograd():
     import torch
     from diffractom_amd.differentiable import IntensityLayer
@@ -69,6 +96,7 @@ def test_loss_gradients_vs_fp64_finite_differences():
     oracle operator."""
     import torch
     from diffractom_amd.differentiable import IntensityLayer, forward_project, poisson_nll, total_variation
+    from oracle.model import deviance, intensity_forward, smooth_tv
     op, orc = _problem()
     rng = np.random.default_rng(7)
     supp = op.support_mask()[None]
@@ -77,12 +105,20 @@ def test_loss_gradients_vs_fp64_finite_differences():
     b = rng.poisson(np.maximum(orc.direct(c0), 0) * 20).astype(np.float64)
     lay = IntensityLayer(op.data_shape, op.N_peaks)
     bt = torch.from_numpy(b)
+    ones = lambda n: np.ones(n)
 
     def loss_fp64(c, f):
-        u = torch.from_numpy(orc.direct(c))
-        prm = lay.params0()
-        prm["angle_scale"] = torch.from_numpy(f)
-        return float(poisson_nll(lay.forward(u, prm), bt) + 1e-3 * total_variation(torch.from_numpy(c)))
+        # every piece from the FP64 oracle: operator, intensity layer, Poisson deviance, smoothed TV
+        d = intensity_forward(orc.direct(c), f, ones(op.My), ones(op.N_peaks), np.zeros(op.N_seg), op.N_peaks)
+        return deviance(d, b) + 1e-3 * smooth_tv(c)
+
+    # the product's loss pieces equal the oracle's at the expansion point
+    prm64 = lay.params0()
+    prm64["angle_scale"] = torch.from_numpy(f0)
+    u0 = torch.from_numpy(orc.direct(c0))
+    ref0 = loss_fp64(c0, f0)
+    own0 = float(poisson_nll(lay.forward(u0, prm64), bt) + 1e-3 * total_variation(torch.from_numpy(c0)))
+    assert abs(own0 / ref0 - 1) < 1e-9, (own0, ref0)        # (the oracle takes eps in float32, as the kernels)
 
     # analytic gradients on the GPU
     c = torch.from_numpy(c0.astype(np.float32)).cuda().requires_grad_(True)

@@ -1,7 +1,8 @@
 """The reference's example run on the HIP path, called as the notebook calls it (examples/
 example_diffractom_pipeline.ipynb cells 7-15): the SPDHG objectives after 10, 20 and 30 epochs equal the printed
 ones. Both the reference and this path compute in float32 (summation orders differ); the FP64 oracle lands within
-8e-6 of these values, the bar here is 5e-5."""
+8e-6 of these values, the bar here is 5e-5 — for the FP32 CSR products and for the headline BF16 matrix-core
+products with the hi/lo residual correction (pf_precision="bf16_residual", dense bf16 fragments)."""
 import numpy as np
 import pytest
 
@@ -10,10 +11,11 @@ from notebook import load, tailored_grid, uniform_grid
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16_residual"])
 @pytest.mark.parametrize("grid_name", ["tailored", "uniform"])
-def test_notebook_objectives(grid_name):
+def test_notebook_objectives(grid_name, precision):
     from diffractom_amd import Grid, Material, SinglePhaseForwardOperator, SPDHGPoisson
-    print(f"case {grid_name}", flush=True)
+    print(f"case {grid_name} {precision}", flush=True)
     p = load()
     g = p["golden"][grid_name]
     mats = tailored_grid() if grid_name == "tailored" else uniform_grid()
@@ -22,7 +24,8 @@ def test_notebook_objectives(grid_name):
     mat = Material.from_lattice_parameters(lattice_matrix=p["lattice"], lattice_matrix_kind="direct",
                                            symmetry_group="cubic", wavelength_A=p["wavelength"], hkl_list=p["hkl"])
     op = SinglePhaseForwardOperator(cfg=p["cfg"], material=mat, grid=grid, max_gb=0.5, normalized=True,
-                                    reserve_coefficient_arrays=0)
+                                    reserve_coefficient_arrays=0, pf_precision=precision)
+    assert op.pf_mode == ("dense" if precision != "fp32" else op.pf_mode)
     solver = SPDHGPoisson(op, subsets=8)
     solver.run(np.zeros(op.coeff_shape, dtype=np.float32), p["b"], n_epochs=30, weights=p["weights"], verbose=1,
                diagnostics_interval=10)

@@ -141,3 +141,29 @@ def test_graph_replay_matches_eager(tv):
     assert len(xs[True]) == len(xs[False]) == 12
     for a, c in zip(xs[True], xs[False]):
         assert np.array_equal(a, c)
+
+
+def test_graph_replay_across_resumed_runs():
+    """A solver with graphs on runs twice (4 epochs, then 2 more continued from solver.state): the second run
+    allocates its arrays anew, so it must capture its own graphs rather than replay the first run's (which hold the
+    first run's buffer addresses). The result equals the same 4 + 2 epochs run eagerly, bit for bit, and so does a
+    third run on inputs allocated between the runs."""
+    import torch
+    from diffractom_amd import SPDHGPoisson
+    print("case graphs resumed", flush=True)
+    op, _, b = _problem(K=16)
+    out = {}
+    for graphs in (False, True):
+        sp = SPDHGPoisson(op, subsets=4, seed=5)
+        sp.graphs = graphs
+        x = sp.run(np.zeros(op.coeff_shape, np.float32), b, 4, diagnostics_interval=2)
+        junk = [torch.empty(1 << 20, device="cuda") for _ in range(4)]     # the allocator hands out other blocks
+        x = sp.run(x, b, 2, diagnostics_interval=2, state=sp.state)
+        del junk
+        x3 = sp.run(np.zeros(op.coeff_shape, np.float32), b * 2, 3, diagnostics_interval=3)
+        out[graphs] = (np.array(x, copy=True), np.array(x3, copy=True), [s["f"] for s in sp.iter_stats])
+        if graphs:
+            assert len(sp._graph_cache) > 0
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
+    assert out[True][2] == out[False][2]

@@ -84,3 +84,32 @@ def test_nodes_per_bin_of_the_reference_benchmark():
     n = lambda s: np.prod([v for v in O.choose_rule(D(0.5), D(1.0), D(s), 0.05, share)][::2]) * \
         np.prod([v for v in O.choose_rule(D(0.5), D(1.0), D(s), 0.05, share)][1::2])
     assert n(0.4) == 4 and n(1.0) == 2 and n(1.3) == 1
+
+
+def test_product_rule_choice_equals_oracle_over_a_sweep():
+    """The product's vectorised search picks the oracle's rule (the reference's search) for every bin size,
+    kernel width and cut of a sweep, including the cases where no rule fits."""
+    share_of = lambda cut: float(np.exp(-cut))
+    for dw in (0.25, 0.5, 1.0, 2.0, 3.0):
+        for de in (0.5, 1.0, 2.0, 4.0):
+            for s in (0.15, 0.2, 0.3, 0.4, 0.6, 1.0, 1.3, 2.0, 3.0, 5.0):
+                for cut in (8.0, 6.0, 4.5):
+                    try:
+                        r = O.choose_rule(D(dw), D(de), D(s), 0.05, share_of(cut))
+                    except ValueError:
+                        r = None
+                    try:
+                        q = Q.choose_rule(D(dw), D(de), D(s), 0.05, share_of(cut))
+                        q = (q.p_omega, q.k_omega, q.p_eta, q.k_eta)
+                    except ValueError:
+                        q = None
+                    assert q == r, (dw, de, s, cut, q, r)
+
+
+@pytest.mark.parametrize("p,k", [(1, 1), (1, 3), (2, 1), (3, 2), (4, 1)])
+def test_product_node_angles_equal_oracle(p, k):
+    """The node angles of every bin are the oracle's to the last bit (midpoint sub-bins and Gauss panels)."""
+    for lo, hi, n in ((0.0, np.pi, 180), (0.0, 2 * np.pi, 360), (np.pi, 0.0, 90)):
+        nodes, w = O.rule_nodes(lo, hi, n, p, k)
+        assert np.array_equal(Q._bin_nodes(lo, hi, n, p, k), nodes)
+        assert np.isclose(w.sum(), 1.0)
