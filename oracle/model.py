@@ -15,6 +15,7 @@ Model (differentiable/reference.py:1-26):
 from __future__ import annotations
 
 import itertools
+import os
 from dataclasses import dataclass
 from math import comb, factorial
 
@@ -385,6 +386,18 @@ class OracleOperator:
         if r not in self._pf:
             self._pf[r] = sp.csr_matrix(self.pf_angle(r).reshape(self.K, self.CP))
         return self._pf[r]
+
+    def prefetch(self, rows=None, workers=None):
+        """Evaluate the PF and projection matrices of the given angles (all by default) on a thread pool (NumPy's
+        products and exponentials release the GIL); the values are those of pf_sparse / radon_matrix."""
+        from concurrent.futures import ThreadPoolExecutor
+        rows = [int(r) for r in (range(self.R) if rows is None else rows) if int(r) not in self._pf]
+        workers = workers or min(16, len(os.sched_getaffinity(0)))
+        with ThreadPoolExecutor(workers) as ex:
+            for r, m in zip(rows, ex.map(lambda r: sp.csr_matrix(self.pf_angle(r).reshape(self.K, self.CP)), rows)):
+                self._pf[r] = m
+        for r in range(self.R) if rows is None else rows:
+            self.radon_matrix(r)
 
     # ------------------------------------------------------------------ projector
     def _kernel(self, d, A):
