@@ -55,6 +55,7 @@ SIGNATURES = {
     "dfm_project_support_k": [P, P, I64, I32, I32, P],
     "dfm_deviance_partial": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P, P],
     "dfm_mean_counted_partial": [P, P, I64, I64, P, P, P],
+    "dfm_deviance_noalloc": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P],
     "dfm_poisson_em_ratio": [P, P, P, I64, P, I64, P, I64, F32, I64, P, P],
     "dfm_poisson_weight_over_gain": [P, I64, P, I64, I64, P, P],
     "dfm_em_update": [P, P, P, P, I64, F32, I64, P],

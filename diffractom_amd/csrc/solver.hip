@@ -1,4 +1,6 @@
 // Element-wise kernels and float64 reductions of the SPDHG-Poisson solver on gfx950.
+#include <mutex>
+
 #include "dfm_common.h"
 
 namespace dfm {
@@ -501,6 +503,53 @@ static dim3 pix_grid(int64_t npix, int kstride) {
   return dim3((unsigned)((kstride + 255) / 256), (unsigned)gy);
 }
 
+// Scratch of the allocation-free deviance (dfm_deviance_noalloc): the per-workgroup float64 partial sums. One copy
+// per GPU, held until the process ends; host code reaches it only through the address hipGetSymbolAddress gives for
+// the current device (noalloc_scratch), never through this name.
+static __device__ __align__(256) double scratch[DFM_MAX_PARTIALS];
+
+// out[0] = partial[0] + partial[1] + ... + partial[n - 1], added in index order (one workgroup: the partials are
+// staged in LDS, one thread adds them), so the sum equals a sequential host sum of the same partials bit for bit.
+__global__ void __launch_bounds__(256) ordered_sum_kernel(const double* __restrict__ partial, int n,
+                                                          double* __restrict__ out) {
+  __shared__ double buf[DFM_MAX_PARTIALS];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) buf[i] = partial[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += buf[i];
+    out[0] = s;
+  }
+}
+
+// The device address of `scratch` on the current device, looked up once per device (each GPU holds its own copy
+// at its own address).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int noalloc_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    set_error("dfm_deviance_noalloc: no current device (%s)", hipGetErrorString(e));
+    return DFM_ERR_LAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (noalloc_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("dfm_deviance_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return DFM_ERR_LAUNCH;
+    }
+    noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = noalloc_ptr[dev];
+  return DFM_OK;
+}
+
 }  // namespace dfm
 
 using namespace dfm;
@@ -605,6 +654,26 @@ int dfm_deviance_partial(const float* mu, const float* b, const float* w, int64_
   hipLaunchKernelGGL(deviance_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, mu, b, w, pw, g, pg, bg, pbg, eps,
                      n, partial);
   DFM_LAUNCH_CHECK("dfm_deviance_partial");
+  return DFM_OK;
+}
+
+int dfm_deviance_noalloc(const float* mu, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                         const float* bg, int64_t pbg, float eps, int64_t n, double* out, dfm_stream_t stream) {
+  DFM_CHECK_ARG(mu && b && out && (pw == 0 || w) && (pg == 0 || g) && (pbg == 0 || bg) && n >= 0,
+                "dfm_deviance_noalloc: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  DFM_CHECK_ARG(nb <= (unsigned)DFM_MAX_PARTIALS, "dfm_deviance_noalloc: %u partial sums exceed the scratch of %d",
+                nb, DFM_MAX_PARTIALS);
+  double* part = nullptr;
+  const int rc = noalloc_scratch(&part);
+  if (rc != DFM_OK) return rc;
+  // every partial 0 .. nb - 1 is written by deviance_kernel before ordered_sum_kernel reads it (same stream), so
+  // nothing left in the scratch by an earlier call is read
+  hipLaunchKernelGGL(deviance_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, mu, b, w, pw, g, pg, bg, pbg, eps,
+                     n, part);
+  DFM_LAUNCH_CHECK("dfm_deviance_noalloc");
+  hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, part, (int)nb, out);
+  DFM_LAUNCH_CHECK("dfm_deviance_noalloc");
   return DFM_OK;
 }
 

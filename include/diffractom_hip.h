@@ -306,6 +306,13 @@ int dfm_deviance_partial(const float* mu, const float* b, const float* w, int64_
                          dfm_stream_t stream);
 int dfm_mean_counted_partial(const float* b, const float* w, int64_t pw, int64_t n, double* partial,
                              int32_t* n_partial, dfm_stream_t stream);
+/* Allocation-free deviance: out[0] (a device double) = the deviance of dfm_deviance_partial, its partial sums added
+ * in index order on the device. No counterpart in the reference: for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). The partial sums live in a file-scope device
+ * array of DFM_MAX_PARTIALS doubles (16 KiB per GPU, held until the process ends), which bounds every input size;
+ * no hipMalloc, hipFree or allocating library call. Calls that overlap on one GPU share that array and race. */
+int dfm_deviance_noalloc(const float* mu, const float* b, const float* w, int64_t pw, const float* g, int64_t pg,
+                         const float* bg, int64_t pbg, float eps, int64_t n, double* out, dfm_stream_t stream);
 /* The EM pieces of the Poisson term for OSEM / MLEM (optimization/poisson_term.py:39-47 pt_ratio, pt_wg;
  * osem_poisson.py:69-73 osem_update), over n data points or n coefficients; w, g, bg periodic as in
  * dfm_deviance_partial (a zero weight: the point is not fitted, e.g. its angle is in no subset):
