@@ -105,20 +105,23 @@ void cpu_spmm_forward(const float* sino, const int32_t* row_ptr, const uint16_t*
     }
 }
 
-/* adjoint CSR rows (r, k) list j; sino[a, y, k] = alpha sum_j val data[omegas[a], y, j] */
+/* adjoint CSR rows (r, k), k < kb, list j; sino[a, y, k] = alpha sum_j val data[omegas[a], y, j] for k < kb and 0
+ * for kb <= k < K (K the sinogram row stride) */
 void cpu_spmm_adjoint(const float* data, const int32_t* row_ptr, const uint16_t* col_j, const float* val,
-                      float* sino, const int32_t* omegas, int na, int ns, int cp, int K, float alpha) {
+                      float* sino, const int32_t* omegas, int na, int ns, int cp, int K, int kb, float alpha) {
 #pragma omp parallel for collapse(2) schedule(dynamic)
   for (int a = 0; a < na; ++a)
     for (int y = 0; y < ns; ++y) {
       const int r = omegas[a];
       const float* D = data + ((int64_t)r * ns + y) * cp;
       float* S = sino + ((int64_t)a * ns + y) * K;
-      for (int k = 0; k < K; ++k) {
+      for (int k = 0; k < kb; ++k) {
         float acc = 0.0f;
-        for (int i = row_ptr[(int64_t)r * K + k]; i < row_ptr[(int64_t)r * K + k + 1]; ++i) acc += val[i] * D[col_j[i]];
+        for (int i = row_ptr[(int64_t)r * kb + k]; i < row_ptr[(int64_t)r * kb + k + 1]; ++i)
+          acc += val[i] * D[col_j[i]];
         S[k] = alpha * acc;
       }
+      for (int k = kb; k < K; ++k) S[k] = 0.0f;
     }
 }
 

@@ -1,5 +1,5 @@
 """Host-core baseline of bench.py (test infrastructure): times the C/OpenMP FP32 restatement of one SPDHG subset step
-(oracle/c/cpu_step.c) on a sample of the benchmark's orientations and scales the rate to the full workload.
+(oracle/c/cpu_step.c) on the benchmark's full workload.
 
 The sample's inputs (CSR pole-figure matrices, step sizes, dual state) are taken from a GPU solver set up on the same
 sample, so the baseline times exactly the per-step work: dual update, CSR adjoint, backprojection + primal update,
@@ -33,7 +33,7 @@ def _load():
     lib.cpu_radon_forward.argtypes = [P, P, P, I, I, I, I, I, F]
     lib.cpu_radon_backward_update.argtypes = [P, P, I, I, I, I, I, P, P, P, F, F]
     lib.cpu_spmm_forward.argtypes = [P, P, P, P, P, P, I, I, I, I]
-    lib.cpu_spmm_adjoint.argtypes = [P, P, P, P, P, P, I, I, I, I, F]
+    lib.cpu_spmm_adjoint.argtypes = [P, P, P, P, P, P, I, I, I, I, I, F]
     lib.cpu_dual_update.argtypes = [P, P, P, P, F, P, I, L]
     return lib
 
@@ -74,7 +74,8 @@ def time_steps(solver, budget_s=15.0, max_steps=400, threads=None):
     def step(m, mn):
         na, nn = len(om[m]), len(om[mn])
         lib.cpu_dual_update(_p(y), _p(D), _p(sig), _p(b), float(np.float32(solver.eps_counts)), _p(om[m]), na, row)
-        lib.cpu_spmm_adjoint(_p(D), _p(rpa), _p(cj), _p(va), _p(sino), _p(om[m]), na, op.My, op.N_seg, K, alpha)
+        lib.cpu_spmm_adjoint(_p(D), _p(rpa), _p(cj), _p(va), _p(sino), _p(om[m]), na, op.My, op.N_seg, K,
+                             op.batches[0]["K_batch"], alpha)
         lib.cpu_radon_backward_update(_p(sino), _p(geo_b[m]), op.Nx, op.Ny, op.My, na, K, _p(x), _p(z), _p(T),
                                       float(solver._inv_p_data), float(solver.lam))
         lib.cpu_radon_forward(_p(x), _p(sino), _p(geo_f[mn]), op.Nx, op.Ny, op.My, nn, K, float(op.radon_scale))
