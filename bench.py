@@ -7,11 +7,12 @@ A step is one SPDHG subset iteration: the dual update of subset m, the PF adjoin
 the primal update, the forward projection and PF product of the next subset. The warm-up and timed steps continue
 one run (the solver's own random subset order). Inputs and all solver arrays are resident in HBM before timing.
 
-Three PF-product precisions run on the same problem: "bf16" (dense PF as bf16 MFMA fragments, FP32 accumulation),
-"bf16_residual" (plus the hi/lo residual correction) and "fp32" (CSR SpMM; the tests pin it to the FP64 oracle
-within 2e-5). The headline (--precision, default bf16_residual: the BF16 path that stays
-within 1e-3 of FP32 on every output) reports its steps/s; every path reports its forward, adjoint and SPDHG-iterate
-relative L2 difference to the fp32 path, all solving the same counts.
+Four PF-product precisions run on the same problem: "fp32" (CSR SpMM; the tests pin it to the FP64 oracle within
+2e-5) and three matrix-core paths on the PF stored as 32 x 32 blocks of its non-zeros: "bf16" (bf16 operands),
+"bf16_residual" (plus the hi/lo residual correction) and "fp16" (fp16 operands with power-of-two scales per ring and
+per row chunk), FP32 accumulation. The headline (--precision, default bf16_residual) reports its steps/s; every path
+reports its forward, adjoint and SPDHG-iterate relative L2 difference to the fp32 path, all solving the same counts,
+and its forward / adjoint error against the FP64 oracle on a sample.
 
 Prints one JSON line (metric, value, roofline of the headline path's dominant kernel from HIP events over the timed
 steps, cpu_baseline from the C/OpenMP restatement on the host cores, forward/adjoint GB/s, per-path figures).
@@ -48,7 +49,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of host-core baseline steps")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--precision", default="bf16_residual", choices=["bf16", "bf16_residual", "fp32"],
+    ap.add_argument("--precision", default="bf16_residual", choices=["bf16", "bf16_residual", "fp16", "fp32"],
                     help="PF product of the headline figure (the other precisions run too, for the comparison)")
     ap.add_argument("--only", action="store_true", help="run the headline precision only")
     ap.add_argument("--no-op-timing", action="store_true", help="skip the full forward / adjoint timing")
@@ -77,8 +78,8 @@ def build_problem(name, K=None, mean_counts=None, precision="fp32", counts=None)
     from diffractom_amd.workloads import CONFIGS, make_grid, poisson_counts, voronoi_truth
     p = CONFIGS[name]
     grid, mat, cfg = make_grid(name, K=K)
-    op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True,
-                                    pf_mode="sparse" if precision == "fp32" else "dense", pf_precision=precision)
+    op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="sparse",
+                                    pf_precision=precision)
     xt = voronoi_truth(op.K, p["n"], p["grains"], p["radius"], p["seed"], p["coeff"])
     if counts is not None:
         return op, counts, xt, p
@@ -137,7 +138,7 @@ def mfma_figures(op, M, kt):
     """The matrix-core PF products' fragment stream (bytes per launch) and MFMA utilisation: FLOPs the MFMAs issue
     per launch (2 Rm My_p CP_p K_p per product pass, the padded tile sizes; 3 passes with the residual) and the
     dense-algorithmic FLOPs 2 Rm My CP K, both over the kernel's HIP-event time against the dense BF16 peak."""
-    if op.mfma_batches is None:
+    if op.mfma_batches is None and op.blk_batches is None:
         return None
     R, My, CP, K = op.N_Omega, op.My, op.N_seg, op.K
     Rm = R / M
@@ -145,12 +146,19 @@ def mfma_figures(op, M, kt):
     for name, keys in (("pf_forward", ("fwd_hi", "fwd_lo")), ("pf_adjoint", ("adj_hi", "adj_lo"))):
         if name not in kt:
             continue
-        frag = sum(mb[k].numel() * 2 for mb in op.mfma_batches for k in keys if mb[k] is not None) * Rm / R
         passes = 3 if op.pf_precision.endswith("residual") else 1
         myp = -(-My // 16) * 16
-        cpp = -(-CP // (16 if name == "pf_forward" else 32)) * (16 if name == "pf_forward" else 32)
-        kp = sum(-(-b["K_batch"] // (32 if name == "pf_forward" else 16)) * (32 if name == "pf_forward" else 16)
-                 for b in op.batches)
+        if op.blk_batches is not None:
+            # blocks of non-zeros: (pos, value) per entry + the block pointers, the subset's share
+            frag = sum(blk["pos"].numel() * 2 + blk["val"].numel() * blk["val"].element_size() +
+                       blk["ptr"].numel() * 4 for blk in op.blk_batches) * Rm / R
+            cpp = -(-CP // 32) * 32
+            kp = sum(-(-b["K_batch"] // 32) * 32 for b in op.batches)
+        else:
+            frag = sum(mb[k].numel() * 2 for mb in op.mfma_batches for k in keys if mb[k] is not None) * Rm / R
+            cpp = -(-CP // (16 if name == "pf_forward" else 32)) * (16 if name == "pf_forward" else 32)
+            kp = sum(-(-b["K_batch"] // (32 if name == "pf_forward" else 16)) * (32 if name == "pf_forward" else 16)
+                     for b in op.batches)
         issued = 2.0 * Rm * myp * cpp * kp * passes
         dense = 2.0 * Rm * My * CP * K
         t = kt[name] * 1e-3
@@ -612,7 +620,7 @@ def main():
         {"osem": run_osem, "pdhg": run_pdhg, "fista": run_fista}[args.solver](args)
         return
     # fp32 first: its counts are the problem every precision solves
-    precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "bf16")]
+    precisions = [args.precision] if args.only else ["fp32"] + [q for q in ("bf16_residual", "fp16", "bf16")]
     results = {}
     counts = None
     leg = {}
@@ -720,9 +728,10 @@ def measured_traffic(kernel, config, precision, variant=None):
     return float(e["bytes_per_launch"])
 
 
-DTYPE = {"bf16": "bf16", "bf16_residual": "bf16", "fp32": "f32"}
-PF_DESC = {"bf16": "dense BF16 MFMA, FP32 accumulation",
-           "bf16_residual": "dense BF16 MFMA with hi/lo residual correction, FP32 accumulation",
+DTYPE = {"bf16": "bf16", "bf16_residual": "bf16", "fp16": "f16", "fp32": "f32"}
+PF_DESC = {"bf16": "block-sparse BF16 MFMA, FP32 accumulation",
+           "bf16_residual": "block-sparse BF16 MFMA with hi/lo residual correction, FP32 accumulation",
+           "fp16": "block-sparse FP16 MFMA, power-of-two scales per ring and per row chunk, FP32 accumulation",
            "fp32": "FP32 CSR SpMM"}
 
 

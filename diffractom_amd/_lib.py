@@ -37,6 +37,7 @@ SIGNATURES = {
     "dfm_smooth_tv": [P, P, I32, I32, I32, F32, P, P, P],
     "dfm_radon_forward_sweep": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_sweep_lines": [I32, I32],
+    "dfm_radon_forward_pairs_sweep": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_forward_band": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_spmm_t": [P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, F32, I32, I32, P],
     "dfm_spmm_t_rows_per_block": [I32],
@@ -78,6 +79,12 @@ SIGNATURES = {
     "dfm_mfma_forward": [P, I32, I32, P, P, P, I32, I32, I32, I32, I32, P, P, I32, I32, I32, P],
     "dfm_mfma_grouped": [I32, P, P, I64, I32, I32, P],
     "dfm_mfma_adjoint": [P, P, P, P, I32, I32, I32, I32, I32, P, I32, F32, I32, P],
+    "dfm_pfblk_count": [P, P, I32, I32, I32, I32, P],
+    "dfm_pfblk_fill": [P, P, P, P, I32, P, I32, I32, I32, I32, I32, P],
+    "dfm_pfblk_forward": [P, I32, I32, P, P, P, I32, P, I32, P, I32, I32, I32, P, P, I32, I32, P],
+    "dfm_pfblk_adjoint": [P, P, P, P, I32, P, I32, P, I32, I32, I32, I32, P, I32, F32, P],
+    "dfm_pfblk_grouped": [I32, P, P, I64, I32, P],
+    "dfm_pfblk_desc_size": [],
 }
 
 DFM_ERR_ARG, DFM_ERR_UNSUPPORTED, DFM_ERR_LAUNCH, DFM_ERR_MEMORY = -1, -2, -3, -4

@@ -37,6 +37,19 @@ class _Desc(ctypes.Structure):
 assert ctypes.sizeof(_Desc) == 104
 
 
+class _BlkDesc(ctypes.Structure):
+    """dfm_pfblk_desc of the C ABI."""
+    _fields_ = [("src", ctypes.c_void_p), ("ptr", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("val", ctypes.c_void_p), ("omegas", ctypes.c_void_p), ("ring_inv", ctypes.c_void_p),
+                ("out", ctypes.c_void_p),
+                ("n_ang", ctypes.c_int32), ("my", ctypes.c_int32), ("cp", ctypes.c_int32), ("sks", ctypes.c_int32),
+                ("kb", ctypes.c_int32), ("P", ctypes.c_int32), ("n_kb", ctypes.c_int32), ("n_jb", ctypes.c_int32),
+                ("n_cb", ctypes.c_int32), ("n_ch", ctypes.c_int32), ("mt_n", ctypes.c_int32),
+                ("n_yg", ctypes.c_int32), ("n_tg", ctypes.c_int32), ("ksplit", ctypes.c_int32),
+                ("ch_per", ctypes.c_int32), ("direct", ctypes.c_int32), ("accumulate", ctypes.c_int32),
+                ("alpha", ctypes.c_float), ("ncols", ctypes.c_int32)]
+
+
 def _y_tiling(my):
     tiles = -(-my // 16)
     n_yg = -(-tiles // 8)
@@ -45,8 +58,9 @@ def _y_tiling(my):
 
 class BatchedSPDHGPoisson:
     """SPDHGPoisson over many slices at once. operators: SinglePhaseForwardOperator of the slices (same geometry,
-    dense PF on the matrix cores, pf_precision "bf16" or "bf16_residual", one orientation batch each); supports: one
-    support option per slice (as SPDHGPoisson's `support`), or one for all."""
+    PF on the matrix cores: dense fragments or blocks of the non-zeros, one 16-bit pf_precision for all, one
+    orientation batch each); supports: one support option per slice (as SPDHGPoisson's `support`), or one for
+    all."""
 
     def __init__(self, operators, subsets=8, support="fov", eps_counts=1e-3, rho="auto", seed=0, variant=2):
         ops = list(operators)
@@ -58,11 +72,11 @@ class BatchedSPDHGPoisson:
                 np.array_equal(op.angles, o0.angles) and op.cor_offset == o0.cor_offset
             if not same:
                 raise ValueError("the slices of a batch must share the detector geometry (cfg)")
-            if op.mfma_batches is None or len(op.batches) != 1:
-                raise ValueError("batched slices need the dense PF on the matrix cores (pf_precision 'bf16' or "
-                                 "'bf16_residual') in one orientation batch")
-            if op.pf_precision != o0.pf_precision:
-                raise ValueError("the slices of a batch must share pf_precision")
+            if (op.mfma_batches is None and op.blk_batches is None) or len(op.batches) != 1:
+                raise ValueError("batched slices need the PF on the matrix cores (pf_precision 'bf16', "
+                                 "'bf16_residual' or 'fp16') in one orientation batch")
+            if op.pf_precision != o0.pf_precision or (op.blk_batches is None) != (o0.blk_batches is None):
+                raise ValueError("the slices of a batch must share pf_precision and the PF storage")
         supports = support if isinstance(support, (list, tuple)) else [support] * len(ops)
         if len(supports) != len(ops):
             raise ValueError("one support per slice")
@@ -75,6 +89,7 @@ class BatchedSPDHGPoisson:
         self._last = None
         self.variant = int(variant)
         self.residual = o0.pf_precision == "bf16_residual"
+        self.blocks = o0.blk_batches is not None
         offs = np.cumsum([0] + [op.K_stride for op in ops])
         self.offsets = [int(v) for v in offs[:-1]]
         self.Kc = int(offs[-1])
@@ -153,6 +168,8 @@ class BatchedSPDHGPoisson:
         R, My, CP = o0.N_Omega, o0.My, o0.N_seg
         n_ang = R if m is None else len(self.subsets[m])
         mt_n, n_yg = _y_tiling(My)
+        if self.blocks:
+            return self._blk_group(forward, m, key, n_ang, mt_n, n_yg)
         descs = (_Desc * len(self.ops))()
         wg = []
         for i, op in enumerate(self.ops):
@@ -186,6 +203,45 @@ class BatchedSPDHGPoisson:
         self._groups[key] = (buf, wgm, len(wg))
         return self._groups[key]
 
+    def _blk_group(self, forward, m, key, n_ang, mt_n, n_yg):
+        """The grouped block-sparse products (dfm_pfblk_grouped): one descriptor per slice, no k split (every
+        slice's forward writes its data rows directly)."""
+        o0 = self.ops[0]
+        R, My, CP = o0.N_Omega, o0.My, o0.N_seg
+        descs = (_BlkDesc * len(self.ops))()
+        assert ctypes.sizeof(_BlkDesc) == _lib.load().dfm_pfblk_desc_size()
+        wg = []
+        for i, op in enumerate(self.ops):
+            blk, b = op.blk_batches[0], op.batches[0]
+            d = descs[i]
+            om = None if m is None else op._sub_omegas[m]
+            d.omegas = om.data_ptr() if om is not None else None
+            d.ptr, d.pos, d.val = blk["ptr"].data_ptr(), blk["pos"].data_ptr(), blk["val"].data_ptr()
+            d.ring_inv = blk["ring_inv"].data_ptr() if blk["ring_inv"] is not None else None
+            d.n_ang, d.my, d.cp, d.sks, d.P = n_ang, My, CP, self.Kc, op.N_peaks
+            d.n_kb, d.n_jb = blk["n_kb"], blk["n_jb"]
+            d.mt_n, d.n_yg, d.ksplit = mt_n, n_yg, 1
+            d.direct, d.accumulate = 1, 0
+            sino_i = self.sino.data_ptr() + 4 * self.offsets[i]
+            data_i = self.D.data_ptr() + 4 * i * R * My * CP
+            if forward:
+                d.src, d.out, d.kb = sino_i, data_i, b["K_batch"]
+                d.n_cb, d.n_ch = blk["n_jb"], blk["n_kb"]
+                d.alpha, d.ncols = 1.0, 0
+            else:
+                d.src, d.out, d.kb = data_i, sino_i, b["kstride"]
+                d.n_cb, d.n_ch = -(-b["kstride"] // 32), blk["n_jb"]
+                d.alpha, d.ncols = float(R / np.pi), b["kstride"]
+            d.ch_per = d.n_ch
+            d.n_tg = -(-d.n_cb // 4)
+            n_wg = n_ang * n_yg * d.n_tg
+            wg.append(np.stack([np.full(n_wg, i, np.int32), np.arange(n_wg, dtype=np.int32)], 1))
+        wg = np.concatenate(wg)
+        buf = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to(o0.device)
+        wgm = torch.from_numpy(np.ascontiguousarray(wg)).to(o0.device)
+        self._groups[key] = (buf, wgm, len(wg))
+        return self._groups[key]
+
     def autotune(self, reps=3):
         """Time the projector variants of the first slice's operator on this batch's buffers (all slices'
         orientations at once; subset 0) and keep the fastest, as SinglePhaseForwardOperator.autotune does for one
@@ -200,6 +256,10 @@ class BatchedSPDHGPoisson:
 
     def _pf(self, forward, m):
         buf, wgm, n_wg = self._group(forward, m)
+        if self.blocks:
+            _lib.call("dfm_pfblk_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg,
+                      self.ops[0]._blk_mode(), self.ops[0]._stream())
+            return
         _lib.call("dfm_mfma_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg, int(self.residual),
                   self.variant, self.ops[0]._stream())
 

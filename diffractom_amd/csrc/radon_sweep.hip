@@ -139,6 +139,155 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------------ ray-pair sweep
+// The same line sweep with two neighbouring rays (bins 2 sp and 2 sp + 1 of one angle) per thread. On a line the two
+// rays' footprints lie within 4 consecutive pixels p0 .. p0 + 3, the first ray's (the one further left on the line)
+// within p0 .. p0 + 2 and the other's within p0 + 1 .. p0 + 3, when 2 hw / |coef| + 1 / |coef| < 4 (any square
+// image with the trapezoid footprint); so each of the 4 pixels is read from LDS once for both rays (4 reads of 32
+// orientations instead of 6) and every candidate's weight is computed from the geometry (0 outside the footprint):
+// no per-lane loops, no divergent branches. The line geometry is float64 (per line, not per pixel), the six weights
+// float32 with one v_med3 clamp each. Pixels are 8 float4 (32 orientations) apart in LDS and a lane reads quad
+// (qq + lane) % 8 at step qq, so the 8 lanes of every LDS cycle hit 8 distinct 16-byte bank groups whatever pixels
+// they read (no bank conflicts); two zero pixels pad each staged line on both sides, so the candidates need no
+// bounds checks. Staging addresses are computed once per thread (the block's line offset is added per block).
+constexpr int PR_TH = 512;
+
+template <int RB, int NV>
+__global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
+    const float4* __restrict__ img, float4* __restrict__ sino, const SweepGeo* __restrict__ geo,
+    const int* __restrict__ grp_ang, const int* __restrict__ grp_cls, int n_grp, int na_grp, int ns, int nx, int ny,
+    int kstride4, int k4_off, int k4_n, int skstride4) {
+  constexpr int CQ = 8;
+  extern __shared__ float4 limg[];   // 2 buffers x RB lines x (len + 4) pixels x CQ float4
+  const int n_cb = (k4_n + CQ - 1) / CQ;
+  int cb, g;
+  if (!xcd_split(n_grp, n_cb, cb, g)) return;                 // uniform over the workgroup
+  const int cls = grp_cls[g];
+  const int len = cls ? ny : nx, nlines = cls ? nx : ny;
+  const int row = (len + 4) * CQ;                            // float4 per staged line
+  const int buf = RB * row;
+  const int t = threadIdx.x, lane = t & 63;
+  const int npair = (ns + 1) >> 1;
+  const int al = t / npair, sp = t - al * npair;
+  const int a = al < na_grp ? grp_ang[g * na_grp + al] : -1;
+  const bool live = a >= 0;
+  const int q0 = cb * CQ;
+
+  // ---- zero pads (pixels -2, -1, len, len + 1 of every line of both buffers); never overwritten
+  for (int i = t; i < 2 * RB * 4 * CQ; i += PR_TH) {
+    const int q = i % CQ, pp = (i / CQ) % 4, l = i / (4 * CQ);       // l over 2 * RB lines
+    const int px = pp < 2 ? pp : len + pp;                            // slots 0, 1, len + 2, len + 3
+    limg[l * row + px * CQ + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+
+  // ---- staging slots of this thread: element i = t + PR_TH u of a block = (line l, position pos, quad q)
+  int lds_u[NV];
+  int64_t gof_u[NV];
+  bool ok_u[NV];
+  const int64_t line_step = cls ? (int64_t)kstride4 : (int64_t)nx * kstride4;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int i = t + PR_TH * u;
+    const int q = i & (CQ - 1), pp = i / CQ;
+    const int l = pp / len, pos = pp - l * len;
+    ok_u[u] = l < RB && q0 + q < k4_n;
+    lds_u[u] = l * row + (pos + 2) * CQ + q;
+    const int64_t pix = cls ? (int64_t)pos * nx + l : (int64_t)l * nx + pos;
+    gof_u[u] = pix * kstride4 + k4_off + q0 + q;
+  }
+  float4 v[NV];
+  auto stage_load = [&](int L0) {
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const int i = t + PR_TH * u;
+      if (ok_u[u] && L0 + i / (CQ * len) < nlines) v[u] = img[gof_u[u] + (int64_t)L0 * line_step];
+    }
+  };
+  auto stage_store = [&](float4* lb) {
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      const int i = t + PR_TH * u;
+      if (i < RB * len * CQ) lb[lds_u[u]] = v[u];
+    }
+  };
+
+  // ---- ray-pair geometry: mid(l) of the left ray = m1 l + mL0 (float64), footprint hinv, float32 weight terms
+  double m1 = 0.0, mL0 = 0.0, hinv = 0.0, ainv = 1.0;
+  float hs = 0.f, ts = 0.f, ds = 0.f;
+  int bin_l = 0, bin_r = 0;
+  if (live) {
+    const SweepGeo gg = geo[a];
+    const double coef = cls ? gg.Y : gg.X, cross = cls ? gg.X : gg.Y;
+    const double inv = 1.0 / coef;
+    const int s = 2 * sp;
+    m1 = -cross * inv;
+    mL0 = -(gg.T0 - (double)s) * inv + (inv < 0.0 ? inv : 0.0);   // the smaller of the two rays' mids
+    ainv = fabs(inv);
+    hinv = gg.hw * ainv;
+    hs = (float)(gg.hw * gg.scale);
+    ts = (float)(gg.top * gg.scale);
+    ds = (float)(coef * gg.scale);
+    bin_l = inv > 0.0 ? s : s + 1;
+    bin_r = inv > 0.0 ? s + 1 : s;
+  }
+  const float c1 = ds, c2 = 2.f * ds, cr1 = (float)((1.0 - ainv) * ds), cr2 = (float)((2.0 - ainv) * ds),
+              cr3 = (float)((3.0 - ainv) * ds);
+  const int rot = (lane & 7);                               // this lane's quad rotation
+  float4 accL[CQ], accR[CQ];
+#pragma unroll
+  for (int qq = 0; qq < CQ; ++qq) accL[qq] = accR[qq] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+  stage_load(0);
+  for (int L0 = 0, bi = 0; L0 < nlines; L0 += RB, ++bi) {
+    float4* lb = limg + (bi & 1) * buf;
+    // buffer bi & 1 was last read for block bi - 2, which every wave finished before the barrier of block bi - 1
+    stage_store(lb);
+    __syncthreads();
+    if (L0 + RB < nlines) stage_load(L0 + RB);
+    if (live) {
+#pragma unroll
+      for (int l = 0; l < RB; ++l) {
+        if (L0 + l >= nlines) break;
+        const double midL = fma(m1, (double)(L0 + l), mL0);
+        double p0 = ceil(midL - hinv);
+        p0 = fmin(fmax(p0, -2.0), (double)(len - 2));
+        const float uu = (float)(p0 - midL);
+        // the six weights: left ray candidates p0, p0 + 1, p0 + 2; right ray p0 + 1 .. p0 + 3
+        const float wl0 = __builtin_amdgcn_fmed3f(hs - fabsf(ds * uu), 0.f, ts);
+        const float wl1 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, c1)), 0.f, ts);
+        const float wl2 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, c2)), 0.f, ts);
+        const float wr1 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr1)), 0.f, ts);
+        const float wr2 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr2)), 0.f, ts);
+        const float wr3 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr3)), 0.f, ts);
+        const float4* base = lb + l * row + ((int)p0 + 2) * CQ;
+#pragma unroll
+        for (int qq = 0; qq < CQ; ++qq) {
+          const float4* px = base + ((qq + rot) & (CQ - 1));
+          const float4 x0 = px[0], x1 = px[CQ], x2 = px[2 * CQ], x3 = px[3 * CQ];
+          accL[qq].x += wl0 * x0.x + wl1 * x1.x + wl2 * x2.x;
+          accL[qq].y += wl0 * x0.y + wl1 * x1.y + wl2 * x2.y;
+          accL[qq].z += wl0 * x0.z + wl1 * x1.z + wl2 * x2.z;
+          accL[qq].w += wl0 * x0.w + wl1 * x1.w + wl2 * x2.w;
+          accR[qq].x += wr1 * x1.x + wr2 * x2.x + wr3 * x3.x;
+          accR[qq].y += wr1 * x1.y + wr2 * x2.y + wr3 * x3.y;
+          accR[qq].z += wr1 * x1.z + wr2 * x2.z + wr3 * x3.z;
+          accR[qq].w += wr1 * x1.w + wr2 * x2.w + wr3 * x3.w;
+        }
+      }
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int qq = 0; qq < CQ; ++qq) {
+      const int q = (qq + rot) & (CQ - 1);
+      if (q0 + q >= k4_n) continue;
+      if (bin_l < ns) sino[((int64_t)a * ns + bin_l) * skstride4 + q0 + q] = accL[qq];
+      if (bin_r < ns) sino[((int64_t)a * ns + bin_r) * skstride4 + q0 + q] = accR[qq];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -149,6 +298,43 @@ int dfm_radon_sweep_lines(int32_t nx, int32_t ny) {
   for (int rb = 8; rb >= 2; rb /= 2)
     if ((size_t)2 * rb * len * 5 * sizeof(float4) <= 163840) return rb;
   return 0;
+}
+
+int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
+                                  const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx,
+                                  int32_t ny, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride,
+                                  int32_t lines, dfm_stream_t stream) {
+  DFM_CHECK_ARG(img_k && sino && geo && grp_ang && grp_cls, "dfm_radon_forward_pairs_sweep: null pointer");
+  const int len = nx > ny ? nx : ny;
+  DFM_CHECK_ARG(n_grp >= 0 && na_grp >= 1 && ns >= 1 && nx >= 1 && ny >= 1 && len <= 256 && kstride % 4 == 0 &&
+                    k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride &&
+                    skstride % 4 == 0 && skstride >= k_n && na_grp * ((ns + 1) / 2) <= PR_TH,
+                "dfm_radon_forward_pairs_sweep: bad sizes (na_grp * ceil(ns / 2) <= 512, image <= 256 x 256)");
+  DFM_CHECK_ARG(lines == 1 || lines == 2 || lines == 4, "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4",
+                lines);
+  const size_t lds = (size_t)2 * lines * (len + 4) * 8 * sizeof(float4);
+  DFM_CHECK_ARG(lds <= 163840, "dfm_radon_forward_pairs_sweep: %d lines of %d pixels exceed the LDS", lines, len);
+  if (n_grp == 0 || k_n == 0) return DFM_OK;
+  const int k4n = k_n / 4;
+  const int n_cb = (k4n + 7) / 8;
+  const unsigned grid = (unsigned)((n_cb + N_XCD - 1) / N_XCD * N_XCD * n_grp);
+  const hipStream_t st = (hipStream_t)stream;
+  // staging elements per thread: lines * len * 8 float4 over 512 threads (len <= 128 or <= 256)
+  const bool small = len <= 128;
+#define DFM_PS(RB, NV)                                                                                           \
+  hipLaunchKernelGGL((radon_forward_pairs_sweep_kernel<RB, NV>), dim3(grid), dim3(PR_TH), lds, st,                 \
+                     (const float4*)img_k, (float4*)sino, (const SweepGeo*)geo, grp_ang, grp_cls, n_grp, na_grp, ns, \
+                     nx, ny, kstride / 4, k_off / 4, k4n, skstride / 4)
+  if (lines == 1) {
+    if (small) DFM_PS(1, 2); else DFM_PS(1, 4);
+  } else if (lines == 2) {
+    if (small) DFM_PS(2, 4); else DFM_PS(2, 8);
+  } else {
+    if (small) DFM_PS(4, 8); else DFM_PS(4, 16);
+  }
+#undef DFM_PS
+  DFM_LAUNCH_CHECK("dfm_radon_forward_pairs_sweep");
+  return DFM_OK;
 }
 
 int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,

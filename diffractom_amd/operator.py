@@ -177,10 +177,10 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
     band_ptr = np.zeros(na * ns * n_bands + 1, np.int64)
     np.cumsum(np.bincount(rows * n_bands + (pix // nx) // bh, minlength=na * ns * n_bands), out=band_ptr[1:])
     # angle groups of one sweep class, as many angles as 1024 (and 512) rays hold (dfm_radon_forward_sweep)
-    def sweep_groups(rays):
-        na_grp = max(1, rays // ns)
-        groups = [(c, grp) for c in (0, 1) for grp in np.array_split(np.flatnonzero(cls == c),
-                                                                      -(-int((cls == c).sum()) // na_grp))]
+    def sweep_groups(rays, per=ns):
+        na_grp = max(1, rays // per)
+        groups = [(c, grp) for c in (0, 1) if (cls == c).any()
+                  for grp in np.array_split(np.flatnonzero(cls == c), -(-int((cls == c).sum()) // na_grp))]
         ga = np.full((max(len(groups), 1), na_grp), -1, np.int32)
         gc = np.zeros(max(len(groups), 1), np.int32)
         for i, (c, grp) in enumerate(groups):
@@ -189,11 +189,19 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
         return ga, gc
     sweep_grp, sweep_gcls = sweep_groups(1024)
     sweep_grp_h, sweep_gcls_h = sweep_groups(512)
+    # ray pairs (dfm_radon_forward_pairs_sweep): ceil(ns / 2) threads per angle, 512 per workgroup; usable when the
+    # two rays' footprints on a line lie within 4 pixels for every angle: (2 hw + 1) / |coef| < 4, coef the
+    # along-line coefficient (the larger of |X|, |Y|)
+    coef = np.maximum(np.abs(g["X"][idx]), np.abs(g["Y"][idx]))
+    pairs_ok = bool(np.all((2.0 * g["hw"][idx] + 1.0) / coef < 4.0 - 1e-9)) and -(-ns // 2) <= 512
+    sweep_grp_p, sweep_gcls_p = sweep_groups(512, -(-ns // 2)) if pairs_ok else (np.full((1, 1), -1, np.int32),
+                                                                         np.zeros(1, np.int32))
     return {"row_ptr": row_ptr.astype(np.int32), "pix": pix[order].astype(np.int32), "band_ptr": band_ptr.astype(np.int32),
             "tbase8": tbase8,
             "sweep_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
                                    radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
+            "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p,
             "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
             "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
             "pair_w": pw.astype(np.float32)}
@@ -248,15 +256,21 @@ class SinglePhaseForwardOperator:
         self.radon_kernel = radon_kernel
         if pf_mode not in ("dense", "sparse", "generated", "auto"):
             raise ValueError(f"pf_mode must be 'dense', 'sparse', 'generated' or 'auto', not {pf_mode!r}")
-        # PF product arithmetic: "fp32" (CSR or dense FP32), "bf16" (dense PF on the matrix cores, bf16 operands,
-        # FP32 accumulation) or "bf16_residual" (the same plus the hi/lo residual correction, ~2^-16 relative)
-        if pf_precision not in ("fp32", "bf16", "bf16_residual"):
-            raise ValueError(f"pf_precision must be 'fp32', 'bf16' or 'bf16_residual', not {pf_precision!r}")
-        if pf_precision != "fp32" and pf_mode not in ("dense", "auto"):
-            raise ValueError(f"pf_precision {pf_precision!r} runs the dense PF product on the matrix cores; "
-                             f"pf_mode must be 'dense' or 'auto', not {pf_mode!r}")
+        # PF product arithmetic on the matrix cores, FP32 accumulation: "fp32" (CSR or dense FP32 products, no
+        # matrix cores), "bf16" (bf16 operands), "bf16_residual" (plus the hi/lo residual correction, ~2^-16
+        # relative), "fp16" (fp16 operands with power-of-two scales per ring and per row chunk, ~2^-11 relative).
+        # pf_mode "sparse" / "auto" keeps the PF as 32 x 32 blocks of its non-zeros (one copy for both directions),
+        # "dense" as dense bf16 fragments (one copy per direction); "generated" is for fp32 only.
+        if pf_precision not in ("fp32", "bf16", "bf16_residual", "fp16"):
+            raise ValueError(f"pf_precision must be 'fp32', 'bf16', 'bf16_residual' or 'fp16', not {pf_precision!r}")
+        if pf_precision != "fp32" and pf_mode == "generated":
+            raise NotImplementedError(f"pf_precision {pf_precision!r} needs a stored PF (pf_mode 'sparse', 'auto' or "
+                                      f"'dense'); the generated mode runs the FP32 products")
+        if pf_precision == "fp16" and pf_mode == "dense":
+            raise NotImplementedError("pf_precision 'fp16' runs on the block-sparse PF (pf_mode 'sparse' or 'auto')")
         self.pf_precision = pf_precision
         self.mfma_variant = {"fwd": 2, "adj": 2, "fwd_waves": 2048}
+        self.blk_variant = {"fwd_waves": 8192}
         self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0,
                             "tdirect": 3}
 
@@ -297,6 +311,7 @@ class SinglePhaseForwardOperator:
         self.sparse_batches = None
         self.dense_batches = None
         self.mfma_batches = None
+        self.blk_batches = None
         self.pf_storage = None
         self.sparse_nnz = 0
         self.sparse_fill = 0.0
@@ -510,7 +525,7 @@ class SinglePhaseForwardOperator:
 
     def _build_pf(self, mode, max_fill, max_gb):
         R, CP = self.N_Omega, self.N_seg
-        if mode == "dense" or self.pf_precision != "fp32":
+        if mode == "dense":
             self.pf_mode = "dense"
             self._store_dense()
             return
@@ -519,6 +534,15 @@ class SinglePhaseForwardOperator:
         dense = self._eval_dense_batch(b0)
         nnz0 = int((dense != 0).sum().item())
         fill = nnz0 / (R * b0["K_batch"] * CP)
+        if self.pf_precision != "fp32":
+            if mode == "auto" and fill > max_fill and self.pf_precision != "fp16":
+                del dense
+                self.pf_mode = "dense"
+                self._store_dense()
+                return
+            self.pf_mode = "sparse"
+            self._store_blocks(dense)
+            return
         if mode == "auto" and fill > max_fill:
             del dense
             self.pf_mode = "dense"
@@ -586,6 +610,82 @@ class SinglePhaseForwardOperator:
         self.sparse_nnz = nnz
         self.sparse_fill = nnz / (R * self.K * CP)
 
+    def _store_blocks(self, dense=None):
+        """The PF of every orientation batch as 32 x 32 blocks (angle, 32 orientations, 32 segments) listing their
+        non-zeros (dfm_pfblk_count / dfm_pfblk_fill): block pointers (int32), positions (uint16) and values (bf16
+        hi/lo pairs, or fp16 times a power-of-two scale per ring putting the ring's largest entry in [2^13, 2^14)).
+        One copy serves both product directions. dense: the first batch's dense evaluation, if already made."""
+        R, CP, P = self.N_Omega, self.N_seg, self.N_peaks
+        st = self._stream()
+        f16 = self.pf_precision == "fp16"
+        self.blk_batches = []
+        nnz = 0
+        for i, b in enumerate(self.batches):
+            if dense is None or dense.shape[1] != b["kstride"]:
+                dense = None
+                dense = self._eval_dense_batch(b)
+            elif i > 0:
+                self._eval_dense_batch(b, out=dense)
+            kb, ks = b["K_batch"], b["kstride"]
+            n_kb, n_jb = -(-kb // 32), -(-CP // 32)
+            cnt = torch.empty(R * n_kb * n_jb, dtype=torch.int32, device=self.device)
+            _lib.call("dfm_pfblk_count", _lib.ptr(dense), _lib.ptr(cnt), R, ks, kb, CP, st)
+            ptr64 = torch.zeros(cnt.numel() + 1, dtype=torch.int64, device=self.device)
+            torch.cumsum(cnt, 0, out=ptr64[1:])
+            n = int(ptr64[-1].item())
+            if n >= 2 ** 31:
+                raise ValueError("Too many non-zeros in one orientation batch for int32 indices; lower max_gb.")
+            del cnt
+            blk = {"ptr": ptr64.to(torch.int32), "n_kb": n_kb, "n_jb": n_jb, "nnz": n,
+                   "pos": torch.empty(max(n, 1), dtype=torch.int16, device=self.device),
+                   "val": torch.empty(max(n, 1), dtype=torch.int16 if f16 else torch.int32, device=self.device),
+                   "ring_scale": None, "ring_inv": None}
+            del ptr64
+            if f16:
+                # per-ring power-of-two scale: max |PF| of ring p of this batch -> [2^13, 2^14)
+                mx = dense.abs().amax(dim=(0, 1, 2)).double().cpu().numpy()          # (P,)
+                e = np.where(mx > 0, np.frexp(np.where(mx > 0, mx, 1.0))[1], 0)
+                sc = np.ldexp(1.0, 14 - e).astype(np.float32)
+                blk["ring_scale"] = torch.from_numpy(sc).to(self.device)
+                blk["ring_inv"] = torch.from_numpy((1.0 / sc.astype(np.float64)).astype(np.float32)).to(self.device)
+            _lib.call("dfm_pfblk_fill", _lib.ptr(dense), _lib.ptr(blk["ptr"]), _lib.ptr(blk["pos"]),
+                      _lib.ptr(blk["val"]), int(f16), _lib.ptr(blk["ring_scale"]), P, R, ks, kb, CP, st)
+            nnz += n
+            self.blk_batches.append(blk)
+        del dense
+        self.pf_storage = {"bf16": "bf16 blocks", "bf16_residual": "bf16 hi/lo blocks",
+                           "fp16": "fp16 blocks, ring-scaled"}[self.pf_precision]
+        self.sparse_nnz = nnz
+        self.sparse_fill = nnz / (R * self.K * CP)
+
+    def _blk_mode(self):
+        return {"bf16": 0, "bf16_residual": 1, "fp16": 2}[self.pf_precision]
+
+    def _blk_split(self, n_ang, blk):
+        """k split of the block-sparse forward: enough waves for blk_variant["fwd_waves"], at most 64 partial sums."""
+        tiles = -(-self.My // 16)
+        n_yg = -(-tiles // 8)
+        waves = n_ang * n_yg * blk["n_jb"]
+        return int(max(1, min(64, blk["n_kb"], -(-int(self.blk_variant.get("fwd_waves", 8192)) // waves))))
+
+    def _blk_forward(self, ib, sino, data, subset, accumulate):
+        b, blk = self.batches[ib], self.blk_batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        ks = self._blk_split(n_ang, blk)
+        part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
+        _lib.call("dfm_pfblk_forward", _lib.ptr(sino), b["kstride"], b["K_batch"], _lib.ptr(blk["ptr"]),
+                  _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode(), _lib.ptr(blk["ring_inv"]),
+                  self.N_peaks, _lib.ptr(om), n_ang, self.My, self.N_seg, _lib.ptr(data), _lib.ptr(part), ks,
+                  int(accumulate), self._stream())
+
+    def _blk_adjoint(self, ib, data, sino, subset, alpha):
+        b, blk = self.batches[ib], self.blk_batches[ib]
+        n_ang, om, _, _ = self._angle_set(subset)
+        _lib.call("dfm_pfblk_adjoint", _lib.ptr(data), _lib.ptr(blk["ptr"]), _lib.ptr(blk["pos"]),
+                  _lib.ptr(blk["val"]), self._blk_mode(), _lib.ptr(blk["ring_inv"]), self.N_peaks, _lib.ptr(om),
+                  n_ang, self.My, self.N_seg, b["K_batch"], _lib.ptr(sino), b["kstride"], float(alpha),
+                  self._stream())
+
     def _count_nnz(self, dense_batches):
         """sparse_nnz / sparse_fill of dense batches (the PF's non-zeros, as the CSR forms would hold them)."""
         self.sparse_nnz = int(sum(int(torch.count_nonzero(d).item()) for d in dense_batches))
@@ -619,7 +719,7 @@ class SinglePhaseForwardOperator:
             n += sum(v.numel() * v.element_size() for v in sb.values() if isinstance(v, torch.Tensor))
         for d in self.dense_batches or []:
             n += d.numel() * 4
-        for mb in self.mfma_batches or []:
+        for mb in (self.mfma_batches or []) + (self.blk_batches or []):
             n += sum(v.numel() * v.element_size() for v in mb.values() if isinstance(v, torch.Tensor))
         return n
 
@@ -627,6 +727,7 @@ class SinglePhaseForwardOperator:
         self.sparse_batches = None
         self.dense_batches = None
         self.mfma_batches = None
+        self.blk_batches = None
         self._scratch = None
 
     def _scratch_buffers(self, n_angles):
@@ -749,6 +850,9 @@ class SinglePhaseForwardOperator:
                 for n, w, cq in ((2, 1, 8), (1, 1, 8)):
                     if 2 * n * ln * (cq + 1) * 16 <= 163840 and (w < 4 or self.My <= 512):
                         cands[f"sweep:{n}+ch{4 * cq}" + ("/512" if w >= 4 else "")] = ("sweep", n + 16 * w)
+            if tab["sweep_grp_p"][0, 0] >= 0:
+                ln = max(self.Nx, self.Ny)
+                cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4) if 2 * n * (ln + 4) * 128 <= 163840})
             if self.radon_bands > 1:
                 cands.update({"band:8+ch128": ("band", 8 + 16), "band:8": ("band", 8), "band:4": ("band", 4),
                               "band:8+ch256": ("band", 8 + 32)})
@@ -769,6 +873,8 @@ class SinglePhaseForwardOperator:
                     rv["fwd_band"] = v
                 elif kern == "sweep":
                     rv["fwd_sweep"] = v
+                elif kern == "psweep":
+                    rv["fwd_psweep"] = v
                 times[name] = timeit(lambda: self._radon_forward(img, img_kstride, koff, kn, sino, geo_f, n_ang, tab))
             best = _pick(times)
             kern, v = cands[best]
@@ -781,6 +887,8 @@ class SinglePhaseForwardOperator:
                 saved["fwd_band"] = v
             elif kern == "sweep":
                 saved["fwd_sweep"] = v
+            elif kern == "psweep":
+                saved["fwd_psweep"] = v
             out["radon_forward"] = (best, times)
             cands = {f"table:{a}x{g}": ("table", a + 64 * g) for a, g in ((4, 4), (4, 2), (8, 4), (8, 2), (8, 8))}
             if tab["tbase8"].numel() > 1:
@@ -874,6 +982,15 @@ class SinglePhaseForwardOperator:
             best = _pick(times)
             self.mfma_variant["adj"] = int(best.split(":")[1])
             out["pf_adjoint"] = (best, times)
+        if self.blk_batches is not None:
+            data = torch.rand(self.data_shape, generator=g, device=dev)
+            times = {}
+            for w in (2048, 4096, 8192, 16384, 32768):
+                self.blk_variant["fwd_waves"] = w
+                times[f"waves:{w}"] = timeit(lambda: self._blk_forward(0, sc["sino"], data, subset, False))
+            best = _pick(times)
+            self.blk_variant["fwd_waves"] = int(best.split(":")[1])
+            out["pf_forward"] = (best, times)
         del x, z, T
         self.tuning = out
         return out
@@ -924,7 +1041,13 @@ class SinglePhaseForwardOperator:
             self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
                                 kn, cw)
             return
-        if tab is not None and self.radon_variant["fwd_kernel"] == "sweep":
+        if tab is not None and self.radon_variant["fwd_kernel"] == "psweep" and tab["sweep_grp_p"][0, 0] >= 0:
+            g = tab["sweep_grp_p"]
+            _lib.call("dfm_radon_forward_pairs_sweep", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["sweep_geo"]),
+                      _lib.ptr(g), _lib.ptr(tab["sweep_gcls_p"]), g.shape[0], g.shape[1], self.My, self.Nx, self.Ny,
+                      img_kstride, koff, kn, kn, int(self.radon_variant.get("fwd_psweep", 2)), self._stream())
+            return
+        if tab is not None and self.radon_variant["fwd_kernel"] in ("sweep", "psweep"):
             lines = self.radon_variant.get("fwd_sweep", 0) or _lib.load().dfm_radon_sweep_lines(self.Nx, self.Ny)
             h = "_h" if lines >= 64 else ""
             g = tab["sweep_grp" + h]
@@ -951,6 +1074,9 @@ class SinglePhaseForwardOperator:
         """data rows (+)= PF_b sino for batch ib (sparse CSR product or dense GEMM)."""
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
+        if self.blk_batches is not None:
+            self._blk_forward(ib, sino, data, subset, accumulate)
+            return
         if self.pf_mode == "dense":
             self._dense_forward(ib, sino, data, subset, accumulate)
             return
@@ -969,6 +1095,9 @@ class SinglePhaseForwardOperator:
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
         alpha = float(self.N_Omega / np.pi)
+        if self.blk_batches is not None:
+            self._blk_adjoint(ib, data, sino, subset, alpha)
+            return
         if self.pf_mode == "dense":
             self._dense_adjoint(ib, data, sino, subset, alpha)
             return

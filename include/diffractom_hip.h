@@ -102,6 +102,13 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
                             dfm_stream_t stream);
 /* The most image lines per LDS block of dfm_radon_forward_sweep for an nx x ny image (0: unsupported). */
 int dfm_radon_sweep_lines(int32_t nx, int32_t ny);
+/* Line sweep with two neighbouring rays per thread (the same result as dfm_radon_forward_sweep): groups of na_grp
+ * angles with na_grp * ceil(ns / 2) <= 512; usable when (2 hw + 1) / max(|X|, |Y|) < 4 for every angle (square
+ * images with the trapezoid footprint); lines: image lines staged per LDS block (1, 2 or 4). */
+int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
+                                  const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx,
+                                  int32_t ny, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride,
+                                  int32_t lines, dfm_stream_t stream);
 /* The forward projection of dfm_radon_forward_table tiled by nb bands of image rows: band_ptr[(a ns + s) nb + b]
  * is the first table entry of ray (a, s) in band b (entries ascend in pixel index), band_ptr[na ns nb] the total.
  * Per band the partial sums go to part (nb * n_angles * ns * skstride floats), then sino = their sum in band order.
@@ -244,6 +251,63 @@ typedef struct dfm_mfma_desc {
 } dfm_mfma_desc;
 int dfm_mfma_grouped(int32_t forward, const dfm_mfma_desc* descs, const int32_t* wg_map, int64_t n_wg,
                      int32_t residual, int32_t variant, dfm_stream_t stream);
+
+/* Block-sparse PF on the matrix cores (the sparse PF mode with 16-bit operands; replaces spmm_pf_forward_c /
+ * spmm_pf_adjoint_c, pf_kernels.cl:597-687, and the subset forms single_phase_forward_operator.py:38-123). The PF of
+ * an orientation batch (kb orientations) is cut into 32 x 32 blocks (angle r, orientation block of 32, segment block
+ * of 32), block index (r * n_kb + kb_i) * n_jb + jb_i with n_kb = ceil(kb / 32), n_jb = ceil(cp / 32); block b lists
+ * its non-zeros ptr[b] .. ptr[b + 1] - 1 (int32 pointers, R * n_kb * n_jb + 1 of them) in pos (uint16: k % 32 in
+ * bits 0-4, j % 32 in bits 5-9) and val. One copy serves both directions.
+ * count: counts[b] = non-zeros of block b of a dense FP32 batch pf[r, k, j] (R x km x cp, rows k < kb).
+ * fill: with ptr = the exclusive scan of counts, the entries in element order (k-major within the block); fp16 = 0:
+ * val uint32 = bf16(v) | bf16(v - bf16(v)) << 16; fp16 = 1: val uint16 = fp16(v * ring_scale[j % P]).
+ * forward: data[omegas[a], y, j] (+)= sum_k PF[omegas[a], k, j] sino[a, y, k] (sino row stride sks, columns < kb);
+ * ksplit > 1 splits the orientation blocks over that many partial sums in part (ksplit * n_ang * my * cp floats),
+ * summed in a fixed order. adjoint: sino[a, y, k] = alpha sum_j PF[omegas[a], k, j] data[omegas[a], y, j] for every
+ * k < sks (0 at and beyond kb). mode: 0 bf16 (val hi only), 1 bf16 with the hi/lo residual correction (hi*hi + hi*lo
+ * + lo*hi), 2 fp16 (val fp16; ring_inv[p] = 1 / ring_scale[p]; the y-side operand scaled by a power of two per row
+ * and chunk of 32). FP32 accumulation. */
+int dfm_pfblk_count(const float* pf, int32_t* counts, int32_t R, int32_t km, int32_t kb, int32_t cp,
+                    dfm_stream_t stream);
+int dfm_pfblk_fill(const float* pf, const int32_t* ptr, uint16_t* pos, void* val, int32_t fp16,
+                   const float* ring_scale, int32_t P, int32_t R, int32_t km, int32_t kb, int32_t cp,
+                   dfm_stream_t stream);
+int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t* ptr, const uint16_t* pos,
+                      const void* val, int32_t mode, const float* ring_inv, int32_t P, const int32_t* omegas,
+                      int32_t n_ang, int32_t my, int32_t cp, float* data, float* part, int32_t ksplit,
+                      int32_t accumulate, dfm_stream_t stream);
+int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos, const void* val, int32_t mode,
+                      const float* ring_inv, int32_t P, const int32_t* omegas, int32_t n_ang, int32_t my, int32_t cp,
+                      int32_t kb, float* sino, int32_t sks, float alpha, dfm_stream_t stream);
+/* Grouped launch of block-sparse products of one direction (forward != 0: the forward) over several PF matrices
+ * (the slices of a multi-slice batch): descs is a device array of dfm_pfblk_desc, wg_map a device array of n_wg
+ * int32 pairs (product, workgroup of that product); a product's workgroups are n_ang * n_yg * n_tg * ksplit,
+ * decomposed (group of 4 column blocks fastest, then y group, k split, angle). Fields as in dfm_pfblk_forward /
+ * dfm_pfblk_adjoint: src the sinogram (forward, first column of the product) or the data; out the data (direct = 1)
+ * or the partial sums (forward), the sinogram (adjoint); n_kb / n_jb the stored blocks; n_cb the output column
+ * blocks of 32 (forward n_jb, adjoint ceil(sks / 32)) and n_ch the contraction chunks; mt_n, n_yg the y tiling
+ * (n_yg = ceil(ceil(my / 16) / 8) groups of mt_n 16-row tiles); n_tg = ceil(n_cb / 4); ch_per chunks per split. */
+typedef struct dfm_pfblk_desc {
+  const float* src;
+  const int32_t* ptr;
+  const uint16_t* pos;
+  const void* val;
+  const int32_t* omegas;
+  const float* ring_inv;
+  float* out;
+  int32_t n_ang, my, cp, sks, kb, P;
+  int32_t n_kb, n_jb;
+  int32_t n_cb, n_ch;
+  int32_t mt_n, n_yg, n_tg;
+  int32_t ksplit, ch_per;
+  int32_t direct, accumulate;
+  float alpha;
+  int32_t ncols;
+} dfm_pfblk_desc;
+int dfm_pfblk_grouped(int32_t forward, const void* descs, const int32_t* wg_map, int64_t n_wg, int32_t mode,
+                      dfm_stream_t stream);
+/* sizeof(dfm_pfblk_desc) as compiled into the library (for bindings that build the descriptor array by hand). */
+int dfm_pfblk_desc_size(void);
 
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.

@@ -2,7 +2,10 @@
 example_diffractom_pipeline.ipynb cells 7-15): the SPDHG objectives after 10, 20 and 30 epochs equal the printed
 ones. Both the reference and this path compute in float32 (summation orders differ); the FP64 oracle lands within
 8e-6 of these values, the bar here is 5e-5 — for the FP32 CSR products and for the headline BF16 matrix-core
-products with the hi/lo residual correction (pf_precision="bf16_residual", dense bf16 fragments)."""
+products with the hi/lo residual correction (pf_precision="bf16_residual"), on both PF storages: the default
+block-sparse one (pf_mode "auto" keeps the PF as 32 x 32 blocks of its non-zeros) and dense bf16 fragments
+(pf_mode "dense"). The fp16 products (11-bit operands, ring and row-chunk scales) are held to 2e-4: a product's
+rounding is about 4e-4, 8x that of the residual's hi/lo pairs."""
 import numpy as np
 import pytest
 
@@ -11,11 +14,12 @@ from notebook import load, tailored_grid, uniform_grid
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16_residual"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16_residual", "bf16_residual/dense", "fp16"])
 @pytest.mark.parametrize("grid_name", ["tailored", "uniform"])
 def test_notebook_objectives(grid_name, precision):
     from diffractom_amd import Grid, Material, SinglePhaseForwardOperator, SPDHGPoisson
     print(f"case {grid_name} {precision}", flush=True)
+    precision, _, mode = precision.partition("/")
     p = load()
     g = p["golden"][grid_name]
     mats = tailored_grid() if grid_name == "tailored" else uniform_grid()
@@ -24,11 +28,14 @@ def test_notebook_objectives(grid_name, precision):
     mat = Material.from_lattice_parameters(lattice_matrix=p["lattice"], lattice_matrix_kind="direct",
                                            symmetry_group="cubic", wavelength_A=p["wavelength"], hkl_list=p["hkl"])
     op = SinglePhaseForwardOperator(cfg=p["cfg"], material=mat, grid=grid, max_gb=0.5, normalized=True,
-                                    reserve_coefficient_arrays=0, pf_precision=precision)
-    assert op.pf_mode == ("dense" if precision != "fp32" else op.pf_mode)
+                                    reserve_coefficient_arrays=0, pf_precision=precision, pf_mode=mode or "auto")
+    if precision != "fp32":
+        assert (op.mfma_batches if mode == "dense" else op.blk_batches) is not None
     solver = SPDHGPoisson(op, subsets=8)
     solver.run(np.zeros(op.coeff_shape, dtype=np.float32), p["b"], n_epochs=30, weights=p["weights"], verbose=1,
                diagnostics_interval=10)
     f = {str(s["epoch"]): s["f"] for s in solver.iter_stats if s["f"] is not None}
+    tol = 2e-4 if precision == "fp16" else 5e-5
     for ep, ref in g["f"].items():
-        assert abs(f[ep] / ref - 1) < 5e-5, (grid_name, ep, f[ep], ref)
+        print(f"  epoch {ep}: {f[ep]:.8e} vs printed {ref:.8e} ({f[ep] / ref - 1:+.1e})", flush=True)
+        assert abs(f[ep] / ref - 1) < tol, (grid_name, ep, f[ep], ref)

@@ -131,7 +131,7 @@ def test_errors_raise_like_the_reference():
                                              ("table_tile8g2", 4), ("table_wide128", 4), ("table_wide256", 4),
                                              ("sweep", 4), ("sweep2", 4),
                                              ("sweep32", 4), ("sweep32h", 4), ("sweep64", 4), ("sweep64_1", 4),
-                                             ("arith", 4)])
+                                             ("psweep1", 4), ("psweep2", 4), ("psweep4", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
     """Every projector kernel choice (LDS-resident image at 4, 2, 1 channels per workgroup; geometry tables;
@@ -160,6 +160,11 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["fwd_sweep"] = {"sweep": 4, "sweep2": 2, "sweep32": 2 + 16, "sweep32h": 2 + 80,
                                          "sweep64": 2 + 96, "sweep64_1": 1 + 96}[kernel]
         op.radon_variant["bwd_kernel"] = "table"
+    elif kernel.startswith("psweep"):            # forward by line sweep with two rays per thread
+        op.radon_variant["fwd_kernel"] = "psweep"
+        op.radon_variant["fwd_psweep"] = int(kernel[-1])
+        op.radon_variant["bwd_kernel"] = "table"
+        assert op._tables_full["sweep_grp_p"][0, 0] >= 0
     elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
         op.radon_variant["bwd"] = 8 + 64 * 4
     elif kernel == "table_tile8":                # backprojection over 8x8 tiles, 4 angles per pass

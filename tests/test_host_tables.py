@@ -134,3 +134,63 @@ def test_sweep_geometry_reproduces_the_table(shape):
     np.testing.assert_array_equal(np.sort(grp[grp >= 0]), idx)
     for row, c in zip(grp, gcls):
         assert np.all(cls[row[row >= 0]] == c)
+
+
+@pytest.mark.parametrize("shape", [(14, 12, 17, (0, 360)), (16, 16, 17, (0, 180)), (99, 99, 103, (0, 180))])
+def test_ray_pair_sweep_rule_reproduces_the_table(shape):
+    """The ray-pair sweep's rule, emulated here with the kernel's arithmetic (float64 line geometry: the left ray's
+    mid m1 L + mL0, p0 = ceil(mid - hw/|coef|) clamped to -2 .. len - 2; float32 weights clamp(hs - |ds (u + j)|, 0,
+    ts) for the left ray's candidates p0 .. p0 + 2 and clamp(hs - |ds (u + j - 1/|coef|)|, 0, ts) for the right
+    ray's p0 + 1 .. p0 + 3, u = p0 - mid), gives every ray of every angle the table's pixels and weights (pixels
+    outside the image carry zero values in the kernel's padded lines); the pair groups hold each angle once, within
+    512 threads."""
+    nx, ny, my, ar = shape
+    n_om = 16
+    op = O.OracleOperator(cfg(nx=nx, ny=ny, my=my, n_omega=n_om, angle_range=ar, cor=0.7), (4.05,) * 3 + (90,) * 3,
+                          "cubic", AL3, 0.35, orientations(2, 1), np.deg2rad(4.0), orthonormalize=False)
+    idx = np.arange(n_om)
+    g = _geo(op, "trapezoid")
+    tab = radon_tables(g, idx, nx, ny, my, op.scale)
+    rp, geo = tab["row_ptr"], tab["sweep_geo"]
+    cls = (np.abs(g["X"][idx]) < np.abs(g["Y"][idx])).astype(int)
+    f32 = np.float32
+    for a in range(n_om):
+        X, Y, T0, hw, top, scale = geo[a]
+        coef, cross, ln, nl = (Y, X, ny, nx) if cls[a] else (X, Y, nx, ny)
+        inv = 1.0 / coef
+        ainv, hinv = abs(inv), hw * abs(inv)
+        hs, ts, ds = f32(hw * scale), f32(top * scale), f32(coef * scale)
+        c1, c2 = ds, f32(2) * ds
+        cr = [f32((j - ainv) * ds) for j in (1.0, 2.0, 3.0)]
+        got = np.zeros((my, nx * ny))
+        for sp in range(-(-my // 2)):
+            s = 2 * sp
+            m1 = -cross * inv
+            mL0 = -(T0 - s) * inv + min(inv, 0.0)
+            bl, br = (s, s + 1) if inv > 0 else (s + 1, s)
+            for L in range(nl):
+                mid = m1 * L + mL0
+                p0 = min(max(np.ceil(mid - hinv), -2.0), float(ln - 2))
+                uu = f32(p0 - mid)
+                wl = [np.clip(hs - abs(ds * uu), 0, ts), np.clip(hs - abs(f32(ds * uu + c1)), 0, ts),
+                      np.clip(hs - abs(f32(ds * uu + c2)), 0, ts)]
+                wr = [np.clip(hs - abs(f32(ds * uu + cr[j])), 0, ts) for j in range(3)]
+                for j in range(4):
+                    p = int(p0) + j
+                    if not 0 <= p < ln:
+                        continue
+                    pix = p * nx + L if cls[a] else L * nx + p
+                    if j <= 2 and bl < my:
+                        got[bl, pix] += wl[j]
+                    if j >= 1 and br < my:
+                        got[br, pix] += wr[j - 1]
+        ref = np.zeros((my, nx * ny))
+        for s in range(my):
+            r = a * my + s
+            ref[s, tab["pix"][rp[r]:rp[r + 1]]] = tab["w"][rp[r]:rp[r + 1]]
+        np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-6 * float(ref.max()))
+    grp = tab["sweep_grp_p"]
+    assert grp[0, 0] >= 0 and grp.shape[1] * (-(-my // 2)) <= 512
+    np.testing.assert_array_equal(np.sort(grp[grp >= 0]), idx)
+    for row, c in zip(grp, tab["sweep_gcls_p"]):
+        assert np.all(cls[row[row >= 0]] == c)
