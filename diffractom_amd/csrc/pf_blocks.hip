@@ -196,11 +196,13 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
     srow[i] = p.src + (src_rows + (sok[i] ? y : 0)) * src_stride;
   }
   const int scol = ((tid & 63) >> 4) * 8;                    // the same for every slot of this thread
-  float raw[NS][8];
-  auto load_raw = [&](int c) {
+  // the ring of column j (adjoint fp16 mode): (c * 32 + scol) % P once per chunk, then stepped
+  auto load_raw = [&](int c, float (&raw)[NS][8]) {
+    const int j0 = c * 32 + scol;
+    int jr = 0;
+    if (F16 && !FWD) jr = j0 % p.P;
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
-      const int j0 = c * 32 + scol;
       if (sok[i] && vec && j0 + 8 <= src_n) {
         const float4 u = *reinterpret_cast<const float4*>(srow[i] + j0);
         const float4 v = *reinterpret_cast<const float4*>(srow[i] + j0 + 4);
@@ -211,16 +213,17 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
         for (int e = 0; e < 8; ++e) raw[i][e] = (sok[i] && j0 + e < src_n) ? srow[i][j0 + e] : 0.f;
       }
       if (F16 && !FWD) {
-        // the ring scale of the PF folded into the data: column j = c * 32 + scol + e, ring j % P
+        // the ring scale of the PF folded into the data: column j = j0 + e, ring j % P
+        int q = jr;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-          const int j = j0 + e;
-          if (j < src_n) raw[i][e] *= p.ring_inv[j % p.P];
+          raw[i][e] *= p.ring_inv[q];
+          q = q + 1 == p.P ? 0 : q + 1;
         }
       }
     }
   };
-  auto store_lds = [&](int buf) {
+  auto store_lds = [&](int buf, float (&raw)[NS][8]) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
       const int s = tid + THREADS * i;
@@ -263,22 +266,27 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   const bool have = cb_live && cb_stored;
   const unsigned* v32 = (const unsigned*)p.val;
   const unsigned short* v16 = (const unsigned short*)p.val;
-  int e0 = 0, e1 = 0, n0 = 0, n1 = 0;                        // entries of the current and the next chunk's block
-  unsigned pe = 0, ve = 0, npe = 0, nve = 0;                  // the first 64 entries (one per lane) in flight
-  auto range = [&](int c, int& lo, int& hi) {
+  // the entries of chunk c's block: its range [lo, hi) and the first ER * 64 (ER per lane), fetched two chunks ahead
+  // (a 32 x 32 block holds 64 entries at 6 % fill; blocks near poles hold several hundred)
+  constexpr int ER = 4;
+  struct Ent {
+    int lo, hi;
+    unsigned pp[ER], vv[ER];
+  };
+  auto fetch = [&](int c, Ent& E) {
+    E.lo = E.hi = 0;
     if (have && c < c1) {
       const int64_t bi = blk_index(c);
-      lo = p.ptr[bi];
-      hi = p.ptr[bi + 1];
-    } else {
-      lo = hi = 0;
-    }
-  };
-  auto load_entries = [&](int lo, int hi, unsigned& pp, unsigned& vv) {
-    const int e = lo + lane;
-    if (e < hi) {
-      pp = p.pos[e];
-      vv = F16 ? (unsigned)v16[e] : v32[e];
+      E.lo = p.ptr[bi];
+      E.hi = p.ptr[bi + 1];
+#pragma unroll
+      for (int u = 0; u < ER; ++u) {
+        const int e = E.lo + 64 * u + lane;
+        if (e < E.hi) {
+          E.pp[u] = p.pos[e];
+          E.vv[u] = F16 ? (unsigned)v16[e] : v32[e];
+        }
+      }
     }
   };
   short* bf = reinterpret_cast<short*>(&b_frag[wv][0][0][0]);   // the element type of the s16x8 fragments
@@ -293,77 +301,80 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   for (int mt = 0; mt < MT_MAX; ++mt) acc[mt][0] = acc[mt][1] = f32x4{0.f, 0.f, 0.f, 0.f};
   const s16x8 zero8 = {};
 
-  load_raw(c0);
-  range(c0, e0, e1);
-  load_entries(e0, e1, pe, ve);
-  range(c0 + 1, n0, n1);
-  store_lds(0);
-  __syncthreads();
-  int cur = 0;
-  for (int c = c0; c < c1; ++c) {
-    const bool more = c + 1 < c1;
-    if (more) {
-      load_raw(c + 1);
-      load_entries(n0, n1, npe, nve);
+  // chunk c: expand its block into the wave's fragments (in-order LDS accesses of one wave: the clear, the scatter
+  // and the read-back need no barrier) and multiply with the y-side operand in LDS buffer buf
+  auto compute = [&](int buf, const Ent& E) {
+    if (!have) return;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      b_frag[wv][0][t][lane] = zero8;
+      if (RES) b_frag[wv][RES ? 1 : 0][t][lane] = zero8;
     }
-    // expand this chunk's block into the wave's fragments (in-order LDS accesses of one wave: the clear, the
-    // scatter and the read-back need no barrier)
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < ER; ++u)
+      if (E.lo + 64 * u + lane < E.hi) scatter(E.pp[u], E.vv[u]);
+    for (int e = E.lo + 64 * ER + lane; e - lane < E.hi; e += 64) {  // blocks of more than 64 ER entries
+      if (e < E.hi) scatter(p.pos[e], F16 ? (unsigned)v16[e] : v32[e]);
+    }
+    __builtin_amdgcn_wave_barrier();
     s16x8 bh[2], bl[2];
-    if (have) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        b_frag[wv][0][t][lane] = zero8;
-        if (RES) b_frag[wv][RES ? 1 : 0][t][lane] = zero8;
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (e0 + lane < e1) scatter(pe, ve);
-      for (int e = e0 + 64 + lane; e - lane < e1; e += 64) {  // blocks of more than 64 entries
-        if (e < e1) scatter(p.pos[e], F16 ? (unsigned)v16[e] : v32[e]);
-      }
-      __builtin_amdgcn_wave_barrier();
+    for (int t = 0; t < 2; ++t) {
+      bh[t] = b_frag[wv][0][t][lane];
+      bl[t] = RES ? b_frag[wv][RES ? 1 : 0][t][lane] : zero8;
+    }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        bh[t] = b_frag[wv][0][t][lane];
-        bl[t] = RES ? b_frag[wv][RES ? 1 : 0][t][lane] : zero8;
-      }
+    for (int mt = 0; mt < MT_MAX; ++mt) {
+      if (mt < p.mt_n) {
+        const s16x8 ah = a_hi[buf * MT_MAX * 64 + mt * 64 + lane];
+        if (F16) {
+          const float4 iv = *reinterpret_cast<const float4*>(&a_inv[buf * MT_MAX * 16 + mt * 16 + (lane >> 4) * 4]);
 #pragma unroll
-      for (int mt = 0; mt < MT_MAX; ++mt) {
-        if (mt < p.mt_n) {
-          const s16x8 ah = a_hi[cur * MT_MAX * 64 + mt * 64 + lane];
-          if (F16) {
-            const float4 iv = *reinterpret_cast<const float4*>(&a_inv[cur * MT_MAX * 16 + mt * 16 + (lane >> 4) * 4]);
+          for (int t = 0; t < 2; ++t) {
+            const f32x4 d = mma_f16(ah, bh[t], f32x4{0.f, 0.f, 0.f, 0.f});
+            acc[mt][t][0] += d[0] * iv.x;
+            acc[mt][t][1] += d[1] * iv.y;
+            acc[mt][t][2] += d[2] * iv.z;
+            acc[mt][t][3] += d[3] * iv.w;
+          }
+          __builtin_amdgcn_sched_barrier(0);   // one row tile's products at a time (register pressure)
+        } else {
+          const s16x8 al = RES ? a_lo[buf * MT_MAX * 64 + mt * 64 + lane] : zero8;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-              const f32x4 d = mma_f16(ah, bh[t], f32x4{0.f, 0.f, 0.f, 0.f});
-              acc[mt][t][0] += d[0] * iv.x;
-              acc[mt][t][1] += d[1] * iv.y;
-              acc[mt][t][2] += d[2] * iv.z;
-              acc[mt][t][3] += d[3] * iv.w;
-            }
-          } else {
-            const s16x8 al = RES ? a_lo[cur * MT_MAX * 64 + mt * 64 + lane] : zero8;
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-              acc[mt][t] = mma_bf16(ah, bh[t], acc[mt][t]);
-              if (RES) {
-                acc[mt][t] = mma_bf16(ah, bl[t], acc[mt][t]);
-                acc[mt][t] = mma_bf16(al, bh[t], acc[mt][t]);
-              }
+          for (int t = 0; t < 2; ++t) {
+            acc[mt][t] = mma_bf16(ah, bh[t], acc[mt][t]);
+            if (RES) {
+              acc[mt][t] = mma_bf16(ah, bl[t], acc[mt][t]);
+              acc[mt][t] = mma_bf16(al, bh[t], acc[mt][t]);
             }
           }
         }
       }
     }
-    if (more) {
-      store_lds(cur ^ 1);
-      e0 = n0;
-      e1 = n1;
-      pe = npe;
-      ve = nve;
-      range(c + 2, n0, n1);
-    }
+  };
+
+  // Software pipeline, two chunks deep: at chunk c the y-side rows of chunk c + 2 and the entries of its block are
+  // fetched, chunk c is computed, and the rows of chunk c + 1 (fetched at chunk c - 1) go to the other LDS buffer.
+  // The loop is unrolled by two so that the register sets alternate roles instead of being copied.
+  float rawA[NS][8], rawB[NS][8];
+  Ent E0, E1;
+  load_raw(c0, rawA);
+  store_lds(0, rawA);
+  if (c0 + 1 < c1) load_raw(c0 + 1, rawA);
+  fetch(c0, E0);
+  fetch(c0 + 1, E1);
+  __syncthreads();
+  auto step = [&](int c, int buf, float (&rin)[NS][8], float (&rout)[NS][8], Ent& E) {
+    if (c + 2 < c1) load_raw(c + 2, rout);
+    compute(buf, E);
+    fetch(c + 2, E);
+    if (c + 1 < c1) store_lds(buf ^ 1, rin);
     __syncthreads();
-    cur ^= 1;
+  };
+  for (int c = c0; c < c1; c += 2) {
+    step(c, 0, rawA, rawB, E0);
+    if (c + 1 < c1) step(c + 1, 1, rawB, rawA, E1);
   }
 
   if (!cb_live) return;
@@ -398,14 +409,14 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
 }
 
 template <int MODE, bool FWD>
-__global__ __launch_bounds__(THREADS, 2) void blk_pf_kernel(BlkArgs p) {
+__global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_kernel(BlkArgs p) {
   blk_pf_body<MODE, FWD>(p, xcd_block());
 }
 
 // Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (the slices of a multi-slice batch,
 // each its own PF blocks, K, sinogram columns and data rows); entries with x < 0 pad the grid.
 template <int MODE, bool FWD>
-__global__ __launch_bounds__(THREADS, 2) void blk_pf_grouped_kernel(const BlkArgs* __restrict__ descs,
+__global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_grouped_kernel(const BlkArgs* __restrict__ descs,
                                                                     const int2* __restrict__ wg_map, int64_t n_map) {
   const int64_t w = xcd_block();
   if (w >= n_map) return;
