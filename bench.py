@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of host-core baseline steps")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--c2-precision", default="fp16", choices=["fp16", "bf16_residual", "fp32"],
+                    help="ti_c2: the PF product (fp16 / bf16_residual: generated per call on the matrix cores; "
+                         "fp32: stored CSR)")
     ap.add_argument("--precision", default="bf16_residual", choices=["bf16", "bf16_residual", "fp16", "fp32"],
                     help="PF product of the headline figure (the other precisions run too, for the comparison)")
     ap.add_argument("--only", action="store_true", help="run the headline precision only")
@@ -376,18 +379,24 @@ def run_c3(args):
 
 
 def run_c2(args):
-    """The large hexagonal workload (Ti, 256 x 256 px, 360 omega, K = 59,440, FP32 CSR PF of about 4 G non-zeros):
-    full forward / adjoint times and GB/s, then SPDHG subset steps per second. One JSON line."""
+    """The large hexagonal workload (Ti, 256 x 256 px, 360 omega, K = 59,440): the PF generated per call in fp16
+    blocks on the matrix cores (--c2-precision fp16, the configuration's own form; bf16_residual possible) or stored
+    as an FP32 CSR of about 4 G non-zeros (--c2-precision fp32). Full forward / adjoint times and GB/s, SPDHG subset
+    steps per second with per-kernel times of the timed steps, the PF products' MFMA figures, and the forward rows /
+    block adjoint against the FP64 oracle on a sample (2 angles x 256 orientations). One JSON line."""
     import torch
     from diffractom_amd import SPDHGPoisson
-    from diffractom_amd.workloads import c2_problem
+    from diffractom_amd.workloads import TI_RINGS, c2_problem
+    q = args.c2_precision
     t0 = time.perf_counter()
-    op, xk, b, p = c2_problem()
+    op, xk, b, p = c2_problem(pf_mode="sparse" if q == "fp32" else "generated", pf_precision=q)
     torch.cuda.synchronize()
-    log(f"[bench] ti_c2: K={op.K} nnz={op.sparse_nnz} fill={op.sparse_fill:.4f} batches={len(op.batches)}, "
-        f"set-up {time.perf_counter() - t0:.1f} s, {torch.cuda.memory_allocated() / 1e9:.0f} GB")
+    setup = time.perf_counter() - t0
+    log(f"[bench] ti_c2 {q}: K={op.K} nnz={op.sparse_nnz} fill={op.sparse_fill:.4f} batches={len(op.batches)}, "
+        f"PF {op.pf_mode} ({op.pf_storage}), {op.device_bytes() / 1e9:.1f} GB, set-up {setup:.1f} s")
     npix = op.Nx * op.Ny
-    op.set_omega_subsets(p["subsets"])
+    M = p["subsets"]
+    op.set_omega_subsets(M)
     if not args.no_autotune:
         tuning = op.autotune(subset=0, reps=3)
         log("[bench] ti_c2: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
@@ -404,34 +413,90 @@ def run_c2(args):
         torch.cuda.synchronize()
         tf.append(a0.elapsed_time(a1))
         ta.append(a1.elapsed_time(a2))
-    del out
-    fbytes = 4.0 * op.K * npix + 4.0 * op.N_Omega * op.My * op.N_seg + 6.0 * op.sparse_nnz
+    del out, data
+    # algorithmic bytes of a full product: the image and the sinogram once, plus the stored CSR's 6 B per non-zero
+    # (4 B value + 2 B column); the generated blocks are made and used within the call (written and read back once
+    # from the scratch, 4 B / 6 B per non-zero, reported as entry traffic, not as algorithmic bytes)
+    fbytes = 4.0 * op.K * npix + 4.0 * op.N_Omega * op.My * op.N_seg + (6.0 * op.sparse_nnz if q == "fp32" else 0.0)
     log(f"[bench] ti_c2: forward {min(tf):.1f} ms, adjoint {min(ta):.1f} ms")
     del xk
     torch.cuda.empty_cache()
-    solver = SPDHGPoisson(op, subsets=p["subsets"], seed=0)
+    fp64 = c2_oracle_errors(op, b) if not args.no_cpu_baseline else None
+    solver = SPDHGPoisson(op, subsets=M, seed=0)
     x0 = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
     solver.start(x0, b)
     torch.cuda.synchronize()
     solver.advance(args.warmup)
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    solver.advance(args.steps)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    line = {"metric": "SPDHG-Poisson iters/s (subset steps/s), FP32 CSR path, hexagonal Ti",
+    ms, kt = timed_steps(solver, args.steps, per_kernel=not args.no_kernel_timing)
+    log(f"[bench] ti_c2: {ms:.2f} ms/step; kernel ms: " + ", ".join(f"{k}={v:.3f}" for k, v in (kt or {}).items()))
+    mf = mfma_figures(op, M, kt) if kt and op.gen_batches is None else None
+    if kt and op.gen_batches is not None:
+        # the generated blocks: MFMA issued per product (3 passes with the residual, 1 otherwise)
+        passes = 3 if q.endswith("residual") else 1
+        eb = 4.0 if q == "fp16" else 6.0             # 2 B position + 2 B fp16 value, or a bf16 hi/lo pair
+        Rm = op.N_Omega / M
+        issued = 2.0 * Rm * (-(-op.My // 16) * 16) * (-(-op.N_seg // 32) * 32) * \
+            sum(-(-bt["K_batch"] // 32) * 32 for bt in op.batches) * passes
+        mf = {k: {"mfma_TFLOPs_issued": round(issued / (kt[k] * 1e-3) / 1e12, 2),
+                  "mfma_frac_issued": round(issued / (kt[k] * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                  "entry_bytes": eb * op.sparse_nnz / M,
+                  "entry_GBps": round(eb * op.sparse_nnz / M / (kt[k] * 1e-3) / 1e9, 1)}
+              for k in ("pf_forward", "pf_adjoint") if k in kt}
+    desc = "FP32 CSR PF" if q == "fp32" else f"PF generated per call, {q} blocks on the matrix cores"
+    line = {"metric": f"SPDHG-Poisson iters/s (subset steps/s), hexagonal Ti, {desc}",
             "value": round(1000.0 / ms, 4), "unit": "SPDHG subset steps/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32", "data": "synthetic (300 Voronoi grains + 10 % fibre background)",
+            "vs_baseline": None, "dtype": DTYPE[q], "data": "synthetic (300 Voronoi grains + 10 % fibre background)",
             "config": {"workload": f"ti_c2: Ti 8 rings, 360 omega x 261 x 360 eta, 256x256 px, K={op.K}, sigma 2.5 "
-                                   f"deg, M={p['subsets']}, FP32 CSR PF (nnz {op.sparse_nnz})",
-                       "K": op.K, "nnz": op.sparse_nnz},
+                                   f"deg, M={M}, {desc} (nnz {op.sparse_nnz})",
+                       "K": op.K, "nnz": op.sparse_nnz, "precision": q, "pf_mode": op.pf_mode},
+            "setup_s": round(setup, 1), "device_GB": round(op.device_bytes() / 1e9, 2),
             "forward_ms": round(min(tf), 2), "adjoint_ms": round(min(ta), 2),
             "forward_GBps": round(fbytes / (min(tf) * 1e-3) / 1e9, 1),
-            "adjoint_GBps": round(fbytes / (min(ta) * 1e-3) / 1e9, 1)}
+            "adjoint_GBps": round(fbytes / (min(ta) * 1e-3) / 1e9, 1),
+            "kernel_ms": {k: round(v, 4) for k, v in (kt or {}).items()}, "matrix_cores": mf,
+            "rel_err_vs_fp64_oracle": fp64}
     print(json.dumps(line), flush=True)
+
+
+def c2_oracle_errors(op, b):
+    """Relative L2 errors of the C2 operator's forward rows (angles 10 and 100) for coefficients on 256 orientations
+    straddling the first two orientation batches, and of the block adjoint of the counts on those rows, against the
+    FP64 oracle (oracle/model.py)."""
+    import torch
+    try:
+        from diffractom_amd.workloads import TI_RINGS
+        from oracle.model import OracleOperator
+        kb0 = op.batches[0]["K_batch"]
+        ks = np.arange(max(kb0 - 128, 0), min(kb0 + 128, op.K))
+        rows = np.array([10, 100])
+        orc = OracleOperator(op.cfg, (2.95, 2.95, 4.68, 90, 90, 120), "hexagonal", TI_RINGS, 0.2,
+                             op.grid_rot_cpu[ks], float(op.sigma_cpu[0]), orthonormalize=False)
+        xb = np.random.default_rng(3).random((len(ks), op.Ny, op.Nx)) * op.support_mask()[None]
+        ref_f = orc.direct_rows(xb, rows)
+        orc.set_omega_subsets([rows])
+        yb = np.zeros(op.data_shape)
+        rt = torch.from_numpy(rows).cuda()
+        yb[rows] = b[rt].double().cpu().numpy()
+        ref_a = orc.adjoint(yb, subset=0)
+        x = torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda")
+        x[int(ks[0]):int(ks[-1]) + 1] = torch.from_numpy(xb.astype(np.float32)).cuda()
+        fwd = op.direct(x)[rt].double().cpu().numpy()
+        del x
+        y = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
+        y[rt] = b[rt]
+        adj = op.adjoint(y)[int(ks[0]):int(ks[-1]) + 1].double().cpu().numpy()
+        del y
+        torch.cuda.empty_cache()
+        e = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+        out = {"forward": e(fwd, ref_f), "adjoint": e(adj, ref_a),
+               "sample": f"angle rows {rows.tolist()} x orientations {int(ks[0])}..{int(ks[-1])}"}
+        log(f"[bench] ti_c2: FP64 oracle sample: forward {out['forward']:.2e}, adjoint {out['adjoint']:.2e}")
+        return out
+    except Exception as exc:
+        log(f"[bench] ti_c2: FP64 oracle sample failed: {exc!r}")
+        return None
 
 
 def run_osem(args):

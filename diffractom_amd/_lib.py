@@ -85,6 +85,8 @@ SIGNATURES = {
     "dfm_pfblk_adjoint": [P, P, P, P, I32, P, I32, P, I32, I32, I32, I32, P, I32, F32, P],
     "dfm_pfblk_grouped": [I32, P, P, I64, I32, P],
     "dfm_pfblk_desc_size": [],
+    "dfm_pfgen": [I32, P, P, P, P, P, P, P, I32, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, F32,
+                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P, P, I32, P, P],
 }
 
 DFM_ERR_ARG, DFM_ERR_UNSUPPORTED, DFM_ERR_LAUNCH, DFM_ERR_MEMORY = -1, -2, -3, -4

@@ -47,7 +47,7 @@ class _BlkDesc(ctypes.Structure):
                 ("n_cb", ctypes.c_int32), ("n_ch", ctypes.c_int32), ("mt_n", ctypes.c_int32),
                 ("n_yg", ctypes.c_int32), ("n_tg", ctypes.c_int32), ("ksplit", ctypes.c_int32),
                 ("ch_per", ctypes.c_int32), ("direct", ctypes.c_int32), ("accumulate", ctypes.c_int32),
-                ("alpha", ctypes.c_float), ("ncols", ctypes.c_int32)]
+                ("alpha", ctypes.c_float), ("ncols", ctypes.c_int32), ("blk_local", ctypes.c_int32)]
 
 
 def _y_tiling(my):

@@ -41,6 +41,43 @@ __device__ __forceinline__ void st_stream(float4* p, const float4& v) {
   __builtin_nontemporal_store(t, reinterpret_cast<f4_t*>(p));
 }
 
+// One pole-figure entry PF[r, kg, c, p] (create_pfo_matrix.py:108-199, pf_generate.cl:62-140 pf_entry): the sum over
+// the quadrature nodes (s, se) of the angle and eta bin (weights qw, or the mean of the nodes) of the ring's pole axes'
+// images count_a g(|n_{kg a} . v|), g(d) = exp(-(1 - d) / sigma^2) + exp(-(1 + d) / sigma^2), each term dropped
+// where its exponent reaches cut, times 1 / (8 pi sigma^2). Directions (R, S, C, Se, P, 3) and poles (K, A, 3) are
+// float64 and so is the arithmetic: near a pole 1 - |n.v| is a difference of nearly equal numbers and the kernel
+// amplifies its error by 1/sigma^2 (3300 at sigma = 1 deg), so float32 inputs alone would cost ~1e-4 of relative
+// accuracy. gfx950 runs float64 at full rate.
+__device__ __forceinline__ float pf_entry(const double* __restrict__ coords, const double* __restrict__ poles,
+                                          const int* __restrict__ axis_start, const float* __restrict__ axis_count,
+                                          const double* __restrict__ inv_sigma2, const double* __restrict__ normf,
+                                          const float* __restrict__ qw, int r, int kg, int c, int p, int C, int P,
+                                          int A, int S, int Se, double cut, int quad_weighted) {
+  const double is2 = inv_sigma2[kg];
+  const int a0 = axis_start[p], a1 = axis_start[p + 1];
+  const double* pk = poles + (int64_t)kg * A * 3;
+  double total_w = 0.0;
+  for (int s = 0; s < S; ++s) {
+    for (int se = 0; se < Se; ++se) {
+      const double* v = coords + (((((int64_t)r * S + s) * C + c) * Se + se) * P + p) * 3;
+      const double vx = v[0], vy = v[1], vz = v[2];
+      double node = 0.0;
+      for (int a = a0; a < a1; ++a) {
+        const double d = fabs(pk[a * 3 + 0] * vx + pk[a * 3 + 1] * vy + pk[a * 3 + 2] * vz);
+        double term = 0.0;
+        const double x1 = (1.0 - d) * is2;
+        if (x1 < cut) term += exp(-x1);
+        const double x2 = (1.0 + d) * is2;
+        if (x2 < cut) term += exp(-x2);
+        node += (double)axis_count[a] * term;
+      }
+      total_w += quad_weighted ? (double)qw[s * Se + se] * node : node;
+    }
+  }
+  if (!quad_weighted) total_w = total_w / (double)(S * Se);
+  return (float)(total_w * normf[kg]);
+}
+
 // data rows omegas[a] (row_len values each) (+)= sum over ks < ksplit of part[ks][a] (fixed order).
 int launch_splitk_reduce(const float* part, float* data, const int* omegas, int n_ang, int64_t row_len, int ksplit,
                          int accumulate, hipStream_t st);

@@ -710,35 +710,9 @@ __global__ void __launch_bounds__(256) pf_eval_kernel(const double* __restrict__
     t /= C;
     const int k = (int)(t % kmax);
     const int r = (int)(t / kmax);
-    if (k >= kb) {
-      out[gid] = 0.0f;
-      continue;
-    }
-    const int kg = k0 + k;
-    const double is2 = inv_sigma2[kg];
-    const double cut = (double)pf_cut;
-    const int a0 = axis_start[p], a1 = axis_start[p + 1];
-    const double* pk = poles + (int64_t)kg * A * 3;
-    double total_w = 0.0;
-    for (int s = 0; s < S; ++s) {
-      for (int se = 0; se < Se; ++se) {
-        const double* v = coords + (((((int64_t)r * S + s) * C + c) * Se + se) * P + p) * 3;
-        const double vx = v[0], vy = v[1], vz = v[2];
-        double node = 0.0;
-        for (int a = a0; a < a1; ++a) {
-          const double d = fabs(pk[a * 3 + 0] * vx + pk[a * 3 + 1] * vy + pk[a * 3 + 2] * vz);
-          double term = 0.0;
-          const double x1 = (1.0 - d) * is2;
-          if (x1 < cut) term += exp(-x1);
-          const double x2 = (1.0 + d) * is2;
-          if (x2 < cut) term += exp(-x2);
-          node += (double)axis_count[a] * term;
-        }
-        total_w += quad_weighted ? (double)qw[s * Se + se] * node : node;
-      }
-    }
-    if (!quad_weighted) total_w = total_w / (double)(S * Se);
-    out[gid] = (float)(total_w * normf[kg]);
+    out[gid] = k >= kb ? 0.0f
+                       : pf_entry(coords, poles, axis_start, axis_count, inv_sigma2, normf, qw, r, k0 + k, c, p, C, P,
+                                  A, S, Se, (double)pf_cut, quad_weighted);
   }
 }
 

@@ -135,6 +135,7 @@ struct BlkArgs {
   int direct, accumulate;
   float alpha;
   int ncols;                // adjoint: sinogram columns written
+  int blk_local;            // blocks indexed by the position a in the angle list (generated PF) instead of omegas[a]
 };
 
 static_assert(sizeof(BlkArgs) == sizeof(dfm_pfblk_desc), "BlkArgs and dfm_pfblk_desc differ");
@@ -261,7 +262,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   // ---- this wave's PF blocks: block (r, kb, jb) with kb = chunk (forward) or the column block (adjoint)
   auto blk_index = [&](int c) -> int64_t {
     const int kbi = FWD ? c : cb, jbi = FWD ? cb : c;
-    return ((int64_t)r * p.n_kb + kbi) * p.n_jb + jbi;
+    return ((int64_t)(p.blk_local ? a : r) * p.n_kb + kbi) * p.n_jb + jbi;
   };
   const bool have = cb_live && cb_stored;
   const unsigned* v32 = (const unsigned*)p.val;
@@ -488,11 +489,14 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
                       int32_t n_ang, int32_t my, int32_t cp, float* data, float* part, int32_t ksplit,
                       int32_t accumulate, dfm_stream_t stream) {
   DFM_CHECK_ARG(sino && ptr && pos && val && data, "dfm_pfblk_forward: null pointer");
+  const int local = (mode >> 4) & 1;
+  mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_forward: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
   DFM_CHECK_ARG(n_ang > 0 && my > 0 && cp > 0 && kb > 0 && sks >= kb && (sks & 3) == 0,
                 "dfm_pfblk_forward: bad sizes (n_ang %d, My %d, CP %d, Kb %d, stride %d)", n_ang, my, cp, kb, sks);
   BlkArgs p{};
+  p.blk_local = local;
   p.n_kb = (kb + 31) / 32;
   p.n_jb = (cp + 31) / 32;
   DFM_CHECK_ARG(ksplit >= 1 && ksplit <= p.n_kb, "dfm_pfblk_forward: k split %d not in 1 .. %d", ksplit, p.n_kb);
@@ -530,11 +534,14 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
                       const float* ring_inv, int32_t P, const int32_t* omegas, int32_t n_ang, int32_t my, int32_t cp,
                       int32_t kb, float* sino, int32_t sks, float alpha, dfm_stream_t stream) {
   DFM_CHECK_ARG(data && ptr && pos && val && sino, "dfm_pfblk_adjoint: null pointer");
+  const int local = (mode >> 4) & 1;
+  mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_adjoint: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
   DFM_CHECK_ARG(n_ang > 0 && my > 0 && cp > 0 && kb > 0 && sks >= kb && (sks & 3) == 0,
                 "dfm_pfblk_adjoint: bad sizes (n_ang %d, My %d, CP %d, Kb %d, stride %d)", n_ang, my, cp, kb, sks);
   BlkArgs p{};
+  p.blk_local = local;
   p.n_kb = (kb + 31) / 32;
   p.n_jb = (cp + 31) / 32;
   p.src = data;
@@ -566,6 +573,7 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
 int dfm_pfblk_grouped(int32_t forward, const void* descs, const int32_t* wg_map, int64_t n_wg, int32_t mode,
                       dfm_stream_t stream) {
   DFM_CHECK_ARG(descs && wg_map && n_wg > 0, "dfm_pfblk_grouped: null pointer or no workgroups");
+  mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16, "dfm_pfblk_grouped: mode %d", mode);
   DFM_CHECK_ARG(n_wg <= ((int64_t)1 << 31) - 8, "dfm_pfblk_grouped: %lld workgroups", (long long)n_wg);
   if (forward)

@@ -12,6 +12,7 @@ the whole run.
 """
 from __future__ import annotations
 
+import ctypes
 import time
 from typing import Any, Mapping
 
@@ -260,12 +261,10 @@ class SinglePhaseForwardOperator:
         # matrix cores), "bf16" (bf16 operands), "bf16_residual" (plus the hi/lo residual correction, ~2^-16
         # relative), "fp16" (fp16 operands with power-of-two scales per ring and per row chunk, ~2^-11 relative).
         # pf_mode "sparse" / "auto" keeps the PF as 32 x 32 blocks of its non-zeros (one copy for both directions),
-        # "dense" as dense bf16 fragments (one copy per direction); "generated" is for fp32 only.
+        # "dense" as dense bf16 fragments (one copy per direction); "generated" evaluates the blocks of each call's
+        # angles on the fly from the candidate eta windows (dfm_pfgen), nothing stored.
         if pf_precision not in ("fp32", "bf16", "bf16_residual", "fp16"):
             raise ValueError(f"pf_precision must be 'fp32', 'bf16', 'bf16_residual' or 'fp16', not {pf_precision!r}")
-        if pf_precision != "fp32" and pf_mode == "generated":
-            raise NotImplementedError(f"pf_precision {pf_precision!r} needs a stored PF (pf_mode 'sparse', 'auto' or "
-                                      f"'dense'); the generated mode runs the FP32 products")
         if pf_precision == "fp16" and pf_mode == "dense":
             raise NotImplementedError("pf_precision 'fp16' runs on the block-sparse PF (pf_mode 'sparse' or 'auto')")
         self.pf_precision = pf_precision
@@ -312,6 +311,7 @@ class SinglePhaseForwardOperator:
         self.dense_batches = None
         self.mfma_batches = None
         self.blk_batches = None
+        self.gen_batches = None
         self.pf_storage = None
         self.sparse_nnz = 0
         self.sparse_fill = 0.0
@@ -529,6 +529,10 @@ class SinglePhaseForwardOperator:
             self.pf_mode = "dense"
             self._store_dense()
             return
+        if mode == "generated" and self.pf_precision != "fp32":
+            self.pf_mode = "generated"
+            self._setup_generated()
+            return
         # fill of the first batch decides "auto" (single_phase_forward_operator.py:1240-1265)
         b0 = self.batches[0]
         dense = self._eval_dense_batch(b0)
@@ -686,6 +690,165 @@ class SinglePhaseForwardOperator:
                   n_ang, self.My, self.N_seg, b["K_batch"], _lib.ptr(sino), b["kstride"], float(alpha),
                   self._stream())
 
+    def _setup_generated(self):
+        """The generated block-sparse PF (dfm_pfgen): the geometry of the candidate windows (omega-node frames R^T d0,
+        R^T d90, R^T p0; sin / cos of half the rings' two-theta; the eta nodes' place in their bin), one count pass
+        over every angle per orientation batch (non-zeros per block (angle, 32 orientations, 32 segments), kept),
+        and for fp16 a power-of-two scale per ring from the entries' bound 2 |G_p| max_k 1 / (8 pi sigma_k^2)
+        (the pole images of a ring, both exponential terms at their peak)."""
+        from scipy.spatial.transform import Rotation
+        R, CP, P = self.N_Omega, self.N_seg, self.N_peaks
+        c = self.cfg
+        e0, e1 = (float(v) for v in self.eta_angle_range)
+        if e1 <= e0:
+            raise NotImplementedError("the generated PF needs an increasing eta range")
+        s2 = np.asarray(self.sigma_cpu, np.float64) ** 2
+        if float(np.max(self.pf_cut * s2)) >= 1.0 - 1e-3:
+            raise NotImplementedError("the generated PF needs cut * sigma^2 < 1 (kernels narrower than about 20 deg)")
+        k0 = np.asarray(c["k_direction_0"], np.float64)
+        Rm = Rotation.from_rotvec(self.angles_subdivided[:, None] * k0).as_matrix()          # (R S, 3, 3)
+        dirs = np.stack([np.asarray(c["detector_direction_origin"], np.float64),
+                         np.asarray(c["detector_direction_positive_90"], np.float64),
+                         np.asarray(c["p_direction_0"], np.float64)])                      # (3, 3)
+        frames = np.einsum("oij,di->odj", Rm, dirs).reshape(R * self.N_Omega_subdivisions, 9)
+        half = self.two_theta_peaks.astype(np.float64) / 2.0
+        deta = (e1 - e0) / self.N_eta
+        x = (np.asarray(self.eta_angles_subdivided[: self.N_eta_subdivisions], np.float64) - e0) / deta
+        dev = self.device
+        g = {"frames": torch.from_numpy(np.ascontiguousarray(frames)).to(dev),
+             "sin_th": torch.from_numpy(np.sin(half)).to(dev), "cos_th": torch.from_numpy(np.cos(half)).to(dev),
+             "eta0": e0, "deta": deta, "xlo": float(x.min()), "xhi": float(x.max())}
+        self._gen_geo = g
+        cnt_axes = np.add.reduceat(self.axis_count_cpu.astype(np.float64), self.axis_start_cpu[:-1])
+        bound = 2.0 * cnt_axes * float(np.max(1.0 / (8.0 * np.pi * s2)))
+        sc = np.ldexp(1.0, 14 - np.ceil(np.log2(bound)).astype(int)).astype(np.float32)
+        all_r = torch.arange(R, dtype=torch.int32, device=dev)
+        self.gen_batches = []
+        nnz = 0
+        for b in self.batches:
+            n_kb, n_jb = -(-b["K_batch"] // 32), -(-CP // 32)
+            counts = torch.empty(R * n_kb * n_jb, dtype=torch.int32, device=dev)
+            self._pfgen(0, b, all_r, R, counts=counts)
+            n = int(counts.sum(dtype=torch.int64).item())
+            nnz += n
+            self.gen_batches.append({"counts": counts.view(R, n_kb * n_jb), "n_kb": n_kb, "n_jb": n_jb, "nnz": n,
+                                     "ring_scale": torch.from_numpy(sc).to(dev),
+                                     "ring_inv": torch.from_numpy((1.0 / sc.astype(np.float64)).astype(np.float32)
+                                                                  ).to(dev),
+                                     "groups": {}})
+        self.pf_storage = {"bf16": "generated per call: bf16 blocks", "bf16_residual": "generated per call: bf16 "
+                           "hi/lo blocks", "fp16": "generated per call: fp16 blocks, ring-scaled"}[self.pf_precision]
+        self.sparse_nnz = nnz
+        self.sparse_fill = nnz / (R * self.K * CP)
+        self._gen_scratch = None
+
+    def _pfgen(self, mode, b, omegas, n_ang, counts=None, ptr=None, pos=None, val=None, ring_scale=None):
+        g = self._gen_geo
+        qw = self.quad_weights_cpu is not None
+        _lib.call("dfm_pfgen", int(mode), _lib.ptr(self.coords_gpu), _lib.ptr(self.poles_gpu),
+                  _lib.ptr(self.axis_start_gpu), _lib.ptr(self.axis_count_gpu), _lib.ptr(self.inv_sigma2_gpu),
+                  _lib.ptr(self.norm_factor_gpu), _lib.ptr(self.quad_weights_gpu), int(qw), _lib.ptr(g["frames"]),
+                  _lib.ptr(g["sin_th"]), _lib.ptr(g["cos_th"]), _lib.ptr(omegas), int(n_ang), b["k_start"],
+                  b["K_batch"], self.N_eta, self.N_peaks, self.N_axes, self.N_Omega_subdivisions,
+                  self.N_eta_subdivisions, float(self.pf_cut), g["eta0"], g["deta"], g["xlo"], g["xhi"],
+                  _lib.ptr(counts), _lib.ptr(ptr), _lib.ptr(pos), _lib.ptr(val),
+                  int(self.pf_precision == "fp16"), _lib.ptr(ring_scale), self._stream())
+
+    def _gen_groups(self, subset):
+        """The angle groups of a generated-PF call: subset m itself, or consecutive chunks of about a tenth of the
+        angles for every angle (each group's PF is generated and used in turn). (key, omegas tensor, first row)."""
+        if subset is not None:
+            m = self._subset(subset)
+            return [(("sub", m, tuple(self.omega_subsets[m][:2]), len(self.omega_subsets[m])),
+                     self._sub_omegas[m], None)]
+        R = self.N_Omega
+        cs = max(1, -(-R // 10))
+        out = []
+        for r0 in range(0, R, cs):
+            r1 = min(R, r0 + cs)
+            key = ("rows", r0, r1)
+            om = getattr(self, "_gen_chunks", {}).get(key)
+            if om is None:
+                om = torch.arange(r0, r1, dtype=torch.int32, device=self.device)
+                self.__dict__.setdefault("_gen_chunks", {})[key] = om
+            out.append((key, om, r0))
+        return out
+
+    def _gen_group(self, ib, key, om):
+        """Block pointers of batch ib for the angle group om (prefix sums of the kept per-block counts; cached)."""
+        gb = self.gen_batches[ib]
+        grp = gb["groups"].get(key)
+        if grp is None:
+            c = gb["counts"].index_select(0, om.long()).reshape(-1)
+            ptr64 = torch.zeros(c.numel() + 1, dtype=torch.int64, device=self.device)
+            torch.cumsum(c, 0, out=ptr64[1:])
+            n = int(ptr64[-1].item())
+            if n >= 2 ** 31:
+                raise ValueError("Too many non-zeros in one generated angle group for int32 indices; lower max_gb.")
+            grp = {"ptr": ptr64.to(torch.int32), "nnz": n}
+            gb["groups"][key] = grp
+        return grp
+
+    def _gen_entry_scratch(self, nnz):
+        """The entry arrays the generation writes and the products read (grown to nnz entries, kept)."""
+        n = max(int(nnz), 1)
+        sc = self._gen_scratch
+        if sc is None or sc["pos"].numel() < n:
+            f16 = self.pf_precision == "fp16"
+            self._gen_scratch = sc = None
+            sc = self._gen_scratch = {
+                "pos": torch.empty(n, dtype=torch.int16, device=self.device),
+                "val": torch.empty(n, dtype=torch.int16 if f16 else torch.int32, device=self.device)}
+            self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
+        return sc
+
+    def _gen_prepare(self, subsets):
+        """Block pointers of every (subset, batch) and the entry scratch for the largest, made when the subsets are
+        set, so that the solver's steps do no host synchronisation or allocation for them."""
+        big = 0
+        for m in subsets:
+            key, om, _ = self._gen_groups(m)[0]
+            for ib in range(len(self.batches)):
+                big = max(big, self._gen_group(ib, key, om)["nnz"])
+        self._gen_entry_scratch(big)
+
+    def _gen_blocks(self, ib, key, om):
+        """Generate the PF blocks of batch ib for the angles om (block pointers cached per angle group) into the
+        scratch entry arrays; returns the block dict the products read (valid until the next generation)."""
+        b, gb = self.batches[ib], self.gen_batches[ib]
+        grp = self._gen_group(ib, key, om)
+        sc = self._gen_entry_scratch(grp["nnz"])
+        self._pfgen(1, b, om, om.numel(), ptr=grp["ptr"], pos=sc["pos"], val=sc["val"],
+                    ring_scale=gb["ring_scale"])
+        self.mark("pf_generate")
+        return {"ptr": grp["ptr"], "pos": sc["pos"], "val": sc["val"], "n_kb": gb["n_kb"], "n_jb": gb["n_jb"],
+                "ring_inv": gb["ring_inv"]}
+
+    def _gen_forward(self, ib, sino, data, subset, accumulate):
+        b = self.batches[ib]
+        row = self.My * b["kstride"]
+        for key, om, r0 in self._gen_groups(subset):
+            blk = self._gen_blocks(ib, key, om)
+            n_ang = om.numel()
+            ks = self._blk_split(n_ang, blk)
+            part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
+            src = _lib.ptr(sino) if r0 is None else ctypes.c_void_p(sino.data_ptr() + 4 * r0 * row)
+            _lib.call("dfm_pfblk_forward", src, b["kstride"], b["K_batch"], _lib.ptr(blk["ptr"]),
+                      _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode() + 16, _lib.ptr(blk["ring_inv"]),
+                      self.N_peaks, _lib.ptr(om), n_ang, self.My, self.N_seg, _lib.ptr(data), _lib.ptr(part), ks,
+                      int(accumulate), self._stream())
+
+    def _gen_adjoint(self, ib, data, sino, subset, alpha):
+        b = self.batches[ib]
+        row = self.My * b["kstride"]
+        for key, om, r0 in self._gen_groups(subset):
+            blk = self._gen_blocks(ib, key, om)
+            dst = _lib.ptr(sino) if r0 is None else ctypes.c_void_p(sino.data_ptr() + 4 * r0 * row)
+            _lib.call("dfm_pfblk_adjoint", _lib.ptr(data), _lib.ptr(blk["ptr"]), _lib.ptr(blk["pos"]),
+                      _lib.ptr(blk["val"]), self._blk_mode() + 16, _lib.ptr(blk["ring_inv"]), self.N_peaks,
+                      _lib.ptr(om), om.numel(), self.My, self.N_seg, b["K_batch"], dst, b["kstride"], float(alpha),
+                      self._stream())
+
     def _count_nnz(self, dense_batches):
         """sparse_nnz / sparse_fill of dense batches (the PF's non-zeros, as the CSR forms would hold them)."""
         self.sparse_nnz = int(sum(int(torch.count_nonzero(d).item()) for d in dense_batches))
@@ -719,7 +882,7 @@ class SinglePhaseForwardOperator:
             n += sum(v.numel() * v.element_size() for v in sb.values() if isinstance(v, torch.Tensor))
         for d in self.dense_batches or []:
             n += d.numel() * 4
-        for mb in (self.mfma_batches or []) + (self.blk_batches or []):
+        for mb in (self.mfma_batches or []) + (self.blk_batches or []) + (self.gen_batches or []):
             n += sum(v.numel() * v.element_size() for v in mb.values() if isinstance(v, torch.Tensor))
         return n
 
@@ -728,6 +891,8 @@ class SinglePhaseForwardOperator:
         self.dense_batches = None
         self.mfma_batches = None
         self.blk_batches = None
+        self.gen_batches = None
+        self._gen_scratch = None
         self._scratch = None
 
     def _scratch_buffers(self, n_angles):
@@ -768,6 +933,8 @@ class SinglePhaseForwardOperator:
         self._sub_geo_fwd = [self.geo_fwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
         self._sub_geo_bwd = [self.geo_bwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
         self._sub_tables = [self._radon_tables(s) for s in out] if self.n_taps == 2 else None
+        if self.gen_batches is not None:
+            self._gen_prepare(range(len(out)))
         return out
 
     def release_omega_subsets(self):
@@ -1077,6 +1244,9 @@ class SinglePhaseForwardOperator:
         if self.blk_batches is not None:
             self._blk_forward(ib, sino, data, subset, accumulate)
             return
+        if self.gen_batches is not None:
+            self._gen_forward(ib, sino, data, subset, accumulate)
+            return
         if self.pf_mode == "dense":
             self._dense_forward(ib, sino, data, subset, accumulate)
             return
@@ -1097,6 +1267,9 @@ class SinglePhaseForwardOperator:
         alpha = float(self.N_Omega / np.pi)
         if self.blk_batches is not None:
             self._blk_adjoint(ib, data, sino, subset, alpha)
+            return
+        if self.gen_batches is not None:
+            self._gen_adjoint(ib, data, sino, subset, alpha)
             return
         if self.pf_mode == "dense":
             self._dense_adjoint(ib, data, sino, subset, alpha)

@@ -175,20 +175,23 @@ C2 = dict(n=256, my=261, n_omega=360, n_eta=360, sigma_deg=2.5, grains=300, fibr
           mean_counts=500.0, seed=2, subsets=10, radius=120.0)
 
 
-def c2_problem(device="cuda", max_gb=20.0):
+def c2_problem(device="cuda", max_gb=20.0, pf_mode="sparse", pf_precision="fp32", counts_from=None):
     """The large hexagonal workload: Ti (8 rings), 360 omega over 180 deg x 261 translations x 360 eta, 256 x 256
-    pixels, a uniform 3-degree hexagonal fundamental-zone grid (59,440 orientations), sigma 2.5 deg, sparse CSR PF
-    (the dense BF16 fragments would need about 250 GB). Truth, in the orientation-fastest layout on the device: 300
-    Voronoi grains in a disc of radius 120 px (coefficients U(0.5, 1.5)) plus a fibre background (orientations with
-    the c axis within 10 deg of the sample z axis) carrying 10 % of the coefficient sum. Returns (operator, truth
-    x_k, counts b as a device tensor, parameters)."""
+    pixels, a uniform 3-degree hexagonal fundamental-zone grid (59,440 orientations), sigma 2.5 deg; the PF stored
+    as FP32 CSR (pf_mode "sparse", about 48 GB for both forms) or generated per call in fp16 blocks (pf_mode
+    "generated", pf_precision "fp16"; the dense BF16 fragments would need about 250 GB). Truth, in the
+    orientation-fastest layout on the device: 300 Voronoi grains in a disc of radius 120 px (coefficients
+    U(0.5, 1.5)) plus a fibre background (orientations with the c axis within 10 deg of the sample z axis) carrying
+    10 % of the coefficient sum. counts_from: an operator of the same grid whose forward projection gives the counts
+    (so that every PF form solves the same data). Returns (operator, truth x_k, counts b as a device tensor,
+    parameters)."""
     import torch
     from .operator import SinglePhaseForwardOperator
     p = C2
     R = uniform_fz(3.0, "hexagonal", seed=p["seed"], thin=0.6)
     cfg = scan_cfg(p["n"], p["my"], p["n_omega"], p["n_eta"])
     op = SinglePhaseForwardOperator(cfg, ti_material(), MatrixGrid(R, np.deg2rad(p["sigma_deg"])), max_gb=max_gb,
-                                    normalized=True, pf_mode="sparse")
+                                    normalized=True, pf_mode=pf_mode, pf_precision=pf_precision)
     K, n = op.K, p["n"]
     rng = np.random.default_rng(p["seed"])
     c = (n - 1) / 2.0
@@ -216,7 +219,7 @@ def c2_problem(device="cuda", max_gb=20.0):
     xk[pt[:, None], fcol[None, :]] += float(bg)
     del kcol
     data = torch.empty(op.data_shape, dtype=torch.float32, device=device)
-    op.direct_k(xk.view(-1), data)
+    (counts_from or op).direct_k(xk.view(-1), data)
     ax = data.double()
     s = p["mean_counts"] / float(ax.mean())
     g = torch.Generator(device=device).manual_seed(p["seed"])

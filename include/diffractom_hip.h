@@ -266,7 +266,8 @@ int dfm_mfma_grouped(int32_t forward, const dfm_mfma_desc* descs, const int32_t*
  * summed in a fixed order. adjoint: sino[a, y, k] = alpha sum_j PF[omegas[a], k, j] data[omegas[a], y, j] for every
  * k < sks (0 at and beyond kb). mode: 0 bf16 (val hi only), 1 bf16 with the hi/lo residual correction (hi*hi + hi*lo
  * + lo*hi), 2 fp16 (val fp16; ring_inv[p] = 1 / ring_scale[p]; the y-side operand scaled by a power of two per row
- * and chunk of 32). FP32 accumulation. */
+ * and chunk of 32); + 16: the blocks are indexed by the position a in the angle list (a generated PF of just these
+ * angles, dfm_pfgen) instead of by omegas[a]. FP32 accumulation. */
 int dfm_pfblk_count(const float* pf, int32_t* counts, int32_t R, int32_t km, int32_t kb, int32_t cp,
                     dfm_stream_t stream);
 int dfm_pfblk_fill(const float* pf, const int32_t* ptr, uint16_t* pos, void* val, int32_t fp16,
@@ -303,11 +304,28 @@ typedef struct dfm_pfblk_desc {
   int32_t direct, accumulate;
   float alpha;
   int32_t ncols;
+  int32_t blk_local;
 } dfm_pfblk_desc;
 int dfm_pfblk_grouped(int32_t forward, const void* descs, const int32_t* wg_map, int64_t n_wg, int32_t mode,
                       dfm_stream_t stream);
 /* sizeof(dfm_pfblk_desc) as compiled into the library (for bindings that build the descriptor array by hand). */
 int dfm_pfblk_desc_size(void);
+/* The PF generated for the angles omegas[0 .. n_ang) of an orientation batch (k0 .. k0 + kb) in the block-sparse
+ * form of dfm_pfblk_*, blocks indexed (a, kb_i, jb_i) (the generated mode, pf_generate.py:218-303 and
+ * pf_generate.cl:146-454: candidates from the eta windows where a pole meets a ring's cone, |n.v| > 1 - cut sigma^2
+ * - 1e-4, then the entries of dfm_pf_eval_dense evaluated for the candidates only). mode 0: counts[block] = the
+ * block's non-zeros; mode 1: the entries at ptr[block] (the exclusive scan of the counts), in element order, values
+ * as dfm_pfblk_fill. coords .. qw: as dfm_pf_eval_dense; frames (R, S, 9) float64: R(omega_node)^T applied to the
+ * detector directions d0, d90 and the beam-normal p0 (the rows of the probed directions' cone); sin_th, cos_th: of
+ * half the rings' two-theta; the eta nodes of bin c lie at eta0 + (c + x) deta, xlo <= x <= xhi (windows are
+ * taken modulo a turn). */
+int dfm_pfgen(int32_t mode, const double* coords, const double* poles, const int32_t* axis_start,
+              const float* axis_count, const double* inv_sigma2, const double* norm, const float* qw,
+              int32_t quad_weighted, const double* frames, const double* sin_th, const double* cos_th,
+              const int32_t* omegas, int32_t n_ang, int32_t k0, int32_t kb, int32_t C, int32_t P, int32_t A, int32_t S,
+              int32_t Se, float pf_cut, double eta0, double deta, double xlo, double xhi,
+              int32_t* counts, const int32_t* ptr, uint16_t* pos, void* val, int32_t fp16, const float* ring_scale,
+              dfm_stream_t stream);
 
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 TOL = {"bf16": (1e-2, 3e-3, 3e-3), "fp16": (2e-3, 8e-4, 8e-4), "bf16_residual": (3e-5, 1e-5, 1e-5)}  # max, L2, adj.
 # (PF storage, precision): pf_mode "dense" = dense fragments, "sparse" = blocks of the non-zeros
 CASES = [("dense", "bf16"), ("dense", "bf16_residual"), ("sparse", "bf16"), ("sparse", "bf16_residual"),
-         ("sparse", "fp16")]
+         ("sparse", "fp16"), ("generated", "bf16_residual"), ("generated", "fp16")]
 
 
 def _inputs(op, seed=1):
@@ -46,7 +46,7 @@ def test_forward_adjoint_vs_oracle(mode, precision, shape):
     else:                           # My = 140: 9 y tiles in 2 groups
         c = cfg(nx=20, ny=20, my=140, n_omega=8, n_eta=20)
         op, orc, _ = build(c, K=40, **kw)
-    assert (op.mfma_batches if mode == "dense" else op.blk_batches) is not None
+    assert {"dense": op.mfma_batches, "sparse": op.blk_batches, "generated": op.gen_batches}[mode] is not None
     tmax, tl2, tadj = TOL[precision]
     if precision == "bf16" and shape == "split_k":
         tl2 = 1e-3
@@ -165,3 +165,40 @@ def test_blocks_hold_the_csr_nonzeros():
         assert np.array_equal(dense != 0, ref != 0)
         tol = 2.0 ** -11 if q == "fp16" else 2.0 ** -16
         assert np.all(np.abs(dense - ref) <= tol * np.abs(ref)), q
+
+
+@pytest.mark.parametrize("precision", ["bf16_residual", "fp16"])
+def test_generated_blocks_equal_stored_blocks(precision):
+    """The PF generated per call (dfm_pfgen: candidate eta windows, then the stored mode's entry function) holds the
+    stored blocks' entries: the same non-zero count, subset products bit for bit equal to the stored mode's (the same
+    fragments), the full products (generated in angle chunks, so their k splits differ) within 1e-6; three
+    orientation batches, Gauss-Legendre eta nodes, an eta range of half a turn."""
+    import torch
+    print(f"case generated vs stored {precision}", flush=True)
+    from diffractom_amd import QuadratureRule
+    c = cfg(nx=14, ny=12, my=17, n_omega=20, angle_range=(0, 360), cor=0.7, n_eta=48)
+    c["eta_angle_range"] = [0, 180]
+    kw = dict(K=70, sigma_deg=3.0, max_gb=3e-4, pf_precision=precision, oracle=False,
+              quadrature=QuadratureRule(1, 1, 2, 1))
+    ops = {m: build(c, pf_mode=m, **kw)[0] for m in ("sparse", "generated")}
+    assert len(ops["sparse"].batches) > 1 and ops["generated"].gen_batches is not None
+    assert ops["sparse"].sparse_nnz == ops["generated"].sparse_nnz > 0
+    x, y = _inputs(ops["sparse"], 4)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    res = {}
+    for m, op in ops.items():
+        op.set_omega_subsets(4)
+        out = []
+        for sub in (1, 3, None):
+            d = torch.zeros(op.data_shape, device="cuda")
+            a = torch.zeros(op.coeff_shape, device="cuda")
+            op.direct_cl(xt, d, subset=sub)
+            op.adjoint_cl(yt, a, subset=sub)
+            out.append((d.cpu().numpy(), a.cpu().numpy()))
+        res[m] = out
+    for i, sub in enumerate((1, 3, None)):
+        for u, v in zip(res["generated"][i], res["sparse"][i]):
+            if sub is None:
+                assert rel_max(u, v) < 1e-6
+            else:
+                assert np.array_equal(u, v)
