@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="al_c1", help="al_c1 (the headline line), al_c0, al_c3 (the multi-slice "
-                    "batch: slices/s and the step-time percentiles), ti_c2 (hexagonal, 256^2 px, K 59,440, CSR), "
+                    "batch: slices/s and the step-time percentiles), ti_c2 (hexagonal, 256^2 px, K 59,440; PF per "
+                    "--c2-precision), "
                     "al_c4 (autograd loss + gradients, 8 slices)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
     ap.add_argument("--batch", type=int, default=32, help="al_c3: slices per batched run")

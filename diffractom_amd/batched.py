@@ -258,7 +258,7 @@ class BatchedSPDHGPoisson:
         buf, wgm, n_wg = self._group(forward, m)
         if self.blocks:
             _lib.call("dfm_pfblk_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg,
-                      self.ops[0]._blk_mode(), self.ops[0]._stream())
+                      self.ops[0]._blk_mode("fwd") & 15, self.ops[0]._stream())
             return
         _lib.call("dfm_mfma_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg, int(self.residual),
                   self.variant, self.ops[0]._stream())

@@ -151,16 +151,17 @@ __device__ __forceinline__ int frag_slot(unsigned p) {
 }
 
 // Workgroup w of one product (decomposition: column-block group fastest, then y group, k split, angle).
-template <int MODE, bool FWD>
+template <int MODE, bool FWD, int WV>
 __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   constexpr bool RES = MODE == MODE_BF16_RES;
   constexpr bool F16 = MODE == MODE_F16;
+  constexpr int TH = 64 * WV;                                // threads of the workgroup
   // y-side operand, double buffered: fragment s = mt * 64 + lane (8 consecutive contraction indices of a row)
   __shared__ s16x8 a_hi[2 * MT_MAX * 64];
   __shared__ s16x8 a_lo[RES ? 2 * MT_MAX * 64 : 1];
   __shared__ __align__(16) float a_inv[F16 ? 2 * MT_MAX * 16 : 4];   // fp16 mode: 1 / scale of each chunk row
   // each wave's expanded PF block: [hi / lo][tile][lane] fragments
-  __shared__ s16x8 b_frag[WAVES][RES ? 2 : 1][2][64];
+  __shared__ s16x8 b_frag[WV][RES ? 2 : 1][2][64];
 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int tg = (int)(w % p.n_tg);
@@ -172,7 +173,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   if (w >= p.n_ang) return;                                  // uniform over the workgroup
   const int a = (int)w;
   const int r = p.omegas ? p.omegas[a] : a;
-  const int cb = tg * WAVES + wv;                            // this wave's column block
+  const int cb = tg * WV + wv;                               // this wave's column block
   const bool cb_live = cb < p.n_cb;
   const bool cb_stored = FWD ? cb < p.n_jb : cb < p.n_kb;    // adjoint: blocks beyond the batch are empty (zeros)
   const int y0 = yg * p.mt_n * 16;
@@ -180,9 +181,9 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   const int c1 = min(p.n_ch, c0 + p.ch_per);
   if (c0 >= c1) return;                                      // uniform
 
-  // ---- y-side operand slots of this thread: s = tid + THREADS i < mt_n * 64
+  // ---- y-side operand slots of this thread: s = tid + TH i < mt_n * 64
   const int n_slots = p.mt_n * 64;
-  constexpr int NS = MT_MAX * 64 / THREADS;
+  constexpr int NS = (MT_MAX * 64 + TH - 1) / TH;
   const float* srow[NS];
   bool sok[NS];
   const int64_t src_rows = FWD ? (int64_t)a * p.my : (int64_t)r * p.my;
@@ -191,7 +192,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   const bool vec = (src_stride & 3) == 0;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    const int s = tid + THREADS * i;
+    const int s = tid + TH * i;
     const int y = y0 + (s >> 6) * 16 + (s & 15);
     sok[i] = s < n_slots && y < p.my;
     srow[i] = p.src + (src_rows + (sok[i] ? y : 0)) * src_stride;
@@ -227,7 +228,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   auto store_lds = [&](int buf, float (&raw)[NS][8]) {
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
-      const int s = tid + THREADS * i;
+      const int s = tid + TH * i;
       if (F16) {
         // row scale: the 4 slots of a row are lanes l, l ^ 16, l ^ 32, l ^ 48 of this wave (same i)
         float m = 0.f;
@@ -274,19 +275,25 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
     int lo, hi;
     unsigned pp[ER], vv[ER];
   };
-  auto fetch = [&](int c, Ent& E) {
-    E.lo = E.hi = 0;
+  // the block range of chunk c (two dependent loads: the pointers are fetched one chunk before the entries, so that
+  // the wait for them never includes the loads issued in the same chunk)
+  auto fetch_ptr = [&](int c, int2& q) {
+    q = make_int2(0, 0);
     if (have && c < c1) {
       const int64_t bi = blk_index(c);
-      E.lo = p.ptr[bi];
-      E.hi = p.ptr[bi + 1];
+      q.x = p.ptr[bi];
+      q.y = p.ptr[bi + 1];
+    }
+  };
+  auto fetch = [&](const int2& q, Ent& E) {
+    E.lo = q.x;
+    E.hi = q.y;
 #pragma unroll
-      for (int u = 0; u < ER; ++u) {
-        const int e = E.lo + 64 * u + lane;
-        if (e < E.hi) {
-          E.pp[u] = p.pos[e];
-          E.vv[u] = F16 ? (unsigned)v16[e] : v32[e];
-        }
+    for (int u = 0; u < ER; ++u) {
+      const int e = E.lo + 64 * u + lane;
+      if (e < E.hi) {
+        E.pp[u] = p.pos[e];
+        E.vv[u] = F16 ? (unsigned)v16[e] : v32[e];
       }
     }
   };
@@ -356,26 +363,34 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   };
 
   // Software pipeline, two chunks deep: at chunk c the y-side rows of chunk c + 2 and the entries of its block are
-  // fetched, chunk c is computed, and the rows of chunk c + 1 (fetched at chunk c - 1) go to the other LDS buffer.
+  // fetched (its block range three chunks ahead), chunk c is computed, and the rows of chunk c + 1 (fetched at chunk
+  // c - 1) go to the other LDS buffer.
   // The loop is unrolled by two so that the register sets alternate roles instead of being copied.
   float rawA[NS][8], rawB[NS][8];
   Ent E0, E1;
+  int2 Q0, Q1;                                               // block ranges of chunks c + 2 and c + 3
   load_raw(c0, rawA);
   store_lds(0, rawA);
   if (c0 + 1 < c1) load_raw(c0 + 1, rawA);
-  fetch(c0, E0);
-  fetch(c0 + 1, E1);
+  fetch_ptr(c0, Q0);
+  fetch_ptr(c0 + 1, Q1);
+  fetch(Q0, E0);
+  fetch(Q1, E1);
+  fetch_ptr(c0 + 2, Q0);
   __syncthreads();
-  auto step = [&](int c, int buf, float (&rin)[NS][8], float (&rout)[NS][8], Ent& E) {
+  // chunk c: the ranges of chunk c + 3 into qn, the rows of chunk c + 2, compute chunk c, the entries of chunk
+  // c + 2 (range qe, fetched one chunk ago), the rows of chunk c + 1 to the other LDS buffer
+  auto step = [&](int c, int buf, float (&rin)[NS][8], float (&rout)[NS][8], Ent& E, const int2& qe, int2& qn) {
+    fetch_ptr(c + 3, qn);
     if (c + 2 < c1) load_raw(c + 2, rout);
     compute(buf, E);
-    fetch(c + 2, E);
+    fetch(qe, E);
     if (c + 1 < c1) store_lds(buf ^ 1, rin);
     __syncthreads();
   };
   for (int c = c0; c < c1; c += 2) {
-    step(c, 0, rawA, rawB, E0);
-    if (c + 1 < c1) step(c + 1, 1, rawB, rawA, E1);
+    step(c, 0, rawA, rawB, E0, Q0, Q1);
+    if (c + 1 < c1) step(c + 1, 1, rawB, rawA, E1, Q1, Q0);
   }
 
   if (!cb_live) return;
@@ -409,9 +424,13 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   }
 }
 
-template <int MODE, bool FWD>
-__global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_kernel(BlkArgs p) {
-  blk_pf_body<MODE, FWD>(p, xcd_block());
+// WV waves per workgroup (WV column blocks share each staged y-side chunk): 4 (three workgroups per CU, two in the
+// fp16 mode) or 8 (half the re-reads of the y-side rows, which every group of WV column blocks stages again; one
+// workgroup per CU).
+template <int MODE, bool FWD, int WV>
+__global__ __launch_bounds__(64 * WV, WV == 4 ? (MODE == MODE_F16 ? 2 : 3) : 2) void blk_pf_kernel(
+    BlkArgs p) {
+  blk_pf_body<MODE, FWD, WV>(p, xcd_block());
 }
 
 // Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (the slices of a multi-slice batch,
@@ -424,7 +443,7 @@ __global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_grou
   const int2 m = wg_map[w];
   if (m.x < 0) return;
   const BlkArgs p = descs[m.x];
-  blk_pf_body<MODE, FWD>(p, m.y);
+  blk_pf_body<MODE, FWD, WAVES>(p, m.y);
 }
 
 template <bool FWD>
@@ -438,15 +457,23 @@ void launch_blk_grouped(const BlkArgs* descs, const int2* wg_map, int64_t n_map,
     hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_F16, FWD>), grid, block, 0, st, descs, wg_map, n_map);
 }
 
-template <bool FWD>
-int launch_blk(const BlkArgs& p, int mode, hipStream_t st) {
+template <bool FWD, int WV>
+void launch_blk_wv(const BlkArgs& p, int mode, hipStream_t st) {
   const int64_t n_wg = (int64_t)p.n_ang * p.ksplit * p.n_yg * p.n_tg;
-  const dim3 grid((unsigned)((n_wg + N_XCD - 1) / N_XCD * N_XCD)), block(THREADS);
-  if (mode == MODE_BF16_RES) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16_RES, FWD>), grid, block, 0, st, p);
-  else if (mode == MODE_BF16) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16, FWD>), grid, block, 0, st, p);
-  else hipLaunchKernelGGL((blk_pf_kernel<MODE_F16, FWD>), grid, block, 0, st, p);
-  return DFM_OK;
+  const dim3 grid((unsigned)((n_wg + N_XCD - 1) / N_XCD * N_XCD)), block(64 * WV);
+  if (mode == MODE_BF16_RES) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16_RES, FWD, WV>), grid, block, 0, st, p);
+  else if (mode == MODE_BF16) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16, FWD, WV>), grid, block, 0, st, p);
+  else hipLaunchKernelGGL((blk_pf_kernel<MODE_F16, FWD, WV>), grid, block, 0, st, p);
 }
+
+template <bool FWD>
+void launch_blk(const BlkArgs& p, int mode, int wv, hipStream_t st) {
+  if (wv == 8) launch_blk_wv<FWD, 8>(p, mode, st);
+  else launch_blk_wv<FWD, 4>(p, mode, st);
+}
+
+// waves per workgroup from bits 5-6 of the mode argument: 0 -> 4, 1 -> 8
+int mode_waves(int mode) { return 4 << ((mode >> 5) & 3); }
 
 void y_tiling(int my, int& mt_n, int& n_yg) {
   const int tiles = (my + 15) / 16;
@@ -490,6 +517,8 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
                       int32_t accumulate, dfm_stream_t stream) {
   DFM_CHECK_ARG(sino && ptr && pos && val && data, "dfm_pfblk_forward: null pointer");
   const int local = (mode >> 4) & 1;
+  const int wv = mode_waves(mode);
+  DFM_CHECK_ARG(wv <= 8, "dfm_pfblk_forward: waves code %d (0, 1: 4, 8 waves per workgroup)", (mode >> 5) & 3);
   mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_forward: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
@@ -516,7 +545,7 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
   p.n_cb = p.n_jb;
   p.n_ch = p.n_kb;
   y_tiling(my, p.mt_n, p.n_yg);
-  p.n_tg = (p.n_cb + WAVES - 1) / WAVES;
+  p.n_tg = (p.n_cb + wv - 1) / wv;
   p.ch_per = (p.n_ch + ksplit - 1) / ksplit;
   p.ksplit = (p.n_ch + p.ch_per - 1) / p.ch_per;
   p.direct = p.ksplit == 1;
@@ -524,7 +553,7 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
   p.out = p.direct ? data : part;
   p.alpha = 1.f;
   const hipStream_t st = (hipStream_t)stream;
-  launch_blk<true>(p, mode, st);
+  launch_blk<true>(p, mode, wv, st);
   DFM_LAUNCH_CHECK("dfm_pfblk_forward");
   if (!p.direct) return dfm::launch_splitk_reduce(part, data, omegas, n_ang, (int64_t)my * cp, p.ksplit, accumulate, st);
   return DFM_OK;
@@ -535,6 +564,8 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
                       int32_t kb, float* sino, int32_t sks, float alpha, dfm_stream_t stream) {
   DFM_CHECK_ARG(data && ptr && pos && val && sino, "dfm_pfblk_adjoint: null pointer");
   const int local = (mode >> 4) & 1;
+  const int wv = mode_waves(mode);
+  DFM_CHECK_ARG(wv <= 8, "dfm_pfblk_adjoint: waves code %d (0, 1: 4, 8 waves per workgroup)", (mode >> 5) & 3);
   mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_adjoint: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
@@ -560,12 +591,12 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
   p.n_cb = (sks + 31) / 32;           // every sinogram column is written (those of no stored block with 0)
   p.n_ch = p.n_jb;
   y_tiling(my, p.mt_n, p.n_yg);
-  p.n_tg = (p.n_cb + WAVES - 1) / WAVES;
+  p.n_tg = (p.n_cb + wv - 1) / wv;
   p.ksplit = 1;
   p.ch_per = p.n_ch;
   p.alpha = alpha;
   p.ncols = sks;
-  launch_blk<false>(p, mode, (hipStream_t)stream);
+  launch_blk<false>(p, mode, wv, (hipStream_t)stream);
   DFM_LAUNCH_CHECK("dfm_pfblk_adjoint");
   return DFM_OK;
 }

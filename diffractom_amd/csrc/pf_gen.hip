@@ -3,24 +3,28 @@
 // angles of one call only, no stored PF.
 //
 // A workgroup owns one angle r and 32 orientations (an orientation block). It first marks the candidate entries in
-// an LDS bitmap (32 orientations x C*P segments): for every orientation k, pole axis a and omega node s the pole
-// direction n = U_k h_a is taken into the detector frame of the node (x, y, z = n . R^T d0, n . R^T d90, n . R^T p0),
-// where the probed directions of ring p form the cone v(eta) = cos(th_p) (cos(eta) R^T d0 + sin(eta) R^T d90) -
-// sin(th_p) R^T p0, so that n . v = A cos(eta - phi) - B with A = cos(th_p) sqrt(x^2 + y^2), B = sin(th_p) z,
-// phi = atan2(y, x). |n . v| > tau = 1 - cut sigma^2 - 1e-4 (a superset of the entries whose exponent
-// (1 - |n . v|) / sigma^2 stays below the cut) holds on eta windows phi +- acos((tau + B) / A) (for n) and
-// phi + pi +- acos((tau - B) / A) (for -n); every eta bin with a quadrature node in a window is a candidate of ring
-// p. Then each wave takes 32 x 32 blocks (orientation block x 32 segments): a lane evaluates its row's candidates
-// of the block with the stored mode's own entry function (pf_entry: all axes of the ring, all nodes, float64), so
-// the values equal the stored PF bit for bit, and the non-zeros are written in element order (orientation-major,
-// then segment) at the block's position in the pointer array — or, in count mode, counted. The counts are taken
-// once per angle set; the pointers of an angle list are their exclusive scan.
+// an LDS bitmap (one word per 32 segments and orientation, words in (segment block, orientation) order): for every
+// orientation k, pole axis a and omega node s the pole direction n = U_k h_a is taken into the detector frame of the
+// node (x, y, z = n . R^T d0, n . R^T d90, n . R^T p0), where the probed directions of ring p form the cone
+// v(eta) = cos(th_p) (cos(eta) R^T d0 + sin(eta) R^T d90) - sin(th_p) R^T p0, so that n . v = A cos(eta - phi) - B
+// with A = cos(th_p) sqrt(x^2 + y^2), B = sin(th_p) z, phi = atan2(y, x). |n . v| > tau = 1 - cut sigma^2 - 1e-4 (a
+// superset of the entries whose exponent (1 - |n . v|) / sigma^2 stays below the cut) holds on eta windows
+// phi +- acos((tau + B) / A) (for n) and phi + pi +- acos((tau - B) / A) (for -n); every eta bin with a quadrature
+// node in a window is a candidate of ring p. The bitmap's set bits, read in word then bit order, are the block-sparse
+// form's element order (segment block, then orientation, then segment). They are expanded into an LDS list (at most
+// LIST_CAP at a time) and evaluated one candidate per thread with the stored mode's own entry function (pf_entry: all
+// axes of the ring, all nodes, float64), so the values equal the stored PF bit for bit; a workgroup-wide scan of the
+// non-zero flags gives each non-zero its place. The blocks of one (angle, orientation block) are consecutive in the
+// pointer array, so the workgroup's non-zeros form one run from the first block's pointer and the stores are
+// coalesced. Count mode adds the non-zeros per block instead. The counts are taken once per angle set; the pointers
+// of an angle list are their exclusive scan.
 #include "dfm_common.h"
 
 namespace {
 
 constexpr int G_THREADS = 256;
 constexpr int G_WAVES = G_THREADS / 64;
+constexpr int LIST_CAP = 4096;            // candidates expanded per pass (16 KB of LDS)
 
 struct GenArgs {
   const double* coords;      // probed directions (R, S, C, Se, P, 3)
@@ -48,16 +52,21 @@ struct GenArgs {
 __device__ __forceinline__ unsigned short f16_bits(float v) { return __builtin_bit_cast(unsigned short, (_Float16)v); }
 
 __global__ void __launch_bounds__(G_THREADS) pf_gen_kernel(GenArgs g) {
-  extern __shared__ unsigned gen_lds[];   // bitmap: 32 rows x n_jb words; values: G_WAVES x 32 x 32 floats
-  unsigned* bm = gen_lds;
-  float* vtmp = reinterpret_cast<float*>(gen_lds + 32 * g.n_jb);
+  extern __shared__ unsigned gen_lds[];
+  unsigned* bm = gen_lds;                                   // word (jb, kl) at jb * 32 + kl
+  unsigned* list = bm + 32 * g.n_jb;                        // LIST_CAP entries: word index << 5 | bit
+  int* wsum = reinterpret_cast<int*>(list + LIST_CAP);      // G_WAVES wave totals
+  int* cnt = wsum + G_WAVES;                                // count mode: non-zeros per segment block
   const int kbi = blockIdx.x % g.n_kb;
   const int a = blockIdx.x / g.n_kb;
   if (a >= g.n_ang) return;
   const int r = g.omegas[a];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int CP = g.C * g.P;
-  for (int i = tid; i < 32 * g.n_jb; i += G_THREADS) bm[i] = 0u;
+  const int W = 32 * g.n_jb;
+  for (int i = tid; i < W; i += G_THREADS) bm[i] = 0u;
+  if (g.mode == 0)
+    for (int i = tid; i < g.n_jb; i += G_THREADS) cnt[i] = 0;
   __syncthreads();
 
   // ---- candidates
@@ -94,69 +103,106 @@ __global__ void __launch_bounds__(G_THREADS) pf_gen_kernel(GenArgs g) {
         m_hi = min(m_hi, g.C - 1);
         for (int c = m_lo; c <= m_hi; ++c) {
           const int j = c * g.P + p;
-          atomicOr(&bm[kl * g.n_jb + (j >> 5)], 1u << (j & 31));
+          atomicOr(&bm[(j >> 5) * 32 + kl], 1u << (j & 31));
         }
       }
     }
   }
   __syncthreads();
 
-  // ---- evaluate: a wave per block, lane = (row kl = lane % 32, half h = lane / 32: the row's odd / even bits)
-  const int kl = lane & 31, h = lane >> 5;
-  const int kloc = kbi * 32 + kl;
-  const bool krow = kloc < g.kb;
-  const int kg = g.k0 + kloc;
-  float* vt = vtmp + (wv * 32 + kl) * 32;
-  for (int jb = wv; jb < g.n_jb; jb += G_WAVES) {
-    const unsigned word = krow ? bm[kl * g.n_jb + jb] : 0u;
-    unsigned mine = word & (h ? 0xAAAAAAAAu : 0x55555555u);
-    unsigned nz = 0u;
-    while (mine) {
-      const int b = __builtin_ctz(mine);
-      mine &= mine - 1u;
-      const int j = jb * 32 + b;
+  // ---- the candidates' order: each thread owns a run of words; exclusive scan of their bit counts
+  const int per = (W + G_THREADS - 1) / G_THREADS;
+  const int w0 = min(tid * per, W), w1 = min(w0 + per, W);
+  int mine = 0;
+  for (int w = w0; w < w1; ++w) mine += __popc(bm[w]);
+  int incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  int before_w = 0, ncand = 0;
+  for (int q = 0; q < G_WAVES; ++q) {
+    const int t = wsum[q];
+    if (q < wv) before_w += t;
+    ncand += t;
+  }
+  const int cbase = before_w + incl - mine;                  // this thread's first candidate
+  const int64_t blk0 = ((int64_t)a * g.n_kb + kbi) * g.n_jb;
+  int64_t out = g.mode == 1 ? (int64_t)g.ptr[blk0] : 0;     // the workgroup's run of non-zeros
+  __syncthreads();                                           // wsum is reused below
+
+  for (int c0 = 0; c0 < ncand; c0 += LIST_CAP) {
+    const int c1 = c0 + LIST_CAP;
+    if (mine > 0 && cbase < c1 && cbase + mine > c0) {
+      int idx = cbase;
+      for (int w = w0; w < w1 && idx < c1; ++w) {
+        unsigned bits = bm[w];
+        const int nb = __popc(bits);
+        if (idx + nb <= c0) {
+          idx += nb;
+          continue;
+        }
+        while (bits && idx < c1) {
+          const int b = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          if (idx >= c0) list[idx - c0] = ((unsigned)w << 5) | (unsigned)b;
+          ++idx;
+        }
+      }
+    }
+    __syncthreads();
+    const int n = min(LIST_CAP, ncand - c0);
+    for (int i0 = 0; i0 < n; i0 += G_THREADS) {
+      const int i = i0 + tid;
       float v = 0.f;
-      if (j < CP) {
-        v = dfm::pf_entry(g.coords, g.poles, g.axis_start, g.axis_count, g.inv_sigma2, g.normf, g.qw, r, kg, j / g.P,
-                     j % g.P, g.C, g.P, g.A, g.S, g.Se, g.cut, g.quad_weighted);
+      int jb = 0, kl = 0, b = 0;
+      if (i < n) {
+        const unsigned e = list[i];
+        const int w = (int)(e >> 5);
+        b = (int)(e & 31u);
+        jb = w >> 5;
+        kl = w & 31;
+        const int j = jb * 32 + b;
+        const int kloc = kbi * 32 + kl;
+        if (j < CP && kloc < g.kb)
+          v = dfm::pf_entry(g.coords, g.poles, g.axis_start, g.axis_count, g.inv_sigma2, g.normf, g.qw, r,
+                            g.k0 + kloc, j / g.P, j % g.P, g.C, g.P, g.A, g.S, g.Se, g.cut, g.quad_weighted);
       }
-      if (v != 0.f) {
-        nz |= 1u << b;
-        vt[b] = v;
+      const bool nzf = v != 0.f;
+      const unsigned long long bal = __ballot(nzf);
+      const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int before = 0, tot = 0;
+      for (int q = 0; q < G_WAVES; ++q) {
+        const int t = wsum[q];
+        if (q < wv) before += t;
+        tot += t;
       }
-    }
-    const unsigned rownz = nz | __shfl_xor(nz, 32, 64);       // the row's non-zeros (both halves)
-    const int cnt = __popc(rownz);
-    // exclusive prefix of the row counts over rows 0..31 (the same in both halves)
-    int incl = cnt;
-    for (int off = 1; off < 32; off <<= 1) {
-      const int t = __shfl_up(incl, off, 32);
-      if (kl >= off) incl += t;
-    }
-    const int total = __shfl(incl, 31, 64);
-    const int64_t blk = ((int64_t)a * g.n_kb + kbi) * g.n_jb + jb;
-    if (g.mode == 0) {
-      if (lane == 0) g.counts[blk] = total;
-      continue;
-    }
-    const int base = g.ptr[blk] + incl - cnt;
-    unsigned m = nz;
-    while (m) {
-      const int b = __builtin_ctz(m);
-      m &= m - 1u;
-      const int o = base + __popc(rownz & ((1u << b) - 1u));
-      const float v = vt[b];
-      g.pos[o] = (unsigned short)(kl | (b << 5));
-      if (g.val16) {
-        g.val16[o] = f16_bits(v * g.ring_scale[(jb * 32 + b) % g.P]);
-      } else {
-        const __bf16 hi = (__bf16)v;
-        const __bf16 lo = (__bf16)(v - (float)hi);
-        g.val32[o] = (unsigned)__builtin_bit_cast(unsigned short, hi) |
-                     ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+      if (nzf) {
+        if (g.mode == 0) {
+          atomicAdd(&cnt[jb], 1);
+        } else {
+          const int64_t o = out + before + rank;
+          g.pos[o] = (unsigned short)(kl | (b << 5));
+          if (g.val16) {
+            g.val16[o] = f16_bits(v * g.ring_scale[(jb * 32 + b) % g.P]);
+          } else {
+            const __bf16 hi = (__bf16)v;
+            const __bf16 lo = (__bf16)(v - (float)hi);
+            g.val32[o] = (unsigned)__builtin_bit_cast(unsigned short, hi) |
+                         ((unsigned)__builtin_bit_cast(unsigned short, lo) << 16);
+          }
+        }
       }
+      out += tot;
+      __syncthreads();                                       // wsum (and, after the last, list) reused
     }
   }
+  if (g.mode == 0)
+    for (int i = tid; i < g.n_jb; i += G_THREADS) g.counts[blk0 + i] = cnt[i];
 }
 
 }  // namespace
@@ -214,7 +260,7 @@ int dfm_pfgen(int32_t mode, const double* coords, const double* poles, const int
   g.xlo = xlo;
   g.xhi = xhi;
   g.mode = mode;
-  const size_t lds = (size_t)32 * g.n_jb * sizeof(unsigned) + (size_t)G_WAVES * 32 * 32 * sizeof(float);
+  const size_t lds = ((size_t)32 * g.n_jb + LIST_CAP + G_WAVES + g.n_jb) * sizeof(unsigned);
   DFM_CHECK_ARG(lds <= 163840, "dfm_pfgen: %d segment blocks exceed the LDS", g.n_jb);
   const int64_t n_wg = (int64_t)n_ang * g.n_kb;
   DFM_CHECK_ARG(n_wg < ((int64_t)1 << 31), "dfm_pfgen: too many workgroups");

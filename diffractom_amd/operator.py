@@ -269,7 +269,7 @@ class SinglePhaseForwardOperator:
             raise NotImplementedError("pf_precision 'fp16' runs on the block-sparse PF (pf_mode 'sparse' or 'auto')")
         self.pf_precision = pf_precision
         self.mfma_variant = {"fwd": 2, "adj": 2, "fwd_waves": 2048}
-        self.blk_variant = {"fwd_waves": 8192}
+        self.blk_variant = {"fwd_waves": 8192, "fwd_wg": 4, "adj_wg": 4}
         self.csr_variant = {"fwd": "auto", "adj": "auto", "ksplit": "auto", "kc": 256, "tsplit": "auto", "tkc": 0,
                             "tdirect": 3}
 
@@ -662,8 +662,10 @@ class SinglePhaseForwardOperator:
         self.sparse_nnz = nnz
         self.sparse_fill = nnz / (R * self.K * CP)
 
-    def _blk_mode(self):
-        return {"bf16": 0, "bf16_residual": 1, "fp16": 2}[self.pf_precision]
+    def _blk_mode(self, direction):
+        """Mode argument of dfm_pfblk_forward / _adjoint: the precision, plus 32 for 8-wave workgroups."""
+        wg = int(self.blk_variant.get(direction + "_wg", 4))
+        return {"bf16": 0, "bf16_residual": 1, "fp16": 2}[self.pf_precision] + (32 if wg == 8 else 0)
 
     def _blk_split(self, n_ang, blk):
         """k split of the block-sparse forward: enough waves for blk_variant["fwd_waves"], at most 64 partial sums."""
@@ -678,7 +680,7 @@ class SinglePhaseForwardOperator:
         ks = self._blk_split(n_ang, blk)
         part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
         _lib.call("dfm_pfblk_forward", _lib.ptr(sino), b["kstride"], b["K_batch"], _lib.ptr(blk["ptr"]),
-                  _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode(), _lib.ptr(blk["ring_inv"]),
+                  _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode("fwd"), _lib.ptr(blk["ring_inv"]),
                   self.N_peaks, _lib.ptr(om), n_ang, self.My, self.N_seg, _lib.ptr(data), _lib.ptr(part), ks,
                   int(accumulate), self._stream())
 
@@ -686,8 +688,8 @@ class SinglePhaseForwardOperator:
         b, blk = self.batches[ib], self.blk_batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
         _lib.call("dfm_pfblk_adjoint", _lib.ptr(data), _lib.ptr(blk["ptr"]), _lib.ptr(blk["pos"]),
-                  _lib.ptr(blk["val"]), self._blk_mode(), _lib.ptr(blk["ring_inv"]), self.N_peaks, _lib.ptr(om),
-                  n_ang, self.My, self.N_seg, b["K_batch"], _lib.ptr(sino), b["kstride"], float(alpha),
+                  _lib.ptr(blk["val"]), self._blk_mode("adj"), _lib.ptr(blk["ring_inv"]), self.N_peaks,
+                  _lib.ptr(om), n_ang, self.My, self.N_seg, b["K_batch"], _lib.ptr(sino), b["kstride"], float(alpha),
                   self._stream())
 
     def _setup_generated(self):
@@ -741,6 +743,7 @@ class SinglePhaseForwardOperator:
         self.sparse_nnz = nnz
         self.sparse_fill = nnz / (R * self.K * CP)
         self._gen_scratch = None
+        self._gen_caps = self._gen_offs = self._gen_tags = None
 
     def _pfgen(self, mode, b, omegas, n_ang, counts=None, ptr=None, pos=None, val=None, ring_scale=None):
         g = self._gen_geo
@@ -759,8 +762,7 @@ class SinglePhaseForwardOperator:
         angles for every angle (each group's PF is generated and used in turn). (key, omegas tensor, first row)."""
         if subset is not None:
             m = self._subset(subset)
-            return [(("sub", m, tuple(self.omega_subsets[m][:2]), len(self.omega_subsets[m])),
-                     self._sub_omegas[m], None)]
+            return [(("sub", self._subsets_version, m), self._sub_omegas[m], None)]
         R = self.N_Omega
         cs = max(1, -(-R // 10))
         out = []
@@ -789,39 +791,54 @@ class SinglePhaseForwardOperator:
             gb["groups"][key] = grp
         return grp
 
-    def _gen_entry_scratch(self, nnz):
-        """The entry arrays the generation writes and the products read (grown to nnz entries, kept)."""
-        n = max(int(nnz), 1)
-        sc = self._gen_scratch
-        if sc is None or sc["pos"].numel() < n:
-            f16 = self.pf_precision == "fp16"
-            self._gen_scratch = sc = None
-            sc = self._gen_scratch = {
-                "pos": torch.empty(n, dtype=torch.int16, device=self.device),
-                "val": torch.empty(n, dtype=torch.int16 if f16 else torch.int32, device=self.device)}
-            self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
-        return sc
+    def _gen_slot(self, ib, nnz):
+        """Batch ib's slot of the entry scratch (pos, val views of at least nnz entries): one slot per orientation
+        batch, so that the blocks of a subset stay in place for all batches until another angle group is generated."""
+        if self._gen_caps is None or self._gen_caps[ib] < nnz:
+            caps = list(self._gen_caps) if self._gen_caps is not None else [1] * len(self.batches)
+            caps[ib] = max(int(nnz), caps[ib])
+            self._gen_alloc(caps)
+        o0, o1 = int(self._gen_offs[ib]), int(self._gen_offs[ib + 1])
+        return self._gen_scratch["pos"][o0:o1], self._gen_scratch["val"][o0:o1]
+
+    def _gen_alloc(self, caps):
+        """(Re)allocate the entry scratch with the slot capacities caps; forgets what the slots held."""
+        f16 = self.pf_precision == "fp16"
+        self._gen_scratch = None
+        tot = int(sum(caps))
+        self._gen_scratch = {"pos": torch.empty(tot, dtype=torch.int16, device=self.device),
+                             "val": torch.empty(tot, dtype=torch.int16 if f16 else torch.int32, device=self.device)}
+        self._gen_caps = list(caps)
+        self._gen_offs = np.concatenate([[0], np.cumsum(caps)]).astype(np.int64)
+        self._gen_tags = [None] * len(self.batches)
+        self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
 
     def _gen_prepare(self, subsets):
-        """Block pointers of every (subset, batch) and the entry scratch for the largest, made when the subsets are
-        set, so that the solver's steps do no host synchronisation or allocation for them."""
-        big = 0
+        """Block pointers of every (subset, batch) and slots for the largest, made when the subsets are set, so
+        that the solver's steps do no host synchronisation or allocation for them."""
+        caps = list(self._gen_caps) if self._gen_caps is not None else [1] * len(self.batches)
+        need = list(caps)
         for m in subsets:
             key, om, _ = self._gen_groups(m)[0]
             for ib in range(len(self.batches)):
-                big = max(big, self._gen_group(ib, key, om)["nnz"])
-        self._gen_entry_scratch(big)
+                need[ib] = max(need[ib], self._gen_group(ib, key, om)["nnz"])
+        if need != caps:
+            self._gen_alloc(need)
 
     def _gen_blocks(self, ib, key, om):
-        """Generate the PF blocks of batch ib for the angles om (block pointers cached per angle group) into the
-        scratch entry arrays; returns the block dict the products read (valid until the next generation)."""
+        """The PF blocks of batch ib for the angles om (block pointers cached per angle group), generated into the
+        batch's scratch slot unless the slot already holds this group: a subset step's forward leaves its blocks for
+        the next step's adjoint on the same subset. Under graph capture the generation is always recorded (a replay
+        cannot know what the slot holds) and the slot is marked unknown. Returns the block dict the products read."""
         b, gb = self.batches[ib], self.gen_batches[ib]
         grp = self._gen_group(ib, key, om)
-        sc = self._gen_entry_scratch(grp["nnz"])
-        self._pfgen(1, b, om, om.numel(), ptr=grp["ptr"], pos=sc["pos"], val=sc["val"],
-                    ring_scale=gb["ring_scale"])
+        pos, val = self._gen_slot(ib, grp["nnz"])
+        capturing = torch.cuda.is_current_stream_capturing()
+        if capturing or self._gen_tags[ib] != key:
+            self._pfgen(1, b, om, om.numel(), ptr=grp["ptr"], pos=pos, val=val, ring_scale=gb["ring_scale"])
+            self._gen_tags[ib] = None if capturing else key
         self.mark("pf_generate")
-        return {"ptr": grp["ptr"], "pos": sc["pos"], "val": sc["val"], "n_kb": gb["n_kb"], "n_jb": gb["n_jb"],
+        return {"ptr": grp["ptr"], "pos": pos, "val": val, "n_kb": gb["n_kb"], "n_jb": gb["n_jb"],
                 "ring_inv": gb["ring_inv"]}
 
     def _gen_forward(self, ib, sino, data, subset, accumulate):
@@ -834,7 +851,8 @@ class SinglePhaseForwardOperator:
             part = self._mfma_part(ks * n_ang * self.My * self.N_seg) if ks > 1 else None
             src = _lib.ptr(sino) if r0 is None else ctypes.c_void_p(sino.data_ptr() + 4 * r0 * row)
             _lib.call("dfm_pfblk_forward", src, b["kstride"], b["K_batch"], _lib.ptr(blk["ptr"]),
-                      _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode() + 16, _lib.ptr(blk["ring_inv"]),
+                      _lib.ptr(blk["pos"]), _lib.ptr(blk["val"]), self._blk_mode("fwd") + 16,
+                      _lib.ptr(blk["ring_inv"]),
                       self.N_peaks, _lib.ptr(om), n_ang, self.My, self.N_seg, _lib.ptr(data), _lib.ptr(part), ks,
                       int(accumulate), self._stream())
 
@@ -845,7 +863,7 @@ class SinglePhaseForwardOperator:
             blk = self._gen_blocks(ib, key, om)
             dst = _lib.ptr(sino) if r0 is None else ctypes.c_void_p(sino.data_ptr() + 4 * r0 * row)
             _lib.call("dfm_pfblk_adjoint", _lib.ptr(data), _lib.ptr(blk["ptr"]), _lib.ptr(blk["pos"]),
-                      _lib.ptr(blk["val"]), self._blk_mode() + 16, _lib.ptr(blk["ring_inv"]), self.N_peaks,
+                      _lib.ptr(blk["val"]), self._blk_mode("adj") + 16, _lib.ptr(blk["ring_inv"]), self.N_peaks,
                       _lib.ptr(om), om.numel(), self.My, self.N_seg, b["K_batch"], dst, b["kstride"], float(alpha),
                       self._stream())
 
@@ -893,6 +911,7 @@ class SinglePhaseForwardOperator:
         self.blk_batches = None
         self.gen_batches = None
         self._gen_scratch = None
+        self._gen_caps = self._gen_offs = self._gen_tags = None
         self._scratch = None
 
     def _scratch_buffers(self, n_angles):
@@ -929,6 +948,7 @@ class SinglePhaseForwardOperator:
         if old is not None and len(old) == len(out) and all(np.array_equal(a, b) for a, b in zip(old, out)):
             return out
         self.omega_subsets = out
+        self._subsets_version = getattr(self, "_subsets_version", 0) + 1
         self._sub_omegas = [torch.from_numpy(s.astype(np.int32)).to(self.device) for s in out]
         self._sub_geo_fwd = [self.geo_fwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
         self._sub_geo_bwd = [self.geo_bwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
@@ -1149,15 +1169,32 @@ class SinglePhaseForwardOperator:
             best = _pick(times)
             self.mfma_variant["adj"] = int(best.split(":")[1])
             out["pf_adjoint"] = (best, times)
-        if self.blk_batches is not None:
+        if self.blk_batches is not None or self.gen_batches is not None:
+            # block-sparse products (stored or generated; a generated subset's blocks are made by the first call and
+            # reused by the timed ones): 4- or 8-wave workgroups, and the forward's k split
             data = torch.rand(self.data_shape, generator=g, device=dev)
-            times = {}
-            for w in (2048, 4096, 8192, 16384, 32768):
-                self.blk_variant["fwd_waves"] = w
-                times[f"waves:{w}"] = timeit(lambda: self._blk_forward(0, sc["sino"], data, subset, False))
+            gen = self.blk_batches is None
+            fwd = (lambda: self._gen_forward(0, sc["sino"], data, subset, False)) if gen else \
+                (lambda: self._blk_forward(0, sc["sino"], data, subset, False))
+            adj = (lambda: self._gen_adjoint(0, data, sc["sino"], subset, 1.0)) if gen else \
+                (lambda: self._blk_adjoint(0, data, sc["sino"], subset, 1.0))
+            times, cands = {}, {}
+            for wg in (4, 8):
+                for w in (2048, 4096, 8192, 16384, 32768):
+                    cands[f"wg:{wg}+waves:{w}"] = (wg, w)
+            for name, (wg, w) in cands.items():
+                self.blk_variant.update(fwd_wg=wg, fwd_waves=w)
+                times[name] = timeit(fwd)
             best = _pick(times)
-            self.blk_variant["fwd_waves"] = int(best.split(":")[1])
+            self.blk_variant.update(fwd_wg=cands[best][0], fwd_waves=cands[best][1])
             out["pf_forward"] = (best, times)
+            times = {}
+            for wg in (4, 8):
+                self.blk_variant["adj_wg"] = wg
+                times[f"wg:{wg}"] = timeit(adj)
+            best = _pick(times)
+            self.blk_variant["adj_wg"] = int(best.split(":")[1])
+            out["pf_adjoint"] = (best, times)
         del x, z, T
         self.tuning = out
         return out

@@ -202,3 +202,65 @@ def test_generated_blocks_equal_stored_blocks(precision):
                 assert rel_max(u, v) < 1e-6
             else:
                 assert np.array_equal(u, v)
+
+
+def test_generated_slots_follow_the_angle_group():
+    """The generated blocks stay in their batch slots for the next product on the same subset and are generated
+    again when another angle group (another subset, a full product's angle chunks, a new subset partition) took the
+    slots: every product in such a sequence equals the stored mode's bit for bit."""
+    import torch
+    print("case generated slot reuse", flush=True)
+    c = cfg(nx=14, ny=12, my=17, n_omega=20, angle_range=(0, 360), cor=0.7)
+    kw = dict(K=70, sigma_deg=3.0, max_gb=3e-4, pf_precision="fp16", oracle=False)
+    ops = {m: build(c, pf_mode=m, **kw)[0] for m in ("sparse", "generated")}
+    x, y = _inputs(ops["sparse"], 5)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    seq = [("set", 4), ("f", 1), ("a", 1), ("f", 3), ("a", 1), ("f", None), ("a", 3), ("a", 3),
+           ("set", [np.arange(0, 20, 2), np.arange(1, 20, 2)]), ("a", 1), ("f", 0), ("a", 0)]
+    res = {}
+    for m, op in ops.items():
+        out = []
+        for what, sub in seq:
+            if what == "set":
+                op.set_omega_subsets(sub)
+                continue
+            if what == "f":
+                d = torch.zeros(op.data_shape, device="cuda")
+                op.direct_cl(xt, d, subset=sub)
+                out.append(d.cpu().numpy())
+            else:
+                a = torch.zeros(op.coeff_shape, device="cuda")
+                op.adjoint_cl(yt, a, subset=sub)
+                out.append(a.cpu().numpy())
+        res[m] = out
+    for i, (u, v) in enumerate(zip(res["generated"], res["sparse"])):
+        if i == 4:                   # the full forward: generated in angle chunks, so its k split differs
+            assert rel_max(u, v) < 1e-6
+        else:
+            assert np.array_equal(u, v), i
+
+
+@pytest.mark.parametrize("mode,precision", [("sparse", "bf16_residual"), ("sparse", "fp16"), ("generated", "fp16")])
+def test_block_workgroup_sizes_agree(mode, precision):
+    """The block-sparse products in 8-wave workgroups (eight column blocks share each staged y-side chunk) give the
+    4-wave results bit for bit: every wave still owns one column block and walks the same chunks in the same order."""
+    import torch
+    print(f"case block workgroups {mode} {precision}", flush=True)
+    c = cfg(nx=20, ny=20, my=41, n_omega=12, angle_range=(0, 180), cor=0.3)
+    op, _, _ = build(c, K=300, pf_mode=mode, pf_precision=precision, oracle=False)
+    op.set_omega_subsets(3)
+    x, y = _inputs(op, 6)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    res = []
+    for wg in (4, 8):
+        op.blk_variant.update(fwd_wg=wg, adj_wg=wg)
+        out = []
+        for sub in (None, 1):
+            d = torch.zeros(op.data_shape, device="cuda")
+            a = torch.zeros(op.coeff_shape, device="cuda")
+            op.direct_cl(xt, d, subset=sub)
+            op.adjoint_cl(yt, a, subset=sub)
+            out += [d.cpu().numpy(), a.cpu().numpy()]
+        res.append(out)
+    for u, v in zip(*res):
+        assert np.array_equal(u, v)
