@@ -44,7 +44,7 @@ def parse():
                     "--c2-precision), "
                     "al_c4 (autograd loss + gradients, 8 slices)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
-    ap.add_argument("--batch", type=int, default=32, help="al_c3: slices per batched run")
+    ap.add_argument("--no-graphs", action="store_true", help="al_c3: eager launches instead of HIP graph replay")
     ap.add_argument("--compare-steps", type=int, default=300, help="the precisions' SPDHG iterates are compared "
                     "after this many steps of the run (the untimed steps after the timed ones)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -314,68 +314,115 @@ def run_path(args, precision, counts=None, leg=None):
 
 
 def run_c3(args):
-    """The multi-slice workload: the slices' SPDHG steps batched (BatchedSPDHGPoisson: every kernel of a step launched
-    once for all slices of a group), plain BF16 PF products (the residual mode's fragments would exceed the 288 GB of
-    HBM). Groups of --batch slices run one after the other (64 slices of K ~ U(500, 8000) need about 260 GB at
-    once). Prints one JSON line: slices per second (a slice = its `steps` subset steps) and the p50 / p95 of the
-    batched step time."""
+    """The multi-slice workload: 64 slices' SPDHG steps batched (BatchedSPDHGPoisson: every kernel of a step
+    launched once for all slices), the PF as BF16 hi/lo blocks of the non-zeros (bf16_residual), the steps replayed
+    from HIP graphs (every ordered subset pair captured before the timed run), projector variants autotuned on the
+    batch. The timed run is `steps` subset steps (each slice's solve: 100 SPDHG iterations at the default). Reports
+    slices/s, the p50 / p95 of the batched step, the per-slice solve time (every slice of the batch finishes with the
+    batch) and, for reference, the p50 / p95 of the slices' solve times run one at a time (SPDHGPoisson on the same
+    operators); the iterates of 3 slices against FP32 single-slice runs (CSR PF) with the same subset order. One
+    JSON line."""
     import torch
-    from diffractom_amd import SinglePhaseForwardOperator
+    from diffractom_amd import SinglePhaseForwardOperator, SPDHGPoisson
     from diffractom_amd.batched import BatchedSPDHGPoisson
     from diffractom_amd.workloads import C3, c3_slices, poisson_counts
     t0 = time.perf_counter()
     cfg, mat, slices = c3_slices(args.slices)
     n_all = len(slices)
-    ms_all, Ks = [], []
-    run_s = 0.0
-    for g0 in range(0, n_all, args.batch):
-        ops, bs, masks = [], [], []
-        for i in range(g0, min(g0 + args.batch, n_all)):
-            grid, mask, truth = slices[i]
-            op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="dense",
-                                            pf_precision="bf16")
-            b, _ = poisson_counts(op.direct(truth), C3["mean_counts"], C3["seed"] + i)
-            ops.append(op)
-            bs.append(b)
-            masks.append(mask)
-            slices[i] = None
-        Ks += [op.K for op in ops]
-        log(f"[bench] al_c3: slices {g0}..{g0 + len(ops) - 1} set up ({time.perf_counter() - t0:.0f} s, "
-            f"{torch.cuda.memory_allocated() / 1e9:.0f} GB)")
-        bat = BatchedSPDHGPoisson(ops, subsets=C3["subsets"], support=masks, seed=0)
-        bat.start(None, bs)
-        if not args.no_autotune:
-            tuning = bat.autotune()
-            log("[bench] al_c3: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    q = "bf16_residual"
+    ops, bs, masks, grids = [], [], [], []
+    for i, (grid, mask, truth) in enumerate(slices):
+        op = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="sparse",
+                                        pf_precision=q)
+        b, _ = poisson_counts(op.direct(truth), C3["mean_counts"], C3["seed"] + i)
+        ops.append(op)
+        bs.append(b)
+        masks.append(mask)
+        grids.append(grid)
+        if (i + 1) % 16 == 0:
+            log(f"[bench] al_c3: {i + 1} slices set up ({time.perf_counter() - t0:.0f} s)")
+    Ks = [op.K for op in ops]
+    log(f"[bench] al_c3: {n_all} slices set up ({time.perf_counter() - t0:.0f} s, "
+        f"{torch.cuda.memory_allocated() / 1e9:.0f} GB, PF {sum(op.device_bytes() for op in ops) / 1e9:.1f} GB)")
+    bat = BatchedSPDHGPoisson(ops, subsets=C3["subsets"], support=masks, seed=0, graphs=not args.no_graphs)
+    bat.start(None, bs)
+    if not args.no_autotune:
+        tuning = bat.autotune()
+        log("[bench] al_c3: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    torch.cuda.synchronize()
+    bat.advance(args.warmup)
+    torch.cuda.synchronize()
+    bat.finish()
+    # the timed solve: a new batch from x = 0 with the subset order of seed 0 (the warm-up drew from the first's)
+    del bat
+    bat = BatchedSPDHGPoisson(ops, subsets=C3["subsets"], support=masks, seed=0, graphs=not args.no_graphs)
+    bat.start(None, bs)
+    if bat.graphs:
+        log(f"[bench] al_c3: {bat.capture_all()} step graphs captured")
+    torch.cuda.synchronize()
+    evs = []
+    w0, w1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0.record()
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        bat.advance(1)
+        e1.record()
+        evs.append((e0, e1))
+    w1.record()
+    torch.cuda.synchronize()
+    ms = np.array([a.elapsed_time(b) for a, b in evs])
+    run_s = w0.elapsed_time(w1) * 1e-3
+    log(f"[bench] al_c3: {args.steps} batched steps in {run_s:.3f} s ({ms.mean():.3f} ms per step)")
+    xs = bat.finish()
+    # FP32 single-slice runs of 3 slices (smallest, median, largest K), the same subset order (seed)
+    cmp = {}
+    for i in sorted({int(np.argmin(Ks)), int(np.argsort(Ks)[len(Ks) // 2]), int(np.argmax(Ks))}):
+        op32 = SinglePhaseForwardOperator(cfg, mat, grids[i], max_gb=1024.0, normalized=True, pf_mode="sparse",
+                                          pf_precision="fp32")
+        sol = SPDHGPoisson(op32, subsets=C3["subsets"], support=masks[i], seed=0)
+        sol.start(torch.zeros(op32.coeff_shape, dtype=torch.float32, device="cuda"), bs[i])
+        sol.advance(args.steps)
+        x32 = sol.finish()
+        xb = xs[i]
+        cmp[f"slice {i} (K {Ks[i]})"] = float(torch.linalg.vector_norm((xb - x32).double()) /
+                                              torch.linalg.vector_norm(x32.double()))
+        del op32, sol, x32
+    log("[bench] al_c3: x vs FP32 after the solve: " + ", ".join(f"{k} {v:.2e}" for k, v in cmp.items()))
+    del bat, xs
+    torch.cuda.empty_cache()
+    # the slices one at a time (their own single-slice solver and launches)
+    solo = []
+    for i, op in enumerate(ops):
+        sol = SPDHGPoisson(op, subsets=C3["subsets"], support=masks[i], seed=0)
+        sol.start(torch.zeros(op.coeff_shape, dtype=torch.float32, device="cuda"), bs[i])
         torch.cuda.synchronize()
-        bat.advance(args.warmup)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sol.advance(args.steps)
+        e1.record()
         torch.cuda.synchronize()
-        evs = []
-        for _ in range(args.steps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            bat.advance(1)
-            e1.record()
-            evs.append((e0, e1))
-        torch.cuda.synchronize()
-        ms = [a.elapsed_time(b) for a, b in evs]
-        ms_all += ms
-        run_s += sum(ms) * 1e-3
-        log(f"[bench] al_c3: group of {len(ops)} slices (K sum {sum(op.K for op in ops)}): "
-            f"{np.mean(ms):.3f} ms per batched step")
-        del bat, ops, bs
-        torch.cuda.empty_cache()
-    ms = np.array(ms_all)
+        solo.append(e0.elapsed_time(e1) * 1e-3)
+        sol.finish()
+        del sol
+    solo = np.array(solo)
+    log(f"[bench] al_c3: one slice at a time: {solo.sum():.2f} s for all, p50 {np.percentile(solo, 50):.3f} s, "
+        f"p95 {np.percentile(solo, 95):.3f} s per slice")
     line = {"metric": "SPDHG multi-slice batch: slices/s", "value": round(n_all / run_s, 4),
             "unit": f"slices/s ({args.steps} subset steps per slice)", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(float(ms.mean()), 4), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[q],
             "data": "synthetic (jittered-grain grids, Voronoi grains in disc masks, Poisson counts)",
-            "config": {"workload": f"al_c3: {n_all} slices in groups of {args.batch}, 128x128 px, 180 omega x 131 "
-                                   f"x 180 eta x 5 rings, K {min(Ks)}..{max(Ks)}, disc masks r 15..60 px, M=10",
-                       "slices": n_all, "K_sum": int(sum(Ks))},
+            "config": {"workload": f"al_c3: {n_all} slices in one batch, 128x128 px, 180 omega x 131 x 180 eta x 5 "
+                                   f"rings, K {min(Ks)}..{max(Ks)}, disc masks r 15..60 px, M=10, {PF_DESC[q]}, "
+                                   f"{'HIP graphs' if not args.no_graphs else 'eager launches'}",
+                       "slices": n_all, "K_sum": int(sum(Ks)), "precision": q},
             "step_ms_p50": round(float(np.percentile(ms, 50)), 4), "step_ms_p95": round(float(np.percentile(ms, 95)), 4),
-            "slice_steps_per_s": round(n_all * args.steps / run_s, 1)}
+            "slice_solve_s_batched": round(run_s, 4),
+            "slice_solve_s_alone_p50": round(float(np.percentile(solo, 50)), 4),
+            "slice_solve_s_alone_p95": round(float(np.percentile(solo, 95)), 4),
+            "slices_per_s_alone": round(n_all / float(solo.sum()), 4),
+            "rel_l2_x_vs_fp32": cmp}
     print(json.dumps(line), flush=True)
 
 

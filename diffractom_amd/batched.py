@@ -62,7 +62,8 @@ class BatchedSPDHGPoisson:
     orientation batch each); supports: one support option per slice (as SPDHGPoisson's `support`), or one for
     all."""
 
-    def __init__(self, operators, subsets=8, support="fov", eps_counts=1e-3, rho="auto", seed=0, variant=2):
+    def __init__(self, operators, subsets=8, support="fov", eps_counts=1e-3, rho="auto", seed=0, variant=2,
+                 graphs=False):
         ops = list(operators)
         if not ops:
             raise ValueError("no slices")
@@ -94,6 +95,11 @@ class BatchedSPDHGPoisson:
         self.offsets = [int(v) for v in offs[:-1]]
         self.Kc = int(offs[-1])
         self.iter_stats = []
+        # graphs: advance() replays each (subset, next subset) step from a captured HIP graph after running it
+        # eagerly once and capturing it the second time (the launches of a step: dual update, grouped PF adjoint,
+        # backprojection, forward projection, grouped PF forward; no host work, no allocation, no memset)
+        self.graphs = bool(graphs)
+        self._graph_cache, self._graph_seen = {}, set()
 
     # ---------------------------------------------------------------- order (spdhg_poisson.py:363-370)
     _epoch_order = SPDHGPoisson._epoch_order
@@ -150,6 +156,8 @@ class BatchedSPDHGPoisson:
         n_max = max(len(s) for s in self.subsets)
         self.sino = torch.empty(max(n_max, R) * My * self.Kc, dtype=torch.float32, device=dev)
         self._groups = {}
+        self._graph_cache.clear()
+        self._graph_seen.clear()
         self._seq = self._epoch_sequence()
         self._pos = 0
         self._forward(self._seq[0])
@@ -270,9 +278,11 @@ class BatchedSPDHGPoisson:
         o0._radon_forward(self.xk, self.Kc, 0, self.Kc, self.sino, geo_f, n_ang, o0._tables(m))
         self._pf(True, m)
 
-    def step(self, next_block=None, full_forward=False):
+    def step(self, next_block=None, full_forward=False, block=None):
+        """The step of subset `block` (default: the current one of the epoch order), then the forward of
+        next_block or, with full_forward, of every angle."""
         o0 = self.ops[0]
-        blk = self._seq[self._pos]
+        blk = self._seq[self._pos] if block is None else block
         row = o0.My * o0.N_seg
         rows = self._rows[blk]
         _lib.call("dfm_dual_update_sub", _lib.ptr(self.y), _lib.ptr(self.D), _lib.ptr(self.sig), _lib.ptr(self.b),
@@ -287,16 +297,57 @@ class BatchedSPDHGPoisson:
         elif next_block is not None:
             self._forward(next_block)
 
+    def _run_step(self, nb):
+        """One step followed by the forward of subset nb (the full forward when nb repeats the current subset),
+        from a captured graph when graphs are on."""
+        blk = self._seq[self._pos]
+        full = nb == blk
+        if not self.graphs or full or self.ops[0].queue is not None:
+            self.step(nb, full_forward=full)
+            return
+        key = (blk, nb)
+        g = self._graph_cache.get(key)
+        if g is None and key in self._graph_seen:
+            g = self._capture(blk, nb)
+        if g is None:
+            self._graph_seen.add(key)
+            self.step(nb)
+        else:
+            g.replay()
+
+    def _capture(self, blk, nb):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.step(nb, block=blk)
+        self._graph_cache[(blk, nb)] = g
+        return g
+
+    def capture_all(self):
+        """Capture the step of every ordered pair of distinct subsets (graphs on; after start()): the descriptors
+        and tables of every subset are made first, eagerly, so nothing is allocated while capturing. Returns the
+        number of graphs."""
+        if not self.graphs:
+            raise ValueError("capture_all needs graphs=True")
+        o0 = self.ops[0]
+        for m in range(self.M):
+            self._group(True, m)
+            self._group(False, m)
+            o0._tables(m)
+        torch.cuda.synchronize(o0.device)
+        for a in range(self.M):
+            for b in range(self.M):
+                if a != b and (a, b) not in self._graph_cache:
+                    self._capture(a, b)
+        return len(self._graph_cache)
+
     def advance(self, n_steps):
         for _ in range(int(n_steps)):
             if self._pos + 1 < len(self._seq):
-                nb = self._seq[self._pos + 1]
-                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self._run_step(self._seq[self._pos + 1])
                 self._pos += 1
             else:
                 nseq = self._epoch_sequence()
-                nb = nseq[0]
-                self.step(nb, full_forward=nb == self._seq[self._pos])
+                self._run_step(nseq[0])
                 self._seq, self._pos = nseq, 0
 
     def deviances(self):

@@ -202,7 +202,7 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
             "sweep_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
                                    radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
-            "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p,
+            "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p, "pairs_ok": pairs_ok,
             "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
             "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
             "pair_w": pw.astype(np.float32)}
@@ -445,7 +445,7 @@ class SinglePhaseForwardOperator:
     def _radon_tables(self, idx):
         """Device copies of radon_tables() for the angles idx (compact order)."""
         t = radon_tables(self._geo64, idx, self.Nx, self.Ny, self.My, self.radon_scale, self.radon_bands)
-        return {k: torch.from_numpy(v).to(self.device) for k, v in t.items()}
+        return {k: torch.from_numpy(v).to(self.device) if isinstance(v, np.ndarray) else v for k, v in t.items()}
 
     def _make_batches(self, max_gb, K_batch=None):
         """Orientation batches (single_phase_forward_operator.py:1026-1093): as many orientations as a dense PF batch
@@ -1037,7 +1037,7 @@ class SinglePhaseForwardOperator:
                 for n, w, cq in ((2, 1, 8), (1, 1, 8)):
                     if 2 * n * ln * (cq + 1) * 16 <= 163840 and (w < 4 or self.My <= 512):
                         cands[f"sweep:{n}+ch{4 * cq}" + ("/512" if w >= 4 else "")] = ("sweep", n + 16 * w)
-            if tab["sweep_grp_p"][0, 0] >= 0:
+            if tab["pairs_ok"]:
                 ln = max(self.Nx, self.Ny)
                 cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4) if 2 * n * (ln + 4) * 128 <= 163840})
             if self.radon_bands > 1:
@@ -1245,7 +1245,7 @@ class SinglePhaseForwardOperator:
             self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
                                 kn, cw)
             return
-        if tab is not None and self.radon_variant["fwd_kernel"] == "psweep" and tab["sweep_grp_p"][0, 0] >= 0:
+        if tab is not None and self.radon_variant["fwd_kernel"] == "psweep" and tab["pairs_ok"]:
             g = tab["sweep_grp_p"]
             _lib.call("dfm_radon_forward_pairs_sweep", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["sweep_geo"]),
                       _lib.ptr(g), _lib.ptr(tab["sweep_gcls_p"]), g.shape[0], g.shape[1], self.My, self.Nx, self.Ny,

@@ -104,3 +104,31 @@ def test_batch_of_block_sparse_slices(precision):
         print(f"  slice {i} (K={op.K}): rel L2 {err:.2e}", flush=True)
         assert err < 1e-5
         assert abs(f_b[i] / sol.iter_stats[-1]["f"] - 1) < 1e-6
+
+
+def test_batched_graph_replay_matches_eager():
+    """Batched steps replayed from HIP graphs (every ordered subset pair captured up front, then the epoch order
+    replayed with the solver's own eager work in between) give the eager iterates bit for bit, over several epochs."""
+    import torch
+    from diffractom_amd.batched import BatchedSPDHGPoisson
+    print("case batched graphs", flush=True)
+    c = cfg(nx=20, ny=20, my=21, n_omega=24, angle_range=(0, 180), cor=0.3)
+    ops, bs = [], []
+    rng = np.random.default_rng(21)
+    for i, K in enumerate((40, 90)):
+        op, orc, _ = build(c, K=K, seed=9 + i, pf_mode="sparse", pf_precision="bf16_residual")
+        xt = (rng.random(op.coeff_shape) * op.support_mask()[None]).astype(np.float32)
+        ax = orc.direct(xt)
+        bs.append(rng.poisson(ax / max(ax.mean(), 1e-9) * 40).astype(np.float32))
+        ops.append(op)
+    res = []
+    for graphs in (False, True):
+        bat = BatchedSPDHGPoisson(ops, subsets=4, seed=3, graphs=graphs)
+        bat.start(None, bs)
+        if graphs:
+            assert bat.capture_all() == 12
+        for _ in range(5):
+            bat.advance(4)
+            torch.cuda.synchronize()
+        res.append(bat.xk.clone())
+    assert torch.equal(res[0], res[1])
