@@ -307,7 +307,9 @@ def run_path(args, precision, counts=None, leg=None):
         leg["cpu"], leg["sample"] = cpu_baseline_leg(args, op, b, M)
     fp64 = fp64_errors(op, leg.get("sample")) if leg is not None else None
     res = {"op": op, "M": M, "ms_step": ms_step, "op_nnz": op.sparse_nnz, "counts": b, "fp64": fp64,
-           "tuning": {k: v[0] for k, v in (op.tuning or {}).items()}, "ktimes": ktimes, "setup_s": setup,
+           "tuning": {k: v[0] for k, v in (op.tuning or {}).items()},
+           "tuning_ms": {k: {c: round(t, 4) for c, t in v[1].items()} for k, v in (op.tuning or {}).items()},
+           "ktimes": ktimes, "setup_s": setup,
            "t_fwd": float(np.median(tf[2:])) if tf else None, "t_adj": float(np.median(ta[2:])) if ta else None,
            "fwd_out": data, "adj_out": out, "x_iter": x_iter, "steps_total": n_done}
     return res
@@ -756,6 +758,7 @@ def main():
         e = {"steps_per_s": round(1000.0 / r["ms_step"], 3), "ms_per_step": round(r["ms_step"], 4),
              "forward_ms": r["t_fwd"] and round(r["t_fwd"], 4), "adjoint_ms": r["t_adj"] and round(r["t_adj"], 4),
              "kernel_ms": {k: round(v, 5) for k, v in (r["ktimes"] or {}).items()}, "tuned": r["tuning"],
+             "autotune_candidates_ms": r.get("tuning_ms"),
              "rel_l2_vs_fp64_oracle": r["fp64"]}
         if ref is not None and q != "fp32":
             e["rel_l2_vs_fp32"] = {"forward": rel_l2(r["fwd_out"], ref["fwd_out"]),

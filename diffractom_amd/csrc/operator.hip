@@ -543,19 +543,16 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
 constexpr int TCH = 32;   // angles whose table rows are staged at once
 
 template <int MODE, int ACH, int G, int TS, int QW>
-__global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_group_kernel(const float4* __restrict__ sino,
-                                                                   float4* __restrict__ out,
-                                                                   const int4* __restrict__ ptab,
-                                                                   const int* __restrict__ tbase, int nx, int ny,
-                                                                   int ns, int na, int kstride4, int k4_off, int k4_n,
-                                                                   int skstride4, float4* __restrict__ x,
-                                                                   float4* __restrict__ z,
-                                                                   const float4* __restrict__ T, float inv_p,
-                                                                   float lam) {
+__global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_group_kernel(
+    const float4* __restrict__ sino, float4* __restrict__ out, const int4* __restrict__ ptab,
+    const int* __restrict__ tbase, int nx, int ny, int ns, int na, int kstride4, int k4_off, int k4_n, int skstride4,
+    float4* __restrict__ x, float4* __restrict__ z, const float4* __restrict__ T, float inv_p, float lam) {
   constexpr int NPX = TS * TS, NTH = NPX * QW;    // QW float4 quads of every pixel per channel block
   constexpr int TBB = TS == 4 ? TB : 16;   // bins a tile reaches at one angle (host-checked)
-  __shared__ float4 lds[ACH * TBB * QW];   // ACH angles x TBB bins x QW quads
-  __shared__ int4 lpt[TCH * NPX];          // pixel-table rows of the tile's pixels for TCH angles
+  __shared__ float4 lds[ACH * TBB * QW];  // ACH angles x TBB bins x QW quads
+  // pixel-table rows of the tile's pixels for TCH angles, as staged: the byte offsets of the two bins' rows within
+  // the angle's LDS slab ((bin - first bin of the tile) x QW quads x 16 B) and the two weights
+  __shared__ int4 lpt[TCH * NPX];
   __shared__ int lbase[TCH];
   static_assert(ACH * TBB * QW % NTH == 0 && TCH % ACH == 0, "whole staging rounds");
   const int tnx = (nx + TS - 1) / TS, tny = (ny + TS - 1) / TS;
@@ -571,14 +568,23 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
   const int64_t pix = (int64_t)py * nx + px;
   const int64_t npix = (int64_t)nx * ny;
   const bool tables_once = na <= TCH;
-  if (tables_once) {
-    if (threadIdx.x < na) lbase[threadIdx.x] = tbase[(int64_t)threadIdx.x * ntiles + tile];
-    for (int i = threadIdx.x; i < na * NPX; i += NTH) {
+  // stage the table rows of angles t0 .. t0 + tn - 1: bins as byte offsets from the tile's first bin
+  auto stage_tables = [&](int t0, int tn) {
+    if (threadIdx.x < tn) lbase[threadIdx.x] = tbase[(int64_t)(t0 + threadIdx.x) * ntiles + tile];
+    for (int i = threadIdx.x; i < tn * NPX; i += NTH) {
       const int al = i / NPX, l = i % NPX;
       const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
-      lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)al * npix + (int64_t)ppy * nx + ppx] : make_int4(0, 0, 0, 0);
+      int4 e = make_int4(0, 0, 0, 0);
+      if (ppx < nx && ppy < ny) {
+        e = ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx];
+        const int b0 = tbase[(int64_t)(t0 + al) * ntiles + tile];
+        e.x = (e.x - b0) * QW * (int)sizeof(float4);
+        e.y = (e.y - b0) * QW * (int)sizeof(float4);
+      }
+      lpt[i] = e;
     }
-  }
+  };
+  if (tables_once) stage_tables(0, na);
   for (int gi = 0; gi < G; ++gi) {
     const int cb = grp * G + gi;
     if (cb >= n_cb) break;                                // uniform over the workgroup
@@ -596,13 +602,7 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
       const int tn = min(TCH, na - t0);
       if (!tables_once) {
         __syncthreads();
-        if (threadIdx.x < tn) lbase[threadIdx.x] = tbase[(int64_t)(t0 + threadIdx.x) * ntiles + tile];
-        for (int i = threadIdx.x; i < tn * NPX; i += NTH) {
-          const int al = i / NPX, l = i % NPX;
-          const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
-          lpt[i] = (ppx < nx && ppy < ny) ? ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx]
-                                          : make_int4(0, 0, 0, 0);
-        }
+        stage_tables(t0, tn);
       }
       // PIPE: software-pipelined passes, the sinogram loads of pass a0 + ACH in flight while pass a0 is summed (for
       // the 512 / 1024-thread forms; in the 256-thread ones the 8 more VGPRs cost more in occupancy than they gain)
@@ -638,17 +638,24 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
         __syncthreads();
         if (PIPE && a0 + ACH < tn) load_pass(a0 + ACH);
         if (live) {
-#pragma unroll 4
-          for (int al = 0; al < nan; ++al) {
+          // per angle: one table read, two LDS addresses (byte offset + this lane's quad), 8 FMAs; a full pass
+          // is unrolled without branches so that its table reads are all issued first
+          const char* lq = reinterpret_cast<const char*>(lds + q);
+          auto angle = [&](int al) {
             const int4 e = lpt[(a0 + al) * NPX + li];
-            const int base = lbase[a0 + al];
             const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
-            const float4 s0 = lds[(al * TBB + (e.x - base)) * QW + q];
-            const float4 s1 = lds[(al * TBB + (e.y - base)) * QW + q];
-            acc.x += w0 * s0.x + w1 * s1.x;
-            acc.y += w0 * s0.y + w1 * s1.y;
-            acc.z += w0 * s0.z + w1 * s1.z;
-            acc.w += w0 * s0.w + w1 * s1.w;
+            const float4 s0 = reinterpret_cast<const float4*>(lq + e.x)[al * TBB * QW];
+            const float4 s1 = reinterpret_cast<const float4*>(lq + e.y)[al * TBB * QW];
+            acc.x = fmaf(w1, s1.x, fmaf(w0, s0.x, acc.x));
+            acc.y = fmaf(w1, s1.y, fmaf(w0, s0.y, acc.y));
+            acc.z = fmaf(w1, s1.z, fmaf(w0, s0.z, acc.z));
+            acc.w = fmaf(w1, s1.w, fmaf(w0, s0.w, acc.w));
+          };
+          if (nan == ACH) {
+#pragma unroll
+            for (int al = 0; al < ACH; ++al) angle(al);
+          } else {
+            for (int al = 0; al < nan; ++al) angle(al);
           }
         }
       }

@@ -277,14 +277,27 @@ __global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
       }
     }
   }
-  if (live) {
+  // ---- stores through a wave-private LDS transpose: lane l holds its two rays' 8 quads (128 B of a sinogram row
+  // each); after the transpose, store instruction j has lanes 8 i .. 8 i + 7 write the 8 quads of lane 8 j + i's
+  // row, i.e. 8 whole 128-byte lines per instruction (with per-lane 16-byte stores the L2 wrote the rows back in
+  // 32-byte pieces: twice the sinogram's bytes)
+  __syncthreads();                                           // every wave is done with the staged lines
+  float4* wl = limg + (t >> 6) * (64 * CQ);
+  const int src = lane >> 3, qd = lane & (CQ - 1);
 #pragma unroll
-    for (int qq = 0; qq < CQ; ++qq) {
-      const int q = (qq + rot) & (CQ - 1);
-      if (q0 + q >= k4_n) continue;
-      if (bin_l < ns) sino[((int64_t)a * ns + bin_l) * skstride4 + q0 + q] = accL[qq];
-      if (bin_r < ns) sino[((int64_t)a * ns + bin_r) * skstride4 + q0 + q] = accR[qq];
+  for (int side = 0; side < 2; ++side) {
+#pragma unroll
+    for (int qq = 0; qq < CQ; ++qq) wl[lane * CQ + ((qq + rot) & (CQ - 1))] = side ? accR[qq] : accL[qq];
+    __builtin_amdgcn_wave_barrier();
+    const int bin = side ? bin_r : bin_l;
+#pragma unroll
+    for (int j = 0; j < CQ; ++j) {
+      const int sl = 8 * j + src;                             // the lane whose row this lane stores a quad of
+      const float4 val = wl[sl * CQ + qd];
+      const int a_s = __shfl(a, sl, 64), bin_s = __shfl(bin, sl, 64);
+      if (a_s >= 0 && bin_s < ns && q0 + qd < k4_n) sino[((int64_t)a_s * ns + bin_s) * skstride4 + q0 + qd] = val;
     }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -312,7 +325,9 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
                 "dfm_radon_forward_pairs_sweep: bad sizes (na_grp * ceil(ns / 2) <= 512, image <= 256 x 256)");
   DFM_CHECK_ARG(lines == 1 || lines == 2 || lines == 4, "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4",
                 lines);
-  const size_t lds = (size_t)2 * lines * (len + 4) * 8 * sizeof(float4);
+  // staged lines, or the store transpose (64 lanes x 8 quads per wave), whichever is larger
+  size_t lds = (size_t)2 * lines * (len + 4) * 8 * sizeof(float4);
+  if (lds < (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4)) lds = (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4);
   DFM_CHECK_ARG(lds <= 163840, "dfm_radon_forward_pairs_sweep: %d lines of %d pixels exceed the LDS", lines, len);
   if (n_grp == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
