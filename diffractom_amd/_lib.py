@@ -87,6 +87,10 @@ SIGNATURES = {
     "dfm_pfblk_desc_size": [],
     "dfm_pfgen": [I32, P, P, P, P, P, P, P, I32, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, F32,
                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P, P, I32, P, P],
+    "dfm_pf_grad": [P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, P, P, P, I32, I32, I32, I32, I32, I32, I32,
+                    I32, I32, F32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P],
+    "dfm_pf_grad_reduce": [P, P, I32, I32, I32, I64, I32, P, P, P],
+    "dfm_angle_dots": [P, P, I32, I32, I32, I32, P, P],
 }
 
 DFM_ERR_ARG, DFM_ERR_UNSUPPORTED, DFM_ERR_LAUNCH, DFM_ERR_MEMORY = -1, -2, -3, -4

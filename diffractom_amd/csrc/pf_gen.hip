@@ -3,22 +3,16 @@
 // angles of one call only, no stored PF.
 //
 // A workgroup owns one angle r and 32 orientations (an orientation block). It first marks the candidate entries in
-// an LDS bitmap (one word per 32 segments and orientation, words in (segment block, orientation) order): for every
-// orientation k, pole axis a and omega node s the pole direction n = U_k h_a is taken into the detector frame of the
-// node (x, y, z = n . R^T d0, n . R^T d90, n . R^T p0), where the probed directions of ring p form the cone
-// v(eta) = cos(th_p) (cos(eta) R^T d0 + sin(eta) R^T d90) - sin(th_p) R^T p0, so that n . v = A cos(eta - phi) - B
-// with A = cos(th_p) sqrt(x^2 + y^2), B = sin(th_p) z, phi = atan2(y, x). |n . v| > tau = 1 - cut sigma^2 - 1e-4 (a
-// superset of the entries whose exponent (1 - |n . v|) / sigma^2 stays below the cut) holds on eta windows
-// phi +- acos((tau + B) / A) (for n) and phi + pi +- acos((tau - B) / A) (for -n); every eta bin with a quadrature
-// node in a window is a candidate of ring p. The bitmap's set bits, read in word then bit order, are the block-sparse
-// form's element order (segment block, then orientation, then segment). They are expanded into an LDS list (at most
-// LIST_CAP at a time) and evaluated one candidate per thread with the stored mode's own entry function (pf_entry: all
-// axes of the ring, all nodes, float64), so the values equal the stored PF bit for bit; a workgroup-wide scan of the
-// non-zero flags gives each non-zero its place. The blocks of one (angle, orientation block) are consecutive in the
-// pointer array, so the workgroup's non-zeros form one run from the first block's pointer and the stores are
-// coalesced. Count mode adds the non-zeros per block instead. The counts are taken once per angle set; the pointers
-// of an angle list are their exclusive scan.
-#include "dfm_common.h"
+// an LDS bitmap (one word per 32 segments and orientation, words in (segment block, orientation) order) from the eta
+// windows of pf_cand.h. The bitmap's set bits, read in word then bit order, are the block-sparse form's element
+// order (segment block, then orientation, then segment). They are expanded into an LDS list (at most LIST_CAP at a
+// time) and evaluated one candidate per thread with the stored mode's own entry function (pf_entry: all axes of the
+// ring, all nodes, float64), so the values equal the stored PF bit for bit; a workgroup-wide scan of the non-zero
+// flags gives each non-zero its place. The blocks of one (angle, orientation block) are consecutive in the pointer
+// array, so the workgroup's non-zeros form one run from the first block's pointer and the stores are coalesced.
+// Count mode adds the non-zeros per block instead. The counts are taken once per angle set; the pointers of an
+// angle list are their exclusive scan.
+#include "pf_cand.h"
 
 namespace {
 
@@ -70,44 +64,9 @@ __global__ void __launch_bounds__(G_THREADS) pf_gen_kernel(GenArgs g) {
   __syncthreads();
 
   // ---- candidates
-  const int items = 32 * g.A * g.S;
-  for (int i = tid; i < items; i += G_THREADS) {
-    const int kl = i & 31;
-    const int rest = i >> 5;
-    const int ax = rest % g.A, s = rest / g.A;
-    const int kloc = kbi * 32 + kl;
-    if (kloc >= g.kb) continue;
-    const int kg = g.k0 + kloc;
-    int p = 0;
-    while (p + 1 < g.P && g.axis_start[p + 1] <= ax) ++p;
-    const double tau = 1.0 - g.cut / g.inv_sigma2[kg] - 1e-4;
-    const double* n = g.poles + ((int64_t)kg * g.A + ax) * 3;
-    const double* f = g.frames + ((int64_t)r * g.S + s) * 9;
-    const double x = n[0] * f[0] + n[1] * f[1] + n[2] * f[2];
-    const double y = n[0] * f[3] + n[1] * f[4] + n[2] * f[5];
-    const double z = n[0] * f[6] + n[1] * f[7] + n[2] * f[8];
-    const double Aa = g.cos_th[p] * sqrt(x * x + y * y), B = g.sin_th[p] * z;
-    if (tau - fabs(B) >= Aa) continue;                       // neither n nor -n reaches the cone's cap
-    const double phi = atan2(y, x);
-    for (int sg = 0; sg < 2; ++sg) {
-      const double ratio = (tau + (sg ? -B : B)) / fmax(Aa, 1e-300);
-      if (ratio >= 1.0) continue;
-      const double w = ratio <= -1.0 ? 3.2 : acos(ratio) + 1e-9;
-      const double center = phi + (sg ? 3.141592653589793 : 0.0);
-      for (int sh = -1; sh <= 1; ++sh) {                    // eta is an angle: the window and its turns
-        const double cc = center + sh * 6.283185307179586;
-        // bin c has its nodes at eta0 + (c + x) deta, xlo <= x <= xhi
-        int m_lo = (int)ceil((cc - w - g.eta0) / g.deta - g.xhi);
-        int m_hi = (int)floor((cc + w - g.eta0) / g.deta - g.xlo);
-        m_lo = max(m_lo, 0);
-        m_hi = min(m_hi, g.C - 1);
-        for (int c = m_lo; c <= m_hi; ++c) {
-          const int j = c * g.P + p;
-          atomicOr(&bm[(j >> 5) * 32 + kl], 1u << (j & 31));
-        }
-      }
-    }
-  }
+  const dfm::CandGeo cg{g.poles, g.frames, g.sin_th, g.cos_th, g.axis_start, g.inv_sigma2, g.k0, g.kb, g.C, g.P,
+                        g.A, g.S, g.cut, g.eta0, g.deta, g.xlo, g.xhi};
+  dfm::mark_candidates(cg, r, kbi, bm, tid, G_THREADS);
   __syncthreads();
 
   // ---- the candidates' order: each thread owns a run of words; exclusive scan of their bit counts

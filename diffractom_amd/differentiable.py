@@ -137,3 +137,9 @@ def total_variation(coeffs, eps=1e-8):
     dx[:, :, :-1] = c[:, :, 1:] - c[:, :, :-1]
     dy[:, :-1, :] = c[:, 1:, :] - c[:, :-1, :]
     return torch.sqrt(dx * dx + dy * dy + eps).sum()
+
+
+# the derivatives with respect to the model's parameters and their updates (the reference's
+# diffractom.differentiable namespace)
+from .gradients import (PF_GROUPS, PFGradient, ProjGradient, rotate_orientations, set_geometry,  # noqa: E402
+                        set_pf_parameters, set_proj_parameters)

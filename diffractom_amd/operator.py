@@ -124,14 +124,18 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
     fwd_scale = (radon_scale * g["inv_a"][idx])[:, None]
     bwd_scale = (g["w"][idx] * g["inv_a"][idx])[:, None]
     na = len(idx)
-    rows, pix, wts = [], [], []
+    rows, pix, wts, dwts = [], [], [], []
     a_idx = np.broadcast_to(np.arange(na)[:, None], t.shape)
     p_idx = np.broadcast_to(np.arange(npix)[None], t.shape)
     for s_, k_, v_ in zip(taps, kern, valid):
         rows.append((a_idx * ns + s_)[v_])
         pix.append(p_idx[v_])
         wts.append((fwd_scale * k_)[v_])
-    rows, pix, wts = np.concatenate(rows), np.concatenate(pix), np.concatenate(wts)
+        # d K / d t of the footprint (the weight of a detector shift): -sign(t - s) where hw - |t - s| lies strictly
+        # between 0 and top (differentiable/reference.py:213-223)
+        q_ = hw - np.abs(t - s_)
+        dwts.append((fwd_scale * np.where((q_ > 0.0) & (q_ < top), -np.sign(t - s_), 0.0))[v_])
+    rows, pix, wts, dwts = np.concatenate(rows), np.concatenate(pix), np.concatenate(wts), np.concatenate(dwts)
     order = np.lexsort((pix, rows))
     row_ptr = np.zeros(na * ns + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=na * ns), out=row_ptr[1:])
@@ -203,7 +207,8 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
                                    radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
             "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p, "pairs_ok": pairs_ok,
-            "w": wts[order].astype(np.float32), "ptab": ptab, "tbase": base.astype(np.int32),
+            "w": wts[order].astype(np.float32), "dw": dwts[order].astype(np.float32), "ptab": ptab,
+            "tbase": base.astype(np.int32),
             "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
             "pair_w": pw.astype(np.float32)}
 
@@ -315,7 +320,9 @@ class SinglePhaseForwardOperator:
         self.pf_storage = None
         self.sparse_nnz = 0
         self.sparse_fill = 0.0
+        self._pf_build_args = (pf_mode, sparse_max_fill, sparse_max_gb)
         self._build_pf(pf_mode, sparse_max_fill, sparse_max_gb)
+        self.pf_version = 0
         self.setup_s = time.perf_counter() - t0
         if verbose:
             print(f"PF matrix: {self.pf_mode} ({self.pf_storage}), {len(self.batches)} batches of up to "
@@ -336,6 +343,8 @@ class SinglePhaseForwardOperator:
         lam = KEV_ANGSTROM / self.cfg["energy"]
         self.two_theta_peaks = (2.0 * np.arcsin(np.linalg.norm(self.h_cpu, axis=1) / (4.0 * np.pi) * lam)
                                 ).astype(np.float32)
+        if getattr(self, "_ring_two_theta", None) is not None:          # set_pf_parameters / set_geometry
+            self.two_theta_peaks = np.asarray(self._ring_two_theta, np.float32).copy()
         Se = self.N_eta_subdivisions
         eta2 = self.eta_angles_subdivided.reshape(self.N_eta, Se)
         d0 = np.array(self.cfg["detector_direction_origin"])
@@ -403,8 +412,10 @@ class SinglePhaseForwardOperator:
         ang = self.angles
         X = np.cos(ang - 0.5 * np.pi) / rho
         Y = np.sin(ang - 0.5 * np.pi) / rho
-        shift = np.full(self.N_Omega, self.cor_offset)
-        T0 = (self.My - 1) / 2.0 - X * (self.Nx - 1) / 2.0 - Y * (self.Ny - 1) / 2.0 - shift / ds
+        # the centre of rotation and a detector shift of every projection (bins; set_proj_parameters)
+        shift = getattr(self, "angle_shift", None)
+        shift = np.zeros(self.N_Omega) if shift is None else np.asarray(shift, np.float64)
+        T0 = (self.My - 1) / 2.0 - X * (self.Nx - 1) / 2.0 - Y * (self.Ny - 1) / 2.0 - self.cor_offset / ds + shift
         self.radon_scale = float(np.float32(dx * dx / ds))
         self._bp_scale = 1.0 if self.voxel_size is None else self.voxel_size ** 2
         w = np.full(self.N_Omega, np.pi / self.N_Omega * self._bp_scale)
@@ -698,29 +709,10 @@ class SinglePhaseForwardOperator:
         over every angle per orientation batch (non-zeros per block (angle, 32 orientations, 32 segments), kept),
         and for fp16 a power-of-two scale per ring from the entries' bound 2 |G_p| max_k 1 / (8 pi sigma_k^2)
         (the pole images of a ring, both exponential terms at their peak)."""
-        from scipy.spatial.transform import Rotation
         R, CP, P = self.N_Omega, self.N_seg, self.N_peaks
-        c = self.cfg
-        e0, e1 = (float(v) for v in self.eta_angle_range)
-        if e1 <= e0:
-            raise NotImplementedError("the generated PF needs an increasing eta range")
         s2 = np.asarray(self.sigma_cpu, np.float64) ** 2
-        if float(np.max(self.pf_cut * s2)) >= 1.0 - 1e-3:
-            raise NotImplementedError("the generated PF needs cut * sigma^2 < 1 (kernels narrower than about 20 deg)")
-        k0 = np.asarray(c["k_direction_0"], np.float64)
-        Rm = Rotation.from_rotvec(self.angles_subdivided[:, None] * k0).as_matrix()          # (R S, 3, 3)
-        dirs = np.stack([np.asarray(c["detector_direction_origin"], np.float64),
-                         np.asarray(c["detector_direction_positive_90"], np.float64),
-                         np.asarray(c["p_direction_0"], np.float64)])                      # (3, 3)
-        frames = np.einsum("oij,di->odj", Rm, dirs).reshape(R * self.N_Omega_subdivisions, 9)
-        half = self.two_theta_peaks.astype(np.float64) / 2.0
-        deta = (e1 - e0) / self.N_eta
-        x = (np.asarray(self.eta_angles_subdivided[: self.N_eta_subdivisions], np.float64) - e0) / deta
         dev = self.device
-        g = {"frames": torch.from_numpy(np.ascontiguousarray(frames)).to(dev),
-             "sin_th": torch.from_numpy(np.sin(half)).to(dev), "cos_th": torch.from_numpy(np.cos(half)).to(dev),
-             "eta0": e0, "deta": deta, "xlo": float(x.min()), "xhi": float(x.max())}
-        self._gen_geo = g
+        self._gen_geo = self._cand_geometry()
         cnt_axes = np.add.reduceat(self.axis_count_cpu.astype(np.float64), self.axis_start_cpu[:-1])
         bound = 2.0 * cnt_axes * float(np.max(1.0 / (8.0 * np.pi * s2)))
         sc = np.ldexp(1.0, 14 - np.ceil(np.log2(bound)).astype(int)).astype(np.float32)
@@ -744,6 +736,66 @@ class SinglePhaseForwardOperator:
         self.sparse_fill = nnz / (R * self.K * CP)
         self._gen_scratch = None
         self._gen_caps = self._gen_offs = self._gen_tags = None
+
+    def rebuild(self, pf=False, geometry=False, projector=False):
+        """Rebuild what depends on changed parameters (the differentiable model's updates): geometry (eta grids,
+        rotation axis, Bragg angles: the probed directions), pf (orientations, widths: the poles), both rebuild the
+        PF as it was set up (same mode, precision and budget); projector (centre of rotation, per-angle detector
+        shifts: the geometry tables; the kernel choices are kept). Omega subsets are set up again; captured graphs
+        and generated scratch are dropped."""
+        subsets = self.omega_subsets
+        if geometry:
+            self._detector_coordinates()
+        if pf or geometry:
+            self._pole_axes()
+            self._cand_geo = None
+            self.sparse_batches = self.dense_batches = self.mfma_batches = self.blk_batches = self.gen_batches = None
+            self._gen_scratch = None
+            self._gen_caps = self._gen_offs = self._gen_tags = None
+            self._build_pf(*self._pf_build_args)
+        if projector:
+            keep = (dict(self.radon_variant), self.tuning, self.fuse_projector, self.fused_channels)
+            self._projector_geometry()
+            self.radon_variant, self.tuning, self.fuse_projector, self.fused_channels = keep
+        self._scratch = None
+        self.omega_subsets = None
+        if subsets is not None:
+            self.set_omega_subsets(subsets)
+        self.scratch_generation = getattr(self, "scratch_generation", 0) + 1
+        self.pf_version = getattr(self, "pf_version", 0) + 1
+        return self
+
+    def _cand_geometry(self):
+        """The geometry of the PF's candidate eta windows (dfm_pfgen, dfm_pf_grad): omega-node frames R^T d0,
+        R^T d90, R^T p0 (R S, 9); sin / cos of half the rings' two-theta; the eta nodes' place in their bin (bin c's
+        nodes at eta0 + (c + x) deta, xlo <= x <= xhi). Device copies, cached until the geometry changes."""
+        g = getattr(self, "_cand_geo", None)
+        if g is not None:
+            return g
+        from scipy.spatial.transform import Rotation
+        c = self.cfg
+        e0, e1 = (float(v) for v in self.eta_angle_range)
+        if e1 <= e0:
+            raise NotImplementedError("the candidate eta windows need an increasing eta range")
+        s2 = np.asarray(self.sigma_cpu, np.float64) ** 2
+        if float(np.max(self.pf_cut * s2)) >= 1.0 - 1e-3:
+            raise NotImplementedError("the candidate eta windows need cut * sigma^2 < 1 (kernels narrower than about "
+                                      "20 deg)")
+        k0 = np.asarray(c["k_direction_0"], np.float64)
+        Rm = Rotation.from_rotvec(self.angles_subdivided[:, None] * k0).as_matrix()          # (R S, 3, 3)
+        dirs = np.stack([np.asarray(c["detector_direction_origin"], np.float64),
+                         np.asarray(c["detector_direction_positive_90"], np.float64),
+                         np.asarray(c["p_direction_0"], np.float64)])                      # (3, 3)
+        frames = np.einsum("oij,di->odj", Rm, dirs).reshape(self.N_Omega * self.N_Omega_subdivisions, 9)
+        half = self.two_theta_peaks.astype(np.float64) / 2.0
+        deta = (e1 - e0) / self.N_eta
+        x = (np.asarray(self.eta_angles_subdivided[: self.N_eta_subdivisions], np.float64) - e0) / deta
+        dev = self.device
+        g = {"frames": torch.from_numpy(np.ascontiguousarray(frames)).to(dev),
+             "sin_th": torch.from_numpy(np.sin(half)).to(dev), "cos_th": torch.from_numpy(np.cos(half)).to(dev),
+             "eta0": e0, "deta": deta, "xlo": float(x.min()), "xhi": float(x.max())}
+        self._cand_geo = g
+        return g
 
     def _pfgen(self, mode, b, omegas, n_ang, counts=None, ptr=None, pos=None, val=None, ring_scale=None):
         g = self._gen_geo
@@ -1298,10 +1350,11 @@ class SinglePhaseForwardOperator:
                   b["K_batch"], 0, int(accumulate), _lib.ptr(part), ks, int(self.csr_variant["kc"]),
                   self._stream())
 
-    def _pf_adjoint(self, ib, data, sino, subset):
+    def _pf_adjoint(self, ib, data, sino, subset, alpha=None):
+        """sino = alpha PF_b^T data rows for batch ib (alpha: the adjoint's R / pi by default)."""
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
-        alpha = float(self.N_Omega / np.pi)
+        alpha = float(self.N_Omega / np.pi) if alpha is None else float(alpha)
         if self.blk_batches is not None:
             self._blk_adjoint(ib, data, sino, subset, alpha)
             return

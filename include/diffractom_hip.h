@@ -327,6 +327,33 @@ int dfm_pfgen(int32_t mode, const double* coords, const double* poles, const int
               int32_t* counts, const int32_t* ptr, uint16_t* pos, void* val, int32_t fp16, const float* ring_scale,
               dfm_stream_t stream);
 
+/* ---------------------------------------------------------------- PF parameter gradients
+ * (differentiable/gradients.py:20-189 PFGradient, pf_grad.cl:42-187 pf_param_vjp / rows_to_orientations)
+ * Per (angle a of omegas, orientation k0 + k of the batch): part_k[(a kb + k) nqk + q] = the angle's share of the
+ * gradient of <r, A c> with respect to the rotation increment (q 0..2) and the width (q 3) of the orientation, with
+ * curv (a data-shaped curvature) also the Gauss-Newton block J^T diag(curv) J of (rotation, width) (q 4..13, upper
+ * triangle row-major; nqk = 14, else 4); per workgroup (a, orientation block of 32): part_g[w (3 + P) + i] = the
+ * shares of the eta offset (i 0), the axis tilt (1, 2) and the rings' 2 theta (3 + p) (a null dv_* skips its
+ * group). sino (n_ang, My, sks): the projection of c for the angle list, orientation-fastest; r (R, My, C P): the
+ * data-space derivative of the loss; dv_eta, dv_t0, dv_t1, dv_tth: float64 derivatives of the probed directions,
+ * shaped as coords; the other arguments as dfm_pfgen. Candidates from the same eta windows as dfm_pfgen. */
+int dfm_pf_grad(const double* coords, const double* poles, const int32_t* axis_start, const float* axis_count,
+                const double* inv_sigma2, const double* norm, const float* qw, const double* frames,
+                const double* sin_th, const double* cos_th, const double* dv_eta, const double* dv_t0,
+                const double* dv_t1, const double* dv_tth, const float* sino, int32_t sks, const float* r,
+                const float* curv, const int32_t* omegas, int32_t n_ang, int32_t k0, int32_t kb, int32_t My,
+                int32_t C, int32_t P, int32_t A, int32_t S, int32_t Se, float pf_cut, double eta0, double deta,
+                double xlo, double xhi, double* part_k, double* part_g, dfm_stream_t stream);
+/* out_k[k nqk + q] = sum over a < n_ang of part_k (in order); out_g[i] = sum over the n_wg rows of part_g (a fixed
+ * tree): the totals of dfm_pf_grad, deterministic. */
+int dfm_pf_grad_reduce(const double* part_k, const double* part_g, int32_t n_ang, int32_t kb, int32_t nqk,
+                       int64_t n_wg, int32_t n_glob, double* out_k, double* out_g, dfm_stream_t stream);
+/* out[a] = sum over y < my, k < kn of u[a, y, k] v[a, y, k] in float64, for sinogram-shaped u, v (n_ang, my, sks)
+ * (gradients.py:220-258 ProjGradient's shift_dot, proj_grad.cl:60-88: the PF adjoint of r against the derivative
+ * projection of c). */
+int dfm_angle_dots(const float* u, const float* v, int32_t n_ang, int32_t my, int32_t sks, int32_t kn, double* out,
+                   dfm_stream_t stream);
+
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
  * Dual update on the rows of a subset (spdhg_poisson.py:69-91 dual_update_sub), float64 arithmetic:
