@@ -91,6 +91,11 @@ SIGNATURES = {
                     I32, I32, F32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P],
     "dfm_pf_grad_reduce": [P, P, I32, I32, I32, I64, I32, P, P, P],
     "dfm_angle_dots": [P, P, I32, I32, I32, I32, P, P],
+    "dfm_nufft_pad": [P, I32, I32, I32, P, P, I64, P, P],
+    "dfm_nufft_interp": [P, P, P, P, P, P, I32, I32, I32, P, P],
+    "dfm_nufft_spread": [P, P, P, P, P, P, I32, I32, I64, P, P],
+    "dfm_nufft_crop": [P, P, P, I64, I32, F32, P, I32, I32, I32, P],
+    "dfm_spdhg_update_2d": [P, P, P, P, I64, I32, I32, I32, F32, F32, P],
 }
 
 DFM_ERR_ARG, DFM_ERR_UNSUPPORTED, DFM_ERR_LAUNCH, DFM_ERR_MEMORY = -1, -2, -3, -4

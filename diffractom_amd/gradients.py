@@ -186,6 +186,8 @@ class ProjGradient:
     cor_offset units) (gradients.py:192-258)."""
 
     def __init__(self, op):
+        if op.projector != "native":
+            raise NotImplementedError("ProjGradient needs the native projector")
         if op._tables_full is None:
             raise NotImplementedError("ProjGradient needs the two-tap projector tables (pixels no larger than a bin)")
         self.op = op

@@ -252,9 +252,9 @@ class SinglePhaseForwardOperator:
             raise NotImplementedError("window_parallax is not implemented")
         if projector not in ("auto", "native", "fft", "gratopy"):
             raise ValueError(f"projector must be 'auto', 'native', 'fft' or 'gratopy', not {projector!r}")
-        if projector in ("fft", "gratopy"):
-            raise NotImplementedError(f"projector {projector!r} is not implemented (the native projector is)")
-        self.projector = "native"
+        if projector == "gratopy":
+            raise NotImplementedError("projector 'gratopy' is not implemented (the native and fft projectors are)")
+        self.projector = "fft" if projector == "fft" else "native"
         if radon_kernel is None:
             radon_kernel = "trapezoid"
         if radon_kernel not in ("trapezoid", "hat"):
@@ -305,6 +305,7 @@ class SinglePhaseForwardOperator:
         self._detector_coordinates()
         self._pole_axes()
         self._projector_geometry()
+        self._fourier_setup()
         self.pf_batch_max_gb = float(max_gb)
         self._make_batches(max_gb)
         self.omega_subsets = None
@@ -438,6 +439,9 @@ class SinglePhaseForwardOperator:
         self.geo_fwd_cpu, self.geo_bwd_cpu = geo_f, geo_b
         self.geo_fwd = torch.from_numpy(geo_f).to(self.device)
         self.geo_bwd = torch.from_numpy(geo_b).to(self.device)
+        # the angle indices of every geometry array the projector is called with (the Fourier projector needs them)
+        all_a = torch.arange(self.N_Omega, dtype=torch.int32, device=self.device)
+        self._geo_angles = {id(self.geo_fwd): (self.geo_fwd, all_a), id(self.geo_bwd): (self.geo_bwd, all_a)}
         # kernel variants (identical results; set by autotune_projector)
         # kernel choices (identical results up to float32 rounding): "lds" (the image of a channel group resident in
         # LDS, when it fits), "table" (geometry tables, 2 taps), "arith" (footprint arithmetic, any taps)
@@ -757,6 +761,7 @@ class SinglePhaseForwardOperator:
             keep = (dict(self.radon_variant), self.tuning, self.fuse_projector, self.fused_channels)
             self._projector_geometry()
             self.radon_variant, self.tuning, self.fuse_projector, self.fused_channels = keep
+            self._fourier_setup()
         self._scratch = None
         self.omega_subsets = None
         if subsets is not None:
@@ -1004,6 +1009,9 @@ class SinglePhaseForwardOperator:
         self._sub_omegas = [torch.from_numpy(s.astype(np.int32)).to(self.device) for s in out]
         self._sub_geo_fwd = [self.geo_fwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
         self._sub_geo_bwd = [self.geo_bwd[torch.from_numpy(s).to(self.device)].contiguous() for s in out]
+        for gf, gb, om in zip(self._sub_geo_fwd, self._sub_geo_bwd, self._sub_omegas):
+            self._geo_angles[id(gf)] = (gf, om)
+            self._geo_angles[id(gb)] = (gb, om)
         self._sub_tables = [self._radon_tables(s) for s in out] if self.n_taps == 2 else None
         if self.gen_batches is not None:
             self._gen_prepare(range(len(out)))
@@ -1034,6 +1042,8 @@ class SinglePhaseForwardOperator:
     def em_fused(self, subset=None):
         """True when the backprojection choice applies the EM or PDHG update itself (a grouped table variant, modes 2
         and 3 of dfm_radon_backward_table)."""
+        if self.fourier is not None:
+            return False
         v = int(self.radon_variant["bwd"]) if self.radon_variant["bwd"] is not None else 0
         grouped = (v // 64) % 16 > 0 or (v // 1024) % 4 == 1 or v // 4096 > 0
         return self.radon_variant["bwd_kernel"] == "table" and grouped and self._tables(subset) is not None
@@ -1045,6 +1055,22 @@ class SinglePhaseForwardOperator:
         lam)), z = 2 x' - x (pdhg: mode 3) from the sinogram of n_ang angles."""
         st = self._stream()
         mode = 0 if x is None else (2 if em else 3 if pdhg else 1)
+        if self.fourier is not None:
+            if mode >= 2:
+                raise ValueError("the fused EM / PDHG updates need a grouped table backprojection (em_fused())")
+            q = self.queue if self.queue is not None else torch.cuda.current_stream(self.device)
+            with torch.cuda.stream(q):
+                ang = self._fourier_angles(geo)
+                if mode == 0:
+                    self.fourier.backward(sino, ang, out_k, kstride, koff, kn, st)
+                    return
+                npix = self.Nx * self.Ny
+                dz = torch.empty(npix * kn, dtype=torch.float32, device=self.device)
+                self.fourier.backward(sino, ang, dz, kn, 0, kn, st)
+                off = lambda t: ctypes.c_void_p(t.data_ptr() + 4 * koff)
+                _lib.call("dfm_spdhg_update_2d", off(x), off(z), off(T), _lib.ptr(dz), npix, kn, kstride, kn,
+                          float(inv_p), float(lam), st)
+            return
         if mode >= 2 and (tab is None or self.radon_variant["bwd_kernel"] != "table"):
             raise ValueError("the fused EM / PDHG updates need a grouped table backprojection (em_fused())")
         cw = self.lds_width() if self.radon_variant["bwd_kernel"] == "lds" else 0
@@ -1073,6 +1099,8 @@ class SinglePhaseForwardOperator:
         """Time the projector candidates on the given buffers (img: orientation-fastest image(s) with stride
         img_kstride, columns koff .. koff + kn; sino: scratch of the subset's angles; x, z, T: the backprojection's
         SPDHG update operands, overwritten) and keep the fastest in radon_variant; {product: (choice, times)}."""
+        if self.fourier is not None:                      # one Fourier-slice projector, nothing to choose
+            return {}
         n_ang, _, geo_f, geo_b = self._angle_set(subset)
         tab = self._tables(subset)
         out = {}
@@ -1264,7 +1292,7 @@ class SinglePhaseForwardOperator:
     def fused_width(self, na_b=None):
         """Channels per workgroup of the fused projector step (0: not available or switched off with
         fuse_projector = False)."""
-        return self.lds_width() if self.fuse_projector else 0
+        return self.lds_width() if self.fuse_projector and self.fourier is None else 0
 
     def _lds_projector(self, mode, sino_b, geo_b, na_b, x, z, T, inv_p, lam, sino_f, geo_f, na_f, kstride, koff, kn,
                        cw):
@@ -1286,7 +1314,26 @@ class SinglePhaseForwardOperator:
                   self.My * self.N_seg, self._stream())
 
     # ================================================================== forward
+    def _fourier_setup(self):
+        """The Fourier-slice projector of the current geometry (projector="fft"), else None."""
+        self.fourier = None
+        if self.projector == "fft":
+            from .fourier import FourierProjector
+            self.fourier = FourierProjector(self)
+            self.fuse_projector = False
+
+    def _fourier_angles(self, geo):
+        entry = self._geo_angles.get(id(geo))
+        if entry is None or entry[0] is not geo:
+            raise ValueError("projector='fft': the geometry array is not one of the operator's angle sets")
+        return entry[1]
+
     def _radon_forward(self, img, img_kstride, koff, kn, sino, geo, n_angles, tab=None):
+        if self.fourier is not None:
+            q = self.queue if self.queue is not None else torch.cuda.current_stream(self.device)
+            with torch.cuda.stream(q):
+                self.fourier.forward(img, img_kstride, koff, kn, sino, self._fourier_angles(geo), self._stream())
+            return
         if tab is not None and self.radon_variant["fwd_kernel"] == "pairs":
             _lib.call("dfm_radon_forward_pairs", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["pair_ptr"]),
                       _lib.ptr(tab["pair_pix"]), _lib.ptr(tab["pair_w"]), self.My, n_angles, img_kstride, koff, kn,

@@ -354,6 +354,26 @@ int dfm_pf_grad_reduce(const double* part_k, const double* part_g, int32_t n_ang
 int dfm_angle_dots(const float* u, const float* v, int32_t n_ang, int32_t my, int32_t sks, int32_t kn, double* out,
                    dfm_stream_t stream);
 
+/* ---------------------------------------------------------------- Fourier-slice projector (projector="fft")
+ * (operators/fourier_radon.py:1-311, fourier_radon.cl fr_pad / fr_slices / fr_spread / fr_crop), every array
+ * (points, channels) with the channels fastest; complex values as float pairs. pad: grid[gidx[p] kn + k] =
+ * img_k[p kstride + k_off + k] deapod[p] (real; the caller zeroes the grid first). interp: spec[(ai H + j) kn + k] =
+ * rowfac[r] sum_e w[e] G[col[e] kn + k] over the CSR row r = angles[ai] H + j (the polar samples' taps). spread:
+ * G[t kn + k] = sum_e tw[e] colfac[r] Z[(apos[a] H + j) kn + k] over the transposed CSR row t, r = smp[e] = a H + j,
+ * skipping angles with apos[a] < 0 (the transpose of interp, as a fixed-order gather). crop: out_k[p kstride + k_off
+ * + k] (+)= scale Re(g[gidx[p] kn + k]) deapod[p]. */
+int dfm_nufft_pad(const float* img_k, int32_t kstride, int32_t k_off, int32_t kn, const float* deapod,
+                  const int32_t* gidx, int64_t npix, void* grid, dfm_stream_t stream);
+int dfm_nufft_interp(const void* G, const int32_t* ptr, const int32_t* col, const float* w, const void* rowfac,
+                     const int32_t* angles, int32_t n_ang, int32_t H, int32_t kn, void* spec, dfm_stream_t stream);
+int dfm_nufft_spread(const void* Z, const int32_t* tptr, const int32_t* smp, const float* tw, const void* colfac,
+                     const int32_t* apos, int32_t H, int32_t kn, int64_t n_grid, void* G, dfm_stream_t stream);
+int dfm_nufft_crop(const void* g, const int32_t* gidx, const float* deapod, int64_t npix, int32_t kn, float scale,
+                   float* out_k, int32_t kstride, int32_t k_off, int32_t accumulate, dfm_stream_t stream);
+/* The SPDHG update of dfm_spdhg_update on a column block: x, z, T (rows, ld), dz (rows, ld_dz), columns < cols. */
+int dfm_spdhg_update_2d(float* x, float* z, const float* T, const float* dz, int64_t rows, int32_t cols, int32_t ld,
+                        int32_t ld_dz, float inv_p, float lam, dfm_stream_t stream);
+
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
  * Dual update on the rows of a subset (spdhg_poisson.py:69-91 dual_update_sub), float64 arithmetic:
