@@ -68,6 +68,8 @@ class BatchedSPDHGPoisson:
         if not ops:
             raise ValueError("no slices")
         o0 = ops[0]
+        if any(getattr(op, "coefficient_layout", "dense") == "local" for op in ops):
+            raise NotImplementedError("BatchedSPDHGPoisson batches operators of the (K, Ny, Nx) coefficient layout")
         for op in ops:
             same = (op.Nx, op.Ny, op.My, op.N_Omega, op.N_seg) == (o0.Nx, o0.Ny, o0.My, o0.N_Omega, o0.N_seg) and \
                 np.array_equal(op.angles, o0.angles) and op.cor_offset == o0.cor_offset

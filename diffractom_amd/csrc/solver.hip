@@ -286,6 +286,62 @@ __global__ void __launch_bounds__(256) tv_value_kernel(const float4* __restrict_
   block_sum<1>(s, partial, gridDim.x);
 }
 
+// ---- total-variation block on a flat coefficient vector whose neighbours are given by index arrays (right, down,
+// left, up; -1: none), the local basis's patches (spdhg_poisson.py:195-236 tv_*_local): the same arithmetic as the
+// channel-fastest kernels above.
+__global__ void __launch_bounds__(256) tv_dual_nbr_kernel(const float* __restrict__ x, const float* __restrict__ T,
+                                                          float* __restrict__ qx, float* __restrict__ qy,
+                                                          float* __restrict__ dqx, float* __restrict__ dqy,
+                                                          const int* __restrict__ right, const int* __restrict__ down,
+                                                          float lam, float ps, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = right[i], d = down[i];
+    const bool vr0 = r >= 0, vd0 = d >= 0;
+    const float ox = qx[i], oy = qy[i];
+    float nx_, ny_;
+    tv_core(T[i], vr0 ? T[r] : 0.0f, vd0 ? T[d] : 0.0f, vr0, vd0, x[i], vr0 ? x[r] : 0.0f, vd0 ? x[d] : 0.0f, ox, oy,
+            lam, ps, nx_, ny_);
+    qx[i] = nx_;
+    qy[i] = ny_;
+    dqx[i] = nx_ - ox;
+    dqy[i] = ny_ - oy;
+  }
+}
+
+__global__ void __launch_bounds__(256) tv_adjoint_update_nbr_kernel(const float* __restrict__ dqx,
+                                                                    const float* __restrict__ dqy,
+                                                                    const int* __restrict__ left,
+                                                                    const int* __restrict__ up,
+                                                                    float* __restrict__ x, float* __restrict__ z,
+                                                                    const float* __restrict__ T, float inv_p,
+                                                                    float lam, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float dz = -dqx[i] - dqy[i];
+    const int l = left[i], u = up[i];
+    if (l >= 0) dz += dqx[l];
+    if (u >= 0) dz += dqy[u];
+    const float zi = z[i] + dz;
+    z[i] = zi;
+    const float tt = T[i];
+    x[i] = tt > 0.0f ? fmaxf(0.0f, x[i] - tt * ((zi + inv_p * dz) + lam)) : 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256) tv_value_nbr_kernel(const float* __restrict__ x, const float* __restrict__ T,
+                                                           const int* __restrict__ right,
+                                                           const int* __restrict__ down, int64_t n,
+                                                           double* __restrict__ partial) {
+  double s[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = right[i], d = down[i];
+    const float tj = T[i], xj = x[i];
+    const float gx = (r >= 0 && tj > 0.0f && T[r] > 0.0f) ? x[r] - xj : 0.0f;
+    const float gy = (d >= 0 && tj > 0.0f && T[d] > 0.0f) ? x[d] - xj : 0.0f;
+    s[0] += (double)sqrtf(gx * gx + gy * gy);
+  }
+  block_sum<1>(s, partial, gridDim.x);
+}
+
 // Smoothed isotropic TV of coefficient images (K, ny, nx) (a differentiable loss term, differentiable/ loss of the
 // C4 refinement): value sum_{k,y,x} sqrt(dx^2 + dy^2 + eps) with forward differences (0 at the last column / row),
 // and its gradient, in one pass: grad(y, x) = -(dx + dy)/s(y, x) + dx(y, x-1)/s(y, x-1) + dy(y-1, x)/s(y-1, x).
@@ -833,6 +889,36 @@ int dfm_tv_adjoint_update_k(const float* dqx, const float* dqy, float* x, float*
                      (const float4*)dqx, (const float4*)dqy, (float4*)x, (float4*)z, (const float4*)T, nx, ny,
                      kstride / 4, inv_p, lam);
   DFM_LAUNCH_CHECK("dfm_tv_adjoint_update_k");
+  return DFM_OK;
+}
+
+int dfm_tv_dual_nbr(const float* x, const float* T, float* qx, float* qy, float* dqx, float* dqy, const int32_t* right,
+                    const int32_t* down, float lam, float ps, int64_t n, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && T && qx && qy && dqx && dqy && right && down && n >= 0, "dfm_tv_dual_nbr: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(tv_dual_nbr_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, x, T,
+                     qx, qy, dqx, dqy, right, down, lam, ps, n);
+  DFM_LAUNCH_CHECK("dfm_tv_dual_nbr");
+  return DFM_OK;
+}
+
+int dfm_tv_adjoint_update_nbr(const float* dqx, const float* dqy, const int32_t* left, const int32_t* up, float* x,
+                              float* z, const float* T, float inv_p, float lam, int64_t n, dfm_stream_t stream) {
+  DFM_CHECK_ARG(dqx && dqy && left && up && x && z && T && n >= 0, "dfm_tv_adjoint_update_nbr: bad arguments");
+  if (n == 0) return DFM_OK;
+  hipLaunchKernelGGL(tv_adjoint_update_nbr_kernel, dim3(grid_stride_blocks(n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, dqx, dqy, left, up, x, z, T, inv_p, lam, n);
+  DFM_LAUNCH_CHECK("dfm_tv_adjoint_update_nbr");
+  return DFM_OK;
+}
+
+int dfm_tv_value_nbr_partial(const float* x, const float* T, const int32_t* right, const int32_t* down, int64_t n,
+                             double* partial, int32_t* n_partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(x && T && right && down && partial && n_partial && n > 0, "dfm_tv_value_nbr_partial: bad arguments");
+  const unsigned nb = reduce_blocks(n);
+  *n_partial = (int32_t)nb;
+  hipLaunchKernelGGL(tv_value_nbr_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, x, T, right, down, n, partial);
+  DFM_LAUNCH_CHECK("dfm_tv_value_nbr_partial");
   return DFM_OK;
 }
 

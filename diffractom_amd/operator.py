@@ -1208,6 +1208,15 @@ class SinglePhaseForwardOperator:
 
         out = self.tune_projector(subset, x, self.K_stride, b["koff"], b["kstride"], sc["sino"], x.clone(), z, T,
                                   timeit)
+        self._autotune_pf(subset, sc, g, timeit, out)
+        del x, z, T
+        self.tuning = out
+        return out
+
+    def _autotune_pf(self, subset, sc, g, timeit, out):
+        """The PF products' part of autotune: times every candidate on random data rows and the sinogram scratch of
+        batch 0, keeps the fastest and records {product: (choice, {candidate: ms})} in out."""
+        dev = self.device
         if self.sparse_batches is not None:
             data = torch.rand(self.data_shape, generator=g, device=dev)
             times = {}
@@ -1275,9 +1284,6 @@ class SinglePhaseForwardOperator:
             best = _pick(times)
             self.blk_variant["adj_wg"] = int(best.split(":")[1])
             out["pf_adjoint"] = (best, times)
-        del x, z, T
-        self.tuning = out
-        return out
 
     def lds_width(self):
         """Channels per workgroup of the LDS-resident projector (4, 2, 1; at most fused_channels), or 0 when the

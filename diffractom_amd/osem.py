@@ -38,6 +38,9 @@ class OSEMPoisson:
         not fitted), or None: MLEM over all angles. lam: weight of the l1 penalty lam sum(x). support: "fov", None
         or an (Ny, Nx) array. seed: the order of the subsets within each epoch. store_sensitivities: keep
         A_m^T(w / g) per subset (False: recompute it every iteration)."""
+        if getattr(operator, "coefficient_layout", "dense") == "local":
+            raise NotImplementedError("OSEMPoisson runs on the (K, Ny, Nx) coefficient layout; with a local basis use "
+                                      "SPDHGPoisson or FISTAPoisson")
         self.op = operator
         if subsets is None:
             self.subsets = None

@@ -36,6 +36,9 @@ class PDHGPoisson:
         number sets it. prox_kind "nonneg" or "nonneg_l1" (lam sum(x) with lam > 0)."""
         if prox_kind not in ("nonneg", "nonneg_l1"):
             raise ValueError("PDHGPoisson supports prox_kind 'nonneg' and 'nonneg_l1'")
+        if getattr(operator, "coefficient_layout", "dense") == "local":
+            raise NotImplementedError("PDHGPoisson runs on the (K, Ny, Nx) coefficient layout; with a local basis use "
+                                      "SPDHGPoisson or FISTAPoisson")
         self.op = operator
         self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
         self.eps_counts = float(eps_counts)

@@ -52,10 +52,16 @@ def _periodic(a, dshape, name, device):
     return torch.from_numpy(h.ravel().copy()).to(device), int(h.size)
 
 
+def is_local(op):
+    """True for an operator whose coefficients are a flat vector of patches (the local basis)."""
+    return getattr(op, "coefficient_layout", "dense") == "local"
+
+
 def support_to_device(op, support):
     """The support option of the solvers as a flat (Ny * Nx) uint8 device mask or None (optimization/prox.py:75-99):
-    "fov" (the operator's field of view), None / False (no support constraint) or an (Ny, Nx) array."""
-    if support is None or support is False:
+    "fov" (the operator's field of view), None / False (no support constraint) or an (Ny, Nx) array. A local-basis
+    operator holds its support itself (spdhg_poisson.py:754-814 runs without a mask): None."""
+    if support is None or support is False or is_local(op):
         return None
     if isinstance(support, str):
         if support != "fov":
@@ -82,6 +88,7 @@ class SPDHGPoisson:
         if tv_tree is not None:
             raise NotImplementedError("tv_tree needs an operator with the local coefficient layout")
         self.op = operator
+        self._local = is_local(operator)
         self.subsets = operator.set_omega_subsets(subsets)
         self.M = len(self.subsets)
         self.lam = float(lam) if prox_kind == "nonneg_l1" else 0.0
@@ -164,6 +171,9 @@ class SPDHGPoisson:
         """z += A_m^T D; x updated (all batches): the CSR adjoint into the sinogram scratch, then the fused
         backprojection + primal update on the batch's columns of x, z, T."""
         op = self.op
+        if self._local:
+            op.adjoint_update(self.D, m, self.xk, self.zk, self.Tk, self._inv_p_data, self.lam)
+            return
         n_ang, _, _, geo = op._angle_set(m)
         tab = op._tables(m)
         sc = op._scratch_buffers(n_ang)
@@ -183,6 +193,15 @@ class SPDHGPoisson:
         1/p_tv."""
         op = self.op
         st = op._stream()
+        if self._local:
+            n = self.xk.numel()
+            _lib.call("dfm_tv_dual_nbr", _lib.ptr(self.xk), _lib.ptr(self.Tk), _lib.ptr(self.qxk), _lib.ptr(self.qyk),
+                      _lib.ptr(self._dqx), _lib.ptr(self._dqy), _lib.ptr(self._nbr[0]), _lib.ptr(self._nbr[1]),
+                      float(self.tv), float(np.float32(TV_SAFETY * self.tv_prob)), n, st)
+            _lib.call("dfm_tv_adjoint_update_nbr", _lib.ptr(self._dqx), _lib.ptr(self._dqy), _lib.ptr(self._nbr[2]),
+                      _lib.ptr(self._nbr[3]), _lib.ptr(self.xk), _lib.ptr(self.zk), _lib.ptr(self.Tk),
+                      float(np.float32(1.0 / self.tv_prob)), float(self.lam), n, st)
+            return
         _lib.call("dfm_tv_dual_k", _lib.ptr(self.xk), _lib.ptr(self.Tk), _lib.ptr(self.qxk), _lib.ptr(self.qyk),
                   _lib.ptr(self._dqx), _lib.ptr(self._dqy), op.Nx, op.Ny, op.K_stride, float(self.tv),
                   float(np.float32(TV_SAFETY * self.tv_prob)), st)
@@ -193,6 +212,10 @@ class SPDHGPoisson:
     def _tv_value(self):
         op = self.op
         n = ctypes.c_int32(0)
+        if self._local:
+            _lib.call("dfm_tv_value_nbr_partial", _lib.ptr(self.xk), _lib.ptr(self.Tk), _lib.ptr(self._nbr[0]),
+                      _lib.ptr(self._nbr[1]), self.xk.numel(), _lib.ptr(self._part), ctypes.byref(n), op._stream())
+            return float(self._part[: n.value].sum().item())
         _lib.call("dfm_tv_value_partial", _lib.ptr(self.xk), _lib.ptr(self.Tk), op.Nx, op.Ny, op.K_stride,
                   _lib.ptr(self._part), ctypes.byref(n), op._stream())
         return float(self._part[: n.value].sum().item())
@@ -247,8 +270,9 @@ class SPDHGPoisson:
         self.g = self.g if self.g is not None else dummy
         self.bg = self.bg if self.bg is not None else dummy
         self._part = torch.empty(2 * DFM_MAX_PARTIALS, dtype=torch.float64, device=dev)
-        nk = op.Nx * op.Ny * op.K_stride
-        npix = op.Nx * op.Ny
+        # the resident layout: channel-fastest (Npix, K_stride), or the local basis's flat vector as (1, n_coef)
+        npix, ks = (1, op.n_coef) if self._local else (op.Nx * op.Ny, op.K_stride)
+        nk = npix * ks
         st = op._stream()
         supp = self.support_mask
         self.y = torch.zeros(dshape, dtype=torch.float32, device=dev)
@@ -271,25 +295,28 @@ class SPDHGPoisson:
         self.resident_used = NAMES
 
         # S from the row sums of A 1_S (spdhg_poisson.py:621-631)
-        ones = torch.empty(nk, dtype=torch.float32, device=dev)
-        _lib.call("dfm_support_ones_k", _lib.ptr(ones), _lib.ptr(supp), npix, op.K_stride, op.K_stride, st)
-        self._zero_padding(ones)
+        if self._local:
+            ones = torch.from_numpy(op.coefficient_mask()).to(dev)
+        else:
+            ones = torch.empty(nk, dtype=torch.float32, device=dev)
+            _lib.call("dfm_support_ones_k", _lib.ptr(ones), _lib.ptr(supp), npix, ks, ks, st)
+            self._zero_padding(ones)
         op.direct_k(ones, self.D)
         self.scales = {"row_sum_mean": self._mean_counted(self.D), "b_mean": self._mean_counted(self.b),
                        "gain_mean": 1.0 if self.pg == 0 else float(self.g.mean().item())}
         _lib.call("dfm_dual_steps", _lib.ptr(self.sig), _lib.ptr(self.D), _lib.ptr(self.w), self.pw, gamma, nd, st)
         # T = min_m p / (gamma A_m^T 1_counted) on the support (spdhg_poisson.py:632-643)
         _lib.call("dfm_set_mask_ones", _lib.ptr(self.D), _lib.ptr(self.w), self.pw, nd, st)
-        _lib.call("dfm_tinit", _lib.ptr(self.Tk), npix, op.K_stride, st)
+        _lib.call("dfm_tinit", _lib.ptr(self.Tk), npix, ks, st)
         colsum = ones
         for m in range(self.M):
             op.adjoint_k(self.D, colsum, subset=m)
-            _lib.call("dfm_tmin", _lib.ptr(self.Tk), _lib.ptr(colsum), _lib.ptr(supp), npix, op.K_stride,
-                      op.K_stride, float(np.float32(p / gamma)), st)
-        _lib.call("dfm_tfinal", _lib.ptr(self.Tk), npix, op.K_stride, op.K_stride, st)
+            _lib.call("dfm_tmin", _lib.ptr(self.Tk), _lib.ptr(colsum), _lib.ptr(supp), npix, ks, ks,
+                      float(np.float32(p / gamma)), st)
+        _lib.call("dfm_tfinal", _lib.ptr(self.Tk), npix, ks, ks, st)
         self._zero_padding(self.Tk)
         del ones, colsum
-        _lib.call("dfm_project_support_k", _lib.ptr(self.xk), _lib.ptr(supp), npix, op.K_stride, op.K_stride, st)
+        _lib.call("dfm_project_support_k", _lib.ptr(self.xk), _lib.ptr(supp), npix, ks, ks, st)
         self._zero_padding(self.xk)
         if state is not None:
             self.y.copy_(torch.from_numpy(as_host(state["y"], dshape, "state['y']")).to(dev))
@@ -306,6 +333,8 @@ class SPDHGPoisson:
                     op.to_k(torch.from_numpy(as_host(state[nm], cshape, f"state['{nm}']")).to(dev), qk)
             self._dqx = torch.empty(nk, dtype=torch.float32, device=dev)
             self._dqy = torch.empty(nk, dtype=torch.float32, device=dev)
+            if self._local:
+                self._nbr = [torch.from_numpy(a).to(dev) for a in op.patch_neighbours()]
             self.resident_used = NAMES + TV_NAMES
         self._ep0 = int(state["epoch"]) if state is not None else 0
         self._seq = self._epoch_sequence()
@@ -401,7 +430,7 @@ class SPDHGPoisson:
             for nm, qk in (("qx", self.qxk), ("qy", self.qyk)):
                 op.from_k(qk, zc)
                 self.state[nm] = zc.cpu().numpy()
-            self.qxk = self.qyk = self._dqx = self._dqy = None
+            self.qxk = self.qyk = self._dqx = self._dqy = self._nbr = None
         if not self._x_on_gpu:
             self._host_x[...] = self._x_dev.cpu().numpy()
         out = self._x_dev if self._x_on_gpu else self._host_x
@@ -474,6 +503,8 @@ class SPDHGPoisson:
     def _zero_padding(self, a_k):
         """Padding channels (K_batch < kstride) of a channel-fastest array to 0."""
         op = self.op
+        if self._local:
+            return
         npix = op.Nx * op.Ny
         v = a_k.view(npix, op.K_stride)
         for bt in op.batches:

@@ -374,6 +374,37 @@ int dfm_nufft_crop(const void* g, const int32_t* gidx, const float* deapod, int6
 int dfm_spdhg_update_2d(float* x, float* z, const float* T, const float* dz, int64_t rows, int32_t cols, int32_t ld,
                         int32_t ld_dz, float inv_p, float lam, dfm_stream_t stream);
 
+/* ---------------------------------------------------------------- local basis (operators/local_forward_operator.py)
+ * Patches of an orientation batch: box[p] = (y0, x0, h, w) int32 x 4, coff[p] the offset of its row-major
+ * coefficients in the flat vector, mask a byte per coefficient (1: allowed); strips of W[p] bins per angle, stored
+ * strips[a * sw + pw_off[p] + j] (sw = the sum of W over the batch), jp[jg] the patch of strip bin jg; ptab (R, P)
+ * int32 pairs (off, tb): the strip's first detector bin at angle a and tb = t(x0, y0) - off as float bits; geo the
+ * operator's geometry records (R x 8 floats). Angles: omegas[0 .. n_ang - 1].
+ * strips (local_kernels.cl:38-117 local_radon_forward): strips = scale / A sum of footprint x allowed coefficients.
+ * assemble (the sinogram rows local_kernels.cl:211-244 local_pf_forward assembles in local memory): the batch's
+ *   orientation-fastest sinogram (n_ang, my, kstride), sino[as, y, k] = sum over the patches of orientation k
+ *   (kp_ptr[k] .. kp_ptr[k + 1]) of their strip bin at line y, 0 elsewhere.
+ * extract (the strip bins local_kernels.cl:289-319 local_pf_adjoint writes): strips = sino at (as, off + j, pk[p]),
+ *   0 off the detector.
+ * backproject (local_kernels.cl:120-174 local_radon_backward): for the batch's coefficients c_begin .. c_begin + n - 1
+ *   (pid: the patch of each) acc = sum_a (w / A)_a sum of footprint x strip bins; mode 0: out = acc (0 where not
+ *   allowed); mode 1: the SPDHG primal update with dz = acc (as dfm_spdhg_update). n_taps 2 .. 4. */
+int dfm_local_strips(const float* coef, const uint8_t* mask, const int32_t* box, const int64_t* coff,
+                     const int32_t* pw_off, int32_t sw, const int32_t* jp, const int32_t* ptab, int32_t n_patches,
+                     const float* geo, const int32_t* omegas, int32_t n_ang, float scale, float* strips,
+                     dfm_stream_t stream);
+int dfm_local_assemble(const float* strips, int32_t sw, const int32_t* pw_off, const int32_t* W, const int32_t* ptab,
+                       int32_t n_patches, const int32_t* kp_ptr, int32_t k_batch, const int32_t* omegas,
+                       int32_t n_ang, int32_t my, int32_t kstride, float* sino, dfm_stream_t stream);
+int dfm_local_extract(const float* sino, int32_t n_ang, int32_t my, int32_t kstride, const int32_t* omegas,
+                      const int32_t* ptab, int32_t n_patches, const int32_t* pk, const int32_t* jp,
+                      const int32_t* pw_off, int32_t sw, float* strips, dfm_stream_t stream);
+int dfm_local_backproject(const float* strips, int32_t sw, const int32_t* pw_off, const int32_t* W,
+                          const int32_t* ptab, int32_t n_patches, const float* geo, const int32_t* omegas,
+                          int32_t n_ang, const int32_t* box, const int64_t* coff, const uint8_t* mask,
+                          const int32_t* pid, int64_t c_begin, int64_t n, int32_t n_taps, int32_t mode, float* out,
+                          float* x, float* z, const float* T, float inv_p, float lam, dfm_stream_t stream);
+
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.
  * Dual update on the rows of a subset (spdhg_poisson.py:69-91 dual_update_sub), float64 arithmetic:
@@ -508,6 +539,15 @@ int dfm_tv_adjoint_update_k(const float* dqx, const float* dqy, float* x, float*
                             int32_t ny, int32_t kstride, float inv_p, float lam, dfm_stream_t stream);
 int dfm_tv_value_partial(const float* x, const float* T, int32_t nx, int32_t ny, int32_t kstride, double* partial,
                          int32_t* n_partial, dfm_stream_t stream);
+/* The same block on a flat coefficient vector whose neighbours are index arrays (right, down, left, up; -1: none):
+ * the local basis's patches (spdhg_poisson.py:195-236 tv_dual_local / tv_adjoint_local / tv_value_local; the
+ * adjoint fused with the primal update as above). n elements. */
+int dfm_tv_dual_nbr(const float* x, const float* T, float* qx, float* qy, float* dqx, float* dqy, const int32_t* right,
+                    const int32_t* down, float lam, float ps, int64_t n, dfm_stream_t stream);
+int dfm_tv_adjoint_update_nbr(const float* dqx, const float* dqy, const int32_t* left, const int32_t* up, float* x,
+                              float* z, const float* T, float inv_p, float lam, int64_t n, dfm_stream_t stream);
+int dfm_tv_value_nbr_partial(const float* x, const float* T, const int32_t* right, const int32_t* down, int64_t n,
+                             double* partial, int32_t* n_partial, dfm_stream_t stream);
 /* a *= s (single_phase_forward_operator.py:2169, 2192 x *= 1/|x|). */
 int dfm_scale(float* a, float s, int64_t n, dfm_stream_t stream);
 

@@ -704,9 +704,11 @@ class OracleSPDHG:
         self.rho_used = rho
         gamma = np.sqrt(rho)
         p = self._p_data
+        # the support: (Ny, Nx) for every orientation, or (K, Ny, Nx) per coefficient (a local basis)
         sup = np.ones((op.Ny, op.Nx), bool) if self.supp is None else self.supp
+        supb = np.broadcast_to(sup, op.coeff_shape)
         # S from the row sums of A 1_S (spdhg_poisson.py:621-631; dual_steps pdhg_poisson.py:79-84)
-        ones = np.broadcast_to(sup, op.coeff_shape).astype(np.float64)
+        ones = supb.astype(np.float64)
         D = op.direct(ones)
         sig = np.where((w != 0) & (D > 0), gamma / np.where(D > 0, D, 1.0), 0.0)
         # T = min_m p / (gamma A_m^T 1_counted) on the support (spdhg_poisson.py:632-643, 252-272)
@@ -714,11 +716,11 @@ class OracleSPDHG:
         T = np.full(op.coeff_shape, np.inf)
         for m in range(self.M):
             col = op.adjoint(mask, subset=m)
-            ok = (col > 0) & sup[None]
+            ok = (col > 0) & supb
             T = np.where(ok, np.minimum(T, (p / gamma) / np.where(col > 0, col, 1.0)), T)
         T[np.isinf(T)] = 0.0
         self.T, self.sig = T, sig
-        x = np.maximum(np.asarray(x0, np.float64).reshape(op.coeff_shape), 0.0) * sup[None]
+        x = np.maximum(np.asarray(x0, np.float64).reshape(op.coeff_shape), 0.0) * supb
         y = np.zeros(op.data_shape) if state is None else np.asarray(state["y"], np.float64).reshape(op.data_shape)
         if state is not None and state.get("z") is not None:
             z = np.asarray(state["z"], np.float64).reshape(op.coeff_shape).copy()
