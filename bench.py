@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--config", default="al_c1", help="al_c1 (the headline line), al_c0, al_c3 (the multi-slice "
                     "batch: slices/s and the step-time percentiles), ti_c2 (hexagonal, 256^2 px, K 59,440; PF per "
                     "--c2-precision), "
-                    "al_c4 (autograd loss + gradients, 8 slices)")
+                    "al_c4 (autograd loss + gradients, 8 slices), al_local (SPDHG on a local basis of the al_c1 "
+                    "scan against the global operator on the same orientations)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
     ap.add_argument("--no-graphs", action="store_true", help="al_c3: eager launches instead of HIP graph replay")
     ap.add_argument("--compare-steps", type=int, default=300, help="the precisions' SPDHG iterates are compared "
@@ -428,6 +429,106 @@ def run_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def run_local(args):
+    """The local-basis workload (al_local: the al_c1 scan, grid and truth; every voxel allows the orientations within
+    3.5 deg of its grain's, grown by 2 voxels): SPDHG subset steps per second of the local operator
+    (LocalSinglePhaseForwardOperator, --precision PF products) with per-kernel times of the timed steps, against the
+    global operator on the same orientations (full images); forward / adjoint of the local operator against the
+    global one and against the FP64 oracle on 4 angle rows. One JSON line."""
+    import torch
+    from diffractom_amd import MatrixGrid, SinglePhaseForwardOperator, SPDHGPoisson
+    from diffractom_amd.workloads import AL_RINGS, al_local_problem
+    from oracle.local import OracleLocalOperator
+    from oracle.model import OracleOperator
+    q = args.precision
+    t0 = time.perf_counter()
+    op, xt, b, p = al_local_problem(precision=q)
+    setup_s = time.perf_counter() - t0
+    sup = op.support
+    log(f"[bench] al_local: {sup.n_pairs} pairs, {op.K} of {op.K_basis} orientations, {op.n_patches} patches, "
+        f"{op.n_coef} coefficients, strips {100 * op.strip_fraction:.1f} % of the sinograms; set up {setup_s:.1f} s")
+    M = p["subsets"]
+    if not args.no_autotune:
+        op.set_omega_subsets(M)
+        tuning = op.autotune()
+        log("[bench] al_local: autotune " + "; ".join(f"{k}: {v[0]}" for k, v in tuning.items()))
+    dev = torch.device("cuda")
+    bd = torch.from_numpy(b).to(dev)
+
+    def solve(o, x0):
+        sol = SPDHGPoisson(o, subsets=M, seed=0)
+        sol.graphs = args.graphs
+        sol.start(x0, bd)
+        sol.advance(args.warmup)
+        torch.cuda.synchronize()
+        ms, _ = timed_steps(sol, args.steps, per_kernel=False)
+        _, kms = timed_steps(sol, min(args.steps, 20), per_kernel=not args.no_kernel_timing)
+        sol.finish()
+        return ms, kms
+
+    ms_l, kms_l = solve(op, torch.zeros(op.coeff_shape, dtype=torch.float32, device=dev))
+    log(f"[bench] al_local: local {ms_l:.4f} ms/step; kernel ms: " +
+        ", ".join(f"{k} {v:.4f}" for k, v in (kms_l or {}).items()))
+    # the global operator on the same orientations (operator order), full images
+    U = op.grid_rot_cpu
+    sig = float(np.asarray(op.sigma_cpu)[0])
+    gop = SinglePhaseForwardOperator(op.cfg, op.material, MatrixGrid(U, sig), max_gb=1024.0, normalized=True,
+                                     pf_mode="sparse", pf_precision=q)
+    if not args.no_autotune:
+        gop.set_omega_subsets(M)
+        gop.autotune()
+    ms_g, kms_g = solve(gop, torch.zeros(gop.coeff_shape, dtype=torch.float32, device=dev))
+    log(f"[bench] al_local: global (same {gop.K} orientations) {ms_g:.4f} ms/step")
+    # parity: the local operator against the global one on the same coefficients, and against the FP64 oracle
+    inv = np.full(op.K_basis, -1, np.int64)
+    inv[op.orientation_index] = np.arange(op.K)
+    k, iy, ix, pi = op._pairs()
+    xd = np.zeros(gop.coeff_shape, np.float32)
+    xd[inv[k], iy, ix] = xt[pi]
+    rng = np.random.default_rng(1)
+    y = rng.random(op.data_shape).astype(np.float32)
+    d_l, d_g = op.direct(xt), gop.direct(xd)
+    a_l, a_g = op.adjoint(y), gop.adjoint(y)
+    rel = lambda u, v: float(np.linalg.norm((np.asarray(u, np.float64) - v).ravel()) /
+                             max(np.linalg.norm(np.asarray(v, np.float64).ravel()), 1e-300))
+    vs_global = {"forward": rel(d_l, d_g), "adjoint": rel(a_l, _gather(a_g, inv[k], iy, ix, pi, op.n_coef))}
+    rows = [0, 45, 90, 135]
+    t1 = time.perf_counter()
+    base = OracleOperator(op.cfg, (4.05, 4.05, 4.05, 90, 90, 90), "cubic", AL_RINGS, 0.2, U, sig,
+                          orthonormalize=False)
+    orc = OracleLocalOperator(base, inv[k], iy, ix, pi, op.n_coef)
+    base.set_omega_subsets([np.asarray(rows)])
+    ym = np.zeros_like(y)
+    ym[rows] = y[rows]
+    vs_oracle = {"forward": rel(d_l[rows], orc.direct_rows(xt, rows)),
+                 "adjoint": rel(op.adjoint(ym), orc.adjoint(y, subset=0)),
+                 "sample": f"angle rows {rows}, all {op.K} orientations"}
+    oracle_s = time.perf_counter() - t1
+    log(f"[bench] al_local: vs global {vs_global}, vs FP64 oracle {vs_oracle} ({oracle_s:.1f} s)")
+    line = {"metric": "SPDHG-Poisson subset steps/s on a local basis (al_c1 scan)", "value": round(1e3 / ms_l, 3),
+            "unit": "SPDHG subset steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(ms_l, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": DTYPE[q], "data": "synthetic (al_c1 Voronoi grains, Poisson counts at a mean of 1000)",
+            "config": {"workload": f"al_local: al_c1 scan and 3-degree grid (K {op.K_basis}), support = "
+                                   f"orientations within {p['tol_deg']} deg of each voxel's grain, grown by "
+                                   f"{p['dilate']} voxels: {sup.n_pairs} pairs, {op.K} orientations, "
+                                   f"{op.n_patches} patches, {op.n_coef} coefficients; M={M}; {PF_DESC[q]}",
+                       "K_used": op.K, "n_coef": op.n_coef, "precision": q},
+            "setup_s": round(setup_s, 2), "kernel_ms": kms_l,
+            "global_same_orientations": {"ms_per_step": round(ms_g, 4), "steps_per_s": round(1e3 / ms_g, 3),
+                                         "kernel_ms": kms_g, "coefficients": int(np.prod(gop.coeff_shape))},
+            "speedup_vs_global": round(ms_g / ms_l, 3),
+            "rel_l2_vs_global": vs_global, "rel_l2_vs_fp64_oracle": vs_oracle}
+    print(json.dumps(line), flush=True)
+
+
+def _gather(c, k, iy, ix, pi, n):
+    """(n,) flat local coefficients of a (K, Ny, Nx) array at the pairs (zero elsewhere)."""
+    out = np.zeros(n, np.float32)
+    out[pi] = c[k, iy, ix]
+    return out
+
+
 def run_c2(args):
     """The large hexagonal workload (Ti, 256 x 256 px, 360 omega, K = 59,440): the PF generated per call in fp16
     blocks on the matrix cores (--c2-precision fp16, the configuration's own form; bf16_residual possible) or stored
@@ -728,8 +829,8 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
-    if args.config in ("al_c3", "ti_c2", "al_c4"):
-        {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4}[args.config](args)
+    if args.config in ("al_c3", "ti_c2", "al_c4", "al_local"):
+        {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4, "al_local": run_local}[args.config](args)
         return
     if args.solver in ("osem", "pdhg", "fista"):
         {"osem": run_osem, "pdhg": run_pdhg, "fista": run_fista}[args.solver](args)

@@ -96,11 +96,11 @@ SIGNATURES = {
     "dfm_nufft_spread": [P, P, P, P, P, P, I32, I32, I64, P, P],
     "dfm_nufft_crop": [P, P, P, I64, I32, F32, P, I32, I32, I32, P],
     "dfm_spdhg_update_2d": [P, P, P, P, I64, I32, I32, I32, F32, F32, P],
-    "dfm_local_strips": [P, P, P, P, P, I32, P, P, I32, P, P, I32, F32, P, P],
+    "dfm_local_strips": [P, P, P, P, P, P, I32, P, P, I32, I32, I32, P, P, I32, F32, P, P],
     "dfm_local_assemble": [P, I32, P, P, P, I32, P, I32, P, I32, I32, I32, P, P],
     "dfm_local_extract": [P, I32, I32, I32, P, P, I32, P, P, P, I32, P, P],
-    "dfm_local_backproject": [P, I32, P, P, P, I32, P, P, I32, P, P, P, P, I64, I64, I32, I32, P, P, P, P, F32, F32,
-                              P],
+    "dfm_local_backproject": [P, I32, P, P, P, I32, I32, P, P, I32, P, P, P, P, I64, I64, P, I32, I32, I32, P, P, P,
+                              P, F32, F32, P],
     "dfm_tv_dual_nbr": [P, P, P, P, P, P, P, P, F32, F32, I64, P],
     "dfm_tv_adjoint_update_nbr": [P, P, P, P, P, P, P, F32, F32, I64, P],
     "dfm_tv_value_nbr_partial": [P, P, P, P, I64, P, P, P],

@@ -110,6 +110,9 @@ class LocalSinglePhaseForwardOperator(SinglePhaseForwardOperator):
             raise ValueError("LocalSinglePhaseForwardOperator: the PF matrix could not be set up sparse")
         if self.n_coef >= 2 ** 31:
             raise ValueError(f"{self.n_coef} coefficients: more than 2^31 - 1")
+        # stage patches (strip projection) and strips (backprojection) in LDS where they fit; False: the global-memory
+        # forms throughout (the same results)
+        self.local_staging = True
         self._setup_local()
         self.setup_s = time.perf_counter() - t0
         if verbose:
@@ -180,8 +183,12 @@ class LocalSinglePhaseForwardOperator(SinglePhaseForwardOperator):
             pk = Pt["pk"][p0:p1] - b["k_start"]
             size = h * w
             coff = Pt["coff"][p0:p1].astype(np.int64)
+            nblk = -(-size // 256)
+            blk = np.stack([np.repeat(np.arange(nb), nblk),
+                            np.concatenate([np.arange(c) * 256 for c in nblk])], 1)
             self.local_batches.append(dict(
-                n=nb, SW=SW, box=i32(np.stack([y0, x0, h, w], 1)),
+                n=nb, SW=SW, max_area=int(size.max()), w_max=int(Wp.max()), blk=i32(blk), n_blk=int(nblk.sum()),
+                box=i32(np.stack([y0, x0, h, w], 1)),
                 coff=torch.from_numpy(coff).to(dev), W=i32(Wp), pw_off=i32(pw_off), ptab=i32(ptab),
                 jp=i32(np.repeat(np.arange(nb), Wp)), pk=i32(pk),
                 kp_ptr=i32(np.searchsorted(pk, np.arange(b["K_batch"] + 1))),
@@ -189,6 +196,11 @@ class LocalSinglePhaseForwardOperator(SinglePhaseForwardOperator):
             max_strip = max(max_strip, R * SW)
             strip_bins += SW * R
         self.strip_fraction = strip_bins / float(R * self.My * self.K)
+        # the most pixels one bin's footprint covers along a patch row, ceil(2 hw / A) over the angles (float32 as the
+        # kernels compute it); 0 (no bound) beyond 4
+        A = np.maximum(np.abs(X.astype(np.float32)), np.abs(Y.astype(np.float32))).astype(np.float64)
+        nr = int(np.ceil(2.0 * float(np.max(hw / A)) - 1e-9))
+        self._n_reach = nr if nr <= 4 else 0
         self._strips = torch.empty(max_strip, dtype=torch.float32, device=dev)
         self._mask = torch.from_numpy(Pt["mask"]).to(dev)
         self._all_omegas = torch.arange(R, dtype=torch.int32, device=dev)
@@ -210,8 +222,9 @@ class LocalSinglePhaseForwardOperator(SinglePhaseForwardOperator):
                 continue
             b = self.batches[ib]
             _lib.call("dfm_local_strips", _lib.ptr(x), _lib.ptr(self._mask), _lib.ptr(lb["box"]), _lib.ptr(lb["coff"]),
-                      _lib.ptr(lb["pw_off"]), lb["SW"], _lib.ptr(lb["jp"]), _lib.ptr(lb["ptab"]), lb["n"],
-                      _lib.ptr(self.geo_fwd), _lib.ptr(om), n_ang, float(self.radon_scale), _lib.ptr(self._strips), st)
+                      _lib.ptr(lb["W"]), _lib.ptr(lb["pw_off"]), lb["SW"], _lib.ptr(lb["jp"]), _lib.ptr(lb["ptab"]),
+                      lb["n"], lb["max_area"] if self.local_staging else 1 << 30, self._n_reach, _lib.ptr(self.geo_fwd), _lib.ptr(om), n_ang, float(self.radon_scale),
+                      _lib.ptr(self._strips), st)
             _lib.call("dfm_local_assemble", _lib.ptr(self._strips), lb["SW"], _lib.ptr(lb["pw_off"]), _lib.ptr(lb["W"]),
                       _lib.ptr(lb["ptab"]), lb["n"], _lib.ptr(lb["kp_ptr"]), b["K_batch"], _lib.ptr(om), n_ang, self.My,
                       b["kstride"], _lib.ptr(sc["sino"]), st)
@@ -240,10 +253,12 @@ class LocalSinglePhaseForwardOperator(SinglePhaseForwardOperator):
                       _lib.ptr(lb["ptab"]), lb["n"], _lib.ptr(lb["pk"]), _lib.ptr(lb["jp"]), _lib.ptr(lb["pw_off"]),
                       lb["SW"], _lib.ptr(self._strips), st)
             _lib.call("dfm_local_backproject", _lib.ptr(self._strips), lb["SW"], _lib.ptr(lb["pw_off"]),
-                      _lib.ptr(lb["W"]), _lib.ptr(lb["ptab"]), lb["n"], _lib.ptr(self.geo_bwd), _lib.ptr(om), n_ang,
-                      _lib.ptr(lb["box"]), _lib.ptr(lb["coff"]), _lib.ptr(self._mask), _lib.ptr(lb["pid"]),
-                      lb["c_begin"], lb["n_c"], int(self.n_taps), int(mode), _lib.ptr(out), _lib.ptr(x), _lib.ptr(z),
-                      _lib.ptr(T), float(inv_p), float(lam), st)
+                      _lib.ptr(lb["W"]), _lib.ptr(lb["ptab"]), lb["n"], lb["w_max"], _lib.ptr(self.geo_bwd),
+                      _lib.ptr(om), n_ang, _lib.ptr(lb["box"]), _lib.ptr(lb["coff"]), _lib.ptr(self._mask),
+                      _lib.ptr(lb["pid"]), lb["c_begin"], lb["n_c"], _lib.ptr(lb["blk"]),
+                      lb["n_blk"] if self.local_staging else 0,
+                      int(self.n_taps), int(mode), _lib.ptr(out), _lib.ptr(x), _lib.ptr(z), _lib.ptr(T),
+                      float(inv_p), float(lam), st)
             self.mark("radon_backward_spdhg" if mode == 1 else "radon_backward")
 
     def direct_cl(self, coeffs, data, subset=None):

@@ -7,6 +7,8 @@ ground truths and Poisson data. Every generator is seeded and deterministic.
   al_c1  The same scan with a uniform cubic fundamental-zone grid at 3 deg spacing (random orientations thinned to
          0.6 x 3 deg: K = 29713), sigma 2 deg;
          80 grains with coefficients U(0.5, 1.5); SPDHG M = 10, 300 steps.
+  al_local  The al_c1 scan, grid and truth on a local basis: every voxel allows the orientations within 3.5 deg of
+         its grain's (grown by 2 voxels).
 """
 from __future__ import annotations
 
@@ -79,6 +81,14 @@ def uniform_fz(spacing_deg, symmetry, seed=0, thin=0.7):
 def voronoi_truth(K, n, n_grains, radius, seed, coeff=None, orientations=None):
     """(K, n, n) coefficients: n_grains Voronoi cells of seeds uniform in a centred disc of `radius` pixels; every
     pixel of the disc gets its cell's orientation with coefficient 1 (coeff None) or U(coeff) per grain."""
+    k, iy, ix, v = voronoi_pairs(K, n, n_grains, radius, seed, coeff, orientations)
+    x = np.zeros((K, n, n), np.float32)
+    x[k, iy, ix] = v
+    return x
+
+
+def voronoi_pairs(K, n, n_grains, radius, seed, coeff=None, orientations=None):
+    """The non-zeros of voronoi_truth as (orientation, row, column, value) arrays, without the (K, n, n) array."""
     rng = np.random.default_rng(seed)
     c = (n - 1) / 2.0
     r = radius * np.sqrt(rng.random(n_grains))
@@ -90,10 +100,39 @@ def voronoi_truth(K, n, n_grains, radius, seed, coeff=None, orientations=None):
     inside = (xx - c) ** 2 + (yy - c) ** 2 <= radius ** 2
     d2 = (xx[..., None] - sx) ** 2 + (yy[..., None] - sy) ** 2
     cell = np.argmin(d2, axis=-1)
-    x = np.zeros((K, n, n), np.float32)
     iy, ix = np.nonzero(inside)
-    x[ori[cell[iy, ix]], iy, ix] = val[cell[iy, ix]]
-    return x
+    return ori[cell[iy, ix]], iy, ix, val[cell[iy, ix]].astype(np.float32)
+
+
+LOCAL = dict(tol_deg=3.5, dilate=2)
+
+
+def al_local_problem(precision="bf16_residual", pf_mode="sparse", max_gb=1024.0):
+    """The local-basis workload: the al_c1 scan and 3-degree grid (K = 29713) and the al_c1 Voronoi truth (80 grains
+    in a disc of radius 45 px); every voxel allows the basis orientations within 3.5 deg of its grain's orientation
+    (what point-by-point indexing finds), grown by 2 voxels (LocalSupport.from_candidates). Counts: the local
+    forward projection of the truth at a mean of 1000. Returns (local operator, truth (n_coef,) float32, counts,
+    parameters)."""
+    from .local_operator import LocalSinglePhaseForwardOperator
+    from .local_support import LocalSupport
+    p = CONFIGS["al_c1"]
+    grid, mat, cfg = make_grid("al_c1")
+    K, n = len(grid), p["n"]
+    k, iy, ix, v = voronoi_pairs(K, n, p["grains"], p["radius"], p["seed"], p["coeff"])
+    sup = LocalSupport.from_candidates(grid.matrices, iy, ix, grid.matrices[k], "cubic", (n, n),
+                                       tol_deg=LOCAL["tol_deg"], dilate=LOCAL["dilate"])
+    op = LocalSinglePhaseForwardOperator(cfg, mat, grid, max_gb, support=sup, normalized=True, pf_mode=pf_mode,
+                                         pf_precision=precision)
+    s = op.support
+    keys = (s.k * n + s.iy) * n + s.ix
+    want = (k * n + iy) * n + ix
+    at = np.searchsorted(keys, want)
+    if not np.array_equal(keys[np.minimum(at, keys.size - 1)], want):
+        raise RuntimeError("the truth lies outside the local support")
+    xt = np.zeros(op.n_coef, np.float32)
+    xt[op._patches["pair_index"][at]] = v
+    b, _ = poisson_counts(op.direct(xt), p["mean_counts"], p["seed"])
+    return op, xt, b, {**p, **LOCAL}
 
 
 def poisson_counts(ax, mean_counts, seed, weights=None):

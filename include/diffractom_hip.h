@@ -380,7 +380,9 @@ int dfm_spdhg_update_2d(float* x, float* z, const float* T, const float* dz, int
  * strips[a * sw + pw_off[p] + j] (sw = the sum of W over the batch), jp[jg] the patch of strip bin jg; ptab (R, P)
  * int32 pairs (off, tb): the strip's first detector bin at angle a and tb = t(x0, y0) - off as float bits; geo the
  * operator's geometry records (R x 8 floats). Angles: omegas[0 .. n_ang - 1].
- * strips (local_kernels.cl:38-117 local_radon_forward): strips = scale / A sum of footprint x allowed coefficients.
+ * strips (local_kernels.cl:38-117 local_radon_forward): strips = scale / A sum of footprint x allowed coefficients;
+ *   max_area: the batch's largest patch (h w), which selects a workgroup per patch staged in LDS (up to 64 KB);
+ *   n_reach: ceil(2 max_a hw_a / A_a), the most pixels a bin's footprint covers along a row (0: no bound). 
  * assemble (the sinogram rows local_kernels.cl:211-244 local_pf_forward assembles in local memory): the batch's
  *   orientation-fastest sinogram (n_ang, my, kstride), sino[as, y, k] = sum over the patches of orientation k
  *   (kp_ptr[k] .. kp_ptr[k + 1]) of their strip bin at line y, 0 elsewhere.
@@ -388,11 +390,13 @@ int dfm_spdhg_update_2d(float* x, float* z, const float* T, const float* dz, int
  *   0 off the detector.
  * backproject (local_kernels.cl:120-174 local_radon_backward): for the batch's coefficients c_begin .. c_begin + n - 1
  *   (pid: the patch of each) acc = sum_a (w / A)_a sum of footprint x strip bins; mode 0: out = acc (0 where not
- *   allowed); mode 1: the SPDHG primal update with dz = acc (as dfm_spdhg_update). n_taps 2 .. 4. */
+ *   allowed); mode 1: the SPDHG primal update with dz = acc (as dfm_spdhg_update). n_taps 2 .. 4. blk: (patch,
+ *   first pixel) per 256 pixels (n_blk of them) for the form that stages a patch's strips in LDS (n_ang (32 + 4
+ *   w_max) bytes up to 64 KB, w_max the batch's widest strip); pid (the patch of each coefficient) for the other. */
 int dfm_local_strips(const float* coef, const uint8_t* mask, const int32_t* box, const int64_t* coff,
-                     const int32_t* pw_off, int32_t sw, const int32_t* jp, const int32_t* ptab, int32_t n_patches,
-                     const float* geo, const int32_t* omegas, int32_t n_ang, float scale, float* strips,
-                     dfm_stream_t stream);
+                     const int32_t* W, const int32_t* pw_off, int32_t sw, const int32_t* jp, const int32_t* ptab,
+                     int32_t n_patches, int32_t max_area, int32_t n_reach, const float* geo, const int32_t* omegas,
+                     int32_t n_ang, float scale, float* strips, dfm_stream_t stream);
 int dfm_local_assemble(const float* strips, int32_t sw, const int32_t* pw_off, const int32_t* W, const int32_t* ptab,
                        int32_t n_patches, const int32_t* kp_ptr, int32_t k_batch, const int32_t* omegas,
                        int32_t n_ang, int32_t my, int32_t kstride, float* sino, dfm_stream_t stream);
@@ -400,10 +404,11 @@ int dfm_local_extract(const float* sino, int32_t n_ang, int32_t my, int32_t kstr
                       const int32_t* ptab, int32_t n_patches, const int32_t* pk, const int32_t* jp,
                       const int32_t* pw_off, int32_t sw, float* strips, dfm_stream_t stream);
 int dfm_local_backproject(const float* strips, int32_t sw, const int32_t* pw_off, const int32_t* W,
-                          const int32_t* ptab, int32_t n_patches, const float* geo, const int32_t* omegas,
-                          int32_t n_ang, const int32_t* box, const int64_t* coff, const uint8_t* mask,
-                          const int32_t* pid, int64_t c_begin, int64_t n, int32_t n_taps, int32_t mode, float* out,
-                          float* x, float* z, const float* T, float inv_p, float lam, dfm_stream_t stream);
+                          const int32_t* ptab, int32_t n_patches, int32_t w_max, const float* geo,
+                          const int32_t* omegas, int32_t n_ang, const int32_t* box, const int64_t* coff,
+                          const uint8_t* mask, const int32_t* pid, int64_t c_begin, int64_t n, const int32_t* blk,
+                          int32_t n_blk, int32_t n_taps, int32_t mode, float* out, float* x, float* z, const float* T,
+                          float inv_p, float lam, dfm_stream_t stream);
 
 /* ---------------------------------------------------------------- SPDHG-Poisson (optimization/spdhg_poisson.py)
  * Periodic arrays w, g, bg: element i uses a[i % period]; period 0 means the constant 1, 1, 0.

@@ -45,12 +45,18 @@ def _local(pf_mode="sparse", split=True, K=30, max_gb=1e-4, full=False, precisio
     return op, OracleLocalOperator(base, k, iy, ix, pi, op.n_coef), base, U
 
 
-@pytest.mark.parametrize("pf_mode,precision", [("sparse", "fp32"), ("generated", "fp32"),
-                                               ("sparse", "bf16_residual"), ("generated", "fp16")])
+@pytest.mark.parametrize("pf_mode,precision,staging", [("sparse", "fp32", True), ("sparse", "fp32", False),
+                                                       ("generated", "fp32", True), ("sparse", "bf16_residual", True),
+                                                       ("generated", "fp16", True)])
 @pytest.mark.parametrize("split", [True, False])
-def test_local_matches_oracle(pf_mode, precision, split):
-    print(f"case pf_mode={pf_mode} precision={precision} split={split}", flush=True)
+def test_local_matches_oracle(pf_mode, precision, staging, split):
+    """staging False: the global-memory forms of the strip projection and the backprojection (taken by patches and
+    strips too large for the LDS) and the counted pixel loop (taken by pixels smaller than about 0.4 bins)."""
+    print(f"case pf_mode={pf_mode} precision={precision} staging={staging} split={split}", flush=True)
     op, orc, _, _ = _local(pf_mode, split, precision=precision)
+    op.local_staging = staging
+    if not staging:
+        op._n_reach = 0
     assert len(op.batches) > 1 and op.n_patches >= op.K
     rng = np.random.default_rng(6)
     x = rng.random(op.coeff_shape).astype(np.float32)
@@ -67,13 +73,14 @@ def test_local_matches_oracle(pf_mode, precision, split):
     assert abs(lhs / rhs - 1) < (1e-5 if precision != "fp16" else 1e-3)
 
 
-def test_local_equals_global_operator():
+@pytest.mark.parametrize("kernel", ["trapezoid", "hat"])
+def test_local_equals_global_operator(kernel):
     """Every orientation on the whole field of view, one patch each: the local operator gives the global operator's
     forward of the same (zero-outside-the-view) coefficients and its adjoint on the view."""
-    print("case full support", flush=True)
-    op, _, _, U = _local("sparse", split=False, full=True)
+    print(f"case full support, kernel {kernel}", flush=True)
+    op, _, _, U = _local("sparse", split=False, full=True, radon_kernel=kernel)
     gop, _, _ = build(cfg(nx=20, ny=20, my=21, n_omega=30, angle_range=(0, 360), cor=0.5), K=30, pf_mode="sparse",
-                      max_gb=1e-4, oracle=False)
+                      max_gb=1e-4, oracle=False, radon_kernel=kernel)
     fov = gop.support_mask()
     rng = np.random.default_rng(2)
     xd = (rng.random(gop.coeff_shape) * fov[None]).astype(np.float32)
