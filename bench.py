@@ -64,6 +64,8 @@ def parse():
                          "counter runs that give roofline.traffic)")
     ap.add_argument("--graphs", action="store_true", help="replay subset steps from captured HIP graphs (pays off on "
                     "small problems, where launches cost more than the kernels)")
+    ap.add_argument("--no-compare", action="store_true", help="al_local: the local operator's steps only (no global "
+                    "operator run, no parity figures; for profiling)")
     ap.add_argument("--solver", default="spdhg", choices=["spdhg", "osem", "pdhg", "fista"],
                     help="al_c0 / al_c1: the solver timed (osem: ordered-subsets EM subset iterations; pdhg: "
                          "Chambolle-Pock iterations over all angles; fista: SQS-metric FISTA iterations over all "
@@ -469,6 +471,8 @@ def run_local(args):
     ms_l, kms_l = solve(op, torch.zeros(op.coeff_shape, dtype=torch.float32, device=dev))
     log(f"[bench] al_local: local {ms_l:.4f} ms/step; kernel ms: " +
         ", ".join(f"{k} {v:.4f}" for k, v in (kms_l or {}).items()))
+    if args.no_compare:
+        return
     # the global operator on the same orientations (operator order), full images
     U = op.grid_rot_cpu
     sig = float(np.asarray(op.sigma_cpu)[0])
