@@ -551,8 +551,10 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
   constexpr int TBB = TS == 4 ? TB : 16;   // bins a tile reaches at one angle (host-checked)
   __shared__ float4 lds[ACH * TBB * QW];  // ACH angles x TBB bins x QW quads
   // pixel-table rows of the tile's pixels for TCH angles, as staged: the byte offsets of the two bins' rows within
-  // the angle's LDS slab ((bin - first bin of the tile) x QW quads x 16 B) and the two weights
-  __shared__ int4 lpt[TCH * NPX];
+  // the angle's LDS slab ((bin - first bin of the tile) x QW quads x 16 B < 2^16, packed in one int) and the two
+  // weights (12 bytes per entry instead of 16: 8 angles' entries in flight take 24 VGPRs, not 32)
+  __shared__ int lofs[TCH * NPX];
+  __shared__ float2 lw[TCH * NPX];
   __shared__ int lbase[TCH];
   static_assert(ACH * TBB * QW % NTH == 0 && TCH % ACH == 0, "whole staging rounds");
   const int tnx = (nx + TS - 1) / TS, tny = (ny + TS - 1) / TS;
@@ -574,14 +576,16 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
     for (int i = threadIdx.x; i < tn * NPX; i += NTH) {
       const int al = i / NPX, l = i % NPX;
       const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
-      int4 e = make_int4(0, 0, 0, 0);
+      int ofs = 0;
+      float2 ww = make_float2(0.f, 0.f);
       if (ppx < nx && ppy < ny) {
-        e = ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx];
+        const int4 e = ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx];
         const int b0 = tbase[(int64_t)(t0 + al) * ntiles + tile];
-        e.x = (e.x - b0) * QW * (int)sizeof(float4);
-        e.y = (e.y - b0) * QW * (int)sizeof(float4);
+        ofs = ((e.x - b0) * QW * (int)sizeof(float4)) | (((e.y - b0) * QW * (int)sizeof(float4)) << 16);
+        ww = make_float2(__int_as_float(e.z), __int_as_float(e.w));
       }
-      lpt[i] = e;
+      lofs[i] = ofs;
+      lw[i] = ww;
     }
   };
   if (tables_once) stage_tables(0, na);
@@ -592,12 +596,6 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
     const bool live = inside && kq < k4_n;
     const int64_t o = pix * kstride4 + kq + k4_off;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 zz = acc, xx = acc, tt = acc;
-    if (MODE >= 1 && live) {
-      if (MODE == 1) zz = ld_stream(z + o);
-      xx = ld_stream(x + o);
-      tt = ld_stream(T + o);                              // MODE 2: the EM sensitivity; 3: the PDHG step
-    }
     for (int t0 = 0; t0 < na; t0 += TCH) {
       const int tn = min(TCH, na - t0);
       if (!tables_once) {
@@ -642,10 +640,11 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
           // is unrolled without branches so that its table reads are all issued first
           const char* lq = reinterpret_cast<const char*>(lds + q);
           auto angle = [&](int al) {
-            const int4 e = lpt[(a0 + al) * NPX + li];
-            const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
-            const float4 s0 = reinterpret_cast<const float4*>(lq + e.x)[al * TBB * QW];
-            const float4 s1 = reinterpret_cast<const float4*>(lq + e.y)[al * TBB * QW];
+            const unsigned ofs = (unsigned)lofs[(a0 + al) * NPX + li];
+            const float2 ww = lw[(a0 + al) * NPX + li];
+            const float w0 = ww.x, w1 = ww.y;
+            const float4 s0 = reinterpret_cast<const float4*>(lq + (ofs & 0xFFFFu))[al * TBB * QW];
+            const float4 s1 = reinterpret_cast<const float4*>(lq + (ofs >> 16))[al * TBB * QW];
             acc.x = fmaf(w1, s1.x, fmaf(w0, s0.x, acc.x));
             acc.y = fmaf(w1, s1.y, fmaf(w0, s0.y, acc.y));
             acc.z = fmaf(w1, s1.z, fmaf(w0, s0.z, acc.z));
@@ -661,6 +660,14 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
       }
     }
     if (live) {
+      // x, z, T (MODE 2: the EM sensitivity; 3: the PDHG step) are read after the accumulation: held across it they
+      // would cost 12 VGPRs of the 64 (8 waves per SIMD), and the other waves hide the latency
+      float4 zz = make_float4(0.f, 0.f, 0.f, 0.f), xx = zz, tt = zz;
+      if (MODE >= 1) {
+        if (MODE == 1) zz = ld_stream(z + o);
+        xx = ld_stream(x + o);
+        tt = ld_stream(T + o);
+      }
       if (MODE == 0) {
         st_stream(out + o, acc);
       } else if (MODE == 1) {
