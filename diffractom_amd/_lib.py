@@ -96,6 +96,8 @@ SIGNATURES = {
     "dfm_nufft_spread": [P, P, P, P, P, P, I32, I32, I64, P, P],
     "dfm_nufft_crop": [P, P, P, I64, I32, F32, P, I32, I32, I32, P],
     "dfm_spdhg_update_2d": [P, P, P, P, I64, I32, I32, I32, F32, F32, P],
+    "dfm_dense_f32": [I32, P, I64, I32, I32, P, I64, I32, P, I64, I32, I32, P, I32, I32, I32, I32, F32, I32, I32, P,
+                      P],
     "dfm_local_strips": [P, P, P, P, P, P, I32, P, P, I32, I32, I32, P, P, I32, F32, P, P],
     "dfm_local_assemble": [P, I32, P, P, P, I32, P, I32, P, I32, I32, I32, P, P],
     "dfm_local_extract": [P, I32, I32, I32, P, P, I32, P, P, P, I32, P, P],

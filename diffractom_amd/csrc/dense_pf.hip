@@ -371,6 +371,116 @@ void y_tiling(int my, int& mt_n, int& n_yg) {
   mt_n = (tiles + n_yg - 1) / n_yg;
 }
 
+
+// The FP32 dense PF products (pf_mode "dense", pf_precision "fp32"; single_phase_forward_operator.py:2063-2127
+// gemmStridedBatched), one angle per blockIdx.z: C[y][c] (+)= alpha sum_r A[y][r] B(r, c) with A row-major (lda),
+// B(r, c) = NT ? Bm[c ldb + r] : Bm[r ldb + c], C row-major (ldc). The forward is NN (sino x PF: B's rows are the
+// orientations), the adjoint NT (data x PF^T: the PF rows read along their segments), both straight from the stored
+// (R, kstride, CP) batch: no transposed or angle-gathered copy of it. A 64 x 64 output tile per 256-thread workgroup,
+// 4 x 4 outputs per thread, the reduction in chunks of 16 staged in LDS (transposed for A, so both operands are read
+// as float4 rows of the chunk); FP32 FMA in a fixed order.
+template <bool NT>
+__global__ void __launch_bounds__(256) dense_f32_kernel(const float* __restrict__ A, int64_t a_sa, int lda, int a_om,
+                                                        const float* __restrict__ Bm, int64_t b_sa, int ldb,
+                                                        float* __restrict__ C, int64_t c_sa, int ldc, int c_om,
+                                                        const int* __restrict__ omegas, int n_ang, int M, int N,
+                                                        int R, int r_len, float alpha, int accumulate,
+                                                        float* __restrict__ partial) {
+  __shared__ float4 As[16][16 + 1];   // As[r][y / 4]: rows y of the tile, chunk column r
+  __shared__ float4 Bs[16][16 + 1];   // Bs[r][c / 4]
+  // blockIdx.z = split * n_ang + as: the split's reduction range r_len * split .. (partial sums with a split)
+  const int as = blockIdx.z % n_ang, split = blockIdx.z / n_ang;
+  const int a = omegas ? omegas[as] : as;
+  const int r_beg = split * r_len, r_end = min(R, r_beg + r_len);
+  const float* Ap = A + (a_om ? a : as) * a_sa;
+  const float* Bp = Bm + (int64_t)a * b_sa;
+  float* Cp = C + (c_om ? a : as) * c_sa;
+  const int y0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int t = threadIdx.x;
+  const int ty = t >> 4, tx = t & 15;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+  float* Asf = reinterpret_cast<float*>(As);
+  float* Bsf = reinterpret_cast<float*>(Bs);
+  for (int r0 = r_beg; r0 < r_end; r0 += 16) {
+    // A chunk: row y = y0 + t / 4, columns r0 + 4 (t % 4) .. + 3, stored transposed
+    {
+      const int yl = t >> 2, rl = (t & 3) * 4;
+      const int y = y0 + yl;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + rl + u;
+        Asf[(rl + u) * (17 * 4) + yl] = (y < M && r < r_end) ? Ap[(int64_t)y * lda + r] : 0.0f;
+      }
+    }
+    // B chunk: Bs[r][c]
+    if (NT) {
+      const int cl = t >> 2, rl = (t & 3) * 4;
+      const int c = c0 + cl;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = r0 + rl + u;
+        Bsf[(rl + u) * (17 * 4) + cl] = (c < N && r < r_end) ? Bp[(int64_t)c * ldb + r] : 0.0f;
+      }
+    } else {
+      const int rl = t >> 4, cl = (t & 15) * 4;
+      const int r = r0 + rl;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + cl + u;
+        Bsf[rl * (17 * 4) + cl + u] = (c < N && r < r_end) ? Bp[(int64_t)r * ldb + c] : 0.0f;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float4 av = As[r][ty], bv = Bs[r][tx];
+      const float ai[4] = {av.x, av.y, av.z, av.w}, bj[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ai[i], bj[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int y = y0 + ty * 4 + i;
+    if (y >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = c0 + tx * 4 + j;
+      if (c >= N) continue;
+      if (partial) {
+        partial[((int64_t)blockIdx.z * M + y) * N + c] = acc[i][j];
+      } else {
+        float* o = Cp + (int64_t)y * ldc + c;
+        *o = accumulate ? *o + alpha * acc[i][j] : alpha * acc[i][j];
+      }
+    }
+  }
+}
+
+// C (+)= alpha sum over the splits (in order) of the partial sums of dense_f32_kernel.
+__global__ void __launch_bounds__(256) dense_f32_reduce_kernel(const float* __restrict__ partial, int n_split,
+                                                               float* __restrict__ C, int64_t c_sa, int ldc, int c_om,
+                                                               const int* __restrict__ omegas, int n_ang, int M,
+                                                               int N, float alpha, int accumulate) {
+  const int64_t per = (int64_t)M * N, total = per * n_ang;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int as = (int)(i / per);
+    const int64_t e = i - (int64_t)as * per;
+    const int y = (int)(e / N), c = (int)(e - (int64_t)y * N);
+    float v = 0.0f;
+    for (int sp = 0; sp < n_split; ++sp) v += partial[((int64_t)sp * n_ang + as) * per + e];
+    const int a = omegas ? omegas[as] : as;
+    float* o = C + (c_om ? a : as) * c_sa + (int64_t)y * ldc + c;
+    *o = accumulate ? *o + alpha * v : alpha * v;
+  }
+}
 }  // namespace
 
 namespace dfm {
@@ -385,6 +495,34 @@ int launch_splitk_reduce(const float* part, float* data, const int* omegas, int 
 }  // namespace dfm
 
 extern "C" {
+
+int dfm_dense_f32(int32_t transpose_b, const float* a, int64_t a_stride, int32_t lda, int32_t a_by_angle,
+                  const float* b, int64_t b_stride, int32_t ldb, float* c, int64_t c_stride, int32_t ldc,
+                  int32_t c_by_angle, const int32_t* omegas, int32_t n_ang, int32_t m, int32_t n, int32_t r,
+                  float alpha, int32_t accumulate, int32_t r_split, float* partial, dfm_stream_t stream) {
+  DFM_CHECK_ARG(a && b && c && n_ang >= 0 && m >= 0 && n >= 0 && r >= 0 && lda >= 0 && ldb >= 0 && ldc >= n &&
+                    a_stride >= 0 && b_stride >= 0 && c_stride >= 0 && r_split >= 1 && (r_split == 1 || partial) &&
+                    (int64_t)n_ang * r_split < 65536,
+                "dfm_dense_f32: bad arguments");
+  if (n_ang == 0 || m == 0 || n == 0) return DFM_OK;
+  const int r_len = r_split == 1 ? (r > 0 ? r : 1) : ((r + r_split - 1) / r_split + 15) / 16 * 16;
+  const int n_split = r_split == 1 ? 1 : (r + r_len - 1) / r_len;
+  float* part = n_split > 1 ? partial : nullptr;
+  const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64), (unsigned)(n_ang * n_split));
+  const hipStream_t st = (hipStream_t)stream;
+  if (transpose_b)
+    hipLaunchKernelGGL(dense_f32_kernel<true>, grid, dim3(256), 0, st, a, a_stride, lda, a_by_angle, b, b_stride, ldb,
+                       c, c_stride, ldc, c_by_angle, omegas, n_ang, m, n, r, r_len, alpha, accumulate, part);
+  else
+    hipLaunchKernelGGL(dense_f32_kernel<false>, grid, dim3(256), 0, st, a, a_stride, lda, a_by_angle, b, b_stride,
+                       ldb, c, c_stride, ldc, c_by_angle, omegas, n_ang, m, n, r, r_len, alpha, accumulate, part);
+  if (part)
+    hipLaunchKernelGGL(dense_f32_reduce_kernel, dim3(dfm::grid_stride_blocks((int64_t)n_ang * m * n, 256)),
+                       dim3(256), 0, st, part, n_split, c, c_stride, ldc, c_by_angle, omegas, n_ang, m, n, alpha,
+                       accumulate);
+  DFM_LAUNCH_CHECK("dfm_dense_f32");
+  return DFM_OK;
+}
 
 int dfm_mfma_pack_fwd(const float* dense, void* hi, void* lo, int R, int km, int kb, int cp, int n_nt, int n_kc,
                       dfm_stream_t stream) {

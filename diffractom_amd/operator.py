@@ -1529,7 +1529,7 @@ class SinglePhaseForwardOperator:
         return p
 
     def _dense_forward(self, ib, sino, data, subset, accumulate):
-        """Dense PF path: data[r] (+)= sino[r] (My x Kb) @ PF[r] (Kb x CP), batched over the angles."""
+        """Dense PF path: data[r] (+)= sino[r] (My x Kb) @ PF[r] (Kb x CP) for the call's angles."""
         b = self.batches[ib]
         n_ang, om, _, _ = self._angle_set(subset)
         if self.mfma_batches is not None:
@@ -1540,16 +1540,16 @@ class SinglePhaseForwardOperator:
                       _lib.ptr(mb["fwd_lo"]), _lib.ptr(om), n_ang, self.My, self.N_seg, mb["n_nt"], mb["n_kc"],
                       _lib.ptr(data), _lib.ptr(part), ks, int(accumulate), self.mfma_variant["fwd"], self._stream())
             return
-        pf = self.dense_batches[ib]
-        if om is not None:
-            pf = pf.index_select(0, om.long())
-        s3 = sino[: n_ang * self.My * b["kstride"]].view(n_ang, self.My, b["kstride"])
-        out = torch.bmm(s3, pf)
-        rows = torch.arange(self.N_Omega, device=self.device) if om is None else om.long()
-        if accumulate:
-            data.index_add_(0, rows, out)
-        else:
-            data.index_copy_(0, rows, out)
+        # FP32: dfm_dense_f32 on the stored (R, kstride, CP) batch, the subset's angles through the angle list
+        ks, CP, My = b["kstride"], self.N_seg, self.My
+        # the long reduction over the orientations in parts (at least 4096 workgroups, parts of >= 512 terms),
+        # summed in order by a second kernel
+        tiles = -(-CP // 64) * -(-My // 64) * n_ang
+        sp = int(max(1, min(-(-4096 // tiles), b["K_batch"] // getattr(self, "dense_split_min", 512), 65535 // n_ang)))
+        part = self._mfma_part(sp * n_ang * My * CP) if sp > 1 else None
+        _lib.call("dfm_dense_f32", 0, _lib.ptr(sino), My * ks, ks, 0, _lib.ptr(self.dense_batches[ib]), ks * CP, CP,
+                  _lib.ptr(data), My * CP, CP, 1, _lib.ptr(om), n_ang, My, CP, b["K_batch"], 1.0, int(accumulate),
+                  sp, _lib.ptr(part), self._stream())
 
     def _dense_adjoint(self, ib, data, sino, subset, alpha):
         b = self.batches[ib]
@@ -1560,15 +1560,10 @@ class SinglePhaseForwardOperator:
                       _lib.ptr(om), n_ang, self.My, self.N_seg, mb["n_kt"], mb["n_jc"], _lib.ptr(sino),
                       b["kstride"], float(alpha), self.mfma_variant["adj"], self._stream())
             return
-        pf = self.dense_batches[ib]
-        d = data
-        if om is not None:
-            pf = pf.index_select(0, om.long())
-            d = data.index_select(0, om.long())
-        pft = pf.transpose(1, 2).contiguous()
-        s3 = sino[: n_ang * self.My * b["kstride"]].view(n_ang, self.My, b["kstride"])
-        torch.bmm(d, pft, out=s3)
-        s3.mul_(alpha)
+        ks, CP, My = b["kstride"], self.N_seg, self.My
+        _lib.call("dfm_dense_f32", 1, _lib.ptr(data), My * CP, CP, 1, _lib.ptr(self.dense_batches[ib]), ks * CP, CP,
+                  _lib.ptr(sino), My * ks, ks, 0, _lib.ptr(om), n_ang, My, ks, CP, float(alpha), 0, 1, None,
+                  self._stream())
 
     def direct_cl(self, coeffs, data, subset=None):
         """In-place forward: data = PF @ Radon(coeffs); with subset=m only the rows of subset m's angles are

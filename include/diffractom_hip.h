@@ -222,6 +222,17 @@ int dfm_mfma_forward(const float* sino, int32_t sks, int32_t kb, const void* hi,
                      dfm_stream_t stream);
 /* adjoint: sino[a, y, k] = alpha sum_j PF[omegas[a], k, j] data[omegas[a], y, j] for k < sks (columns >= kb are
  * 0). variant: k tiles per wave (2 or 4). */
+/* The FP32 dense PF products (gemmStridedBatched, single_phase_forward_operator.py:2063-2127), per angle as of
+ * the call's list (omegas, NULL: 0 .. n_ang - 1): C[y][c] (+)= alpha sum_r A[y][r] B(r, c), A row-major (lda), B(r, c)
+ * = b[c ldb + r] with transpose_b (the adjoint: data x PF^T) else b[r ldb + c] (the forward: sino x PF), C
+ * row-major (ldc); M x N outputs, R terms. Each operand's angle block is at its stride times the angle (a_by_angle /
+ * c_by_angle: the angle index omegas[as]) or times the position as in the list; b always by angle. r_split > 1:
+ * the R terms in r_split parts (multiples of 16) summed in order from partial (r_split n_ang M N floats);
+ * n_ang r_split < 65536. */
+int dfm_dense_f32(int32_t transpose_b, const float* a, int64_t a_stride, int32_t lda, int32_t a_by_angle,
+                  const float* b, int64_t b_stride, int32_t ldb, float* c, int64_t c_stride, int32_t ldc,
+                  int32_t c_by_angle, const int32_t* omegas, int32_t n_ang, int32_t m, int32_t n, int32_t r,
+                  float alpha, int32_t accumulate, int32_t r_split, float* partial, dfm_stream_t stream);
 int dfm_mfma_adjoint(const float* data, const void* hi, const void* lo, const int32_t* omegas, int32_t n_ang,
                      int32_t my, int32_t cp, int32_t n_kt, int32_t n_jc, float* sino, int32_t sks, float alpha,
                      int32_t variant, dfm_stream_t stream);

@@ -46,7 +46,7 @@ def test_storage_forms_and_rectangular_grid(pf_mode):
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5
 
 
-@pytest.mark.parametrize("pf_mode", ["sparse", "generated"])
+@pytest.mark.parametrize("pf_mode", ["sparse", "generated", "dense"])
 @pytest.mark.parametrize("subsets", [4, "blocks"])
 def test_subset_forward_and_adjoint(pf_mode, subsets):
     """tests/test_subsets.py:49-86: subset rows equal the full rows, other rows untouched, each subset adjoint is
@@ -319,3 +319,19 @@ def test_csr_transposed_kernels_match_oracle(my, tsplit, tkc, direct):
     out = torch.empty(op.coeff_shape, dtype=torch.float32, device="cuda")
     op.adjoint_cl(torch.from_numpy(y).cuda(), out, subset=2)
     assert rel_max(out.cpu().numpy(), orc.adjoint(ysub)) < 2e-5
+
+
+def test_dense_fp32_reduction_split():
+    """The FP32 dense forward with its reduction over the orientations split in parts summed in order (taken at
+    large K; forced here with parts of 32 terms): the oracle's forward within 2e-5, subsets included."""
+    print("case dense fp32 split", flush=True)
+    op, orc, _ = build(cfg(n_omega=12), K=300, sigma_deg=6.0, pf_mode="dense")
+    op.dense_split_min = 32
+    x, y = _inputs(op, 4)
+    assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
+    op.set_omega_subsets(3)
+    orc.set_omega_subsets(3)
+    import torch
+    d = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
+    op.direct_cl(torch.from_numpy(x).cuda(), d, subset=1)
+    assert rel_max(d.cpu().numpy(), orc.direct(x, subset=1)) < 2e-5
