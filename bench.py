@@ -137,7 +137,6 @@ def kernel_bytes(op, M):
     rows = 4.0 * Rm * My * CP
     pf = 6.0 * pf_nnz(op) / M
     return {"radon_backward_spdhg": 20.0 * K * npix + sino, "radon_forward": 4.0 * K * npix + sino,
-            "spdhg_fused": 20.0 * K * npix + 2.0 * sino,
             "pf_adjoint": pf + rows + sino, "pf_forward": pf + sino + rows, "dual_update": 6.0 * rows}
 
 

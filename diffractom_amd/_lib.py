@@ -25,7 +25,6 @@ SIGNATURES = {
     "dfm_radon_backward": [P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_backward_spdhg": [P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P, P, P, F32, F32, P],
     "dfm_radon_forward_table": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
-    "dfm_radon_forward_pairs": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_backward_table": [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P, P, P, F32, F32, I32, I32, P],
     "dfm_pf_eval_dense": [P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, F32, I32, P],
     "dfm_pf_count_fwd": [P, P, I32, I32, I32, I32, P],
@@ -72,8 +71,6 @@ SIGNATURES = {
     "dfm_tv_dual_k": [P, P, P, P, P, P, I32, I32, I32, F32, F32, P],
     "dfm_tv_adjoint_update_k": [P, P, P, P, P, I32, I32, I32, F32, F32, P],
     "dfm_tv_value_partial": [P, P, I32, I32, I32, P, P, P],
-    "dfm_projector_lds_width": [I32, I32, I32],
-    "dfm_projector_lds": [I32, P, P, I32, P, P, P, F32, F32, P, P, I32, F32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_pack_fwd": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_pack_adj": [P, P, P, I32, I32, I32, I32, I32, I32, P],
     "dfm_mfma_forward": [P, I32, I32, P, P, P, I32, I32, I32, I32, I32, P, P, I32, I32, I32, P],

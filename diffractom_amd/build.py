@@ -10,7 +10,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libdiffractom_hip.so"
-SOURCES = ("operator.hip", "solver.hip", "dense_pf.hip", "radon_fused.hip", "spmm_t.hip", "radon_band.hip", "radon_sweep.hip",
+SOURCES = ("operator.hip", "solver.hip", "dense_pf.hip", "spmm_t.hip", "radon_band.hip", "radon_sweep.hip",
            "pf_blocks.hip", "pf_gen.hip", "pf_grad.hip", "fourier.hip",
            "local_basis.hip")
 ARCH = os.environ.get("DFM_OFFLOAD_ARCH", "gfx950")

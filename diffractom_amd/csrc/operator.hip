@@ -353,86 +353,6 @@ __global__ void __launch_bounds__(256) radon_forward_table_lds_kernel(const floa
   if (live) st_stream(sino + (int64_t)r * skstride4 + kq, acc);
 }
 
-// Forward from the bin-pair table: a thread per (channel quad, bin pair 2q, 2q + 1), a workgroup 16 quads x 16 pairs
-// of one angle, entries staged in LDS as above. A pixel whose two footprint taps fall in one pair is gathered once
-// for both bins, so a projection gathers about 1.5 instead of 2 image values per pixel and angle.
-template <int UNR>
-__global__ void __launch_bounds__(256) radon_forward_pairs_kernel(const float4* __restrict__ img,
-                                                                  float4* __restrict__ sino,
-                                                                  const int* __restrict__ pair_ptr,
-                                                                  const int* __restrict__ ent_pix,
-                                                                  const float2* __restrict__ ent_w, int ns, int na,
-                                                                  int kstride4, int k4_off, int k4_n, int skstride4) {
-  __shared__ int lp[FT_CHUNK];
-  __shared__ float2 lw[FT_CHUNK];
-  const int np2 = (ns + 1) / 2;
-  const int qblocks = (np2 + 15) / 16;
-  const int n_blk = (k4_n + QB - 1) / QB;
-  int blk, local;
-  if (!xcd_split(na * qblocks, n_blk, blk, local)) return;       // uniform over the workgroup
-  const int a = local / qblocks;
-  const int q0 = (local % qblocks) * 16;
-  const int q = q0 + (threadIdx.x >> 4);
-  const int kq = blk * QB + (threadIdx.x & (QB - 1));
-  const bool live = kq < k4_n && q < np2;
-  const int g0 = pair_ptr[a * np2 + q0];
-  const int g1 = pair_ptr[a * np2 + min(q0 + 16, np2)];
-  const int r = a * np2 + min(q, np2 - 1);
-  const int e0 = live ? pair_ptr[r] : 0;
-  const int e1 = live ? pair_ptr[r + 1] : 0;
-  const float4* col = img + (live ? kq : 0) + k4_off;
-  float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
-  for (int c0 = g0; c0 < g1; c0 += FT_CHUNK) {
-    const int c1 = min(c0 + FT_CHUNK, g1);
-    __syncthreads();
-    for (int i = threadIdx.x; i < c1 - c0; i += 256) {
-      lp[i] = ent_pix[c0 + i];
-      lw[i] = ent_w[c0 + i];
-    }
-    __syncthreads();
-    const int en = min(e1, c1);
-    int e = max(e0, c0);
-    for (; e + UNR <= en; e += UNR) {
-      int p[UNR];
-      float2 w[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        p[u] = lp[e + u - c0];
-        w[u] = lw[e + u - c0];
-      }
-      float4 v[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) v[u] = col[(int64_t)p[u] * kstride4];
-#pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        a0.x += w[u].x * v[u].x;
-        a0.y += w[u].x * v[u].y;
-        a0.z += w[u].x * v[u].z;
-        a0.w += w[u].x * v[u].w;
-        a1.x += w[u].y * v[u].x;
-        a1.y += w[u].y * v[u].y;
-        a1.z += w[u].y * v[u].z;
-        a1.w += w[u].y * v[u].w;
-      }
-    }
-    for (; e < en; ++e) {
-      const float2 w = lw[e - c0];
-      const float4 v = col[(int64_t)lp[e - c0] * kstride4];
-      a0.x += w.x * v.x;
-      a0.y += w.x * v.y;
-      a0.z += w.x * v.z;
-      a0.w += w.x * v.w;
-      a1.x += w.y * v.x;
-      a1.y += w.y * v.y;
-      a1.z += w.y * v.z;
-      a1.w += w.y * v.w;
-    }
-  }
-  if (!live) return;
-  st_stream(sino + ((int64_t)a * ns + 2 * q) * skstride4 + kq, a0);
-  if (2 * q + 1 < ns) st_stream(sino + ((int64_t)a * ns + 2 * q + 1) * skstride4 + kq, a1);
-}
-
 // Backward from the tables, one 4 x 4 pixel tile x 16 channel quads per workgroup: the TB bins of every angle the
 // tile reaches are staged in LDS (TB x 16 quads x 16 B per angle, angles in chunks of ACH), then every thread (pixel,
 // quad) reads its two taps from LDS. MODE 0 writes the image, MODE 1 applies the SPDHG primal update.
@@ -1088,33 +1008,6 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
 #undef DFM_FTQ
 #undef DFM_FT
   DFM_LAUNCH_CHECK("dfm_radon_forward_table");
-  return DFM_OK;
-}
-
-int dfm_radon_forward_pairs(const float* img_k, float* sino, const int32_t* pair_ptr, const int32_t* ent_pix,
-                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
-                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream) {
-  DFM_CHECK_ARG(img_k && sino && pair_ptr && ent_pix && ent_w, "dfm_radon_forward_pairs: null pointer");
-  DFM_CHECK_ARG(ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
-                    k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
-                "dfm_radon_forward_pairs: bad sizes");
-  DFM_CHECK_ARG(variant == 1 || variant == 4 || variant == 8, "dfm_radon_forward_pairs: variant must be 1, 4 or 8");
-  if (n_angles == 0 || k_n == 0) return DFM_OK;
-  const int k4n = k_n / 4;
-  const unsigned grid = xcd_grid(n_angles * (((ns + 1) / 2 + 15) / 16), (k4n + QB - 1) / QB);
-#define DFM_FP(U)                                                                                                 \
-  hipLaunchKernelGGL(radon_forward_pairs_kernel<U>, dim3(grid), dim3(256), 0, (hipStream_t)stream,               \
-                     (const float4*)img_k, (float4*)sino, pair_ptr, ent_pix, (const float2*)ent_w, ns, n_angles,  \
-                     kstride / 4, k_off / 4, k4n, skstride / 4)
-  if (variant == 1) {
-    DFM_FP(1);
-  } else if (variant == 4) {
-    DFM_FP(4);
-  } else {
-    DFM_FP(8);
-  }
-#undef DFM_FP
-  DFM_LAUNCH_CHECK("dfm_radon_forward_pairs");
   return DFM_OK;
 }
 

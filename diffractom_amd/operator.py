@@ -139,18 +139,6 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
     order = np.lexsort((pix, rows))
     row_ptr = np.zeros(na * ns + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=na * ns), out=row_ptr[1:])
-    # bin-pair table: rows (angle, q) for the bins 2q, 2q + 1; an entry (pixel, weight to 2q, weight to 2q + 1), a
-    # pixel whose two taps fall in one pair listed once
-    npairs = -(-ns // 2)
-    bins = np.concatenate([np.broadcast_to(s_, t.shape)[v_] for s_, v_ in zip(taps, valid)])
-    arow = rows // ns
-    key = (arow * npairs + bins // 2) * npix + pix
-    ukey, inv = np.unique(key, return_inverse=True)
-    pw = np.zeros((len(ukey), 2))
-    np.add.at(pw, (inv, bins % 2), wts)
-    prow = ukey // npix
-    pair_ptr = np.zeros(na * npairs + 1, np.int64)
-    np.cumsum(np.bincount(prow, minlength=na * npairs), out=pair_ptr[1:])
     sc = [np.clip(s_, 0, ns - 1) for s_ in taps]
     wb = [np.where(v_, bwd_scale * k_, 0.0) for k_, v_ in zip(kern, valid)]
     ptab = np.empty((na, npix, 4), np.int32)
@@ -208,9 +196,7 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
             "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p, "pairs_ok": pairs_ok,
             "w": wts[order].astype(np.float32), "dw": dwts[order].astype(np.float32), "ptab": ptab,
-            "tbase": base.astype(np.int32),
-            "pair_ptr": pair_ptr.astype(np.int32), "pair_pix": (ukey % npix).astype(np.int32),
-            "pair_w": pw.astype(np.float32)}
+            "tbase": base.astype(np.int32)}
 
 
 class SinglePhaseForwardOperator:
@@ -442,15 +428,12 @@ class SinglePhaseForwardOperator:
         # the angle indices of every geometry array the projector is called with (the Fourier projector needs them)
         all_a = torch.arange(self.N_Omega, dtype=torch.int32, device=self.device)
         self._geo_angles = {id(self.geo_fwd): (self.geo_fwd, all_a), id(self.geo_bwd): (self.geo_bwd, all_a)}
-        # kernel variants (identical results; set by autotune_projector)
-        # kernel choices (identical results up to float32 rounding): "lds" (the image of a channel group resident in
-        # LDS, when it fits), "table" (geometry tables, 2 taps), "arith" (footprint arithmetic, any taps)
-        self.radon_variant = {"fwd": 8 + 256, "bwd": 4 + 64 * 4, "fwd_kernel": "table", "bwd_kernel": "table",
-                              "fwd_pairs": 8}
+        # kernel choices (identical results up to float32 rounding; set by autotune): forward "table" (geometry
+        # tables, 2 taps), "band", "sweep", "psweep" (line sweeps), "arith" (footprint arithmetic, any taps);
+        # backward "table" (tile and grouped forms by variant number) or "arith"
+        self.radon_variant = {"fwd": 8 + 256, "bwd": 4 + 64 * 4, "fwd_kernel": "table", "bwd_kernel": "table"}
         self.tuning = None
         self.marks = None                       # a list while a profiling pass records per-launch events (mark())
-        self.fuse_projector = False
-        self.fused_channels = 4
         # image bands of the banded forward projection: one XCD's band of 64 channels within about 2 MB of L2
         self.radon_bands = int(min(8, max(1, -(-self.Nx * self.Ny // 8192))))
         if self.radon_bands >= 4:
@@ -758,9 +741,9 @@ class SinglePhaseForwardOperator:
             self._gen_caps = self._gen_offs = self._gen_tags = None
             self._build_pf(*self._pf_build_args)
         if projector:
-            keep = (dict(self.radon_variant), self.tuning, self.fuse_projector, self.fused_channels)
+            keep = (dict(self.radon_variant), self.tuning)
             self._projector_geometry()
-            self.radon_variant, self.tuning, self.fuse_projector, self.fused_channels = keep
+            self.radon_variant, self.tuning = keep
             self._fourier_setup()
         self._scratch = None
         self.omega_subsets = None
@@ -1073,11 +1056,6 @@ class SinglePhaseForwardOperator:
             return
         if mode >= 2 and (tab is None or self.radon_variant["bwd_kernel"] != "table"):
             raise ValueError("the fused EM / PDHG updates need a grouped table backprojection (em_fused())")
-        cw = self.lds_width() if self.radon_variant["bwd_kernel"] == "lds" else 0
-        if cw and koff % cw == 0 and kn % cw == 0:
-            self._lds_projector(mode, sino, geo, n_ang, out_k if mode == 0 else x, z, T, inv_p, lam, None, None, 0,
-                                kstride, koff, kn, cw)
-            return
         if tab is not None and self.radon_variant["bwd_kernel"] != "arith":
             t8 = (self.radon_variant["bwd"] // 1024) % 4 == 1       # 8x8 tiles: their first-bin table
             if t8 and tab["tbase8"].numel() == 1:
@@ -1126,16 +1104,12 @@ class SinglePhaseForwardOperator:
             cands.update({"table:8+lds+ch128": ("table", 8 + 256 + 1536), "table:8+lds": ("table", 8 + 256),
                           "table:4+lds": ("table", 4 + 256), "table:8": ("table", 8),
                           "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "table:8+lds+ch32": ("table", 8 + 256 + 512),
-                          "table:8+lds+ch256": ("table", 8 + 256 + 2048), "pairs:8": ("pairs", 8)})
-            if self.lds_width():
-                cands["lds"] = ("lds", None)
+                          "table:8+lds+ch256": ("table", 8 + 256 + 2048)})
             times = {}
             for name, (kern, v) in cands.items():
                 rv["fwd_kernel"] = kern
                 if kern == "table":
                     rv["fwd"] = v
-                elif kern == "pairs":
-                    rv["fwd_pairs"] = v
                 elif kern == "band":
                     rv["fwd_band"] = v
                 elif kern == "sweep":
@@ -1148,8 +1122,6 @@ class SinglePhaseForwardOperator:
             saved["fwd_kernel"] = kern
             if kern == "table":
                 saved["fwd"] = v
-            elif kern == "pairs":
-                saved["fwd_pairs"] = v
             elif kern == "band":
                 saved["fwd_band"] = v
             elif kern == "sweep":
@@ -1164,8 +1136,6 @@ class SinglePhaseForwardOperator:
             cands.update({f"table:{a}x{g}:ch{64 << w}": ("table", a + 64 * g + 4096 * w)
                           for w in (1, 2) for a, g in ((4, 1), (8, 1), (4, 2))})
             cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})
-            if self.lds_width():
-                cands["lds"] = ("lds", None)
             times = {}
             for name, (kern, v) in cands.items():
                 rv["bwd_kernel"] = kern
@@ -1186,9 +1156,10 @@ class SinglePhaseForwardOperator:
         """Pick each product's fastest kernel variant by timing it on this operator's shapes (HIP events, median of
         reps after 2 warm-up launches), as the reference times its tiled and plain projectors
         (parallel_radon.py:240-281). Every candidate computes the same values (up to float32 summation order).
-        Candidates: forward projection (geometry table with 4 or 8 entries fetched ahead, staged in LDS or not,
-        capped occupancy; bin-pair table; LDS-resident image), backprojection with the SPDHG update (4, 8 or 16
-        angles staged per pass; LDS-resident image), the matrix-core PF products (2 or 4 output tiles per wave).
+        Candidates: forward projection (line sweeps with one or two rays per thread, pixel bands, geometry table
+        with 4 or 8 entries fetched ahead, staged in LDS or not, capped occupancy), backprojection with the SPDHG
+        update (4x4 and 8x8 tiles, 4, 8 or 16 angles staged per pass, 1-8 channel blocks per workgroup), the PF
+        products (block workgroup size and k split, CSR forms).
         Returns {product: (choice, {candidate: ms})} and keeps it in self.tuning."""
         if self.omega_subsets is None:
             self.set_omega_subsets(10)
@@ -1285,35 +1256,6 @@ class SinglePhaseForwardOperator:
             self.blk_variant["adj_wg"] = int(best.split(":")[1])
             out["pf_adjoint"] = (best, times)
 
-    def lds_width(self):
-        """Channels per workgroup of the LDS-resident projector (4, 2, 1; at most fused_channels), or 0 when the
-        image does not fit the LDS or the projector needs more than 2 taps."""
-        if self.n_taps != 2:
-            return 0
-        cw = int(_lib.load().dfm_projector_lds_width(self.Nx, self.Ny, self.My))
-        while cw > self.fused_channels:
-            cw //= 2
-        return cw
-
-    def fused_width(self, na_b=None):
-        """Channels per workgroup of the fused projector step (0: not available or switched off with
-        fuse_projector = False)."""
-        return self.lds_width() if self.fuse_projector and self.fourier is None else 0
-
-    def _lds_projector(self, mode, sino_b, geo_b, na_b, x, z, T, inv_p, lam, sino_f, geo_f, na_f, kstride, koff, kn,
-                       cw):
-        _lib.call("dfm_projector_lds", int(mode), _lib.ptr(sino_b), _lib.ptr(geo_b), int(na_b), _lib.ptr(x),
-                  _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), _lib.ptr(sino_f), _lib.ptr(geo_f), int(na_f),
-                  float(self.radon_scale), self.Nx, self.Ny, self.My, int(kstride), int(koff), int(kn), int(kn),
-                  int(cw), self._stream())
-
-    def _spdhg_fused(self, sino, m, mn, x, z, T, inv_p, lam, koff, kn, cw):
-        """Backprojection of subset m's sinogram with the primal update of (x, z), then the forward projection of
-        subset mn from the updated x into the same sinogram scratch (dfm_projector_lds mode 3)."""
-        na_b, _, _, geo_b = self._angle_set(m)
-        na_f, _, geo_f, _ = self._angle_set(mn)
-        self._lds_projector(3, sino, geo_b, na_b, x, z, T, inv_p, lam, sino, geo_f, na_f, self.K_stride, koff, kn, cw)
-
     def zero_subset_rows(self, data, subset):
         m = self._subset(subset)
         _lib.call("dfm_zero_rows", _lib.ptr(data), _lib.ptr(self._sub_omegas[m]), len(self.omega_subsets[m]),
@@ -1326,7 +1268,6 @@ class SinglePhaseForwardOperator:
         if self.projector == "fft":
             from .fourier import FourierProjector
             self.fourier = FourierProjector(self)
-            self.fuse_projector = False
 
     def _fourier_angles(self, geo):
         entry = self._geo_angles.get(id(geo))
@@ -1339,16 +1280,6 @@ class SinglePhaseForwardOperator:
             q = self.queue if self.queue is not None else torch.cuda.current_stream(self.device)
             with torch.cuda.stream(q):
                 self.fourier.forward(img, img_kstride, koff, kn, sino, self._fourier_angles(geo), self._stream())
-            return
-        if tab is not None and self.radon_variant["fwd_kernel"] == "pairs":
-            _lib.call("dfm_radon_forward_pairs", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["pair_ptr"]),
-                      _lib.ptr(tab["pair_pix"]), _lib.ptr(tab["pair_w"]), self.My, n_angles, img_kstride, koff, kn,
-                      kn, self.radon_variant["fwd_pairs"], self._stream())
-            return
-        cw = self.lds_width() if self.radon_variant["fwd_kernel"] == "lds" else 0
-        if cw and koff % cw == 0 and kn % cw == 0:
-            self._lds_projector(2, None, None, 0, img, None, None, 0.0, 0.0, sino, geo, n_angles, img_kstride, koff,
-                                kn, cw)
             return
         if tab is not None and self.radon_variant["fwd_kernel"] == "psweep" and tab["pairs_ok"]:
             g = tab["sweep_grp_p"]

@@ -108,8 +108,8 @@ class SPDHGPoisson:
         self.support_mask = self._support(support)
         self.state = None
         # HIP graphs of the subset steps: the launches of a step are two graphs, ("update", m) = dual update,
-        # adjoint and primal update of subset m, and ("forward", m') = the forward of the next subset m' (with the
-        # fused projector the pair (m, m') is one graph); each is captured the second time it occurs and replayed
+        # adjoint and primal update of subset m, and ("forward", m') = the forward of the next subset m'; each is
+        # captured the second time it occurs and replayed
         # from then on (every array of the step is allocated before; a reallocated operator scratch buffer drops the
         # captured graphs). Only on the current stream (operator queue None).
         self.graphs = False
@@ -219,25 +219,6 @@ class SPDHGPoisson:
         _lib.call("dfm_tv_value_partial", _lib.ptr(self.xk), _lib.ptr(self.Tk), op.Nx, op.Ny, op.K_stride,
                   _lib.ptr(self._part), ctypes.byref(n), op._stream())
         return float(self._part[: n.value].sum().item())
-
-    def _fused_step(self, m, mn):
-        """The adjoint update of subset m and the forward of subset mn with one projector launch per batch (the PF
-        adjoint, the fused backprojection + primal update + forward projection, the PF forward). False (nothing
-        done) when the fused projector does not apply to this operator."""
-        op = self.op
-        cw = op.fused_width(len(self.subsets[m]))
-        if cw == 0 or any(bt["koff"] % cw or bt["kstride"] % cw for bt in op.batches):
-            return False
-        sc = op._scratch_buffers(max(len(self.subsets[m]), len(self.subsets[mn])))
-        for ib, bt in enumerate(op.batches):
-            op._pf_adjoint(ib, self.D, sc["sino"], m)
-            op.mark("pf_adjoint")
-            op._spdhg_fused(sc["sino"], m, mn, self.xk, self.zk, self.Tk, self._inv_p_data, self.lam, bt["koff"],
-                            bt["kstride"], cw)
-            op.mark("spdhg_fused")
-            op._pf_forward(ib, sc["sino"], self.D, mn, accumulate=ib > 0)
-            op.mark("pf_forward")
-        return True
 
     # ---------------------------------------------------------------- run
     def start(self, x0, b, weights=None, gain=None, background=None, state=None):
@@ -353,11 +334,8 @@ class SPDHGPoisson:
         m_next = next_block if next_block is not None and next_block >= 0 else None
         if self.graphs and self.op.queue is None and not full_forward and m_next is not None and \
                 self.op.marks is None:
-            if blk >= 0 and self.op.fused_width(len(self.subsets[blk])):
-                self._graph_run(("step", blk, m_next), lambda: self._step_launches(blk, m_next, False))
-            else:
-                self._graph_run(("update", blk), lambda: self._step_launches(blk, None, False))
-                self._graph_run(("forward", m_next), lambda: self._forward(m_next))
+            self._graph_run(("update", blk), lambda: self._step_launches(blk, None, False))
+            self._graph_run(("forward", m_next), lambda: self._forward(m_next))
             return
         self._step_launches(blk, m_next, full_forward)
 
@@ -390,8 +368,6 @@ class SPDHGPoisson:
         if blk >= 0:
             self._dual(blk)
             self.op.mark("dual_update")
-            if not full_forward and m_next is not None and self._fused_step(blk, m_next):
-                return
             self._adjoint_update(blk)
         else:
             self._tv_update()

@@ -117,12 +117,6 @@ int dfm_radon_forward_band(const float* img_k, float* sino, float* part, const i
                            const int32_t* ent_pix, const float* ent_w, int32_t ns, int32_t n_angles, int32_t nb,
                            int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t variant,
                            dfm_stream_t stream);
-/* Forward from the bin-pair table (the same projection, radon_kernels.cl:44-101): rows (angle, q) for the bins 2q and
- * 2q + 1, pair_ptr (n_angles * ceil(ns/2) + 1), entries (pixel, weight to 2q, weight to 2q + 1) with ent_w float2; a
- * pixel whose two taps fall in one pair appears once. variant: entries fetched ahead, 1, 4 or 8. */
-int dfm_radon_forward_pairs(const float* img_k, float* sino, const int32_t* pair_ptr, const int32_t* ent_pix,
-                            const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
-                            int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
 int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,
                              int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
                              int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
@@ -434,22 +428,6 @@ int dfm_dual_update_sub(float* y, float* d, const float* sig, const float* b, co
 int dfm_radon_backward_spdhg(const float* sino, const float* geo, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
                              int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t ntap, float* x,
                              float* z, const float* T, float inv_p, float lam, dfm_stream_t stream);
-/* Projections with a channel group's whole image in LDS (radon_kernels.cl:44-141 radon_forward_k /
- * radon_backward_k; spdhg_poisson.py:95-106 spdhg_update; spdhg_poisson.py:430-484 _pass for the fused step). A
- * workgroup holds cw channels (4, 2 or 1) of all nx * ny pixels: nx ny cw <= 40960 and ns cw <= 40960;
- * dfm_projector_lds_width returns the largest cw that fits (0: none). 2 taps only (pixels no larger than a bin).
- * mode 0: x[p, k] = backward(sino_b) over the na_b angles of geo_b (backward records of dfm_radon_backward);
- * mode 1: the SPDHG primal update of (x, z) with step T, as dfm_radon_backward_spdhg;
- * mode 2: sino_f[a, s, k] = scale wscale_a sum_p K(t_a(p) - s) x[p, k] over the na_f angles of geo_f (forward records
- *         of dfm_radon_forward);
- * mode 3: mode 1, then mode 2 on the updated x (one SPDHG step's projector work). sino_f may be sino_b (each
- *         workgroup reads its channels of sino_b before it writes them).
- * x, z, T: (Npix, kstride) channels k_off .. k_off + k_n - 1; sinograms (angles, ns, skstride) channels 0 .. k_n - 1. */
-int dfm_projector_lds_width(int32_t nx, int32_t ny, int32_t ns);
-int dfm_projector_lds(int32_t mode, const float* sino_b, const void* geo_b, int32_t na_b, float* x, float* z,
-                      const float* T, float inv_p, float lam, float* sino_f, const void* geo_f, int32_t na_f,
-                      float scale, int32_t nx, int32_t ny, int32_t ns, int32_t kstride, int32_t k_off, int32_t k_n,
-                      int32_t skstride, int32_t cw, dfm_stream_t stream);
 /* The plain update (spdhg_poisson.py:95-106) on n elements: z += dz; x = T > 0 ? max(0, x - T (z + inv_p dz + lam)) : 0 */
 int dfm_spdhg_update(float* x, float* z, const float* T, const float* dz, float inv_p, float lam, int64_t n,
                      dfm_stream_t stream);

@@ -123,8 +123,7 @@ def test_errors_raise_like_the_reference():
         op.direct_cl(torch.zeros(op.coeff_shape, device="cuda"), torch.zeros(op.data_shape, device="cuda"), subset=0)
 
 
-@pytest.mark.parametrize("kernel,channels", [("lds", 4), ("lds", 2), ("lds", 1), ("table", 4), ("table_staged", 4),
-                                             ("table_capped", 4), ("table_pairs", 4), ("table_band", 4),
+@pytest.mark.parametrize("kernel,channels", [("table", 4), ("table_staged", 4), ("table_capped", 4), ("table_band", 4),
                                              ("table_ch32", 4), ("table_ch16", 4), ("table_ch128", 4),
                                              ("table_ch256", 4), ("table_band128", 4), ("table_band256", 4),
                                              ("table_group", 4), ("table_group16", 4), ("table_tile8", 4),
@@ -134,9 +133,9 @@ def test_errors_raise_like_the_reference():
                                              ("psweep1", 4), ("psweep2", 4), ("psweep4", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
-    """Every projector kernel choice (LDS-resident image at 4, 2, 1 channels per workgroup; geometry tables;
-    footprint arithmetic) against the oracle; "chunked": 120 angles x 103 bins, so the LDS backprojection stages the
-    sinogram in two angle chunks at 4 channels."""
+    """Every projector kernel choice (geometry tables, staged or not, capped; pixel bands; line sweeps with one or two
+    rays per thread; the backprojection's tile, grouped, 8x8 and wide forms; footprint arithmetic) against the
+    oracle; "chunked": 120 angles x 103 bins (several table chunks of angles)."""
     print(f"case kernel={kernel} channels={channels} shape={shape}", flush=True)
     if shape == "square":
         c = cfg(nx=14, ny=12, my=17, n_omega=16, angle_range=(0, 360), cor=0.7, n_eta=24)
@@ -148,8 +147,6 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["fwd"] = 8 + 256
     elif kernel == "table_capped":               # at most 6 workgroups per CU, 4 entries ahead
         op.radon_variant["fwd"] = 4 + 16 * 6
-    elif kernel == "table_pairs":                # the bin-pair table forward
-        op.radon_variant["fwd_kernel"] = "pairs"
     elif kernel.startswith("table_band"):        # forward tiled by 3 pixel bands, band partial sums
         op.radon_bands = 3
         op._tables_full = op._radon_tables(np.arange(op.N_Omega))
@@ -178,9 +175,6 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
     elif kernel.startswith("table_ch"):          # channel blocks of 32, 16, 128, 256 channels
         op.radon_variant["fwd"] = 8 + 256 + 512 * {"table_ch32": 1, "table_ch16": 2, "table_ch128": 3,
                                                     "table_ch256": 4}[kernel]
-    op.fused_channels = channels
-    if kernel == "lds":
-        assert op.lds_width() == channels
     x, y = _inputs(op, 5)
     assert rel_max(op.direct(x), orc.direct(x)) < 2e-5
     assert rel_max(op.adjoint(y), orc.adjoint(y)) < 2e-5

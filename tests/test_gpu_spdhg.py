@@ -67,29 +67,24 @@ def test_spdhg_resume_and_torch_x0():
     assert three.state["epoch"] == 7
 
 
-@pytest.mark.parametrize("channels,batches", [(4, 1), (2, 1), (1, 1), (4, 3)])
-def test_fused_projector_step_matches_separate_kernels(channels, batches):
-    """The fused projector step (backprojection + primal update + next forward projection in one launch, at 4, 2
-    or 1 channels per workgroup, one or several orientation batches) gives the iterates of the separate table
-    kernels: 6 epochs, relative L2 within 1e-5; both paths are checked against the FP64 oracle above."""
+def test_spdhg_several_batches_matches_oracle():
+    """Three orientation batches (the backprojection-update and forward of each batch in turn): iterates within 1e-4
+    (relative L2) of the FP64 oracle after 6 epochs, objectives within 1e-5."""
     from diffractom_amd import SPDHGPoisson
-    print(f"case channels={channels} batches={batches}", flush=True)
-    op, _, b = _problem(K=24)
-    if batches > 1:
-        op2, _, _ = build(cfg(nx=16, ny=16, my=17, n_omega=24, angle_range=(0, 180), cor=0.0), K=24, sigma_deg=4.0,
-                          pf_mode="sparse", max_gb=8.5 * 24 * 36 * 3 * 4 / 1024 ** 3, oracle=False)
-        assert len(op2.batches) == batches
-        op = op2
-    xs = []
-    for fuse in (True, False):
-        op.fuse_projector = fuse
-        op.fused_channels = channels
-        sp = SPDHGPoisson(op, subsets=4, seed=3)
-        xs.append(sp.run(np.zeros(op.coeff_shape, np.float32), b, 6, diagnostics_interval=3))
-        if fuse:
-            assert op.fused_width(6) == channels
-    op.fuse_projector = True
-    assert rel_l2(xs[0], xs[1]) < 1e-5
+    op, orc, _ = build(cfg(nx=16, ny=16, my=17, n_omega=24, angle_range=(0, 180), cor=0.0), K=24, sigma_deg=4.0,
+                       pf_mode="sparse", max_gb=8.5 * 24 * 36 * 3 * 4 / 1024 ** 3)
+    assert len(op.batches) == 3
+    rng = np.random.default_rng(7)
+    xt = rng.random(op.coeff_shape) * (rng.random(op.coeff_shape) < 0.2) * op.support_mask()[None]
+    ax = orc.direct(xt)
+    b = rng.poisson(ax / ax.mean() * 50).astype(np.float32)
+    sp = SPDHGPoisson(op, subsets=4, seed=3)
+    x = sp.run(np.zeros(op.coeff_shape, np.float32), b, 6, diagnostics_interval=3)
+    ref = OracleSPDHG(orc, subsets=4, seed=3)
+    x_ref = ref.run(np.zeros(op.coeff_shape), b, 6, diagnostics_interval=3)
+    assert rel_l2(x, x_ref) < 1e-4
+    np.testing.assert_allclose([s["f"] for s in sp.iter_stats if s["f"] is not None],
+                               [s["f"] for s in ref.iter_stats if s["f"] is not None], rtol=1e-5)
 
 
 @pytest.mark.parametrize("tv,tv_prob", [(0.5, None), (2.0, 0.4)])

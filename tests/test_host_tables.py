@@ -54,30 +54,6 @@ def test_tables_equal_projection_matrices(kernel, shape):
         assert np.all((pt[:, :2] - base8[:, None] >= 0) & (pt[:, :2] - base8[:, None] < 16))
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 17, (0, 180)), (14, 12, 16, (0, 360)), (99, 99, 103, (0, 180))])
-def test_pair_table_equals_projection_matrices(shape):
-    """The bin-pair table (rows (angle, q) for the bins 2q, 2q + 1; an entry per pixel with its weights to both) holds
-    exactly the projection matrix of the oracle, each pixel at most once per pair, and it lists fewer entries than
-    the per-bin table (about 1.5 instead of 2 per pixel and angle)."""
-    nx, ny, my, ar = shape
-    op = O.OracleOperator(cfg(nx=nx, ny=ny, my=my, n_omega=30, angle_range=ar, cor=0.7), (4.05,) * 3 + (90,) * 3,
-                          "cubic", AL3, 0.35, orientations(2, 1), np.deg2rad(4.0), orthonormalize=False)
-    idx = np.array([0, 5, 13, 21, 29])
-    tab = radon_tables(_geo(op, "trapezoid"), idx, nx, ny, my, op.scale)
-    npairs = -(-my // 2)
-    for i, r in enumerate(idx):
-        P = op.radon_matrix(r).toarray()
-        F = np.zeros((2 * npairs, nx * ny))
-        for q in range(npairs):
-            lo, hi = tab["pair_ptr"][i * npairs + q], tab["pair_ptr"][i * npairs + q + 1]
-            pix = tab["pair_pix"][lo:hi]
-            assert len(np.unique(pix)) == len(pix)
-            F[2 * q, pix] += tab["pair_w"][lo:hi, 0]
-            F[2 * q + 1, pix] += tab["pair_w"][lo:hi, 1]
-        assert not F[my:].any()
-        np.testing.assert_allclose(F[:my], P, rtol=2e-6, atol=1e-7 * P.max())
-    assert len(tab["pair_pix"]) < 0.8 * len(tab["pix"])
-
 @pytest.mark.parametrize("n_bands", [1, 3, 5])
 def test_band_pointers_split_each_ray_by_image_rows(n_bands):
     """band_ptr cuts every ray's entries into n_bands contiguous ranges, range b holding exactly the entries whose
