@@ -25,7 +25,8 @@ SIGNATURES = {
     "dfm_radon_backward": [P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P],
     "dfm_radon_backward_spdhg": [P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, P, P, P, F32, F32, P],
     "dfm_radon_forward_table": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, P],
-    "dfm_radon_backward_table": [P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P, P, P, F32, F32, I32, I32, P],
+    "dfm_radon_backward_table": [P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, P, P, P, F32, F32, I32, I32,
+                                 P],
     "dfm_pf_eval_dense": [P, P, P, P, P, P, P, P, I32, I32, I32, I32, I32, I32, I32, I32, I32, F32, I32, P],
     "dfm_pf_count_fwd": [P, P, I32, I32, I32, I32, P],
     "dfm_pf_count_adj": [P, P, I32, I32, I32, I32, P],

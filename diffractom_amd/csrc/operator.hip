@@ -464,7 +464,7 @@ constexpr int TCH = 32;   // angles whose table rows are staged at once
 
 template <int MODE, int ACH, int G, int TS, int QW>
 __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_group_kernel(
-    const float4* __restrict__ sino, float4* __restrict__ out, const int4* __restrict__ ptab,
+    const float4* __restrict__ sino, float4* __restrict__ out, const double* __restrict__ bgeo,
     const int* __restrict__ tbase, int nx, int ny, int ns, int na, int kstride4, int k4_off, int k4_n, int skstride4,
     float4* __restrict__ x, float4* __restrict__ z, const float4* __restrict__ T, float inv_p, float lam) {
   constexpr int NPX = TS * TS, NTH = NPX * QW;    // QW float4 quads of every pixel per channel block
@@ -487,22 +487,37 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
   const int li = threadIdx.x / QW;                       // pixel of the tile, row-major TS x TS
   const int px = (tile % tnx) * TS + li % TS, py = (tile / tnx) * TS + li / TS;
   const bool inside = px < nx && py < ny;
-  const int64_t pix = (int64_t)py * nx + px;
-  const int64_t npix = (int64_t)nx * ny;
+  const int pix = py * nx + px;                          // < 2^31 pixels (a 32-bit value: one VGPR less live)
   const bool tables_once = na <= TCH;
-  // stage the table rows of angles t0 .. t0 + tn - 1: bins as byte offsets from the tile's first bin
+  // stage the table rows of angles t0 .. t0 + tn - 1, computed from the angles' float64 geometry (bgeo: X, Y, T0,
+  // hw, top, w / A per angle) by the rule of the host's pixel table (radon_tables: taps floor(t - hw) + 1 and the
+  // next bin, weights (w / A) clamp(hw - |t - s|, 0, top) where the bin is on the detector, bins clamped to it), so
+  // no table is read from memory; the bins become byte offsets from the tile's first bin, clamped into the staged
+  // range (a bin outside it has weight 0)
   auto stage_tables = [&](int t0, int tn) {
     if (threadIdx.x < tn) lbase[threadIdx.x] = tbase[(int64_t)(t0 + threadIdx.x) * ntiles + tile];
-    for (int i = threadIdx.x; i < tn * NPX; i += NTH) {
+    // (the loop's first index passes an empty asm: its geometry address is formed here, not hoisted out of the
+    // channel-block and angle loops to stay live, and spilled, across the whole kernel)
+    int i0 = threadIdx.x;
+    asm volatile("" : "+v"(i0));
+    for (int i = i0; i < tn * NPX; i += NTH) {
       const int al = i / NPX, l = i % NPX;
       const int ppx = (tile % tnx) * TS + l % TS, ppy = (tile / tnx) * TS + l / TS;
       int ofs = 0;
       float2 ww = make_float2(0.f, 0.f);
       if (ppx < nx && ppy < ny) {
-        const int4 e = ptab[(int64_t)(t0 + al) * npix + (int64_t)ppy * nx + ppx];
+        const double* g = bgeo + (int64_t)(t0 + al) * 6;
+        const double t = g[0] * (double)ppx + g[1] * (double)ppy + g[2];
+        const double hw = g[3], tp = g[4], sc = g[5];
+        const int s0 = (int)floor(t - hw) + 1, s1 = s0 + 1;
+        const double k0 = fmin(fmax(hw - fabs(t - (double)s0), 0.0), tp);
+        const double k1 = fmin(fmax(hw - fabs(t - (double)s1), 0.0), tp);
+        ww.x = (s0 >= 0 && s0 < ns && k0 > 0.0) ? (float)(sc * k0) : 0.0f;
+        ww.y = (s1 >= 0 && s1 < ns && k1 > 0.0) ? (float)(sc * k1) : 0.0f;
         const int b0 = tbase[(int64_t)(t0 + al) * ntiles + tile];
-        ofs = ((e.x - b0) * QW * (int)sizeof(float4)) | (((e.y - b0) * QW * (int)sizeof(float4)) << 16);
-        ww = make_float2(__int_as_float(e.z), __int_as_float(e.w));
+        const int o0 = min(max(min(max(s0, 0), ns - 1) - b0, 0), TBB - 1);
+        const int o1 = min(max(min(max(s1, 0), ns - 1) - b0, 0), TBB - 1);
+        ofs = (o0 * QW * (int)sizeof(float4)) | ((o1 * QW * (int)sizeof(float4)) << 16);
       }
       lofs[i] = ofs;
       lw[i] = ww;
@@ -514,7 +529,6 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
     if (cb >= n_cb) break;                                // uniform over the workgroup
     const int kq = cb * QW + q;
     const bool live = inside && kq < k4_n;
-    const int64_t o = pix * kstride4 + kq + k4_off;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int t0 = 0; t0 < na; t0 += TCH) {
       const int tn = min(TCH, na - t0);
@@ -580,6 +594,11 @@ __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_
       }
     }
     if (live) {
+      // the element offset is formed here from the 32-bit pixel index: the empty asm keeps the compiler from hoisting
+      // the 64-bit product out of the loops, where it would stay live across the accumulation (2 VGPRs of the 64)
+      int pe = pix;
+      asm volatile("" : "+v"(pe));
+      const int64_t o = (int64_t)pe * kstride4 + kq + k4_off;
       // x, z, T (MODE 2: the EM sensitivity; 3: the PDHG step) are read after the accumulation: held across it they
       // would cost 12 VGPRs of the 64 (8 waves per SIMD), and the other waves hide the latency
       float4 zz = make_float4(0.f, 0.f, 0.f, 0.f), xx = zz, tt = zz;
@@ -1011,11 +1030,12 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
   return DFM_OK;
 }
 
-int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,
-                             int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
-                             int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
-                             int32_t mode, int32_t variant, dfm_stream_t stream) {
-  DFM_CHECK_ARG(sino && ptab && tbase &&
+int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const double* bgeo,
+                             const int32_t* tbase, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
+                             int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, float* x, float* z,
+                             const float* T, float inv_p, float lam, int32_t mode, int32_t variant,
+                             dfm_stream_t stream) {
+  DFM_CHECK_ARG(sino && ptab && bgeo && tbase &&
                     (mode == 0 ? img_k != nullptr : mode == 2 ? (x && T) : (x && z && T)),
                 "dfm_radon_backward_table: null pointer");
   DFM_CHECK_ARG(nx > 0 && ny > 0 && ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 &&
@@ -1045,7 +1065,7 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
     const unsigned ggrid = xcd_grid(ntiles, n_grp);
 #define DFM_BW(M, A, G, W)                                                                                        \
   hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4, W>), dim3(ggrid), dim3(16 * W), 0,                  \
-                     (hipStream_t)stream, (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny,  \
+                     (hipStream_t)stream, (const float4*)sino, (float4*)img_k, bgeo, tbase, nx, ny,  \
                      ns, n_angles, kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z,             \
                      (const float4*)T, inv_p, lam)
 #define DFM_BWG(M, A, W)     \
@@ -1082,7 +1102,7 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
 #define DFM_B8(M, A, G)                                                                                           \
   hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 8, QB>), dim3(ggrid), dim3(1024), 0,                   \
                      (hipStream_t)stream,                                                                         \
-                     (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
+                     (const float4*)sino, (float4*)img_k, bgeo, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
 #define DFM_B8G(M, A)        \
   if (grp == 1) {            \
@@ -1112,7 +1132,7 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
 #define DFM_BG(M, A, G)                                                                                           \
   hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4, QB>), dim3(ggrid), dim3(256), 0,                    \
                      (hipStream_t)stream,                                                                         \
-                     (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
+                     (const float4*)sino, (float4*)img_k, bgeo, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
 #define DFM_BGA(M, G)            \
   if (ach == 4) {                \

@@ -193,6 +193,10 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
             "tbase8": tbase8,
             "sweep_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
                                    radon_scale * g["inv_a"][idx]], 1).astype(np.float64),
+            # the backprojection's per-angle geometry (X, Y, T0, hw, top, w / A): the grouped kernels derive the
+            # pixel table's taps and weights from it on the device, by the rule of ptab above
+            "bp_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
+                                g["w"][idx] * g["inv_a"][idx]], 1).astype(np.float64),
             "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
             "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p, "pairs_ok": pairs_ok,
             "w": wts[order].astype(np.float32), "dw": dwts[order].astype(np.float32), "ptab": ptab,
@@ -1061,7 +1065,7 @@ class SinglePhaseForwardOperator:
             if t8 and tab["tbase8"].numel() == 1:
                 raise ValueError("8x8 backprojection tiles reach more than 16 bins for this geometry")
             _lib.call("dfm_radon_backward_table", _lib.ptr(sino), _lib.ptr(out_k), _lib.ptr(tab["ptab"]),
-                      _lib.ptr(tab["tbase8" if t8 else "tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff,
+                      _lib.ptr(tab["bp_geo"]), _lib.ptr(tab["tbase8" if t8 else "tbase"]), self.Nx, self.Ny, self.My, n_ang, kstride, koff,
                       kn, kn, _lib.ptr(x),
                       _lib.ptr(z), _lib.ptr(T), float(inv_p), float(lam), mode, self.radon_variant["bwd"], st)
         elif mode == 0:
@@ -1132,7 +1136,7 @@ class SinglePhaseForwardOperator:
             cands = {f"table:{a}x{g}": ("table", a + 64 * g) for a, g in ((4, 4), (4, 2), (8, 4), (8, 2), (8, 8))}
             if tab["tbase8"].numel() > 1:
                 cands.update({f"table8:{a}x{g}": ("table", a + 64 * g + 1024)
-                              for a, g in ((4, 1), (4, 2), (8, 1), (8, 2), (4, 4))})
+                              for a, g in ((4, 1), (4, 2), (8, 1), (8, 2), (4, 4), (8, 4))})
             cands.update({f"table:{a}x{g}:ch{64 << w}": ("table", a + 64 * g + 4096 * w)
                           for w in (1, 2) for a, g in ((4, 1), (8, 1), (4, 2))})
             cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})

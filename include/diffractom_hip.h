@@ -69,7 +69,9 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  * forward table: CSR over (angle, bin) rows, row_ptr (n_angles*ns + 1), entries (pixel, weight) with weight =
  *   scale * wscale_a * K(t - s);  sino[a, s, k] = sum_e weight_e img[pixel_e, k]
  * pixel table ptab (n_angles, Npix, 4) int32: bin s0, bin s1 (>= 0, < ns), float bits of w0, w1 (the backprojection
- *   weights, wscale_a folded in; 0 for taps off the detector);
+ *   weights, wscale_a folded in; 0 for taps off the detector): read by the plain tile variants; the grouped variants
+ *   derive the same entries on the device from bgeo (n_angles, 6) float64: X, Y, T0, hw, top, wscale_a / A
+ *   (s0 = floor(t - hw) + 1, s1 = s0 + 1, w = wscale_a / A clamp(hw - |t - s|, 0, top));
  * tile bases tbase (n_angles, ceil(nx/4) * ceil(ny/4)): the first of the 8 bins every 4 x 4 pixel tile reaches.
  * mode 0: img_k = backward(sino); mode 1: the SPDHG primal update of x, z with step T, as dfm_radon_backward_spdhg;
  * mode 2 (grouped variants: + 64 G, + 1024, + 4096 w): the EM update x = T > 0 ? x backward(sino) / (T + lam) : x with
@@ -117,10 +119,11 @@ int dfm_radon_forward_band(const float* img_k, float* sino, float* part, const i
                            const int32_t* ent_pix, const float* ent_w, int32_t ns, int32_t n_angles, int32_t nb,
                            int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t variant,
                            dfm_stream_t stream);
-int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const int32_t* tbase, int32_t nx,
-                             int32_t ny, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off, int32_t k_n,
-                             int32_t skstride, float* x, float* z, const float* T, float inv_p, float lam,
-                             int32_t mode, int32_t variant, dfm_stream_t stream);
+int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* ptab, const double* bgeo,
+                             const int32_t* tbase, int32_t nx, int32_t ny, int32_t ns, int32_t n_angles,
+                             int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, float* x, float* z,
+                             const float* T, float inv_p, float lam, int32_t mode, int32_t variant,
+                             dfm_stream_t stream);
 
 /* ---------------------------------------------------------------- pole-figure matrix
  * Dense PF batch (create_pfo_matrix.py:108-199 pfmatrix_eval_poles): out (R, kmax, C, P) for orientations
