@@ -43,7 +43,8 @@ def parse():
                     "batch: slices/s and the step-time percentiles), ti_c2 (hexagonal, 256^2 px, K 59,440; PF per "
                     "--c2-precision), "
                     "al_c4 (autograd loss + gradients, 8 slices), al_local (SPDHG on a local basis of the al_c1 "
-                    "scan against the global operator on the same orientations)")
+                    "scan against the global operator on the same orientations), al_extras (parameter gradients, "
+                    "Fourier-slice projector and host streaming on the al_c1 workload)")
     ap.add_argument("--slices", type=int, default=None, help="al_c3: number of slices (default 64)")
     ap.add_argument("--no-graphs", action="store_true", help="al_c3: eager launches instead of HIP graph replay")
     ap.add_argument("--compare-steps", type=int, default=300, help="the precisions' SPDHG iterates are compared "
@@ -525,6 +526,91 @@ def run_local(args):
     print(json.dumps(line), flush=True)
 
 
+def run_extras(args):
+    """The widened paths on the al_c1 workload (K = 29713, 99 x 99 pixels, 180 angles; the al_c1 truth and counts),
+    one JSON line of times (HIP events, the median of 3 calls after one warm-up call):
+      parameter gradients: PFGradient (rotation, width, eta offset, axis tilt, Bragg angles; with the Gauss-Newton
+        blocks) and ProjGradient (per-angle shifts, centre of rotation) of <r, A c>, r = 1 - b / (A c + 1e-3);
+      Fourier-slice projector (projector="fft"): full forward and adjoint against the native projector's, and the
+        relative difference of the two forwards (the Fourier model differs from the native footprint by design);
+      host streaming (HostStreamer): forward and adjoint of (K, Ny, Nx) host arrays against the device products."""
+    import torch
+    from diffractom_amd import SinglePhaseForwardOperator
+    from diffractom_amd.gradients import PFGradient, ProjGradient
+    from diffractom_amd.streaming import HostStreamer
+    from diffractom_amd.workloads import make_grid
+    q = args.precision
+    op, b, xt, p = build_problem("al_c1", precision=q)
+    dev = torch.device("cuda")
+    c = torch.from_numpy(xt).to(dev)
+    bd = torch.from_numpy(b).to(dev)
+    ax = op.direct(c)
+    r = (1.0 - bd / (ax + 1e-3)).contiguous()
+
+    def med(fn, n=3, label=""):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        log(f"[bench] extras: {label} warm-up call {time.perf_counter() - t0:.2f} s")
+        ts = []
+        for _ in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    out = {}
+    pg, pgh, jg = PFGradient(op), PFGradient(op, hessian=True), ProjGradient(op)
+    out["pf_gradient_ms"] = round(med(lambda: pg(c, r), label="pf_gradient"), 3)
+    out["pf_gradient_gauss_newton_ms"] = round(med(lambda: pgh(c, r), label="pf_gradient_gauss_newton"), 3)
+    out["proj_gradient_ms"] = round(med(lambda: jg(c, r), label="proj_gradient"), 3)
+    log(f"[bench] extras: PFGradient {out['pf_gradient_ms']} ms, with Gauss-Newton blocks "
+        f"{out['pf_gradient_gauss_newton_ms']} ms, ProjGradient {out['proj_gradient_ms']} ms")
+    data = torch.empty(op.data_shape, dtype=torch.float32, device=dev)
+    img = torch.empty(op.coeff_shape, dtype=torch.float32, device=dev)
+    out["native_forward_ms"] = round(med(lambda: op.direct_cl(c, data), label="native_forward"), 3)
+    out["native_adjoint_ms"] = round(med(lambda: op.adjoint_cl(bd, img), label="native_adjoint"), 3)
+    # host streaming of (K, Ny, Nx) host arrays
+    hs = HostStreamer(op)
+    xh = np.ascontiguousarray(xt, dtype=np.float32)
+    oh = np.empty_like(xh)
+    gb = xh.nbytes / 1e9
+    out["stream_forward_ms"] = round(med(lambda: hs.forward(xh, data), label="stream_forward"), 3)
+    out["stream_adjoint_ms"] = round(med(lambda: hs.adjoint(bd, oh), label="stream_adjoint"), 3)
+    out["stream_host_GBps"] = {"forward": round(gb / out["stream_forward_ms"] * 1e3, 1),
+                               "adjoint": round(gb / out["stream_adjoint_ms"] * 1e3, 1)}
+    hs.release()
+    log(f"[bench] extras: native forward {out['native_forward_ms']} / adjoint {out['native_adjoint_ms']} ms; "
+        f"host-streamed {out['stream_forward_ms']} / {out['stream_adjoint_ms']} ms ({gb:.2f} GB host arrays)")
+    d_native = data.clone()
+    del op, pg, pgh, jg
+    torch.cuda.empty_cache()
+    # the Fourier-slice projector on the same geometry and PF
+    grid, mat, cfg = make_grid("al_c1")
+    t0 = time.perf_counter()
+    fop = SinglePhaseForwardOperator(cfg, mat, grid, max_gb=1024.0, normalized=True, pf_mode="sparse",
+                                     pf_precision=q, projector="fft")
+    out["fft_setup_s"] = round(time.perf_counter() - t0, 1)
+    out["fft_forward_ms"] = round(med(lambda: fop.direct_cl(c, data), label="fft_forward"), 3)
+    out["fft_adjoint_ms"] = round(med(lambda: fop.adjoint_cl(bd, img), label="fft_adjoint"), 3)
+    fop.direct_cl(c, data)
+    out["fft_vs_native_forward_rel_l2"] = float(torch.linalg.vector_norm((data - d_native).double()) /
+                                                torch.linalg.vector_norm(d_native.double()))
+    log(f"[bench] extras: fft forward {out['fft_forward_ms']} / adjoint {out['fft_adjoint_ms']} ms, "
+        f"vs native forward {out['fft_vs_native_forward_rel_l2']:.2e}")
+    line = {"metric": "widened paths on al_c1: parameter gradients, Fourier-slice projector, host streaming",
+            "value": out["pf_gradient_ms"], "unit": "ms per PFGradient call", "n_gpus": 1, "steps": 3, "warmup": 1,
+            "ms_per_step": out["pf_gradient_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": DTYPE[q], "data": "synthetic (al_c1 Voronoi grains, Poisson counts at a mean of 1000)",
+            "config": {"workload": f"al_c1: K={xt.shape[0]}, 99x99 px, 180 omega x 103 x 180 eta x 5 rings; "
+                                   f"{PF_DESC[q]}", "precision": q},
+            **out}
+    print(json.dumps(line), flush=True)
+
+
 def _gather(c, k, iy, ix, pi, n):
     """(n,) flat local coefficients of a (K, Ny, Nx) array at the pairs (zero elsewhere)."""
     out = np.zeros(n, np.float32)
@@ -832,8 +918,9 @@ def main():
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     import torch
-    if args.config in ("al_c3", "ti_c2", "al_c4", "al_local"):
-        {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4, "al_local": run_local}[args.config](args)
+    if args.config in ("al_c3", "ti_c2", "al_c4", "al_local", "al_extras"):
+        {"al_c3": run_c3, "ti_c2": run_c2, "al_c4": run_c4, "al_local": run_local,
+         "al_extras": run_extras}[args.config](args)
         return
     if args.solver in ("osem", "pdhg", "fista"):
         {"osem": run_osem, "pdhg": run_pdhg, "fista": run_fista}[args.solver](args)
