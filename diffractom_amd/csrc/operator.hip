@@ -432,10 +432,10 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
         const float w0 = __int_as_float(e.z), w1 = __int_as_float(e.w);
         const float4 s0 = lds[(al * TB + (e.x - base)) * QB + q];
         const float4 s1 = lds[(al * TB + (e.y - base)) * QB + q];
-        acc.x += w0 * s0.x + w1 * s1.x;
-        acc.y += w0 * s0.y + w1 * s1.y;
-        acc.z += w0 * s0.z + w1 * s1.z;
-        acc.w += w0 * s0.w + w1 * s1.w;
+        acc.x = fmaf(w1, s1.x, fmaf(w0, s0.x, acc.x));
+        acc.y = fmaf(w1, s1.y, fmaf(w0, s0.y, acc.y));
+        acc.z = fmaf(w1, s1.z, fmaf(w0, s0.z, acc.z));
+        acc.w = fmaf(w1, s1.w, fmaf(w0, s0.w, acc.w));
       }
     }
   }

@@ -152,8 +152,12 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
 // bounds checks. Staging addresses are computed once per thread (the block's line offset is added per block).
 constexpr int PR_TH = 512;
 
-template <int RB, int NV>
-__global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
+// PIPE: the next block's image loads are issued before the current block's sums and held in registers across them
+// (about 155 VGPRs: one 512-thread workgroup per CU); otherwise each block's loads are issued, waited for and stored
+// at its start, their addresses formed there from the thread index (at most 128 VGPRs: two workgroups per CU, the
+// other one's sums hiding the loads)
+template <int RB, int NV, bool PIPE>
+__global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep_kernel(
     const float4* __restrict__ img, float4* __restrict__ sino, const SweepGeo* __restrict__ geo,
     const int* __restrict__ grp_ang, const int* __restrict__ grp_cls, int n_grp, int na_grp, int ns, int nx, int ny,
     int kstride4, int k4_off, int k4_n, int skstride4) {
@@ -185,16 +189,19 @@ __global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
   int64_t gof_u[NV];
   bool ok_u[NV];
   const int64_t line_step = cls ? (int64_t)kstride4 : (int64_t)nx * kstride4;
+  auto slots = [&](int tt) {
 #pragma unroll
-  for (int u = 0; u < NV; ++u) {
-    const int i = t + PR_TH * u;
-    const int q = i & (CQ - 1), pp = i / CQ;
-    const int l = pp / len, pos = pp - l * len;
-    ok_u[u] = l < RB && q0 + q < k4_n;
-    lds_u[u] = l * row + (pos + 2) * CQ + q;
-    const int64_t pix = cls ? (int64_t)pos * nx + l : (int64_t)l * nx + pos;
-    gof_u[u] = pix * kstride4 + k4_off + q0 + q;
-  }
+    for (int u = 0; u < NV; ++u) {
+      const int i = tt + PR_TH * u;
+      const int q = i & (CQ - 1), pp = i / CQ;
+      const int l = pp / len, pos = pp - l * len;
+      ok_u[u] = l < RB && q0 + q < k4_n;
+      lds_u[u] = l * row + (pos + 2) * CQ + q;
+      const int64_t pix = cls ? (int64_t)pos * nx + l : (int64_t)l * nx + pos;
+      gof_u[u] = pix * kstride4 + k4_off + q0 + q;
+    }
+  };
+  if (PIPE) slots(t);
   float4 v[NV];
   auto stage_load = [&](int L0) {
 #pragma unroll
@@ -238,13 +245,20 @@ __global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
 #pragma unroll
   for (int qq = 0; qq < CQ; ++qq) accL[qq] = accR[qq] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-  stage_load(0);
+  if (PIPE) stage_load(0);
   for (int L0 = 0, bi = 0; L0 < nlines; L0 += RB, ++bi) {
     float4* lb = limg + (bi & 1) * buf;
     // buffer bi & 1 was last read for block bi - 2, which every wave finished before the barrier of block bi - 1
+    if (!PIPE) {
+      // the slots from a thread index passed through an empty asm: formed here, not hoisted to stay live
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      slots(tt);
+      stage_load(L0);
+    }
     stage_store(lb);
     __syncthreads();
-    if (L0 + RB < nlines) stage_load(L0 + RB);
+    if (PIPE && L0 + RB < nlines) stage_load(L0 + RB);
     if (live) {
 #pragma unroll
       for (int l = 0; l < RB; ++l) {
@@ -265,14 +279,14 @@ __global__ void __launch_bounds__(PR_TH) radon_forward_pairs_sweep_kernel(
         for (int qq = 0; qq < CQ; ++qq) {
           const float4* px = base + ((qq + rot) & (CQ - 1));
           const float4 x0 = px[0], x1 = px[CQ], x2 = px[2 * CQ], x3 = px[3 * CQ];
-          accL[qq].x += wl0 * x0.x + wl1 * x1.x + wl2 * x2.x;
-          accL[qq].y += wl0 * x0.y + wl1 * x1.y + wl2 * x2.y;
-          accL[qq].z += wl0 * x0.z + wl1 * x1.z + wl2 * x2.z;
-          accL[qq].w += wl0 * x0.w + wl1 * x1.w + wl2 * x2.w;
-          accR[qq].x += wr1 * x1.x + wr2 * x2.x + wr3 * x3.x;
-          accR[qq].y += wr1 * x1.y + wr2 * x2.y + wr3 * x3.y;
-          accR[qq].z += wr1 * x1.z + wr2 * x2.z + wr3 * x3.z;
-          accR[qq].w += wr1 * x1.w + wr2 * x2.w + wr3 * x3.w;
+          // FMA chains into the accumulators (acc += w0 x0 + w1 x1 + w2 x2 as written would cost a multiply and an
+          // add besides the two FMAs: 4 VALU operations per 3 taps instead of 3)
+#define DFM_PS3(acc, c, wa, xa, wb, xb, wc, xc) acc.c = fmaf(wc, xc.c, fmaf(wb, xb.c, fmaf(wa, xa.c, acc.c)));
+          DFM_PS3(accL[qq], x, wl0, x0, wl1, x1, wl2, x2) DFM_PS3(accL[qq], y, wl0, x0, wl1, x1, wl2, x2)
+          DFM_PS3(accL[qq], z, wl0, x0, wl1, x1, wl2, x2) DFM_PS3(accL[qq], w, wl0, x0, wl1, x1, wl2, x2)
+          DFM_PS3(accR[qq], x, wr1, x1, wr2, x2, wr3, x3) DFM_PS3(accR[qq], y, wr1, x1, wr2, x2, wr3, x3)
+          DFM_PS3(accR[qq], z, wr1, x1, wr2, x2, wr3, x3) DFM_PS3(accR[qq], w, wr1, x1, wr2, x2, wr3, x3)
+#undef DFM_PS3
         }
       }
     }
@@ -323,12 +337,18 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
                     k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride &&
                     skstride % 4 == 0 && skstride >= k_n && na_grp * ((ns + 1) / 2) <= PR_TH,
                 "dfm_radon_forward_pairs_sweep: bad sizes (na_grp * ceil(ns / 2) <= 512, image <= 256 x 256)");
-  DFM_CHECK_ARG(lines == 1 || lines == 2 || lines == 4, "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4",
+  // lines: lines per staged block (1, 2, 4), + 8 for the form that loads each block at its start (no loads held
+  // across the sums: two workgroups per CU)
+  const int rb = lines & 7;
+  const bool pipe = (lines & 8) == 0;
+  DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4) && (lines & ~15) == 0 &&
+                    (pipe || rb == 1 || (rb == 2 && len <= 128)),
+                "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4, or 1 + 8 or (lines <= 128 pixels) 2 + 8",
                 lines);
   // staged lines, or the store transpose (64 lanes x 8 quads per wave), whichever is larger
-  size_t lds = (size_t)2 * lines * (len + 4) * 8 * sizeof(float4);
+  size_t lds = (size_t)2 * rb * (len + 4) * 8 * sizeof(float4);
   if (lds < (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4)) lds = (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4);
-  DFM_CHECK_ARG(lds <= 163840, "dfm_radon_forward_pairs_sweep: %d lines of %d pixels exceed the LDS", lines, len);
+  DFM_CHECK_ARG(lds <= 163840, "dfm_radon_forward_pairs_sweep: %d lines of %d pixels exceed the LDS", rb, len);
   if (n_grp == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int n_cb = (k4n + 7) / 8;
@@ -336,17 +356,24 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
   const hipStream_t st = (hipStream_t)stream;
   // staging elements per thread: lines * len * 8 float4 over 512 threads (len <= 128 or <= 256)
   const bool small = len <= 128;
-#define DFM_PS(RB, NV)                                                                                           \
-  hipLaunchKernelGGL((radon_forward_pairs_sweep_kernel<RB, NV>), dim3(grid), dim3(PR_TH), lds, st,                 \
+#define DFM_PS(RB, NV)              \
+  if (pipe) {                       \
+    DFM_PS1(RB, NV, true);          \
+  } else {                          \
+    DFM_PS1(RB, NV, false);         \
+  }
+#define DFM_PS1(RB, NV, P)                                                                                       \
+  hipLaunchKernelGGL((radon_forward_pairs_sweep_kernel<RB, NV, P>), dim3(grid), dim3(PR_TH), lds, st,              \
                      (const float4*)img_k, (float4*)sino, (const SweepGeo*)geo, grp_ang, grp_cls, n_grp, na_grp, ns, \
                      nx, ny, kstride / 4, k_off / 4, k4n, skstride / 4)
-  if (lines == 1) {
-    if (small) DFM_PS(1, 2); else DFM_PS(1, 4);
-  } else if (lines == 2) {
-    if (small) DFM_PS(2, 4); else DFM_PS(2, 8);
+  if (rb == 1) {
+    if (small) { DFM_PS(1, 2) } else { DFM_PS(1, 4) }
+  } else if (rb == 2) {
+    if (small) { DFM_PS(2, 4) } else { DFM_PS1(2, 8, true); }
   } else {
-    if (small) DFM_PS(4, 8); else DFM_PS(4, 16);
+    if (small) { DFM_PS1(4, 8, true); } else { DFM_PS1(4, 16, true); }
   }
+#undef DFM_PS1
 #undef DFM_PS
   DFM_LAUNCH_CHECK("dfm_radon_forward_pairs_sweep");
   return DFM_OK;
