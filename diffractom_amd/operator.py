@@ -1101,10 +1101,9 @@ class SinglePhaseForwardOperator:
                         cands[f"sweep:{n}+ch{4 * cq}" + ("/512" if w >= 4 else "")] = ("sweep", n + 16 * w)
             if tab["pairs_ok"]:
                 ln = max(self.Nx, self.Ny)
-                # first the forms that load each staged block at its start (at most 128 VGPRs: two workgroups per
-                # CU), then those that hold the next block's loads in registers across the sums (one per CU)
-                cands.update({f"psweep:{n}+np": ("psweep", n + 8) for n in (2, 1) if n == 1 or ln <= 128})
                 cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4) if 2 * n * (ln + 4) * 128 <= 163840})
+                # the forms that load each staged block at its start (at most 128 VGPRs: two workgroups per CU)
+                cands.update({f"psweep:{n}+np": ("psweep", n + 8) for n in (2, 1) if n == 1 or ln <= 128})
             if self.radon_bands > 1:
                 cands.update({"band:8+ch128": ("band", 8 + 16), "band:8": ("band", 8), "band:4": ("band", 4),
                               "band:8+ch256": ("band", 8 + 32)})
