@@ -225,64 +225,11 @@ __global__ void __launch_bounds__(256) radon_backward_kernel(const float4* __res
 //   pixel table:    per (angle, pixel) its two bins and the backprojection weights (angle weight and 1/A folded in);
 //   tile bases:     per (angle, 4 x 4 pixel tile) the first of the TB bins the tile reaches.
 
-// Forward from the table: a thread per (channel quad, bin); a workgroup covers 16 quads x 16 consecutive bins of one
-// angle; the channel blocks are split over the XCDs as above. UNR entries are fetched before they are used, so a
-// wave keeps UNR independent loads in flight instead of one dependent chain per entry.
-template <int UNR, int QBT>
-__global__ void __launch_bounds__(256) radon_forward_table_kernel(const float4* __restrict__ img,
-                                                                  float4* __restrict__ sino,
-                                                                  const int* __restrict__ row_ptr,
-                                                                  const int* __restrict__ ent_pix,
-                                                                  const float* __restrict__ ent_w, int ns, int na,
-                                                                  int kstride4, int k4_off, int k4_n, int skstride4) {
-  constexpr int BINS = 256 / QBT;
-  const int sblocks = (ns + BINS - 1) / BINS;
-  const int n_blk = (k4_n + QBT - 1) / QBT;
-  int blk, local;
-  if (!xcd_split(na * sblocks, n_blk, blk, local)) return;
-  const int a = local / sblocks;
-  const int s = (local % sblocks) * BINS + threadIdx.x / QBT;
-  const int kq = blk * QBT + (threadIdx.x & (QBT - 1));
-  if (kq >= k4_n || s >= ns) return;
-  const float4* col = img + kq + k4_off;
-  const int r = a * ns + s;
-  const int e1 = row_ptr[r + 1];
-  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  int e = row_ptr[r];
-  for (; e + UNR <= e1; e += UNR) {
-    int p[UNR];
-    float w[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      p[u] = ent_pix[e + u];
-      w[u] = ent_w[e + u];
-    }
-    float4 v[UNR];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) v[u] = col[(int64_t)p[u] * kstride4];
-#pragma unroll
-    for (int u = 0; u < UNR; ++u) {
-      acc.x += w[u] * v[u].x;
-      acc.y += w[u] * v[u].y;
-      acc.z += w[u] * v[u].z;
-      acc.w += w[u] * v[u].w;
-    }
-  }
-  for (; e < e1; ++e) {
-    const float w = ent_w[e];
-    const float4 v = col[(int64_t)ent_pix[e] * kstride4];
-    acc.x += w * v.x;
-    acc.y += w * v.y;
-    acc.z += w * v.z;
-    acc.w += w * v.w;
-  }
-  st_stream(sino + (int64_t)r * skstride4 + kq, acc);
-}
-
-// The same product with the workgroup's table entries staged in LDS: the 16 bins of a workgroup are consecutive
+// Forward from the table: a thread per (channel quad, bin); a workgroup covers QBT quads x 256 / QBT consecutive bins
+// of one angle; the channel blocks are split over the XCDs as above. The bins of a workgroup are consecutive
 // rows of the CSR, i.e. one contiguous entry range, loaded once with coalesced loads (FT_CHUNK entries per pass) and
-// then read from LDS by all 16 channel quads of a bin (a broadcast); the vector memory path then carries the image
-// gathers only (with per-lane entry loads it also carries every entry 16 times).
+// then read from LDS by all channel quads of a bin (a broadcast): the vector memory path carries the image gathers
+// only. UNR entries are fetched before they are used, so a wave keeps UNR independent image loads in flight.
 constexpr int FT_CHUNK = 2048;
 
 template <int UNR, int QBT>
@@ -458,8 +405,7 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
 // tile's bases and pixel-table rows (channel-independent) are staged once per TCH angles and reused by every block, so
 // the per-block work is the sinogram staging (all of a pass's loads in flight) and the pixel sums. Blocks
 // g*G .. g*G+G-1 run on XCD g % 8, so their sinogram slabs stay in its L2. The blocks of a workgroup run one after
-// another; wider blocks (QW 32 or 64) instead put a pixel's 128 or 256 channels in flight at once, x, z, T loads and
-// each pass's sinogram loads of all of them.
+// another (QW = 16 quads: 64 channels per block).
 constexpr int TCH = 32;   // angles whose table rows are staged at once
 
 template <int MODE, int ACH, int G, int TS, int QW>
@@ -982,49 +928,25 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
   DFM_CHECK_ARG(ns > 0 && n_angles >= 0 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
                     k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
                 "dfm_radon_forward_table: bad sizes");
-  // variant = unr + 16 * cap: entries fetched ahead (1, 4, 8) and, cap > 0, at most cap workgroups per CU (an
-  // otherwise unused LDS allocation of 160 KB / cap), so that fewer channel blocks share an XCD's L2
-  // + 256: the workgroup's entries staged in LDS
-  // + 512 * q: channel block of 64 (q 0), 32 (q 1), 16 (q 2), 128 (q 3) or 256 (q 4) channels per workgroup (the
-  //   image slice an XCD streams for all angles at once: that many channels x the pixels)
-  const int unr = variant % 16, cap = (variant / 16) % 16, staged = (variant / 256) % 2, q = variant / 512;
-  DFM_CHECK_ARG((unr == 1 || unr == 4 || unr == 8) && (cap == 0 || cap >= 3) && q <= 4,
-                "dfm_radon_forward_table: variant %d: entries ahead 1, 4 or 8 (+ 16 * cap, cap 0 or 3 .. 15, + 256 "
-                "for LDS-staged entries, + 512 * q for 64, 32, 16, 128 or 256 channels per block)", variant);
+  // variant = unr + 256 + 512 q: table entries fetched ahead (4 or 8), staged in LDS (the + 256), channel blocks of
+  // 64 (q 0) or 128 (q 3) channels per workgroup (the image slice an XCD streams for all angles at once)
+  const int unr = variant % 256, staged = (variant / 256) % 2, q = variant / 512;
+  DFM_CHECK_ARG((unr == 4 || unr == 8) && staged == 1 && (q == 0 || q == 3),
+                "dfm_radon_forward_table: variant %d: entries ahead 4 or 8, + 256 (LDS-staged entries), + 512 * q "
+                "for 64 (q 0) or 128 (q 3) channels per block", variant);
   if (n_angles == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
-  const int qb = q == 4 ? 4 * QB : q == 3 ? 2 * QB : QB >> q;
+  const int qb = q == 3 ? 2 * QB : QB;
   const unsigned grid = xcd_grid(n_angles * ((ns + 256 / qb - 1) / (256 / qb)), (k4n + qb - 1) / qb);
-  const size_t lds_cap = cap > 0 ? (size_t)(163840 / cap) & ~(size_t)255 : 0;
 #define DFM_FT(U, Q)                                                                                              \
-  if (staged)                                                                                                     \
-    hipLaunchKernelGGL((radon_forward_table_lds_kernel<U, Q>), dim3(grid), dim3(256), lds_cap,                   \
-                       (hipStream_t)stream, (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns,      \
-                       n_angles, kstride / 4, k_off / 4, k4n, skstride / 4);                                      \
-  else                                                                                                            \
-    hipLaunchKernelGGL((radon_forward_table_kernel<U, Q>), dim3(grid), dim3(256), lds_cap, (hipStream_t)stream,  \
-                       (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,    \
-                       k_off / 4, k4n, skstride / 4)
-#define DFM_FTQ(U)        \
-  if (qb == 16) {         \
-    DFM_FT(U, 16);        \
-  } else if (qb == 32) {  \
-    DFM_FT(U, 32);        \
-  } else if (qb == 64) {  \
-    DFM_FT(U, 64);        \
-  } else if (qb == 8) {   \
-    DFM_FT(U, 8);         \
-  } else {                \
-    DFM_FT(U, 4);         \
-  }
-  if (unr == 1) {
-    DFM_FTQ(1);
-  } else if (unr == 4) {
-    DFM_FTQ(4);
+  hipLaunchKernelGGL((radon_forward_table_lds_kernel<U, Q>), dim3(grid), dim3(256), 0, (hipStream_t)stream,      \
+                     (const float4*)img_k, (float4*)sino, row_ptr, ent_pix, ent_w, ns, n_angles, kstride / 4,      \
+                     k_off / 4, k4n, skstride / 4)
+  if (unr == 4) {
+    if (qb == 16) DFM_FT(4, 16); else DFM_FT(4, 32);
   } else {
-    DFM_FTQ(8);
+    if (qb == 16) DFM_FT(8, 16); else DFM_FT(8, 32);
   }
-#undef DFM_FTQ
 #undef DFM_FT
   DFM_LAUNCH_CHECK("dfm_radon_forward_table");
   return DFM_OK;
@@ -1042,59 +964,20 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                     k_n % 4 == 0 && k_off >= 0 && k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 &&
                     skstride >= k_n && mode >= 0 && mode <= 3 && (lam >= 0.0f || mode != 2),
                 "dfm_radon_backward_table: bad sizes");
-  // variant = angles staged per pass (4, 8, 16, 32: a workgroup per 4x4 tile and channel block), or ACH + 64 G
-  // with ACH 4, 8 or 16 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged
-  // once; + 1024: 8x8 tiles (tbase then holds the 8x8 tiles' first bins), ACH 4 or 8, G 1, 2 or 4
-  // + 4096 w (w 1, 2): 4x4 tiles with channel blocks of 16 << w quads (128 or 256 channels; 512 or 1024 threads),
-  // ACH 4 or 8, G 1 or 2
-  const int ach = variant % 64, grp = (variant / 64) % 16, t8 = (variant / 1024) % 4, wq = variant / 4096;
-  DFM_CHECK_ARG((wq == 0 && t8 == 0 && grp == 0 && (ach == 4 || ach == 8 || ach == 16 || ach == 32)) ||
-                    (wq == 0 && t8 == 0 && (grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8 || ach == 16)) ||
-                    (wq == 0 && t8 == 1 && (grp == 1 || grp == 2 || grp == 4) && (ach == 4 || ach == 8)) ||
-                    ((wq == 1 || wq == 2) && t8 == 0 && (grp == 1 || grp == 2) && (ach == 4 || ach == 8)),
-                "dfm_radon_backward_table: variant %d: angles staged per pass 4, 8, 16 or 32, or 4 / 8 / 16 + 64 G "
-                "for G = 2, 4, 8 channel blocks per workgroup, or 4 / 8 + 64 G + 1024 (8x8 tiles, G = 1, 2, 4), or "
-                "4 / 8 + 64 G + 4096 w (G = 1, 2; blocks of 16 << w quads, w = 1, 2)",
+  // variant = 4: a workgroup per 4x4 tile and channel block (4 angles per pass; modes 0 and 1), or ACH + 64 G with
+  // ACH 4 or 8 and G 2, 4 or 8: a workgroup per 4x4 tile and G channel blocks, the tile's tables staged once;
+  // + 1024: 8x8 tiles (tbase then holds the 8x8 tiles' first bins), ACH 4 or 8, G 1, 2 or 4
+  const int ach = variant % 64, grp = (variant / 64) % 16, t8 = (variant / 1024) % 4;
+  DFM_CHECK_ARG(variant < 4096 &&
+                    ((t8 == 0 && grp == 0 && ach == 4) ||
+                     (t8 == 0 && (grp == 2 || grp == 4 || grp == 8) && (ach == 4 || ach == 8)) ||
+                     (t8 == 1 && (grp == 1 || grp == 2 || grp == 4) && (ach == 4 || ach == 8))),
+                "dfm_radon_backward_table: variant %d: 4, or 4 / 8 + 64 G for G = 2, 4, 8 channel blocks per "
+                "workgroup, or 4 / 8 + 64 G + 1024 (8x8 tiles, G = 1, 2, 4)",
                 variant);
   if (k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int ntiles = ((nx + 3) / 4) * ((ny + 3) / 4);
-  if (wq) {
-    const int qw = QB << wq;
-    const int n_grp = ((k4n + qw - 1) / qw + grp - 1) / grp;
-    const unsigned ggrid = xcd_grid(ntiles, n_grp);
-#define DFM_BW(M, A, G, W)                                                                                        \
-  hipLaunchKernelGGL((radon_backward_group_kernel<M, A, G, 4, W>), dim3(ggrid), dim3(16 * W), 0,                  \
-                     (hipStream_t)stream, (const float4*)sino, (float4*)img_k, bgeo, tbase, nx, ny,  \
-                     ns, n_angles, kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z,             \
-                     (const float4*)T, inv_p, lam)
-#define DFM_BWG(M, A, W)     \
-  if (grp == 1) {            \
-    DFM_BW(M, A, 1, W);      \
-  } else {                   \
-    DFM_BW(M, A, 2, W);      \
-  }
-#define DFM_BWA(M, W)                 \
-  if (ach == 4) {                     \
-    DFM_BWG(M, 4, W)                  \
-  } else {                            \
-    DFM_BWG(M, 8, W)                  \
-  }
-    if (mode == 0) {
-      if (wq == 1) { DFM_BWA(0, 32) } else { DFM_BWA(0, 64) }
-    } else if (mode == 1) {
-      if (wq == 1) { DFM_BWA(1, 32) } else { DFM_BWA(1, 64) }
-    } else if (mode == 2) {
-      if (wq == 1) { DFM_BWA(2, 32) } else { DFM_BWA(2, 64) }
-    } else {
-      if (wq == 1) { DFM_BWA(3, 32) } else { DFM_BWA(3, 64) }
-    }
-#undef DFM_BWA
-#undef DFM_BWG
-#undef DFM_BW
-    DFM_LAUNCH_CHECK("dfm_radon_backward_table");
-    return DFM_OK;
-  }
   if (t8) {
     const int ntiles8 = ((nx + 7) / 8) * ((ny + 7) / 8);
     const int n_grp = ((k4n + QB - 1) / QB + grp - 1) / grp;
@@ -1137,10 +1020,8 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
 #define DFM_BGA(M, G)            \
   if (ach == 4) {                \
     DFM_BG(M, 4, G);             \
-  } else if (ach == 8) {         \
-    DFM_BG(M, 8, G);             \
   } else {                       \
-    DFM_BG(M, 16, G);            \
+    DFM_BG(M, 8, G);             \
   }
 #define DFM_BGG(M)               \
   if (grp == 2) {                \
@@ -1167,7 +1048,7 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
   }
   if (mode >= 2) {
     dfm::set_error("dfm_radon_backward_table: mode %d (the EM / PDHG update) needs a grouped variant (+ 64 G, "
-                   "+ 1024 or + 4096 w), not %d", mode, variant);
+                   "+ 1024), not %d", mode, variant);
     return DFM_ERR_UNSUPPORTED;
   }
   const unsigned grid = xcd_grid(ntiles, (k4n + QB - 1) / QB);
@@ -1176,11 +1057,9 @@ int dfm_radon_backward_table(const float* sino, float* img_k, const int32_t* pta
                      (const float4*)sino, (float4*)img_k, (const int4*)ptab, tbase, nx, ny, ns, n_angles,          \
                      kstride / 4, k_off / 4, k4n, skstride / 4, (float4*)x, (float4*)z, (const float4*)T, inv_p, lam)
   if (mode == 0) {
-    if (ach == 4) DFM_BT(0, 4); else if (ach == 8) DFM_BT(0, 8); else if (ach == 16) DFM_BT(0, 16);
-    else DFM_BT(0, 32);
+    DFM_BT(0, 4);
   } else {
-    if (ach == 4) DFM_BT(1, 4); else if (ach == 8) DFM_BT(1, 8); else if (ach == 16) DFM_BT(1, 16);
-    else DFM_BT(1, 32);
+    DFM_BT(1, 4);
   }
 #undef DFM_BT
   DFM_LAUNCH_CHECK("dfm_radon_backward_table");

@@ -388,40 +388,33 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
                     nx <= 256 && ny <= 256 && kstride % 4 == 0 && k_off % 4 == 0 && k_n % 4 == 0 && k_off >= 0 &&
                     k_n >= 0 && k_off + k_n <= kstride && skstride % 4 == 0 && skstride >= k_n,
                 "dfm_radon_forward_sweep: bad sizes (na_grp * ns <= 1024, image <= 256 x 256)");
-  // variant = lines per LDS block (1, 2, 4, 8) + 16 w: 16 (w 0), 32 (w 1) or 64 (w 2) orientations per workgroup,
-  // + 64 for workgroups of 512 threads (na_grp * ns <= 512; 32 or 64 orientations); instantiated: 16 x 2/4/8 and
-  // 32 x 1/2/4 at 1024 threads, 32 x 2 and 64 x 1/2 at 512 (the others spill registers)
+  // variant = lines per LDS block + 16 w: 16 orientations per workgroup (w 0) with 2, 4 or 8 lines, or 32 (w 1) with
+  // 1 or 2 lines (1024 threads each)
   const int len = nx > ny ? nx : ny;
-  const int rb = lines % 16, w = (lines / 16) % 4, half = lines / 64;
-  const int cq = 4 << (w > 2 ? 0 : w);
-  const int th = half ? 512 : SW_THREADS;
-  const bool inst = half ? ((w == 2 && rb <= 2) || (w == 1 && rb == 2)) : ((w == 0 && rb >= 2) || (w == 1 && rb <= 4));
-  DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4 || rb == 8) && w <= 2 && half <= 1 && inst && na_grp * ns <= th &&
-                    (size_t)2 * rb * len * (cq + 1) * sizeof(float4) <= 163840,
-                "dfm_radon_forward_sweep: variant %d: %d lines of %d pixels x %d orientations, %d threads not "
-                "available", lines, rb, len, 4 * cq, th);
+  const int rb = lines % 16, w = lines / 16;
+  const int cq = w ? 8 : 4;
+  const bool inst = w == 0 ? (rb == 2 || rb == 4 || rb == 8) : (w == 1 && (rb == 1 || rb == 2));
+  DFM_CHECK_ARG(inst && na_grp * ns <= SW_THREADS && (size_t)2 * rb * len * (cq + 1) * sizeof(float4) <= 163840,
+                "dfm_radon_forward_sweep: variant %d: %d lines of %d pixels x %d orientations not available (16 "
+                "orientations x 2, 4, 8 lines or 32 x 1, 2 lines, within the LDS; na_grp * ns <= 1024)",
+                lines, rb, len, 4 * cq);
   if (n_grp == 0 || k_n == 0) return DFM_OK;
   const int k4n = k_n / 4;
   const int n_cb = (k4n + cq - 1) / cq;
   const unsigned grid = (unsigned)((n_cb + N_XCD - 1) / N_XCD * N_XCD * n_grp);
   const size_t lds = (size_t)2 * rb * len * (cq + 1) * sizeof(float4);
   const hipStream_t st = (hipStream_t)stream;
-#define DFM_SW(RB, CQ, TH)                                                                                        \
-  hipLaunchKernelGGL((radon_forward_sweep_kernel<RB, CQ, TH>), dim3(grid), dim3(TH), lds, st,                     \
+#define DFM_SW(RB, CQ)                                                                                            \
+  hipLaunchKernelGGL((radon_forward_sweep_kernel<RB, CQ, SW_THREADS>), dim3(grid), dim3(SW_THREADS), lds, st,     \
                      (const float4*)img_k, (float4*)sino, (const SweepGeo*)geo, grp_ang, grp_cls, n_grp, na_grp, ns,  \
                      nx, ny, kstride / 4, k_off / 4, k4n, skstride / 4)
-  if (half) {
-    if (cq == 8) DFM_SW(2, 8, 512);
-    else if (rb == 1) DFM_SW(1, 16, 512);
-    else DFM_SW(2, 16, 512);
-  } else if (cq == 4) {
-    if (rb == 2) DFM_SW(2, 4, 1024);
-    else if (rb == 4) DFM_SW(4, 4, 1024);
-    else DFM_SW(8, 4, 1024);
+  if (cq == 4) {
+    if (rb == 2) DFM_SW(2, 4);
+    else if (rb == 4) DFM_SW(4, 4);
+    else DFM_SW(8, 4);
   } else {
-    if (rb == 1) DFM_SW(1, 8, 1024);
-    else if (rb == 2) DFM_SW(2, 8, 1024);
-    else DFM_SW(4, 8, 1024);
+    if (rb == 1) DFM_SW(1, 8);
+    else DFM_SW(2, 8);
   }
 #undef DFM_SW
   DFM_LAUNCH_CHECK("dfm_radon_forward_sweep");

@@ -181,7 +181,6 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
             gc[i] = c
         return ga, gc
     sweep_grp, sweep_gcls = sweep_groups(1024)
-    sweep_grp_h, sweep_gcls_h = sweep_groups(512)
     # ray pairs (dfm_radon_forward_pairs_sweep): ceil(ns / 2) threads per angle, 512 per workgroup; usable when the
     # two rays' footprints on a line lie within 4 pixels for every angle: (2 hw + 1) / |coef| < 4, coef the
     # along-line coefficient (the larger of |X|, |Y|)
@@ -197,7 +196,7 @@ def radon_tables(geo64, idx, nx, ny, ns, radon_scale, n_bands=1):
             # pixel table's taps and weights from it on the device, by the rule of ptab above
             "bp_geo": np.stack([g["X"][idx], g["Y"][idx], g["T0"][idx], g["hw"][idx], g["top"][idx],
                                 g["w"][idx] * g["inv_a"][idx]], 1).astype(np.float64),
-            "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls, "sweep_grp_h": sweep_grp_h, "sweep_gcls_h": sweep_gcls_h,
+            "sweep_grp": sweep_grp, "sweep_gcls": sweep_gcls,
             "sweep_grp_p": sweep_grp_p, "sweep_gcls_p": sweep_gcls_p, "pairs_ok": pairs_ok,
             "w": wts[order].astype(np.float32), "dw": dwts[order].astype(np.float32), "ptab": ptab,
             "tbase": base.astype(np.int32)}
@@ -1032,7 +1031,7 @@ class SinglePhaseForwardOperator:
         if self.fourier is not None:
             return False
         v = int(self.radon_variant["bwd"]) if self.radon_variant["bwd"] is not None else 0
-        grouped = (v // 64) % 16 > 0 or (v // 1024) % 4 == 1 or v // 4096 > 0
+        grouped = (v // 64) % 16 > 0 or (v // 1024) % 4 == 1
         return self.radon_variant["bwd_kernel"] == "table" and grouped and self._tables(subset) is not None
 
     def _radon_backward(self, sino, geo, tab, n_ang, out_k, kstride, koff, kn, x=None, z=None, T=None, inv_p=0.0,
@@ -1094,11 +1093,9 @@ class SinglePhaseForwardOperator:
             if sl and self.My <= 1024:         # 1024 rays per workgroup (512 for the 64-orientation form)
                 cands.update({f"sweep:{n}": ("sweep", n) for n in (8, 4, 2) if n <= sl})
                 ln = max(self.Nx, self.Ny)
-                # (the 512-thread forms, 32 or 64 orientations in half-size angle groups, stage the image twice as
-                # often and measured 1.8-1.9x slower at 99x99 and 256x256 pixels; they stay callable)
-                for n, w, cq in ((2, 1, 8), (1, 1, 8)):
-                    if 2 * n * ln * (cq + 1) * 16 <= 163840 and (w < 4 or self.My <= 512):
-                        cands[f"sweep:{n}+ch{4 * cq}" + ("/512" if w >= 4 else "")] = ("sweep", n + 16 * w)
+                for n in (2, 1):                 # 32 orientations per workgroup
+                    if 2 * n * ln * 9 * 16 <= 163840:
+                        cands[f"sweep:{n}+ch32"] = ("sweep", n + 16)
             if tab["pairs_ok"]:
                 ln = max(self.Nx, self.Ny)
                 cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4) if 2 * n * (ln + 4) * 128 <= 163840})
@@ -1108,9 +1105,7 @@ class SinglePhaseForwardOperator:
                 cands.update({"band:8+ch128": ("band", 8 + 16), "band:8": ("band", 8), "band:4": ("band", 4),
                               "band:8+ch256": ("band", 8 + 32)})
             cands.update({"table:8+lds+ch128": ("table", 8 + 256 + 1536), "table:8+lds": ("table", 8 + 256),
-                          "table:4+lds": ("table", 4 + 256), "table:8": ("table", 8),
-                          "table:8+lds+cap6": ("table", 8 + 256 + 16 * 6), "table:8+lds+ch32": ("table", 8 + 256 + 512),
-                          "table:8+lds+ch256": ("table", 8 + 256 + 2048)})
+                          "table:4+lds": ("table", 4 + 256)})
             times = {}
             for name, (kern, v) in cands.items():
                 rv["fwd_kernel"] = kern
@@ -1139,9 +1134,6 @@ class SinglePhaseForwardOperator:
             if tab["tbase8"].numel() > 1:
                 cands.update({f"table8:{a}x{g}": ("table", a + 64 * g + 1024)
                               for a, g in ((4, 1), (4, 2), (8, 1), (8, 2), (4, 4), (8, 4))})
-            cands.update({f"table:{a}x{g}:ch{64 << w}": ("table", a + 64 * g + 4096 * w)
-                          for w in (1, 2) for a, g in ((4, 1), (8, 1), (4, 2))})
-            cands.update({f"table:{a}": ("table", a) for a in (4, 8, 16)})
             times = {}
             for name, (kern, v) in cands.items():
                 rv["bwd_kernel"] = kern
@@ -1295,10 +1287,9 @@ class SinglePhaseForwardOperator:
             return
         if tab is not None and self.radon_variant["fwd_kernel"] in ("sweep", "psweep"):
             lines = self.radon_variant.get("fwd_sweep", 0) or _lib.load().dfm_radon_sweep_lines(self.Nx, self.Ny)
-            h = "_h" if lines >= 64 else ""
-            g = tab["sweep_grp" + h]
+            g = tab["sweep_grp"]
             _lib.call("dfm_radon_forward_sweep", _lib.ptr(img), _lib.ptr(sino), _lib.ptr(tab["sweep_geo"]),
-                      _lib.ptr(g), _lib.ptr(tab["sweep_gcls" + h]), g.shape[0], g.shape[1], self.My, self.Nx,
+                      _lib.ptr(g), _lib.ptr(tab["sweep_gcls"]), g.shape[0], g.shape[1], self.My, self.Nx,
                       self.Ny, img_kstride, koff, kn, kn, int(lines), self._stream())
             return
         if tab is not None and self.radon_variant["fwd_kernel"] == "band":

@@ -79,14 +79,12 @@ int dfm_radon_backward(const float* sino, float* img_k, const float* geo, int32_
  * radon_backward_k); mode 3 (grouped variants): the PDHG primal step and extrapolation x' = T > 0 ? max(0, x - T
  * (backward(sino) + lam)) : 0, z = 2 x' - x (x_bar), x = x' (pdhg_poisson.py:69-80 primal_update applied to the
  * output of radon_backward_k); the plain tile variants return DFM_ERR_UNSUPPORTED for modes 2 and 3.
- * variant: a tuning choice with identical results (forward: table entries fetched ahead, 1, 4 or 8, plus 16 * cap
- * to hold at most cap (3 .. 15) workgroups per CU, 0 for no cap, plus 256 to stage a workgroup's table entries in LDS,
- * plus 512 q for channel blocks of 32, 16, 128 or 256 channels (q 1 .. 4; 64 for q 0);
- * backward: angles staged in LDS per pass, 4, 8, 16 or 32, a workgroup per 4 x 4 tile and 64 channels; or ACH + 64 G
- * (ACH 4, 8 or 16; G 2, 4 or 8): a workgroup per tile and G channel blocks, the tile's table rows staged once; + 1024
- * (ACH 4 or 8, G 1, 2 or 4): 8 x 8 tiles, tbase then holding each 8 x 8 tile's first of 16 bins; + 4096 w (ACH 4 or 8,
- * G 1 or 2): 4 x 4 tiles with blocks of 64 << w channels, w 1 or 2), picked by timing (the reference times its tiled /
- * plain projectors the same way, parallel_radon.py:240-281). */
+ * variant: a tuning choice with identical results (forward: table entries fetched ahead, 4 or 8, plus 256 (the
+ * workgroup's table entries staged in LDS), plus 512 q for channel blocks of 64 (q 0) or 128 (q 3) channels;
+ * backward: 4, a workgroup per 4 x 4 tile and 64 channels (4 angles staged per pass); or ACH + 64 G (ACH 4 or 8
+ * angles per pass; G 2, 4 or 8): a workgroup per tile and G channel blocks, the tile's table rows staged once; + 1024
+ * (ACH 4 or 8, G 1, 2 or 4): 8 x 8 tiles, tbase then holding each 8 x 8 tile's first of 16 bins), picked by timing
+ * (the reference times its tiled / plain projectors the same way, parallel_radon.py:240-281). */
 int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_ptr, const int32_t* ent_pix,
                             const float* ent_w, int32_t ns, int32_t n_angles, int32_t kstride, int32_t k_off,
                             int32_t k_n, int32_t skstride, int32_t variant, dfm_stream_t stream);
@@ -94,10 +92,9 @@ int dfm_radon_forward_table(const float* img_k, float* sino, const int32_t* row_
  * of na_grp angle indices, -1 for none; grp_cls: 0 = sweep image rows, for |X| >= |Y|, 1 = columns) and 16
  * orientations, `lines` image lines are staged in LDS at a time and each ray adds its pixels on them; geo: per angle
  * (X, Y, T0, hw, top, scale) in float64, the weight scale * min(hw - |X x + Y y + T0 - s|, top) where positive.
- * Needs nx, ny <= 256; lines: image lines per LDS block (1, 2, 4 or 8, within the LDS: dfm_radon_sweep_lines)
- * + 16 w for 16 << w orientations per workgroup (w 0, 1, 2) + 64 for workgroups of 512 threads; instantiated: 16
- * orientations x 2, 4 or 8 lines and 32 x 1, 2 or 4 at 1024 threads, 32 x 2 and 64 x 1 or 2 at 512 threads;
- * na_grp * ns <= the workgroup's threads. */
+ * Needs nx, ny <= 256; lines: image lines per LDS block (within the LDS: dfm_radon_sweep_lines) + 16 w for 16 << w
+ * orientations per workgroup: 16 orientations (w 0) x 2, 4 or 8 lines or 32 (w 1) x 1 or 2 lines, 1024 threads;
+ * na_grp * ns <= 1024. */
 int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
                             const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx, int32_t ny,
                             int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride, int32_t lines,
@@ -106,7 +103,9 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
 int dfm_radon_sweep_lines(int32_t nx, int32_t ny);
 /* Line sweep with two neighbouring rays per thread (the same result as dfm_radon_forward_sweep): groups of na_grp
  * angles with na_grp * ceil(ns / 2) <= 512; usable when (2 hw + 1) / max(|X|, |Y|) < 4 for every angle (square
- * images with the trapezoid footprint); lines: image lines staged per LDS block (1, 2 or 4). */
+ * images with the trapezoid footprint); lines: image lines staged per LDS block (1, 2 or 4), + 8 for the form that
+ * loads each block at its start instead of holding the next block's loads across the sums (lines 1, or 2 for images
+ * of at most 128 pixels a side). */
 int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
                                   const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx,
                                   int32_t ny, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride,

@@ -23,10 +23,11 @@ def _problem(K=12, seed=7, gain=0.05):
 
 @pytest.mark.parametrize("subsets,lam,extras,bwd", [(4, 0.0, False, None), (None, 0.0, False, None),
                                                     (3, 20.0, True, None), (4, 5.0, True, 4),
-                                                    (3, 0.0, False, 4 + 64 + 1024), (4, 0.0, False, 8 + 64 + 8192)])
+                                                    (3, 0.0, False, 4 + 64 + 1024), (4, 0.0, False, 8 + 64 * 8)])
 def test_osem_matches_oracle(subsets, lam, extras, bwd):
     """bwd: the backprojection variant (None: the operator's default, a grouped kernel applying the EM update
-    itself; 4: the plain tile kernel, backprojection then the element-wise update; 8x8 tiles; 256-channel blocks)."""
+    itself; 4: the plain tile kernel, backprojection then the element-wise update; 8x8 tiles; 4x4 tiles with 8
+    channel blocks per workgroup)."""
     from diffractom_amd import OSEMPoisson
     print(f"case subsets={subsets} lam={lam} extras={extras} bwd={bwd}", flush=True)
     op, orc, b = _problem()

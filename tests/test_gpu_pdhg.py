@@ -24,7 +24,7 @@ def _problem(K=12, seed=7, gain=0.05):
 
 @pytest.mark.parametrize("kind,extras,bwd", [("nonneg", False, None), ("nonneg", False, 4),
                                              ("nonneg_l1", True, None), ("nonneg", False, 4 + 64 + 1024),
-                                             ("nonneg_l1", False, 8 + 64 * 2 + 8192)])
+                                             ("nonneg_l1", False, 8 + 64 * 8)])
 def test_pdhg_matches_oracle(kind, extras, bwd):
     from diffractom_amd import PDHGPoisson
     print(f"case kind={kind} extras={extras} bwd={bwd}", flush=True)
