@@ -156,19 +156,27 @@ constexpr int PR_TH = 512;
 // (about 155 VGPRs: one 512-thread workgroup per CU); otherwise each block's loads are issued, waited for and stored
 // at its start, their addresses formed there from the thread index (at most 128 VGPRs: two workgroups per CU, the
 // other one's sums hiding the loads)
-template <int RB, int NV, bool PIPE>
+//
+// DUP: a b128 LDS read serves 16 lanes per cycle from 64 banks (16 groups of 16 bytes); with pixels 8 quads apart a
+// lane's group is 8 (pixel parity) + its quad, so lanes i and i + 8 (the same rotation) collide whenever their
+// pixels have equal parity: 1.8 cycles per 16 lanes on average at al_c1. In the DUP layout every staged pixel is
+// held twice, in the first half of its 16-quad block and in the second half of the block before it; lanes 0-7 of
+// each 16 read the first copies and lanes 8-15 the second (groups 0-7 and 8-15), so no two lanes of a cycle meet,
+// for twice the LDS and twice the staging stores.
+template <int RB, int NV, bool PIPE, bool DUP>
 __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep_kernel(
     const float4* __restrict__ img, float4* __restrict__ sino, const SweepGeo* __restrict__ geo,
     const int* __restrict__ grp_ang, const int* __restrict__ grp_cls, int n_grp, int na_grp, int ns, int nx, int ny,
     int kstride4, int k4_off, int k4_n, int skstride4) {
   constexpr int CQ = 8;
-  extern __shared__ float4 limg[];   // 2 buffers x RB lines x (len + 4) pixels x CQ float4
+  extern __shared__ float4 limg[];   // 2 buffers x RB lines x (len + 4) pixels x CQ float4 (DUP: (len + 5) x 2 CQ)
+  constexpr int PS = DUP ? 2 * CQ : CQ;                      // float4 per pixel slot
   const int n_cb = (k4_n + CQ - 1) / CQ;
   int cb, g;
   if (!xcd_split(n_grp, n_cb, cb, g)) return;                 // uniform over the workgroup
   const int cls = grp_cls[g];
   const int len = cls ? ny : nx, nlines = cls ? nx : ny;
-  const int row = (len + 4) * CQ;                            // float4 per staged line
+  const int row = (len + 4 + (DUP ? 1 : 0)) * PS;          // float4 per staged line
   const int buf = RB * row;
   const int t = threadIdx.x, lane = t & 63;
   const int npair = (ns + 1) >> 1;
@@ -177,11 +185,17 @@ __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep
   const bool live = a >= 0;
   const int q0 = cb * CQ;
 
-  // ---- zero pads (pixels -2, -1, len, len + 1 of every line of both buffers); never overwritten
+  // ---- zero pads (pixels -2, -1, len, len + 1 of every line of both buffers, both copies); never overwritten. Slot
+  // S (pixel S - 2) lies at S PS + q, or with DUP at (S + 1) PS + q and S PS + CQ + q
   for (int i = t; i < 2 * RB * 4 * CQ; i += PR_TH) {
     const int q = i % CQ, pp = (i / CQ) % 4, l = i / (4 * CQ);       // l over 2 * RB lines
     const int px = pp < 2 ? pp : len + pp;                            // slots 0, 1, len + 2, len + 3
-    limg[l * row + px * CQ + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (DUP) {
+      limg[l * row + (px + 1) * PS + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      limg[l * row + px * PS + CQ + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      limg[l * row + px * CQ + q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
 
   // ---- staging slots of this thread: element i = t + PR_TH u of a block = (line l, position pos, quad q)
@@ -196,7 +210,7 @@ __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep
       const int q = i & (CQ - 1), pp = i / CQ;
       const int l = pp / len, pos = pp - l * len;
       ok_u[u] = l < RB && q0 + q < k4_n;
-      lds_u[u] = l * row + (pos + 2) * CQ + q;
+      lds_u[u] = l * row + (pos + 2 + (DUP ? 1 : 0)) * PS + q;    // (DUP: the first copy; the second CQ before)
       const int64_t pix = cls ? (int64_t)pos * nx + l : (int64_t)l * nx + pos;
       gof_u[u] = pix * kstride4 + k4_off + q0 + q;
     }
@@ -215,7 +229,10 @@ __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep
 #pragma unroll
     for (int u = 0; u < NV; ++u) {
       const int i = t + PR_TH * u;
-      if (i < RB * len * CQ) lb[lds_u[u]] = v[u];
+      if (i < RB * len * CQ) {
+        lb[lds_u[u]] = v[u];
+        if (DUP) lb[lds_u[u] - CQ] = v[u];
+      }
     }
   };
 
@@ -241,6 +258,7 @@ __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep
   const float c1 = ds, c2 = 2.f * ds, cr1 = (float)((1.0 - ainv) * ds), cr2 = (float)((2.0 - ainv) * ds),
               cr3 = (float)((3.0 - ainv) * ds);
   const int rot = (lane & 7);                               // this lane's quad rotation
+  const int hoff = DUP ? (lane & 8) : 0;                     // DUP: lanes 8-15 of 16 read the second copies
   float4 accL[CQ], accR[CQ];
 #pragma unroll
   for (int qq = 0; qq < CQ; ++qq) accL[qq] = accR[qq] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -274,11 +292,11 @@ __global__ void __launch_bounds__(PR_TH, PIPE ? 1 : 4) radon_forward_pairs_sweep
         const float wr1 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr1)), 0.f, ts);
         const float wr2 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr2)), 0.f, ts);
         const float wr3 = __builtin_amdgcn_fmed3f(hs - fabsf(fmaf(ds, uu, cr3)), 0.f, ts);
-        const float4* base = lb + l * row + ((int)p0 + 2) * CQ;
+        const float4* base = lb + l * row + ((int)p0 + 2 + (DUP ? 1 : 0)) * PS - hoff;
 #pragma unroll
         for (int qq = 0; qq < CQ; ++qq) {
           const float4* px = base + ((qq + rot) & (CQ - 1));
-          const float4 x0 = px[0], x1 = px[CQ], x2 = px[2 * CQ], x3 = px[3 * CQ];
+          const float4 x0 = px[0], x1 = px[PS], x2 = px[2 * PS], x3 = px[3 * PS];
           // FMA chains into the accumulators (acc += w0 x0 + w1 x1 + w2 x2 as written would cost a multiply and an
           // add besides the two FMAs: 4 VALU operations per 3 taps instead of 3)
 #define DFM_PS3(acc, c, wa, xa, wb, xb, wc, xc) acc.c = fmaf(wc, xc.c, fmaf(wb, xb.c, fmaf(wa, xa.c, acc.c)));
@@ -338,15 +356,18 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
                     skstride % 4 == 0 && skstride >= k_n && na_grp * ((ns + 1) / 2) <= PR_TH,
                 "dfm_radon_forward_pairs_sweep: bad sizes (na_grp * ceil(ns / 2) <= 512, image <= 256 x 256)");
   // lines: lines per staged block (1, 2, 4), + 8 for the form that loads each block at its start (no loads held
-  // across the sums: two workgroups per CU)
+  // across the sums: two workgroups per CU), or + 16 for the duplicated-pixel layout (lines 1 or 2; conflict-free LDS
+  // reads, twice the LDS)
   const int rb = lines & 7;
-  const bool pipe = (lines & 8) == 0;
-  DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4) && (lines & ~15) == 0 &&
-                    (pipe || rb == 1 || (rb == 2 && len <= 128)),
-                "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4, or 1 + 8 or (lines <= 128 pixels) 2 + 8",
-                lines);
+  const bool pipe = (lines & 8) == 0, dup = (lines & 16) != 0;
+  DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4) && (lines & ~31) == 0 &&
+                    (pipe || rb == 1 || (rb == 2 && len <= 128)) &&
+                    (!dup || (pipe && (rb == 1 || (rb == 2 && len <= 128)))),
+                "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4, or 1 + 8 or (lines <= 128 pixels) 2 + 8, or "
+                "1 + 16 or (lines <= 128 pixels) 2 + 16", lines);
   // staged lines, or the store transpose (64 lanes x 8 quads per wave), whichever is larger
-  size_t lds = (size_t)2 * rb * (len + 4) * 8 * sizeof(float4);
+  size_t lds = dup ? (size_t)2 * rb * (len + 5) * 16 * sizeof(float4)
+                   : (size_t)2 * rb * (len + 4) * 8 * sizeof(float4);
   if (lds < (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4)) lds = (size_t)(PR_TH / 64) * 64 * 8 * sizeof(float4);
   DFM_CHECK_ARG(lds <= 163840, "dfm_radon_forward_pairs_sweep: %d lines of %d pixels exceed the LDS", rb, len);
   if (n_grp == 0 || k_n == 0) return DFM_OK;
@@ -357,21 +378,23 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
   // staging elements per thread: lines * len * 8 float4 over 512 threads (len <= 128 or <= 256)
   const bool small = len <= 128;
 #define DFM_PS(RB, NV)              \
-  if (pipe) {                       \
-    DFM_PS1(RB, NV, true);          \
+  if (dup) {                        \
+    DFM_PS1(RB, NV, true, true);    \
+  } else if (pipe) {                \
+    DFM_PS1(RB, NV, true, false);   \
   } else {                          \
-    DFM_PS1(RB, NV, false);         \
+    DFM_PS1(RB, NV, false, false);  \
   }
-#define DFM_PS1(RB, NV, P)                                                                                       \
-  hipLaunchKernelGGL((radon_forward_pairs_sweep_kernel<RB, NV, P>), dim3(grid), dim3(PR_TH), lds, st,              \
+#define DFM_PS1(RB, NV, P, D)                                                                                    \
+  hipLaunchKernelGGL((radon_forward_pairs_sweep_kernel<RB, NV, P, D>), dim3(grid), dim3(PR_TH), lds, st,           \
                      (const float4*)img_k, (float4*)sino, (const SweepGeo*)geo, grp_ang, grp_cls, n_grp, na_grp, ns, \
                      nx, ny, kstride / 4, k_off / 4, k4n, skstride / 4)
   if (rb == 1) {
     if (small) { DFM_PS(1, 2) } else { DFM_PS(1, 4) }
   } else if (rb == 2) {
-    if (small) { DFM_PS(2, 4) } else { DFM_PS1(2, 8, true); }
+    if (small) { DFM_PS(2, 4) } else { DFM_PS1(2, 8, true, false); }
   } else {
-    if (small) { DFM_PS1(4, 8, true); } else { DFM_PS1(4, 16, true); }
+    if (small) { DFM_PS1(4, 8, true, false); } else { DFM_PS1(4, 16, true, false); }
   }
 #undef DFM_PS1
 #undef DFM_PS

@@ -128,7 +128,7 @@ def test_errors_raise_like_the_reference():
                                              ("table_group", 4), ("table_tile8", 4), ("table_tile8g2", 4),
                                              ("sweep", 4), ("sweep2", 4), ("sweep32", 4), ("sweep32_1", 4),
                                              ("psweep1", 4), ("psweep2", 4), ("psweep4", 4), ("psweep1np", 4),
-                                             ("psweep2np", 4), ("arith", 4)])
+                                             ("psweep2np", 4), ("psweep1dup", 4), ("psweep2dup", 4), ("arith", 4)])
 @pytest.mark.parametrize("shape", ["square", "chunked"])
 def test_projector_kernels_match_oracle(kernel, channels, shape):
     """Every projector kernel choice (geometry tables, staged or not, capped; pixel bands; line sweeps with one or two
@@ -154,8 +154,8 @@ def test_projector_kernels_match_oracle(kernel, channels, shape):
         op.radon_variant["bwd_kernel"] = "table"
     elif kernel.startswith("psweep"):            # forward by line sweep with two rays per thread
         op.radon_variant["fwd_kernel"] = "psweep"
-        # (np: each staged block loaded at its start, two workgroups per CU)
-        op.radon_variant["fwd_psweep"] = int(kernel[6]) + (8 if kernel.endswith("np") else 0)
+        # (np: each staged block loaded at its start, two workgroups per CU; dup: the duplicated-pixel LDS layout)
+        op.radon_variant["fwd_psweep"] = int(kernel[6]) + {"np": 8, "up": 16}.get(kernel[-2:], 0)
         op.radon_variant["bwd_kernel"] = "table"
         assert op._tables_full["sweep_grp_p"][0, 0] >= 0
     elif kernel == "table_group":                # backprojection: a workgroup per tile and 4 channel blocks
