@@ -147,9 +147,10 @@ __global__ void __launch_bounds__(TH) radon_forward_sweep_kernel(
 // orientations instead of 6) and every candidate's weight is computed from the geometry (0 outside the footprint):
 // no per-lane loops, no divergent branches. The line geometry is float64 (per line, not per pixel), the six weights
 // float32 with one v_med3 clamp each. Pixels are 8 float4 (32 orientations) apart in LDS and a lane reads quad
-// (qq + lane) % 8 at step qq, so the 8 lanes of every LDS cycle hit 8 distinct 16-byte bank groups whatever pixels
-// they read (no bank conflicts); two zero pixels pad each staged line on both sides, so the candidates need no
-// bounds checks. Staging addresses are computed once per thread (the block's line offset is added per block).
+// (qq + lane) % 8 at step qq, so lanes 0-7 (and 8-15) of a read hit 8 distinct 16-byte bank groups whatever pixels
+// they read (the DUP layout below also keeps the two halves apart); two zero pixels pad each staged line on both
+// sides, so the candidates need no bounds checks. Staging addresses are computed once per thread (the block's line
+// offset is added per block).
 constexpr int PR_TH = 512;
 
 // PIPE: the next block's image loads are issued before the current block's sums and held in registers across them
