@@ -150,8 +150,10 @@ __device__ __forceinline__ int frag_slot(unsigned p) {
   return (c >> 4) * 512 + ((c & 15) + 16 * (q >> 3)) * 8 + (q & 7);
 }
 
-// Workgroup w of one product (decomposition: column-block group fastest, then y group, k split, angle).
-template <int MODE, bool FWD, int WV>
+// Workgroup w of one product (decomposition: column-block group fastest, then y group, k split, angle). MTN > 0:
+// the y tiles per group as a compile-time count (equal to p.mt_n), so the tile loops carry no run-time bound and the
+// chunk's LDS reads and MFMAs form one straight-line block the scheduler can interleave.
+template <int MODE, bool FWD, int WV, int MTN = 0>
 __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   constexpr bool RES = MODE == MODE_BF16_RES;
   constexpr bool F16 = MODE == MODE_F16;
@@ -176,13 +178,14 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   const int cb = tg * WV + wv;                               // this wave's column block
   const bool cb_live = cb < p.n_cb;
   const bool cb_stored = FWD ? cb < p.n_jb : cb < p.n_kb;    // adjoint: blocks beyond the batch are empty (zeros)
-  const int y0 = yg * p.mt_n * 16;
+  const int mt_n = MTN > 0 ? MTN : p.mt_n;
+  const int y0 = yg * mt_n * 16;
   const int c0 = ks * p.ch_per;
   const int c1 = min(p.n_ch, c0 + p.ch_per);
   if (c0 >= c1) return;                                      // uniform
 
   // ---- y-side operand slots of this thread: s = tid + TH i < mt_n * 64
-  const int n_slots = p.mt_n * 64;
+  const int n_slots = mt_n * 64;
   constexpr int NS = (MT_MAX * 64 + TH - 1) / TH;
   const float* srow[NS];
   bool sok[NS];
@@ -334,7 +337,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
     }
 #pragma unroll
     for (int mt = 0; mt < MT_MAX; ++mt) {
-      if (mt < p.mt_n) {
+      if (mt < mt_n) {
         const s16x8 ah = a_hi[buf * MT_MAX * 64 + mt * 64 + lane];
         if (F16) {
           const float4 iv = *reinterpret_cast<const float4*>(&a_inv[buf * MT_MAX * 16 + mt * 16 + (lane >> 4) * 4]);
@@ -398,7 +401,7 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
   const int row = lane & 15, grp = lane >> 4;
 #pragma unroll
   for (int mt = 0; mt < MT_MAX; ++mt) {
-    if (mt >= p.mt_n) continue;
+    if (mt >= mt_n) continue;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
       const int col = cb * 32 + t * 16 + row;
@@ -427,11 +430,14 @@ __device__ __forceinline__ void blk_pf_body(const BlkArgs& p, int64_t w) {
 // WV waves per workgroup (WV column blocks share each staged y-side chunk): 4 (three workgroups per CU, two in the
 // fp16 mode) or 8 (half the re-reads of the y-side rows, which every group of WV column blocks stages again; one
 // workgroup per CU).
-template <int MODE, bool FWD, int WV>
+template <int MODE, bool FWD, int WV, int MTN>
 __global__ __launch_bounds__(64 * WV, WV == 4 ? (MODE == MODE_F16 ? 2 : 3) : 2) void blk_pf_kernel(
     BlkArgs p) {
-  blk_pf_body<MODE, FWD, WV>(p, xcd_block());
+  blk_pf_body<MODE, FWD, WV, MTN>(p, xcd_block());
 }
+
+// the y-tile count compiled in (MTN) for the 4-wave form: 7 (99-112 detector rows in one group)
+constexpr int MT_FIXED = 7;
 
 // Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (the slices of a multi-slice batch,
 // each its own PF blocks, K, sinogram columns and data rows); entries with x < 0 pad the grid.
@@ -457,22 +463,24 @@ void launch_blk_grouped(const BlkArgs* descs, const int2* wg_map, int64_t n_map,
     hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_F16, FWD>), grid, block, 0, st, descs, wg_map, n_map);
 }
 
-template <bool FWD, int WV>
+template <bool FWD, int WV, int MTN>
 void launch_blk_wv(const BlkArgs& p, int mode, hipStream_t st) {
   const int64_t n_wg = (int64_t)p.n_ang * p.ksplit * p.n_yg * p.n_tg;
   const dim3 grid((unsigned)((n_wg + N_XCD - 1) / N_XCD * N_XCD)), block(64 * WV);
-  if (mode == MODE_BF16_RES) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16_RES, FWD, WV>), grid, block, 0, st, p);
-  else if (mode == MODE_BF16) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16, FWD, WV>), grid, block, 0, st, p);
-  else hipLaunchKernelGGL((blk_pf_kernel<MODE_F16, FWD, WV>), grid, block, 0, st, p);
+  if (mode == MODE_BF16_RES) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16_RES, FWD, WV, MTN>), grid, block, 0, st, p);
+  else if (mode == MODE_BF16) hipLaunchKernelGGL((blk_pf_kernel<MODE_BF16, FWD, WV, MTN>), grid, block, 0, st, p);
+  else hipLaunchKernelGGL((blk_pf_kernel<MODE_F16, FWD, WV, MTN>), grid, block, 0, st, p);
 }
 
 template <bool FWD>
-void launch_blk(const BlkArgs& p, int mode, int wv, hipStream_t st) {
-  if (wv == 8) launch_blk_wv<FWD, 8>(p, mode, st);
-  else launch_blk_wv<FWD, 4>(p, mode, st);
+void launch_blk(const BlkArgs& p, int mode, int wv, bool fixed, hipStream_t st) {
+  if (wv == 8) launch_blk_wv<FWD, 8, 0>(p, mode, st);
+  else if (fixed && p.mt_n == MT_FIXED) launch_blk_wv<FWD, 4, MT_FIXED>(p, mode, st);
+  else launch_blk_wv<FWD, 4, 0>(p, mode, st);
 }
 
-// waves per workgroup from bits 5-6 of the mode argument: 0 -> 4, 1 -> 8
+// waves per workgroup from bits 5-6 of the mode argument: 0 -> 4, 1 -> 8; bit 7: the 4-wave form with the y-tile
+// count compiled in where it is MT_FIXED (the same results)
 int mode_waves(int mode) { return 4 << ((mode >> 5) & 3); }
 
 void y_tiling(int my, int& mt_n, int& n_yg) {
@@ -519,6 +527,7 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
   const int local = (mode >> 4) & 1;
   const int wv = mode_waves(mode);
   DFM_CHECK_ARG(wv <= 8, "dfm_pfblk_forward: waves code %d (0, 1: 4, 8 waves per workgroup)", (mode >> 5) & 3);
+  const bool fixed = (mode >> 7) & 1;
   mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_forward: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
@@ -553,7 +562,7 @@ int dfm_pfblk_forward(const float* sino, int32_t sks, int32_t kb, const int32_t*
   p.out = p.direct ? data : part;
   p.alpha = 1.f;
   const hipStream_t st = (hipStream_t)stream;
-  launch_blk<true>(p, mode, wv, st);
+  launch_blk<true>(p, mode, wv, fixed, st);
   DFM_LAUNCH_CHECK("dfm_pfblk_forward");
   if (!p.direct) return dfm::launch_splitk_reduce(part, data, omegas, n_ang, (int64_t)my * cp, p.ksplit, accumulate, st);
   return DFM_OK;
@@ -566,6 +575,7 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
   const int local = (mode >> 4) & 1;
   const int wv = mode_waves(mode);
   DFM_CHECK_ARG(wv <= 8, "dfm_pfblk_adjoint: waves code %d (0, 1: 4, 8 waves per workgroup)", (mode >> 5) & 3);
+  const bool fixed = (mode >> 7) & 1;
   mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16 && (mode != MODE_F16 || (ring_inv && P > 0)),
                 "dfm_pfblk_adjoint: mode %d (0 bf16, 1 bf16 residual, 2 fp16 with ring scales)", mode);
@@ -596,7 +606,7 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
   p.ch_per = p.n_ch;
   p.alpha = alpha;
   p.ncols = sks;
-  launch_blk<false>(p, mode, wv, (hipStream_t)stream);
+  launch_blk<false>(p, mode, wv, fixed, (hipStream_t)stream);
   DFM_LAUNCH_CHECK("dfm_pfblk_adjoint");
   return DFM_OK;
 }

@@ -664,9 +664,12 @@ class SinglePhaseForwardOperator:
         self.sparse_fill = nnz / (R * self.K * CP)
 
     def _blk_mode(self, direction):
-        """Mode argument of dfm_pfblk_forward / _adjoint: the precision, plus 32 for 8-wave workgroups."""
+        """Mode argument of dfm_pfblk_forward / _adjoint: the precision, plus 32 for 8-wave workgroups, plus 128
+        for the 4-wave form with the y-tile count compiled in (used where the count matches; same results)."""
         wg = int(self.blk_variant.get(direction + "_wg", 4))
-        return {"bf16": 0, "bf16_residual": 1, "fp16": 2}[self.pf_precision] + (32 if wg == 8 else 0)
+        fixed = bool(self.blk_variant.get(direction + "_fixed", False))
+        return ({"bf16": 0, "bf16_residual": 1, "fp16": 2}[self.pf_precision] + (32 if wg == 8 else 0)
+                + (128 if fixed else 0))
 
     def _blk_split(self, n_ang, blk):
         """k split of the block-sparse forward: enough waves for blk_variant["fwd_waves"], at most 64 partial sums."""
@@ -1244,21 +1247,23 @@ class SinglePhaseForwardOperator:
             adj = (lambda: self._gen_adjoint(0, data, sc["sino"], subset, 1.0)) if gen else \
                 (lambda: self._blk_adjoint(0, data, sc["sino"], subset, 1.0))
             times, cands = {}, {}
+            for w in (4096, 8192, 16384):             # the y-tile count compiled in (4-wave form)
+                cands[f"wg:4+waves:{w}+mt"] = (4, w, True)
             for wg in (4, 8):
                 for w in (2048, 4096, 8192, 16384, 32768):
-                    cands[f"wg:{wg}+waves:{w}"] = (wg, w)
-            for name, (wg, w) in cands.items():
-                self.blk_variant.update(fwd_wg=wg, fwd_waves=w)
+                    cands[f"wg:{wg}+waves:{w}"] = (wg, w, False)
+            for name, (wg, w, fx) in cands.items():
+                self.blk_variant.update(fwd_wg=wg, fwd_waves=w, fwd_fixed=fx)
                 times[name] = timeit(fwd)
             best = _pick(times)
-            self.blk_variant.update(fwd_wg=cands[best][0], fwd_waves=cands[best][1])
+            self.blk_variant.update(fwd_wg=cands[best][0], fwd_waves=cands[best][1], fwd_fixed=cands[best][2])
             out["pf_forward"] = (best, times)
-            times = {}
-            for wg in (4, 8):
-                self.blk_variant["adj_wg"] = wg
-                times[f"wg:{wg}"] = timeit(adj)
+            times, cands = {}, {"wg:4+mt": (4, True), "wg:4": (4, False), "wg:8": (8, False)}
+            for name, (wg, fx) in cands.items():
+                self.blk_variant.update(adj_wg=wg, adj_fixed=fx)
+                times[name] = timeit(adj)
             best = _pick(times)
-            self.blk_variant["adj_wg"] = int(best.split(":")[1])
+            self.blk_variant.update(adj_wg=cands[best][0], adj_fixed=cands[best][1])
             out["pf_adjoint"] = (best, times)
 
     def zero_subset_rows(self, data, subset):

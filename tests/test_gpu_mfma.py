@@ -264,3 +264,29 @@ def test_block_workgroup_sizes_agree(mode, precision):
         res.append(out)
     for u, v in zip(*res):
         assert np.array_equal(u, v)
+
+
+@pytest.mark.parametrize("mode,precision", [("sparse", "bf16_residual"), ("sparse", "fp16"), ("generated", "fp16")])
+def test_block_fixed_tile_count_agrees(mode, precision):
+    """The 4-wave block products with the y-tile count compiled in (7 tiles: 103 detector rows) give the run-time
+    count's results bit for bit (the same tiles, chunks and accumulation order)."""
+    import torch
+    print(f"case block fixed tile count {mode} {precision}", flush=True)
+    c = cfg(nx=12, ny=12, my=103, n_omega=12, angle_range=(0, 180), cor=0.3)
+    op, _, _ = build(c, K=300, pf_mode=mode, pf_precision=precision, oracle=False)
+    op.set_omega_subsets(3)
+    x, y = _inputs(op, 7)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    res = []
+    for fixed in (False, True):
+        op.blk_variant.update(fwd_wg=4, adj_wg=4, fwd_fixed=fixed, adj_fixed=fixed)
+        out = []
+        for sub in (None, 1):
+            d = torch.zeros(op.data_shape, device="cuda")
+            a = torch.zeros(op.coeff_shape, device="cuda")
+            op.direct_cl(xt, d, subset=sub)
+            op.adjoint_cl(yt, a, subset=sub)
+            out += [d.cpu().numpy(), a.cpu().numpy()]
+        res.append(out)
+    for u, v in zip(*res):
+        assert np.array_equal(u, v)
