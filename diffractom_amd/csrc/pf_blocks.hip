@@ -436,8 +436,8 @@ __global__ __launch_bounds__(64 * WV, WV == 4 ? (MODE == MODE_F16 ? 2 : 3) : 2) 
   blk_pf_body<MODE, FWD, WV, MTN>(p, xcd_block());
 }
 
-// the y-tile count compiled in (MTN) for the 4-wave form: 7 (99-112 detector rows in one group)
-constexpr int MT_FIXED = 7;
+// the y-tile counts compiled in (MTN) for the 4-wave form: 7 (97-112 detector rows in one group) and 6 (groups of
+// 6 tiles, e.g. 261 rows in three)
 
 // Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (the slices of a multi-slice batch,
 // each its own PF blocks, K, sinogram columns and data rows); entries with x < 0 pad the grid.
@@ -475,12 +475,13 @@ void launch_blk_wv(const BlkArgs& p, int mode, hipStream_t st) {
 template <bool FWD>
 void launch_blk(const BlkArgs& p, int mode, int wv, bool fixed, hipStream_t st) {
   if (wv == 8) launch_blk_wv<FWD, 8, 0>(p, mode, st);
-  else if (fixed && p.mt_n == MT_FIXED) launch_blk_wv<FWD, 4, MT_FIXED>(p, mode, st);
+  else if (fixed && p.mt_n == 7) launch_blk_wv<FWD, 4, 7>(p, mode, st);
+  else if (fixed && p.mt_n == 6) launch_blk_wv<FWD, 4, 6>(p, mode, st);
   else launch_blk_wv<FWD, 4, 0>(p, mode, st);
 }
 
 // waves per workgroup from bits 5-6 of the mode argument: 0 -> 4, 1 -> 8; bit 7: the 4-wave form with the y-tile
-// count compiled in where it is MT_FIXED (the same results)
+// count compiled in where it is 6 or 7 (the same results)
 int mode_waves(int mode) { return 4 << ((mode >> 5) & 3); }
 
 void y_tiling(int my, int& mt_n, int& n_yg) {

@@ -266,13 +266,14 @@ def test_block_workgroup_sizes_agree(mode, precision):
         assert np.array_equal(u, v)
 
 
+@pytest.mark.parametrize("my", [103, 90])
 @pytest.mark.parametrize("mode,precision", [("sparse", "bf16_residual"), ("sparse", "fp16"), ("generated", "fp16")])
-def test_block_fixed_tile_count_agrees(mode, precision):
-    """The 4-wave block products with the y-tile count compiled in (7 tiles: 103 detector rows) give the run-time
-    count's results bit for bit (the same tiles, chunks and accumulation order)."""
+def test_block_fixed_tile_count_agrees(mode, precision, my):
+    """The 4-wave block products with the y-tile count compiled in (7 tiles at 103 detector rows, 6 at 90) give the
+    run-time count's results bit for bit (the same tiles, chunks and accumulation order)."""
     import torch
-    print(f"case block fixed tile count {mode} {precision}", flush=True)
-    c = cfg(nx=12, ny=12, my=103, n_omega=12, angle_range=(0, 180), cor=0.3)
+    print(f"case block fixed tile count {mode} {precision} my={my}", flush=True)
+    c = cfg(nx=12, ny=12, my=my, n_omega=12, angle_range=(0, 180), cor=0.3)
     op, _, _ = build(c, K=300, pf_mode=mode, pf_precision=precision, oracle=False)
     op.set_omega_subsets(3)
     x, y = _inputs(op, 7)
