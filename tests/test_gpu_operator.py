@@ -320,3 +320,38 @@ def test_dense_fp32_reduction_split():
     d = torch.zeros(op.data_shape, dtype=torch.float32, device="cuda")
     op.direct_cl(torch.from_numpy(x).cuda(), d, subset=1)
     assert rel_max(d.cpu().numpy(), orc.direct(x, subset=1)) < 2e-5
+
+
+@pytest.mark.parametrize("nx", [99, 150])
+def test_pairs_sweep_forms_agree_on_long_lines(nx):
+    """The ray-pair sweep's forms (1, 2 or 4 staged lines; each block loaded at its start; every pixel held twice in
+    LDS) compute every ray's taps in the same order, so their sinograms agree bit for bit; at 99 and 150 pixels a
+    line (the 128-pixel limit of the two-line loaded-at-start and duplicated forms in between), 64 orientations,
+    a subset of the angles; and one of them against the table forward (the projector the tests above pin)."""
+    import torch
+    print(f"case pairs sweep forms nx={nx}", flush=True)
+    c = cfg(nx=nx, ny=nx, my=int(nx * 1.05) + 3, n_omega=20, angle_range=(0, 180), cor=0.4, n_eta=12)
+    op, _, _ = build(c, K=64, pf_mode="sparse", oracle=False)
+    op.set_omega_subsets(2)
+    tab = op._tables(1)
+    assert tab["pairs_ok"]
+    n_ang, _, geo_f, _ = op._angle_set(1)
+    b = op.batches[0]
+    ks, kn = op.K_stride, b["kstride"]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    img = torch.rand(op.Nx * op.Ny * ks, generator=g, device="cuda")
+    forms = [1, 2, 4, 9, 17] + ([10, 18] if nx <= 128 else [])
+    res = {}
+    for v in forms + ["table"]:
+        sino = torch.zeros(n_ang * op.My * kn, device="cuda")
+        if v == "table":
+            op.radon_variant.update(fwd_kernel="table", fwd=8 + 256)
+        else:
+            op.radon_variant.update(fwd_kernel="psweep", fwd_psweep=v)
+        op._radon_forward(img, ks, b["koff"], kn, sino, geo_f, n_ang, tab)
+        torch.cuda.synchronize()
+        res[v] = sino.cpu().numpy()
+    for v in forms[1:]:
+        assert np.array_equal(res[v], res[forms[0]]), v
+    assert rel_max(res[2], res["table"]) < 2e-5
