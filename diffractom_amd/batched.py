@@ -267,8 +267,10 @@ class BatchedSPDHGPoisson:
     def _pf(self, forward, m):
         buf, wgm, n_wg = self._group(forward, m)
         if self.blocks:
+            # + the slices' common y-tile count in bits 8-11 (a compiled form where there is one; same results)
+            mt_n, _ = _y_tiling(self.ops[0].My)
             _lib.call("dfm_pfblk_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg,
-                      self.ops[0]._blk_mode("fwd") & 15, self.ops[0]._stream())
+                      (self.ops[0]._blk_mode("fwd") & 15) | (mt_n << 8), self.ops[0]._stream())
             return
         _lib.call("dfm_mfma_grouped", int(forward), _lib.ptr(buf), _lib.ptr(wgm), n_wg, int(self.residual),
                   self.variant, self.ops[0]._stream())

@@ -441,7 +441,8 @@ __global__ __launch_bounds__(64 * WV, WV == 4 ? (MODE == MODE_F16 ? 2 : 3) : 2) 
 
 // Grouped launch: workgroup w runs workgroup wg_map[w].y of product wg_map[w].x (the slices of a multi-slice batch,
 // each its own PF blocks, K, sinogram columns and data rows); entries with x < 0 pad the grid.
-template <int MODE, bool FWD>
+// (MTN > 0: every descriptor's y-tile count, compiled in)
+template <int MODE, bool FWD, int MTN>
 __global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_grouped_kernel(const BlkArgs* __restrict__ descs,
                                                                     const int2* __restrict__ wg_map, int64_t n_map) {
   const int64_t w = xcd_block();
@@ -449,18 +450,24 @@ __global__ __launch_bounds__(THREADS, MODE == MODE_F16 ? 2 : 3) void blk_pf_grou
   const int2 m = wg_map[w];
   if (m.x < 0) return;
   const BlkArgs p = descs[m.x];
-  blk_pf_body<MODE, FWD, WAVES>(p, m.y);
+  blk_pf_body<MODE, FWD, WAVES, MTN>(p, m.y);
+}
+
+template <bool FWD, int MTN>
+void launch_blk_grouped_mt(const BlkArgs* descs, const int2* wg_map, int64_t n_map, int mode, hipStream_t st) {
+  const dim3 grid((unsigned)((n_map + N_XCD - 1) / N_XCD * N_XCD)), block(THREADS);
+  if (mode == MODE_BF16_RES)
+    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_BF16_RES, FWD, MTN>), grid, block, 0, st, descs, wg_map, n_map);
+  else if (mode == MODE_BF16)
+    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_BF16, FWD, MTN>), grid, block, 0, st, descs, wg_map, n_map);
+  else
+    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_F16, FWD, MTN>), grid, block, 0, st, descs, wg_map, n_map);
 }
 
 template <bool FWD>
-void launch_blk_grouped(const BlkArgs* descs, const int2* wg_map, int64_t n_map, int mode, hipStream_t st) {
-  const dim3 grid((unsigned)((n_map + N_XCD - 1) / N_XCD * N_XCD)), block(THREADS);
-  if (mode == MODE_BF16_RES)
-    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_BF16_RES, FWD>), grid, block, 0, st, descs, wg_map, n_map);
-  else if (mode == MODE_BF16)
-    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_BF16, FWD>), grid, block, 0, st, descs, wg_map, n_map);
-  else
-    hipLaunchKernelGGL((blk_pf_grouped_kernel<MODE_F16, FWD>), grid, block, 0, st, descs, wg_map, n_map);
+void launch_blk_grouped(const BlkArgs* descs, const int2* wg_map, int64_t n_map, int mode, int mtn, hipStream_t st) {
+  if (mtn == 5) launch_blk_grouped_mt<FWD, 5>(descs, wg_map, n_map, mode, st);
+  else launch_blk_grouped_mt<FWD, 0>(descs, wg_map, n_map, mode, st);
 }
 
 template <bool FWD, int WV, int MTN>
@@ -615,13 +622,15 @@ int dfm_pfblk_adjoint(const float* data, const int32_t* ptr, const uint16_t* pos
 int dfm_pfblk_grouped(int32_t forward, const void* descs, const int32_t* wg_map, int64_t n_wg, int32_t mode,
                       dfm_stream_t stream) {
   DFM_CHECK_ARG(descs && wg_map && n_wg > 0, "dfm_pfblk_grouped: null pointer or no workgroups");
+  // bits 8-11: every descriptor's y-tile count mt_n, when the caller knows it (5 has a compiled form; 0: unknown)
+  const int mtn = (mode >> 8) & 15;
   mode &= 15;
   DFM_CHECK_ARG(mode >= MODE_BF16 && mode <= MODE_F16, "dfm_pfblk_grouped: mode %d", mode);
   DFM_CHECK_ARG(n_wg <= ((int64_t)1 << 31) - 8, "dfm_pfblk_grouped: %lld workgroups", (long long)n_wg);
   if (forward)
-    launch_blk_grouped<true>((const BlkArgs*)descs, (const int2*)wg_map, n_wg, mode, (hipStream_t)stream);
+    launch_blk_grouped<true>((const BlkArgs*)descs, (const int2*)wg_map, n_wg, mode, mtn, (hipStream_t)stream);
   else
-    launch_blk_grouped<false>((const BlkArgs*)descs, (const int2*)wg_map, n_wg, mode, (hipStream_t)stream);
+    launch_blk_grouped<false>((const BlkArgs*)descs, (const int2*)wg_map, n_wg, mode, mtn, (hipStream_t)stream);
   DFM_LAUNCH_CHECK("dfm_pfblk_grouped");
   return DFM_OK;
 }

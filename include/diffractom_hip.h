@@ -314,6 +314,8 @@ typedef struct dfm_pfblk_desc {
   int32_t ncols;
   int32_t blk_local;
 } dfm_pfblk_desc;
+/* mode: 0 bf16, 1 bf16 residual, 2 fp16, + 256 mt_n when every descriptor has the same y-tile count mt_n (a form
+ * with that count compiled in where there is one, for 5; the same results). */
 int dfm_pfblk_grouped(int32_t forward, const void* descs, const int32_t* wg_map, int64_t n_wg, int32_t mode,
                       dfm_stream_t stream);
 /* sizeof(dfm_pfblk_desc) as compiled into the library (for bindings that build the descriptor array by hand). */

@@ -75,14 +75,16 @@ def test_batch_autotune_keeps_the_iterates():
     assert rel_l2(bat.xk.cpu().numpy(), x_ref.cpu().numpy()) < 1e-5
 
 
+@pytest.mark.parametrize("my", [21, 131])
 @pytest.mark.parametrize("precision", ["bf16_residual", "fp16"])
-def test_batch_of_block_sparse_slices(precision):
+def test_batch_of_block_sparse_slices(precision, my):
     """The grouped block-sparse PF products (dfm_pfblk_grouped: one launch for every slice's blocks) give each
-    slice the iterates and objectives of its single-slice run on the same blocks (1e-5 / 1e-6)."""
+    slice the iterates and objectives of its single-slice run on the same blocks (1e-5 / 1e-6); at 131 detector rows
+    (two groups of 5 y tiles) the grouped launch runs its form with the tile count compiled in."""
     from diffractom_amd import SPDHGPoisson
     from diffractom_amd.batched import BatchedSPDHGPoisson
-    print(f"case batched blocks {precision}", flush=True)
-    c = cfg(nx=20, ny=20, my=21, n_omega=24, angle_range=(0, 180), cor=0.3)
+    print(f"case batched blocks {precision} my={my}", flush=True)
+    c = cfg(nx=20, ny=20, my=my, n_omega=24, angle_range=(0, 180), cor=0.3)
     ops, bs = [], []
     rng = np.random.default_rng(12)
     for i, K in enumerate((20, 75, 130)):
