@@ -361,11 +361,11 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
   // reads, twice the LDS)
   const int rb = lines & 7;
   const bool pipe = (lines & 8) == 0, dup = (lines & 16) != 0;
-  DFM_CHECK_ARG((rb == 1 || rb == 2 || rb == 4) && (lines & ~31) == 0 &&
+  DFM_CHECK_ARG((rb == 1 || rb == 2 || (rb == 4 && len <= 128)) && (lines & ~31) == 0 &&
                     (pipe || rb == 1 || (rb == 2 && len <= 128)) &&
                     (!dup || (pipe && (rb == 1 || (rb == 2 && len <= 128)))),
-                "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or 4, or 1 + 8 or (lines <= 128 pixels) 2 + 8, or "
-                "1 + 16 or (lines <= 128 pixels) 2 + 16", lines);
+                "dfm_radon_forward_pairs_sweep: lines %d not 1, 2 or (lines <= 128 pixels) 4, or 1 + 8 or (lines <= "
+                "128 pixels) 2 + 8, or 1 + 16 or (lines <= 128 pixels) 2 + 16", lines);
   // staged lines, or the store transpose (64 lanes x 8 quads per wave), whichever is larger
   size_t lds = dup ? (size_t)2 * rb * (len + 5) * 16 * sizeof(float4)
                    : (size_t)2 * rb * (len + 4) * 8 * sizeof(float4);
@@ -395,7 +395,7 @@ int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double*
   } else if (rb == 2) {
     if (small) { DFM_PS(2, 4) } else { DFM_PS1(2, 8, true, false); }
   } else {
-    if (small) { DFM_PS1(4, 8, true, false); } else { DFM_PS1(4, 16, true, false); }
+    DFM_PS1(4, 8, true, false);
   }
 #undef DFM_PS1
 #undef DFM_PS

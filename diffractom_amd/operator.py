@@ -1106,7 +1106,8 @@ class SinglePhaseForwardOperator:
                 # VGPRs: two workgroups per CU)
                 cands.update({f"psweep:{n}+dup": ("psweep", n + 16) for n in (2, 1)
                               if (n == 1 or ln <= 128) and 2 * n * (ln + 5) * 256 <= 163840})
-                cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4) if 2 * n * (ln + 4) * 128 <= 163840})
+                cands.update({f"psweep:{n}": ("psweep", n) for n in (2, 1, 4)
+                              if 2 * n * (ln + 4) * 128 <= 163840 and (n < 4 or ln <= 128)})
                 cands.update({f"psweep:{n}+np": ("psweep", n + 8) for n in (2, 1) if n == 1 or ln <= 128})
             if self.radon_bands > 1:
                 cands.update({"band:8+ch128": ("band", 8 + 16), "band:8": ("band", 8), "band:4": ("band", 4),

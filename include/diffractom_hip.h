@@ -103,10 +103,10 @@ int dfm_radon_forward_sweep(const float* img_k, float* sino, const double* geo, 
 int dfm_radon_sweep_lines(int32_t nx, int32_t ny);
 /* Line sweep with two neighbouring rays per thread (the same result as dfm_radon_forward_sweep): groups of na_grp
  * angles with na_grp * ceil(ns / 2) <= 512; usable when (2 hw + 1) / max(|X|, |Y|) < 4 for every angle (square
- * images with the trapezoid footprint); lines: image lines staged per LDS block (1, 2 or 4), + 8 for the form that
- * loads each block at its start instead of holding the next block's loads across the sums (lines 1, or 2 for images
- * of at most 128 pixels a side), or + 16 for the layout holding every staged pixel twice so that the LDS reads of
- * 16 lanes never share a bank (the same lines; twice the LDS). */
+ * images with the trapezoid footprint); lines: image lines staged per LDS block (1, 2, or 4 for images of at most
+ * 128 pixels a side), + 8 for the form that loads each block at its start instead of holding the next block's loads
+ * across the sums (lines 1, or 2 for images of at most 128 pixels a side), or + 16 for the layout holding every
+ * staged pixel twice so that the LDS reads of 16 lanes never share a bank (the same lines; twice the LDS). */
 int dfm_radon_forward_pairs_sweep(const float* img_k, float* sino, const double* geo, const int32_t* grp_ang,
                                   const int32_t* grp_cls, int32_t n_grp, int32_t na_grp, int32_t ns, int32_t nx,
                                   int32_t ny, int32_t kstride, int32_t k_off, int32_t k_n, int32_t skstride,

@@ -326,7 +326,8 @@ def test_dense_fp32_reduction_split():
 def test_pairs_sweep_forms_agree_on_long_lines(nx):
     """The ray-pair sweep's forms (1, 2 or 4 staged lines; each block loaded at its start; every pixel held twice in
     LDS) compute every ray's taps in the same order, so their sinograms agree bit for bit; at 99 and 150 pixels a
-    line (the 128-pixel limit of the two-line loaded-at-start and duplicated forms in between), 64 orientations,
+    line (the 128-pixel limit of the four-line and the two-line loaded-at-start and duplicated forms in between),
+    64 orientations,
     a subset of the angles; and one of them against the table forward (the projector the tests above pin)."""
     import torch
     print(f"case pairs sweep forms nx={nx}", flush=True)
@@ -341,7 +342,7 @@ def test_pairs_sweep_forms_agree_on_long_lines(nx):
     g = torch.Generator(device="cuda")
     g.manual_seed(11)
     img = torch.rand(op.Nx * op.Ny * ks, generator=g, device="cuda")
-    forms = [1, 2, 4, 9, 17] + ([10, 18] if nx <= 128 else [])
+    forms = [1, 2, 9, 17] + ([4, 10, 18] if nx <= 128 else [])
     res = {}
     for v in forms + ["table"]:
         sino = torch.zeros(n_ang * op.My * kn, device="cuda")
