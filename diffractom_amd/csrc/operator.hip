@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(256) radon_backward_tile_kernel(const float4* 
 // the per-block work is the sinogram staging (all of a pass's loads in flight) and the pixel sums. Blocks
 // g*G .. g*G+G-1 run on XCD g % 8, so their sinogram slabs stay in its L2. The blocks of a workgroup run one after
 // another (QW = 16 quads: 64 channels per block).
-constexpr int TCH = 32;   // angles whose table rows are staged at once
+constexpr int TCH = 40;   // angles whose table rows are staged at once (a subset of 36 at C2 in one go)
 
 template <int MODE, int ACH, int G, int TS, int QW>
 __global__ void __launch_bounds__(TS * TS * QW, TS == 8 ? 8 : 1) radon_backward_group_kernel(
